@@ -1,0 +1,68 @@
+"""Mesh and matrix file formats used by the reference's scenes.
+
+* Medit ``.mesh`` tetrahedral meshes (read via libigl's readMESH in the
+  reference, src/simulation_state.cpp:62-68; 1-based indices).
+* libigl ASCII ``.dmat`` matrices (``cols rows`` header, then column-major
+  values; reference initial-state files scenes/data/*.dmat and
+  src/simulation_state.cpp:263-308).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def read_mesh(path: str):
+    """Read a Medit .mesh file. Returns V (nV x 3 float64), T (nT x 4 int32, 0-based)."""
+    with open(path) as f:
+        toks = []
+        for line in f:
+            line = line.split("#", 1)[0]
+            toks.extend(line.split())
+    i = 0
+    V = T = None
+    while i < len(toks):
+        t = toks[i]
+        if t == "Vertices":
+            n = int(toks[i + 1])
+            vals = np.array(toks[i + 2:i + 2 + 4 * n], dtype=np.float64).reshape(n, 4)
+            V = vals[:, :3].copy()
+            i += 2 + 4 * n
+        elif t == "Tetrahedra":
+            n = int(toks[i + 1])
+            vals = np.array(toks[i + 2:i + 2 + 5 * n], dtype=np.int64).reshape(n, 5)
+            T = (vals[:, :4] - 1).astype(np.int32)
+            i += 2 + 5 * n
+        elif t == "Triangles":
+            n = int(toks[i + 1])
+            i += 2 + 4 * n
+        elif t == "Edges":
+            n = int(toks[i + 1])
+            i += 2 + 3 * n
+        else:
+            i += 1
+    if V is None or T is None:
+        raise ValueError(f"{path}: missing Vertices or Tetrahedra")
+    return V, T
+
+
+def read_dmat(path: str) -> np.ndarray:
+    """Read a libigl ASCII .dmat file into a (rows x cols) array."""
+    with open(path) as f:
+        head = f.readline().split()
+        cols, rows = int(head[0]), int(head[1])
+        data = np.array(f.read().split(), dtype=np.float64)
+    if data.size != rows * cols:
+        raise ValueError(f"{path}: expected {rows * cols} values, got {data.size}")
+    return data.reshape(cols, rows).T.copy()
+
+
+def write_dmat(path: str, A: np.ndarray) -> None:
+    """Write a 2D array as a libigl ASCII .dmat file."""
+    A = np.atleast_2d(np.asarray(A, dtype=np.float64))
+    if A.ndim == 2 and A.shape[0] == 1 and A.shape[1] > 1:
+        A = A.T
+    rows, cols = A.shape
+    with open(path, "w") as f:
+        f.write(f"{cols} {rows}\n")
+        for v in A.T.reshape(-1):
+            f.write(f"{v:.17g}\n")

@@ -1,0 +1,97 @@
+// ORACLE — test infrastructure only. Not part of the product library.
+//
+// FP64 CPU restatement of the reference's per-Newton-step mixed-FEM path
+// (tytrusty/mfem). Only tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline may load it, and only as the checker; the product library never
+// links or calls it.
+#pragma once
+#include <cstdint>
+
+#ifdef __cplusplus
+#include "omath.h"
+namespace orc {
+enum { MAT_SNH = 0, MAT_NH = 1, MAT_COROT = 2, MAT_FCR = 3, MAT_ARAP = 4, MAT_FUNG = 5 };
+int newton_procrustes(M3& R, const M3& A, const M3& B, bool compute_gradients,
+                      Mat<9, 9>* dRdF, double tol, int max_iter, int* iters);
+int polar_svd3(M3& R, Vec<6>& s, const M3& F, bool compute_gradients,
+               Mat<6, 9>* dsdF, int* iters);
+void material_s6(int type, double mu, double la, const Vec<6>& s, bool psd_fix,
+                 double* E, Vec<6>* g, Mat<6, 6>* H);
+void snh_f9(double mu, double la, const Vec<9>& f, double* E, Vec<9>* g,
+            Mat<9, 9>* H);
+void snh_f4(double mu, double la, const Vec<4>& f, double* E, Vec<4>* g,
+            Mat<4, 4>* H);
+}  // namespace orc
+extern "C" {
+#endif
+
+// Simulation parameters (mirrors the reference's SimConfig + scene objects).
+typedef struct {
+  int dim;            // 2 or 3
+  int nV, nT;
+  const double* V;    // nV*dim, row per vertex
+  const int* T;       // nT*(dim+1)
+  const double* mu;   // nT per-element Lame mu
+  const double* la;   // nT per-element Lame lambda
+  const int* mat;     // nT material type (MAT_*)
+  const int* fixed;   // nV, 1 = pinned
+  const double* v0;   // nV*dim initial velocity, or NULL
+  double density;
+  double dt;
+  double gravity[3];
+  int bdf_order;      // 1..6
+  int mixed;          // 1: "mixed-stretch", 0: "stretch"
+  int solver;         // 0: dense LLT ("eigen-llt"), 1: Jacobi CG ("eigen-pcg-diag")
+  double itr_tol;
+  int max_iters;
+  int max_newton;
+  double newton_tol;
+  int ls_iters;
+  int threads;        // OpenMP threads (<= 0: runtime default)
+} oracle_params;
+
+typedef struct {
+  int newton_iters;
+  int pcg_iters_total;
+  int ls_fail;
+  double last_decrement;
+  double last_energy;
+  double last_alpha;
+  double last_res;
+} oracle_stats;
+
+// element-level entry points
+int oracle_newton_procrustes(double* R, const double* A, const double* B,
+                             int compute_gradients, double* dRdF, double tol,
+                             int max_iter, int* iters);
+int oracle_polar_svd3(double* R, double* s, const double* F, int grad,
+                      double* dsdF, int* iters);
+void oracle_material_s6(int type, double mu, double la, const double* s,
+                        int psd_fix, double* E, double* g, double* H);
+void oracle_snh_f9(double mu, double la, const double* F, double* E,
+                   double* g, double* H);
+void oracle_snh_f4(double mu, double la, const double* F, double* E,
+                   double* g, double* H);
+void oracle_psd_fix(int n, double* A, double tol);
+void oracle_bc_fixed(int type, const double* V, int nV, int dim, int axis,
+                     double ratio, int* fixed);
+void oracle_tet_geometry(const double* V, const int* T, int nT, double* vols,
+                         double* dX);
+
+// simulation-level entry points
+void* oracle_sim_create(const oracle_params* p);
+void oracle_sim_destroy(void* sim);
+int oracle_sim_step(void* sim);              // one timestep, 0 on success
+int oracle_sim_update_system(void* sim);     // only update_system() of step
+int oracle_sim_solve(void* sim);             // only substep() of step
+void oracle_sim_stats(void* sim, oracle_stats* st);
+void oracle_sim_get_positions(void* sim, double* out);  // nV*dim
+// Fetch an internal array by name into out (size from oracle_sim_size).
+long oracle_sim_size(void* sim, const char* name);
+int oracle_sim_get(void* sim, const char* name, double* out);
+int oracle_sim_get_int(void* sim, const char* name, int* out);
+double oracle_sim_energy(void* sim, double alpha);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,257 @@
+"""ORACLE — test infrastructure only (ctypes binding of oracle/lib/liboracle.so).
+
+The FP64 CPU restatement of the reference path. Only tests/, the smoke check
+and bench.py's CPU-baseline leg import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
+_lib = None
+
+SOLVERS = {"eigen-llt": 0, "eigen-ldlt": 0, "cholmod": 0, "eigen-pcg-diag": 1}
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("nV", C.c_int), ("nT", C.c_int),
+        ("V", C.POINTER(C.c_double)), ("T", C.POINTER(C.c_int)),
+        ("mu", C.POINTER(C.c_double)), ("la", C.POINTER(C.c_double)),
+        ("mat", C.POINTER(C.c_int)), ("fixed", C.POINTER(C.c_int)),
+        ("v0", C.POINTER(C.c_double)),
+        ("density", C.c_double), ("dt", C.c_double), ("gravity", C.c_double * 3),
+        ("bdf_order", C.c_int), ("mixed", C.c_int), ("solver", C.c_int),
+        ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
+        ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("threads", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("newton_iters", C.c_int), ("pcg_iters_total", C.c_int), ("ls_fail", C.c_int),
+        ("last_decrement", C.c_double), ("last_energy", C.c_double),
+        ("last_alpha", C.c_double), ("last_res", C.c_double),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.oracle_newton_procrustes.argtypes = [dp, dp, dp, C.c_int, dp, C.c_double, C.c_int, ip]
+        L.oracle_polar_svd3.argtypes = [dp, dp, dp, C.c_int, dp, ip]
+        L.oracle_material_s6.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, dp]
+        L.oracle_snh_f9.argtypes = [C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_snh_f4.argtypes = [C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_psd_fix.argtypes = [C.c_int, dp, C.c_double]
+        L.oracle_bc_fixed.argtypes = [C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_double, ip]
+        L.oracle_tet_geometry.argtypes = [dp, ip, C.c_int, dp, dp]
+        L.oracle_sim_create.argtypes = [C.POINTER(Params)]
+        L.oracle_sim_create.restype = C.c_void_p
+        for f in ("oracle_sim_destroy",):
+            getattr(L, f).argtypes = [C.c_void_p]
+        for f in ("oracle_sim_step", "oracle_sim_update_system", "oracle_sim_solve"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_int
+        L.oracle_sim_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.oracle_sim_get_positions.argtypes = [C.c_void_p, dp]
+        L.oracle_sim_size.argtypes = [C.c_void_p, C.c_char_p]
+        L.oracle_sim_size.restype = C.c_long
+        L.oracle_sim_get.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.oracle_sim_get.restype = C.c_int
+        L.oracle_sim_get_int.argtypes = [C.c_void_p, C.c_char_p, ip]
+        L.oracle_sim_get_int.restype = C.c_int
+        L.oracle_sim_energy.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_sim_energy.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---- element level (column-major 3x3 / 6x6 / 9x9 as flat arrays) ----------
+
+def newton_procrustes(R, A, B, compute_gradients=False, tol=1e-6, max_iter=100):
+    R = f64(np.asarray(R).reshape(3, 3).T.reshape(-1)).copy()
+    A = f64(np.asarray(A).reshape(3, 3).T.reshape(-1))
+    B = f64(np.asarray(B).reshape(3, 3).T.reshape(-1))
+    d = np.zeros(81)
+    it = C.c_int(0)
+    rc = lib().oracle_newton_procrustes(_dp(R), _dp(A), _dp(B), int(compute_gradients), _dp(d),
+                                        tol, max_iter, C.byref(it))
+    return R.reshape(3, 3).T.copy(), d.reshape(9, 9).T.copy(), rc, it.value
+
+
+def polar_svd3(F, R0=None, grad=True):
+    """F (3x3 array). Returns R (3x3), s (6), dsdF (6x9, column-major vec F)."""
+    R = np.eye(3) if R0 is None else np.asarray(R0, dtype=np.float64)
+    Rc = f64(R.T.reshape(-1)).copy()
+    Fc = f64(np.asarray(F, dtype=np.float64).T.reshape(-1))
+    s = np.zeros(6)
+    d = np.zeros(54)
+    it = C.c_int(0)
+    rc = lib().oracle_polar_svd3(_dp(Rc), _dp(s), _dp(Fc), int(grad), _dp(d), C.byref(it))
+    return Rc.reshape(3, 3).T.copy(), s, d.reshape(9, 6).T.copy(), rc, it.value
+
+
+def material_s6(mtype, mu, la, s, psd_fix=True):
+    s = f64(s)
+    E = np.zeros(1)
+    g = np.zeros(6)
+    H = np.zeros(36)
+    lib().oracle_material_s6(int(mtype), float(mu), float(la), _dp(s), int(psd_fix),
+                             _dp(E), _dp(g), _dp(H))
+    return E[0], g, H.reshape(6, 6).T.copy()
+
+
+def snh_f9(mu, la, F9):
+    F9 = f64(F9)
+    E = np.zeros(1)
+    g = np.zeros(9)
+    H = np.zeros(81)
+    lib().oracle_snh_f9(float(mu), float(la), _dp(F9), _dp(E), _dp(g), _dp(H))
+    return E[0], g, H.reshape(9, 9).T.copy()
+
+
+def snh_f4(mu, la, F4):
+    F4 = f64(F4)
+    E = np.zeros(1)
+    g = np.zeros(4)
+    H = np.zeros(16)
+    lib().oracle_snh_f4(float(mu), float(la), _dp(F4), _dp(E), _dp(g), _dp(H))
+    return E[0], g, H.reshape(4, 4).T.copy()
+
+
+def psd_fix(A, tol=1e-6):
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    a = f64(A.T.reshape(-1)).copy()
+    lib().oracle_psd_fix(n, _dp(a), float(tol))
+    return a.reshape(n, n).T.copy()
+
+
+def bc_fixed(kind_id, V, axis, ratio):
+    V = f64(V)
+    out = np.zeros(V.shape[0], dtype=np.int32)
+    lib().oracle_bc_fixed(int(kind_id), _dp(V), V.shape[0], V.shape[1], int(axis), float(ratio),
+                          _ip(out))
+    return out
+
+
+def tet_geometry(V, T):
+    V, T = f64(V), i32(T)
+    vols = np.zeros(T.shape[0])
+    dX = np.zeros(T.shape[0] * 12)
+    lib().oracle_tet_geometry(_dp(V), _ip(T), T.shape[0], _dp(vols), _dp(dX))
+    return vols, dX.reshape(-1, 4, 3)
+
+
+# ---- simulation level --------------------------------------------------------
+
+class Sim:
+    """One reference-style simulation (SimState + NewtonOptimizer) in FP64 on the CPU."""
+
+    def __init__(self, scene, threads=0):
+        self._keep = []
+        p = Params()
+        sc = scene
+        p.dim = sc.dim
+        p.nV, p.nT = sc.nV, sc.nT
+        arrs = dict(V=f64(sc.V), T=i32(sc.T), mu=f64(sc.mu), la=f64(sc.la), mat=i32(sc.mat),
+                    fixed=i32(sc.fixed))
+        self._keep.append(arrs)
+        p.V, p.T = _dp(arrs["V"]), _ip(arrs["T"])
+        p.mu, p.la, p.mat, p.fixed = _dp(arrs["mu"]), _dp(arrs["la"]), _ip(arrs["mat"]), _ip(arrs["fixed"])
+        if sc.v0 is not None:
+            v0 = f64(sc.v0)
+            self._keep.append(v0)
+            p.v0 = _dp(v0)
+        else:
+            p.v0 = C.POINTER(C.c_double)()
+        p.density, p.dt = sc.density, sc.dt
+        g = list(sc.gravity) + [0.0] * (3 - len(sc.gravity))
+        p.gravity = (C.c_double * 3)(*g[:3])
+        p.bdf_order = sc.bdf_order
+        p.mixed = int(sc.mixed)
+        p.solver = SOLVERS[sc.solver]
+        p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
+        p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
+        p.threads = threads
+        self.p = p
+        self.scene = sc
+        self.h = lib().oracle_sim_create(C.byref(p))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().oracle_sim_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self):
+        rc = lib().oracle_sim_step(self.h)
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed ({rc})")
+
+    def update_system(self):
+        lib().oracle_sim_update_system(self.h)
+
+    def solve(self):
+        rc = lib().oracle_sim_solve(self.h)
+        if rc != 0:
+            raise RuntimeError(f"oracle solve failed ({rc})")
+
+    def energy(self, alpha):
+        return lib().oracle_sim_energy(self.h, float(alpha))
+
+    def stats(self):
+        st = Stats()
+        lib().oracle_sim_stats(self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def positions(self):
+        out = np.zeros(self.scene.nV * self.scene.dim)
+        lib().oracle_sim_get_positions(self.h, _dp(out))
+        return out.reshape(self.scene.nV, self.scene.dim)
+
+    def get(self, name):
+        n = lib().oracle_sim_size(self.h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        lib().oracle_sim_get(self.h, name.encode(), _dp(out))
+        return out
+
+    def get_int(self, name, n):
+        out = np.zeros(n, dtype=np.int32)
+        if lib().oracle_sim_get_int(self.h, name.encode(), _ip(out)) != 0:
+            raise KeyError(name)
+        return out

@@ -1,0 +1,1012 @@
+// ORACLE — test infrastructure only. Not part of the product library.
+//
+// FP64 CPU restatement of one timestep of the reference (tytrusty/mfem):
+//   mesh setup        reference src/mesh/mesh.cpp:44-95, tet_mesh.cpp:16-80,
+//                     tri2d_mesh.cpp:15-77, utils/pinning_matrix.cpp:5-34
+//   boundary groups   boundary_conditions/boundary_condition.cpp:5-21,
+//                     fixed_boundary_conditions.h:75-166
+//   Assembler         utils/sparse_utils.cpp:19-203 (free-free pairs sorted by
+//                     global (row, col); duplicates summed in sorted order)
+//   Displacement      variables/displacement.cpp:21-98
+//   Stretch (plain)   variables/stretch.cpp:15-89
+//   MixedStretch      variables/mixed_stretch.cpp:83-332
+//   SystemMatrixPD    linear_solvers/linear_system.h:18-40
+//   Eigen CG(Jacobi)  linear_solver_factory.cpp:144-154 (x0 = 0, stop at
+//                     |r| < tol |b|, Jacobi preconditioner)
+//   Newton + LS       optimizers/newton_optimizer.cpp:11-140, linesearch.h:78-131
+//   BDF1..6           time_integrators/BDF.cpp:7-93, BDF.h:19-26
+// Element-local loops run under OpenMP (as the reference's #pragma omp loops);
+// every reduction here is serial and therefore deterministic.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle.h"
+
+using namespace orc;
+
+namespace {
+
+// Free-free vertex-pair scatter map (reference sparse_utils.cpp:19-138).
+// Duplicate (row, col) entries are kept in (element, local j, local k) order,
+// a total order the reference leaves to std::sort.
+struct Assembler {
+  int nodes = 0;  // vertices per element
+  int dim = 0;
+  std::vector<int> element_ids;
+  std::vector<std::pair<int, int>> global_pairs, local_pairs;
+  std::vector<int> multiplicity, row_offsets, offsets;
+  // CSR of the scalar matrix (full DIMxDIM blocks)
+  int n = 0;
+  std::vector<int> rowptr, colind;
+  void build(const std::vector<int>& T, int nT, int nodes_, int dim_,
+             const std::vector<int>& free_map, int nfree) {
+    nodes = nodes_; dim = dim_;
+    std::vector<int> eids; std::vector<std::pair<int, int>> gp, lp;
+    for (int i = 0; i < nT; ++i)
+      for (int j = 0; j < nodes; ++j) {
+        int vj = T[i * nodes + j];
+        if (vj < 0 || free_map[vj] < 0) continue;
+        for (int k = 0; k < nodes; ++k) {
+          int vk = T[i * nodes + k];
+          if (vk < 0 || free_map[vk] < 0) continue;
+          eids.push_back(i);
+          gp.push_back({free_map[vj], free_map[vk]});
+          lp.push_back({j, k});
+        }
+      }
+    std::vector<int> ids(eids.size());
+    std::iota(ids.begin(), ids.end(), 0);
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
+      return gp[a] < gp[b];
+    });
+    element_ids.resize(ids.size()); global_pairs.resize(ids.size());
+    local_pairs.resize(ids.size());
+    for (size_t i = 0; i < ids.size(); ++i) {
+      element_ids[i] = eids[ids[i]];
+      global_pairs[i] = gp[ids[i]];
+      local_pairs[i] = lp[ids[i]];
+    }
+    multiplicity.assign(ids.size(), 1);
+    row_offsets.clear(); offsets.clear();
+    size_t i = 0; int curr_row = -1;
+    while (i < multiplicity.size()) {
+      auto cp = global_pairs[i];
+      if (cp.first != curr_row) { row_offsets.push_back((int)i); curr_row = cp.first; }
+      while (i + multiplicity[i] < multiplicity.size() &&
+             global_pairs[i + multiplicity[i]] == cp) ++multiplicity[i];
+      offsets.push_back((int)i);
+      i += multiplicity[i];
+    }
+    row_offsets.push_back((int)multiplicity.size());
+    // scalar CSR pattern: row (dim*br + r) lists blocks in column order
+    n = dim * nfree;
+    rowptr.assign(n + 1, 0);
+    int nb_rows = (int)row_offsets.size() - 1;
+    std::vector<std::vector<int>> bcols(nfree);
+    for (int ii = 0; ii < nb_rows; ++ii) {
+      int beg = row_offsets[ii], end = row_offsets[ii + 1];
+      int br = global_pairs[beg].first;
+      while (beg < end) {
+        bcols[br].push_back(global_pairs[beg].second);
+        beg += multiplicity[beg];
+      }
+    }
+    for (int br = 0; br < nfree; ++br)
+      for (int r = 0; r < dim; ++r)
+        rowptr[dim * br + r + 1] = (int)bcols[br].size() * dim;
+    for (int k = 0; k < n; ++k) rowptr[k + 1] += rowptr[k];
+    colind.assign(rowptr[n], 0);
+    for (int br = 0; br < nfree; ++br)
+      for (int r = 0; r < dim; ++r) {
+        int p = rowptr[dim * br + r];
+        for (int bc : bcols[br])
+          for (int c = 0; c < dim; ++c) colind[p++] = dim * bc + c;
+      }
+  }
+  // blocks: per element, a (nodes*dim)^2 column-major matrix
+  // (reference sparse_utils.cpp:141-203)
+  void update_matrix(const std::vector<double>& blocks, std::vector<double>& vals) const {
+    int M = nodes * dim;
+    vals.assign(rowptr[n], 0.0);
+    int nb_rows = (int)row_offsets.size() - 1;
+#pragma omp parallel for schedule(static)
+    for (int ii = 0; ii < nb_rows; ++ii) {
+      int beg = row_offsets[ii], end = row_offsets[ii + 1];
+      int block_col = 0;
+      while (beg < end) {
+        int nm = multiplicity[beg];
+        auto g = global_pairs[beg];
+        double lb[9] = {0};
+        for (int q = 0; q < nm; ++q) {
+          int e = element_ids[beg + q];
+          auto l = local_pairs[beg + q];
+          const double* B = &blocks[(size_t)e * M * M];
+          for (int r = 0; r < dim; ++r)
+            for (int c = 0; c < dim; ++c)
+              lb[r * 3 + c] += B[(dim * l.first + r) + M * (dim * l.second + c)];
+        }
+        for (int r = 0; r < dim; ++r) {
+          int start = rowptr[dim * g.first + r] + dim * block_col;
+          for (int c = 0; c < dim; ++c) vals[start + c] = lb[r * 3 + c];
+        }
+        beg += nm;
+        ++block_col;
+      }
+    }
+  }
+};
+
+double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+}  // namespace
+
+struct OracleSim {
+  oracle_params p;
+  int dim, nodes, nV, nT;
+  std::vector<double> Vref, Vcur;
+  std::vector<int> T, mat, fixed, free_map;
+  std::vector<double> mu, la;
+  int nfree = 0, n = 0;
+  double h = 0;  // integrator dt() = beta * h
+  // mesh quantities
+  std::vector<double> vols, dX;  // dX[e][k][c], nodes x dim per element
+  Assembler asmb;
+  std::vector<double> PMP;  // values on asmb pattern
+  // displacement
+  std::vector<double> x, b, dx, f_ext, grad_x_disp;
+  std::deque<std::vector<double>> x_prevs, v_prevs;
+  std::vector<double> alphas;
+  double beta = 1;
+  // mixed stretch (3D)
+  std::vector<M3> R;
+  std::vector<Vec<6>> S, g, gl;
+  std::vector<double> s, lam, ds;
+  std::vector<Mat<9, 6>> dSdF;
+  std::vector<Mat<6, 6>> Hm, Hinv, Hloc;
+  std::vector<double> Aloc;  // per element (nodes*dim)^2
+  std::vector<double> grad_x_mixed, grad_s;
+  // plain stretch
+  std::vector<double> Hp, gp, grad_plain;
+  // system
+  std::vector<double> lhs, rhs;
+  oracle_stats st{};
+  int pcg_iters_last = 0;
+  double pcg_err_last = 0;
+};
+
+namespace {
+
+int mat_dim(const OracleSim* S_) { return S_->dim * S_->dim; }
+
+// x_full = P^T x + b  (reference displacement.h:49-53)
+void unproject(const OracleSim* S_, const std::vector<double>& xf, std::vector<double>& full) {
+  full = S_->b;
+  for (int v = 0; v < S_->nV; ++v) {
+    int f = S_->free_map[v];
+    if (f < 0) continue;
+    for (int d = 0; d < S_->dim; ++d) full[S_->dim * v + d] += xf[S_->dim * f + d];
+  }
+}
+
+// F_e = Jloc_e q_e  (column-major vec; reference tet_mesh.cpp:16-27)
+void def_grad(const OracleSim* S_, const std::vector<double>& full, int e, double* F) {
+  int D = S_->dim;
+  for (int i = 0; i < D * D; ++i) F[i] = 0;
+  for (int k = 0; k < S_->nodes; ++k) {
+    int v = S_->T[e * S_->nodes + k];
+    const double* dx = &S_->dX[((size_t)e * S_->nodes + k) * D];
+    for (int c = 0; c < D; ++c)
+      for (int r = 0; r < D; ++r) F[r + D * c] += full[D * v + r] * dx[c];
+  }
+}
+
+// out += sign * PJW * tmp, tmp holding one DxD column-major vector per
+// element (reference mesh.cpp:63: PJW = P J^T W).
+void apply_PJW(const OracleSim* S_, const std::vector<double>& tmp, std::vector<double>& out, double sign) {
+  int D = S_->dim, M = D * D;
+  std::vector<double> full(S_->nV * D, 0.0);
+  for (int e = 0; e < S_->nT; ++e) {
+    double vol = S_->vols[e];
+    for (int k = 0; k < S_->nodes; ++k) {
+      int v = S_->T[e * S_->nodes + k];
+      const double* dx = &S_->dX[((size_t)e * S_->nodes + k) * D];
+      for (int r = 0; r < D; ++r) {
+        double a = 0;
+        for (int c = 0; c < D; ++c) a += dx[c] * tmp[(size_t)M * e + r + D * c];
+        full[D * v + r] += vol * a;
+      }
+    }
+  }
+  for (int v = 0; v < S_->nV; ++v) {
+    int f = S_->free_map[v];
+    if (f < 0) continue;
+    for (int r = 0; r < D; ++r) out[D * f + r] += sign * full[D * v + r];
+  }
+}
+
+// full mass product  y = M d  (consistent linear-element mass)
+void mass_mult_full(const OracleSim* S_, const std::vector<double>& d, std::vector<double>& y) {
+  int D = S_->dim, N = S_->nodes;
+  y.assign(S_->nV * D, 0.0);
+  double denom = (D == 3) ? 20.0 : 12.0;
+  for (int e = 0; e < S_->nT; ++e) {
+    double m = S_->p.density * S_->vols[e] / denom;
+    for (int a = 0; a < N; ++a) {
+      int va = S_->T[e * N + a];
+      for (int bb = 0; bb < N; ++bb) {
+        int vb = S_->T[e * N + bb];
+        double c = m * (a == bb ? 2.0 : 1.0);
+        for (int r = 0; r < D; ++r) y[D * va + r] += c * d[D * vb + r];
+      }
+    }
+  }
+}
+
+std::vector<double> x_tilde(const OracleSim* S_) {
+  size_t N = S_->x_prevs.front().size();
+  std::vector<double> wx(N, 0.0), wv(N, 0.0);
+  for (size_t i = 0; i < S_->x_prevs.size(); ++i)
+    for (size_t k = 0; k < N; ++k) {
+      wx[k] += S_->alphas[i] * S_->x_prevs[i][k];
+      wv[k] += S_->alphas[i] * S_->v_prevs[i][k];
+    }
+  for (size_t k = 0; k < N; ++k) wx[k] += S_->h * wv[k];
+  return wx;
+}
+
+// d = P^T x + b - x_tilde - h^2 f_ext  (reference displacement.cpp:21-31,64-73)
+std::vector<double> inertia_diff(const OracleSim* S_, const std::vector<double>& xf) {
+  std::vector<double> full;
+  unproject(S_, xf, full);
+  std::vector<double> xt = x_tilde(S_);
+  double h2 = S_->h * S_->h;
+  for (size_t k = 0; k < full.size(); ++k) full[k] = full[k] - xt[k] - h2 * S_->f_ext[k];
+  return full;
+}
+
+double disp_energy(const OracleSim* S_, const std::vector<double>& xf) {
+  std::vector<double> d = inertia_diff(S_, xf), Md;
+  mass_mult_full(S_, d, Md);
+  return 0.5 * dotv(d, Md);
+}
+
+void disp_gradient(OracleSim* S_) {
+  std::vector<double> d = inertia_diff(S_, S_->x), Md;
+  mass_mult_full(S_, d, Md);
+  S_->grad_x_disp.assign(S_->n, 0.0);
+  for (int v = 0; v < S_->nV; ++v) {
+    int f = S_->free_map[v];
+    if (f < 0) continue;
+    for (int r = 0; r < S_->dim; ++r) S_->grad_x_disp[S_->dim * f + r] = Md[S_->dim * v + r];
+  }
+}
+
+const double kSym[6] = {1, 1, 1, 2, 2, 2};
+const double kSyminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+
+// ---------------- MixedStretch (3D) ----------------
+void mixed_update_rotations(OracleSim* S_, const std::vector<double>& full) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    double f[9]; def_grad(S_, full, e, f);
+    M3 F; for (int i = 0; i < 9; ++i) F[i] = f[i];
+    Mat<6, 9> dsdF;
+    polar_svd3(S_->R[e], S_->S[e], F, true, &dsdF, nullptr);
+    S_->dSdF[e] = dsdF.T() * diag<6>(kSym);
+  }
+}
+
+void mixed_update_derivatives(OracleSim* S_) {
+  double h2 = S_->h * S_->h;
+  int M = 12;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    double vol = S_->vols[e];
+    Vec<6> si; for (int i = 0; i < 6; ++i) si[i] = S_->s[6 * e + i];
+    Vec<6> gg; Mat<6, 6> HH;
+    material_s6(S_->mat[e], S_->mu[e], S_->la[e], si, true, nullptr, &gg, &HH);
+    Mat<6, 6> H = h2 * HH;
+    S_->Hinv[e] = lu_inverse<6>(H);
+    S_->g[e] = h2 * gg;
+    S_->Hm[e] = (1.0 / vol) * (diag<6>(kSyminv) * H * diag<6>(kSyminv));
+    S_->Hloc[e] = vol * H;
+  }
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    double vol = S_->vols[e];
+    Mat<9, 9> inner = S_->dSdF[e] * S_->Hm[e] * S_->dSdF[e].T();
+    // Jloc^T inner Jloc, Jloc(r + 3c, 3k + r) = dX(k, c)
+    double* A = &S_->Aloc[(size_t)e * M * M];
+    const double* dx = &S_->dX[(size_t)e * 12];
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r)
+        for (int l = 0; l < 4; ++l)
+          for (int q = 0; q < 3; ++q) {
+            double a = 0;
+            for (int c = 0; c < 3; ++c)
+              for (int c2 = 0; c2 < 3; ++c2)
+                a += dx[3 * k + c] * inner(r + 3 * c, q + 3 * c2) * dx[3 * l + c2];
+            A[(3 * k + r) + M * (3 * l + q)] = a * vol * vol;
+          }
+  }
+  S_->asmb.update_matrix(S_->Aloc, S_->lhs);  // temporarily holds A
+  // grad_x = -PJW (dSdF la)
+  std::vector<double> tmp(9 * S_->nT);
+  for (int e = 0; e < S_->nT; ++e) {
+    Vec<6> l; for (int i = 0; i < 6; ++i) l[i] = S_->lam[6 * e + i];
+    Vec<9> t = S_->dSdF[e] * l;
+    for (int i = 0; i < 9; ++i) tmp[9 * e + i] = t[i];
+  }
+  S_->grad_x_mixed.assign(S_->n, 0.0);
+  apply_PJW(S_, tmp, S_->grad_x_mixed, -1.0);
+  S_->grad_s.resize(6 * S_->nT);
+  for (int e = 0; e < S_->nT; ++e)
+    for (int i = 0; i < 6; ++i) S_->grad_s[6 * e + i] = S_->vols[e] * S_->g[e][i];
+}
+
+// rhs of the mixed variable (reference mixed_stretch.cpp:207-226)
+std::vector<double> mixed_rhs(OracleSim* S_) {
+  std::vector<double> out(S_->n, 0.0), tmp(9 * S_->nT);
+  for (int e = 0; e < S_->nT; ++e) {
+    double vol = S_->vols[e];
+    Vec<6> d; for (int i = 0; i < 6; ++i) d[i] = S_->S[e][i] - S_->s[6 * e + i];
+    S_->gl[e] = vol * (S_->Hm[e] * (diag<6>(kSym) * d)) + diag<6>(kSyminv) * S_->g[e];
+    Vec<9> t = S_->dSdF[e] * S_->gl[e];
+    for (int i = 0; i < 9; ++i) tmp[9 * e + i] = t[i];
+  }
+  apply_PJW(S_, tmp, out, -1.0);
+  return out;
+}
+
+// local solve (reference mixed_stretch.cpp:244-259)
+void mixed_solve(OracleSim* S_, const std::vector<double>& dxv) {
+  // Jdx = -PJW^T dx
+  std::vector<double> full(S_->nV * 3, 0.0);
+  for (int v = 0; v < S_->nV; ++v) {
+    int f = S_->free_map[v];
+    if (f < 0) continue;
+    for (int r = 0; r < 3; ++r) full[3 * v + r] = dxv[3 * f + r];
+  }
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    double Fd[9]; def_grad(S_, full, e, Fd);
+    Vec<9> jdx; for (int i = 0; i < 9; ++i) jdx[i] = -S_->vols[e] * Fd[i];
+    Vec<6> l = -S_->gl[e] + S_->Hm[e] * (S_->dSdF[e].T() * jdx);
+    Vec<6> ls = diag<6>(kSym) * l + S_->g[e];
+    Vec<6> d = -(S_->Hinv[e] * ls);
+    for (int i = 0; i < 6; ++i) { S_->lam[6 * e + i] = l[i]; S_->ds[6 * e + i] = d[i]; }
+  }
+}
+
+double mixed_energy(const OracleSim* S_, const std::vector<double>& sv) {
+  double e = 0;
+  for (int i = 0; i < S_->nT; ++i) {
+    Vec<6> si; for (int k = 0; k < 6; ++k) si[k] = sv[6 * i + k];
+    double E;
+    material_s6(S_->mat[i], S_->mu[i], S_->la[i], si, false, &E, nullptr, nullptr);
+    e += E * S_->vols[i];
+  }
+  return e;
+}
+
+double mixed_constraint_value(const OracleSim* S_, const std::vector<double>& full,
+                              const std::vector<double>& sv) {
+  std::vector<double> per(S_->nT);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < S_->nT; ++i) {
+    double f[9]; def_grad(S_, full, i, f);
+    M3 F; for (int k = 0; k < 9; ++k) F[k] = f[k];
+    M3 Rc = S_->R[i];
+    Vec<6> stmp;
+    polar_svd3(Rc, stmp, F, false, nullptr, nullptr);
+    double el = 0;
+    for (int k = 0; k < 6; ++k)
+      el += S_->lam[6 * i + k] * kSym[k] * (stmp[k] - sv[6 * i + k]);
+    per[i] = el * S_->vols[i];
+  }
+  double e = 0;
+  for (int i = 0; i < S_->nT; ++i) e += per[i];
+  return e;
+}
+
+// ---------------- Stretch (plain) ----------------
+void plain_update(OracleSim* S_, const std::vector<double>& full) {
+  int D = S_->dim, N = S_->nodes, M = D * N;
+  double h2 = S_->h * S_->h;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    double f[9]; def_grad(S_, full, e, f);
+    double vol = S_->vols[e];
+    double Hf[81], gf[9];
+    if (D == 3) {
+      Vec<9> fv; for (int i = 0; i < 9; ++i) fv[i] = f[i];
+      Vec<9> gg; Mat<9, 9> HH;
+      snh_f9(S_->mu[e], S_->la[e], fv, nullptr, &gg, &HH);
+      for (int i = 0; i < 81; ++i) Hf[i] = HH[i];
+      for (int i = 0; i < 9; ++i) gf[i] = gg[i];
+    } else {
+      Vec<4> fv; for (int i = 0; i < 4; ++i) fv[i] = f[i];
+      Vec<4> gg; Mat<4, 4> HH;
+      snh_f4(S_->mu[e], S_->la[e], fv, nullptr, &gg, &HH);
+      for (int i = 0; i < 16; ++i) Hf[i] = HH[i];
+      for (int i = 0; i < 4; ++i) gf[i] = gg[i];
+    }
+    const double* dx = &S_->dX[(size_t)e * N * D];
+    double* A = &S_->Hp[(size_t)e * M * M];
+    int MM = D * D;
+    for (int k = 0; k < N; ++k)
+      for (int r = 0; r < D; ++r) {
+        for (int l = 0; l < N; ++l)
+          for (int q = 0; q < D; ++q) {
+            double a = 0;
+            for (int c = 0; c < D; ++c)
+              for (int c2 = 0; c2 < D; ++c2)
+                a += dx[D * k + c] * Hf[(r + D * c) + MM * (q + D * c2)] * dx[D * l + c2];
+            A[(D * k + r) + M * (D * l + q)] = a * vol * h2;
+          }
+        double a = 0;
+        for (int c = 0; c < D; ++c) a += dx[D * k + c] * gf[r + D * c];
+        S_->gp[(size_t)e * M + D * k + r] = a * vol * h2;
+      }
+  }
+  // VecAssembler::assemble (reference sparse_utils.cpp:285-323)
+  S_->grad_plain.assign(S_->n, 0.0);
+  std::vector<double> full_g(S_->nV * D, 0.0);
+  for (int e = 0; e < S_->nT; ++e)
+    for (int k = 0; k < N; ++k) {
+      int v = S_->T[e * N + k];
+      for (int r = 0; r < D; ++r) full_g[D * v + r] += S_->gp[(size_t)e * M + D * k + r];
+    }
+  for (int v = 0; v < S_->nV; ++v) {
+    int f = S_->free_map[v];
+    if (f < 0) continue;
+    for (int r = 0; r < D; ++r) S_->grad_plain[D * f + r] = full_g[D * v + r];
+  }
+}
+
+double plain_energy(const OracleSim* S_, const std::vector<double>& full) {
+  std::vector<double> per(S_->nT);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    double f[9]; def_grad(S_, full, e, f);
+    double E;
+    if (S_->dim == 3) {
+      Vec<9> fv; for (int i = 0; i < 9; ++i) fv[i] = f[i];
+      snh_f9(S_->mu[e], S_->la[e], fv, &E, nullptr, nullptr);
+    } else {
+      Vec<4> fv; for (int i = 0; i < 4; ++i) fv[i] = f[i];
+      snh_f4(S_->mu[e], S_->la[e], fv, &E, nullptr, nullptr);
+    }
+    per[e] = E * S_->vols[e];
+  }
+  double s = 0;
+  for (int e = 0; e < S_->nT; ++e) s += per[e];
+  return s;
+}
+
+// ---------------- linear solvers ----------------
+void csr_mult(const OracleSim* S_, const std::vector<double>& vals, const std::vector<double>& xv,
+              std::vector<double>& y) {
+  const auto& rp = S_->asmb.rowptr;
+  const auto& ci = S_->asmb.colind;
+  y.assign(S_->n, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < S_->n; ++i) {
+    double a = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) a += vals[k] * xv[ci[k]];
+    y[i] = a;
+  }
+}
+
+// Conjugate gradient with Jacobi preconditioner and Eigen's stopping rule.
+int pcg(OracleSim* S_, std::vector<double>& xo) {
+  int n = S_->n;
+  const auto& A = S_->lhs;
+  const auto& bb = S_->rhs;
+  xo.assign(n, 0.0);
+  std::vector<double> invd(n, 1.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = S_->asmb.rowptr[i]; k < S_->asmb.rowptr[i + 1]; ++k)
+      if (S_->asmb.colind[k] == i && A[k] != 0.0) invd[i] = 1.0 / A[k];
+  double rhsNorm2 = dotv(bb, bb);
+  if (rhsNorm2 == 0) { S_->pcg_iters_last = 0; S_->pcg_err_last = 0; return 0; }
+  double tol = S_->p.itr_tol;
+  double threshold = std::max(tol * tol * rhsNorm2, 2.2250738585072014e-308);
+  std::vector<double> r = bb, z(n), p(n), tmp(n);
+  double rn2 = dotv(r, r);
+  if (rn2 < threshold) {
+    S_->pcg_iters_last = 0; S_->pcg_err_last = std::sqrt(rn2 / rhsNorm2); return 0;
+  }
+  for (int i = 0; i < n; ++i) p[i] = invd[i] * r[i];
+  double absNew = dotv(r, p);
+  int it = 0;
+  while (it < S_->p.max_iters) {
+    csr_mult(S_, A, p, tmp);
+    double alpha = absNew / dotv(p, tmp);
+    for (int i = 0; i < n; ++i) { xo[i] += alpha * p[i]; r[i] -= alpha * tmp[i]; }
+    rn2 = dotv(r, r);
+    if (rn2 < threshold) break;
+    for (int i = 0; i < n; ++i) z[i] = invd[i] * r[i];
+    double absOld = absNew;
+    absNew = dotv(r, z);
+    double beta = absNew / absOld;
+    for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  S_->pcg_iters_last = it;
+  S_->pcg_err_last = std::sqrt(rn2 / rhsNorm2);
+  return 0;
+}
+
+// dense Cholesky (stands in for the reference's SimplicialLLT / CHOLMOD;
+// exact-arithmetic solution is identical). Returns -1 if not PD.
+int dense_llt(OracleSim* S_, std::vector<double>& xo) {
+  int n = S_->n;
+  std::vector<double> L((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = S_->asmb.rowptr[i]; k < S_->asmb.rowptr[i + 1]; ++k)
+      L[(size_t)i * n + S_->asmb.colind[k]] = S_->lhs[k];
+  for (int j = 0; j < n; ++j) {
+    double d = L[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+    if (!(d > 0)) return -1;
+    d = std::sqrt(d);
+    L[(size_t)j * n + j] = d;
+#pragma omp parallel for schedule(static)
+    for (int i = j + 1; i < n; ++i) {
+      double s = L[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+      L[(size_t)i * n + j] = s / d;
+    }
+  }
+  std::vector<double> y(n);
+  for (int i = 0; i < n; ++i) {
+    double s = S_->rhs[i];
+    for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * y[k];
+    y[i] = s / L[(size_t)i * n + i];
+  }
+  xo.assign(n, 0.0);
+  for (int i = n - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < n; ++k) s -= L[(size_t)k * n + i] * xo[k];
+    xo[i] = s / L[(size_t)i * n + i];
+  }
+  S_->pcg_iters_last = 0;
+  return 0;
+}
+
+// ---------------- Newton optimizer ----------------
+void update_system(OracleSim* S_) {
+  std::vector<double> full;
+  unproject(S_, S_->x, full);
+  if (S_->p.mixed) {
+    mixed_update_rotations(S_, full);
+    mixed_update_derivatives(S_);
+  } else {
+    plain_update(S_, full);
+  }
+}
+
+int substep(OracleSim* S_, double& decrement) {
+  // SystemMatrixPD::pre_solve: lhs = PMP + var lhs; rhs = x rhs + var rhs
+  std::vector<double> Avals;
+  if (S_->p.mixed) {
+    Avals = S_->lhs;  // assembled in mixed_update_derivatives
+  } else {
+    S_->asmb.update_matrix(S_->Hp, Avals);
+  }
+  S_->lhs = S_->PMP;
+  for (size_t k = 0; k < S_->lhs.size(); ++k) S_->lhs[k] += Avals[k];
+  disp_gradient(S_);
+  S_->rhs.assign(S_->n, 0.0);
+  for (int i = 0; i < S_->n; ++i) S_->rhs[i] = -S_->grad_x_disp[i];
+  if (S_->p.mixed) {
+    std::vector<double> mr = mixed_rhs(S_);
+    for (int i = 0; i < S_->n; ++i) S_->rhs[i] += mr[i];
+  } else {
+    for (int i = 0; i < S_->n; ++i) S_->rhs[i] += -S_->grad_plain[i];
+  }
+  std::vector<double> sol;
+  int rc = (S_->p.solver == 0) ? dense_llt(S_, sol) : pcg(S_, sol);
+  if (rc != 0) return rc;
+  S_->st.pcg_iters_total += S_->pcg_iters_last;
+  // post_solve
+  S_->dx = sol;
+  if (S_->p.mixed) mixed_solve(S_, sol);
+  decrement = 0;
+  for (double v : S_->dx) decrement = std::max(decrement, std::fabs(v));
+  if (S_->p.mixed)
+    for (double v : S_->ds) decrement = std::max(decrement, std::fabs(v));
+  return 0;
+}
+
+double energy_func(const OracleSim* S_, double a) {
+  double h2 = S_->h * S_->h;
+  std::vector<double> x0(S_->n);
+  for (int i = 0; i < S_->n; ++i) x0[i] = S_->x[i] + a * S_->dx[i];
+  double val = disp_energy(S_, x0);
+  std::vector<double> full;
+  unproject(S_, x0, full);
+  if (S_->p.mixed) {
+    std::vector<double> si(S_->s.size());
+    for (size_t i = 0; i < si.size(); ++i) si[i] = S_->s[i] + a * S_->ds[i];
+    val += h2 * mixed_energy(S_, si) - mixed_constraint_value(S_, full, si);
+  } else {
+    val += h2 * plain_energy(S_, full);
+  }
+  return val;
+}
+
+int step(OracleSim* S_) {
+  int i = 0;
+  double grad_norm = 0, E = 0, E_prev = 0, res = 0;
+  S_->st.newton_iters = 0;
+  S_->st.pcg_iters_total = 0;
+  S_->st.ls_fail = 0;
+  do {
+    update_system(S_);
+    int rc = substep(S_, grad_norm);
+    if (rc != 0) return rc;
+    double alpha = 1.0;
+    E = energy_func(S_, 0.0);
+    res = std::fabs((E - E_prev) / (E + 1e-6));
+    E_prev = E;
+    // linesearch_backtracking with c = 0, p = 0.5
+    double fx0 = energy_func(S_, 0.0);
+    int iter = 0;
+    while (iter < S_->p.ls_iters) {
+      double fx = energy_func(S_, alpha);
+      if (fx < fx0 + (alpha * 0.0) * 0.0) break;
+      alpha *= 0.5;
+      ++iter;
+    }
+    if (iter < S_->p.ls_iters) {
+      for (int k = 0; k < S_->n; ++k) S_->x[k] += alpha * S_->dx[k];
+      if (S_->p.mixed)
+        for (size_t k = 0; k < S_->s.size(); ++k) S_->s[k] += alpha * S_->ds[k];
+    } else {
+      S_->st.ls_fail++;
+    }
+    S_->st.last_alpha = alpha;
+    ++i;
+  } while (i < S_->p.max_newton && grad_norm > S_->p.newton_tol && res > 1e-12);
+  S_->st.newton_iters = i;
+  S_->st.last_decrement = grad_norm;
+  S_->st.last_energy = E;
+  S_->st.last_res = res;
+  // post_solve: Displacement (b unchanged for static pins), BDF update
+  std::vector<double> full;
+  unproject(S_, S_->x, full);
+  {
+    size_t N = full.size();
+    std::vector<double> wx(N, 0.0);
+    for (size_t k = 0; k < S_->x_prevs.size(); ++k)
+      for (size_t q = 0; q < N; ++q) wx[q] += S_->alphas[k] * S_->x_prevs[k][q];
+    std::vector<double> v(N);
+    for (size_t q = 0; q < N; ++q) v[q] = (full[q] - wx[q]) / S_->h;
+    S_->x_prevs.push_front(full);
+    S_->v_prevs.push_front(v);
+    if ((int)S_->x_prevs.size() > S_->p.bdf_order) {
+      S_->x_prevs.pop_back();
+      S_->v_prevs.pop_back();
+    }
+  }
+  S_->Vcur = full;
+  if (S_->p.mixed) std::fill(S_->lam.begin(), S_->lam.end(), 0.0);
+  return 0;
+}
+
+void bdf_tables(int I, std::vector<double>& a, double& beta) {
+  switch (I) {
+    case 1: a = {1.0}; beta = 1.0; break;
+    case 2: a = {4.0 / 3.0, -1.0 / 3.0}; beta = 2.0 / 3.0; break;
+    case 3: a = {18.0 / 11.0, -9.0 / 11.0, 2.0 / 11.0}; beta = 6.0 / 11.0; break;
+    case 4: a = {48.0 / 25.0, -36.0 / 25.0, 16.0 / 25.0, -3.0 / 25.0}; beta = 12.0 / 25.0; break;
+    case 5: a = {300.0 / 137.0, -300.0 / 137.0, 200.0 / 137.0, -75.0 / 137.0, 12.0 / 137.0};
+            beta = 60.0 / 137.0; break;
+    default: a = {360.0 / 147.0, -450.0 / 147.0, 400.0 / 147.0, -225.0 / 147.0, 72.0 / 147.0,
+                  -10.0 / 147.0}; beta = 60.0 / 147.0; break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_newton_procrustes(double* R, const double* A, const double* B, int cg,
+                             double* dRdF, double tol, int max_iter, int* iters) {
+  M3 Rm, Am, Bm;
+  for (int i = 0; i < 9; ++i) { Rm[i] = R[i]; Am[i] = A[i]; Bm[i] = B[i]; }
+  Mat<9, 9> d;
+  int rc = newton_procrustes(Rm, Am, Bm, cg != 0, &d, tol, max_iter, iters);
+  for (int i = 0; i < 9; ++i) R[i] = Rm[i];
+  if (cg && dRdF) for (int i = 0; i < 81; ++i) dRdF[i] = d[i];
+  return rc;
+}
+
+int oracle_polar_svd3(double* R, double* s, const double* F, int grad, double* dsdF,
+                      int* iters) {
+  M3 Rm, Fm;
+  for (int i = 0; i < 9; ++i) { Rm[i] = R[i]; Fm[i] = F[i]; }
+  Vec<6> sv; Mat<6, 9> d;
+  int rc = polar_svd3(Rm, sv, Fm, grad != 0, &d, iters);
+  for (int i = 0; i < 9; ++i) R[i] = Rm[i];
+  for (int i = 0; i < 6; ++i) s[i] = sv[i];
+  if (grad && dsdF) for (int i = 0; i < 54; ++i) dsdF[i] = d[i];
+  return rc;
+}
+
+void oracle_material_s6(int type, double mu, double la, const double* s, int psd,
+                        double* E, double* g, double* H) {
+  Vec<6> sv; for (int i = 0; i < 6; ++i) sv[i] = s[i];
+  Vec<6> gg; Mat<6, 6> HH;
+  material_s6(type, mu, la, sv, psd != 0, E, &gg, &HH);
+  if (g) for (int i = 0; i < 6; ++i) g[i] = gg[i];
+  if (H) for (int i = 0; i < 36; ++i) H[i] = HH[i];
+}
+
+void oracle_snh_f9(double mu, double la, const double* F, double* E, double* g, double* H) {
+  Vec<9> f; for (int i = 0; i < 9; ++i) f[i] = F[i];
+  Vec<9> gg; Mat<9, 9> HH;
+  snh_f9(mu, la, f, E, &gg, H ? &HH : nullptr);
+  if (g) for (int i = 0; i < 9; ++i) g[i] = gg[i];
+  if (H) for (int i = 0; i < 81; ++i) H[i] = HH[i];
+}
+
+void oracle_snh_f4(double mu, double la, const double* F, double* E, double* g, double* H) {
+  Vec<4> f; for (int i = 0; i < 4; ++i) f[i] = F[i];
+  Vec<4> gg; Mat<4, 4> HH;
+  snh_f4(mu, la, f, E, &gg, &HH);
+  if (g) for (int i = 0; i < 4; ++i) g[i] = gg[i];
+  if (H) for (int i = 0; i < 16; ++i) H[i] = HH[i];
+}
+
+void oracle_psd_fix(int n, double* A, double tol) {
+  if (n == 6) { Mat<6, 6> M; for (int i = 0; i < 36; ++i) M[i] = A[i]; simple_psd_fix<6>(M, tol); for (int i = 0; i < 36; ++i) A[i] = M[i]; }
+  else if (n == 9) { Mat<9, 9> M; for (int i = 0; i < 81; ++i) M[i] = A[i]; simple_psd_fix<9>(M, tol); for (int i = 0; i < 81; ++i) A[i] = M[i]; }
+  else if (n == 3) { Mat<3, 3> M; for (int i = 0; i < 9; ++i) M[i] = A[i]; simple_psd_fix<3>(M, tol); for (int i = 0; i < 9; ++i) A[i] = M[i]; }
+}
+
+// Boundary groups and pinned sets (reference boundary_condition.cpp:5-21,
+// fixed_boundary_conditions.h:75-166, bend/stretch/twist headers).
+// type: 0 null, 1 scale, 2 hang, 3 hangends, 4 stretch, 7 bend, 8 twist,
+//       13 onepoint, 14 randomize (reference config.h:70-87 numbering).
+void oracle_bc_fixed(int type, const double* V, int nV, int dim, int axis, double ratio,
+                     int* fixed) {
+  std::vector<double> mn(dim, 1e300), mx(dim, -1e300);
+  for (int i = 0; i < nV; ++i)
+    for (int d = 0; d < dim; ++d) {
+      mn[d] = std::min(mn[d], V[i * dim + d]);
+      mx[d] = std::max(mx[d], V[i * dim + d]);
+    }
+  double range = mx[axis] - mn[axis];
+  std::vector<int> g0, g1;
+  for (int i = 0; i < nV; ++i) {
+    double v = V[i * dim + axis];
+    if (v < mn[axis] + range * ratio) g0.push_back(i);
+    else if (v > mx[axis] - range * ratio) g1.push_back(i);
+  }
+  for (int i = 0; i < nV; ++i) fixed[i] = 0;
+  switch (type) {
+    case 2: if (!g0.empty()) fixed[g0.back()] = 1; if (!g1.empty()) fixed[g1.back()] = 1; break;
+    case 3: for (int i : g1) fixed[i] = 1; break;
+    case 4: case 7: case 8: for (int i : g0) fixed[i] = 1; for (int i : g1) fixed[i] = 1; break;
+    case 13: if (nV > 0) fixed[0] = 1; break;
+    default: break;
+  }
+}
+
+// Per-tet |det Dm|/6 and shape-function gradients dX[e][k][c] = dphi_k/dX_c
+// (phi_1..3 rows of Dm^{-1}; phi_0 = -sum).
+void oracle_tet_geometry(const double* V, const int* T, int nT, double* vols, double* dX) {
+  for (int e = 0; e < nT; ++e) {
+    const double* X0 = &V[3 * T[4 * e + 0]];
+    M3 Dm;
+    for (int k = 1; k < 4; ++k) {
+      const double* Xk = &V[3 * T[4 * e + k]];
+      for (int r = 0; r < 3; ++r) Dm(r, k - 1) = Xk[r] - X0[r];
+    }
+    vols[e] = std::fabs(det3(Dm)) / 6.0;
+    M3 Di = inv3(Dm);
+    for (int c = 0; c < 3; ++c) {
+      double s = 0;
+      for (int k = 1; k < 4; ++k) { dX[e * 12 + 3 * k + c] = Di(k - 1, c); s += Di(k - 1, c); }
+      dX[e * 12 + c] = -s;
+    }
+  }
+}
+
+void* oracle_sim_create(const oracle_params* p) {
+  OracleSim* S_ = new OracleSim();
+  S_->p = *p;
+#ifdef _OPENMP
+  if (p->threads > 0) omp_set_num_threads(p->threads);
+#endif
+  int D = p->dim;
+  S_->dim = D; S_->nodes = D + 1; S_->nV = p->nV; S_->nT = p->nT;
+  S_->Vref.assign(p->V, p->V + (size_t)p->nV * D);
+  S_->Vcur = S_->Vref;
+  S_->T.assign(p->T, p->T + (size_t)p->nT * S_->nodes);
+  S_->mu.assign(p->mu, p->mu + p->nT);
+  S_->la.assign(p->la, p->la + p->nT);
+  S_->mat.assign(p->mat, p->mat + p->nT);
+  S_->fixed.assign(p->fixed, p->fixed + p->nV);
+  S_->free_map.assign(p->nV, -1);
+  int curr = 0;
+  for (int i = 0; i < p->nV; ++i) if (!S_->fixed[i]) S_->free_map[i] = curr++;
+  S_->nfree = curr; S_->n = D * curr;
+  // geometry
+  S_->vols.resize(p->nT); S_->dX.resize((size_t)p->nT * S_->nodes * D);
+  if (D == 3) {
+    oracle_tet_geometry(p->V, p->T, p->nT, S_->vols.data(), S_->dX.data());
+  } else {
+    // doublearea (signed, reference tri2d_mesh.cpp:32-34) and barycentric gradients
+    for (int e = 0; e < p->nT; ++e) {
+      const double* X0 = &p->V[2 * p->T[3 * e]];
+      const double* X1 = &p->V[2 * p->T[3 * e + 1]];
+      const double* X2 = &p->V[2 * p->T[3 * e + 2]];
+      double a = X1[0] - X0[0], bq = X2[0] - X0[0], c = X1[1] - X0[1], d = X2[1] - X0[1];
+      double det = a * d - bq * c;
+      S_->vols[e] = det;
+      // Dm^{-1} = [d -b; -c a]/det, rows are grad phi_1, grad phi_2
+      double g1x = d / det, g1y = -bq / det, g2x = -c / det, g2y = a / det;
+      double* o = &S_->dX[(size_t)e * 6];
+      o[2] = g1x; o[3] = g1y; o[4] = g2x; o[5] = g2y;
+      o[0] = -(g1x + g2x); o[1] = -(g1y + g2y);
+    }
+  }
+  S_->asmb.build(S_->T, p->nT, S_->nodes, D, S_->free_map, S_->nfree);
+  // PMP on the assembler pattern: element mass blocks m_ab * I
+  {
+    int M = S_->nodes * D;
+    std::vector<double> mb((size_t)p->nT * M * M, 0.0);
+    double denom = (D == 3) ? 20.0 : 12.0;
+    for (int e = 0; e < p->nT; ++e) {
+      double m = p->density * S_->vols[e] / denom;
+      for (int a = 0; a < S_->nodes; ++a)
+        for (int bb = 0; bb < S_->nodes; ++bb)
+          for (int r = 0; r < D; ++r)
+            mb[(size_t)e * M * M + (D * a + r) + M * (D * bb + r)] = m * (a == bb ? 2.0 : 1.0);
+    }
+    S_->asmb.update_matrix(mb, S_->PMP);
+  }
+  // Displacement::reset
+  bdf_tables(p->bdf_order, S_->alphas, S_->beta);
+  S_->h = S_->beta * p->dt;
+  std::vector<double> x0 = S_->Vref;
+  std::vector<double> v0(x0.size(), 0.0);
+  if (p->v0) v0.assign(p->v0, p->v0 + x0.size());
+  for (int k = 0; k < p->bdf_order; ++k) { S_->x_prevs.push_front(x0); S_->v_prevs.push_front(v0); }
+  S_->b.assign(x0.size(), 0.0);
+  S_->x.assign(S_->n, 0.0);
+  for (int v = 0; v < p->nV; ++v) {
+    int f = S_->free_map[v];
+    for (int d = 0; d < D; ++d) {
+      if (f < 0) S_->b[D * v + d] = x0[D * v + d];
+      else S_->x[D * f + d] = x0[D * v + d];
+    }
+  }
+  S_->dx.assign(S_->n, 0.0);
+  S_->f_ext.resize(x0.size());
+  for (int v = 0; v < p->nV; ++v)
+    for (int d = 0; d < D; ++d) S_->f_ext[D * v + d] = (double)(float)p->gravity[d];
+  // variables
+  int M = S_->nodes * D;
+  if (p->mixed) {
+    int nT = p->nT;
+    S_->R.assign(nT, M3::Identity());
+    S_->S.assign(nT, Vec<6>()); S_->g.assign(nT, Vec<6>()); S_->gl.assign(nT, Vec<6>());
+    S_->s.assign(6 * nT, 0.0); S_->lam.assign(6 * nT, 0.0); S_->ds.assign(6 * nT, 0.0);
+    S_->dSdF.assign(nT, Mat<9, 6>());
+    S_->Hm.assign(nT, Mat<6, 6>::Identity()); S_->Hinv.assign(nT, Mat<6, 6>::Identity());
+    S_->Hloc.assign(nT, Mat<6, 6>());
+    S_->Aloc.assign((size_t)nT * M * M, 0.0);
+    std::vector<double> full = S_->Vcur;
+    for (int e = 0; e < nT; ++e) {
+      double f[9]; def_grad(S_, full, e, f);
+      M3 F; for (int i = 0; i < 9; ++i) F[i] = f[i];
+      Mat<6, 9> dsdF;
+      polar_svd3(S_->R[e], S_->S[e], F, true, &dsdF, nullptr);
+      S_->dSdF[e] = dsdF.T() * diag<6>(kSym);
+      for (int i = 0; i < 6; ++i) S_->s[6 * e + i] = S_->S[e][i];
+    }
+  } else {
+    S_->Hp.assign((size_t)p->nT * M * M, 0.0);
+    S_->gp.assign((size_t)p->nT * M, 0.0);
+  }
+  return S_;
+}
+
+void oracle_sim_destroy(void* sim) { delete (OracleSim*)sim; }
+
+int oracle_sim_step(void* sim) { return step((OracleSim*)sim); }
+
+int oracle_sim_update_system(void* sim) { update_system((OracleSim*)sim); return 0; }
+
+int oracle_sim_solve(void* sim) {
+  double dec; return substep((OracleSim*)sim, dec);
+}
+
+double oracle_sim_energy(void* sim, double alpha) { return energy_func((OracleSim*)sim, alpha); }
+
+void oracle_sim_stats(void* sim, oracle_stats* st) { *st = ((OracleSim*)sim)->st; }
+
+void oracle_sim_get_positions(void* sim, double* out) {
+  OracleSim* S_ = (OracleSim*)sim;
+  std::copy(S_->Vcur.begin(), S_->Vcur.end(), out);
+}
+
+long oracle_sim_size(void* sim, const char* name) {
+  OracleSim* S_ = (OracleSim*)sim;
+  std::string k(name);
+  long nT = S_->nT;
+  if (k == "R") return 9 * nT;
+  if (k == "S" || k == "s" || k == "lambda" || k == "ds" || k == "g" || k == "gl") return 6 * nT;
+  if (k == "dSdF") return 54 * nT;
+  if (k == "H" || k == "Hinv") return 36 * nT;
+  if (k == "Aloc") return (long)S_->Aloc.size();
+  if (k == "Hplain") return (long)S_->Hp.size();
+  if (k == "gplain") return (long)S_->gp.size();
+  if (k == "lhs" || k == "PMP") return (long)S_->lhs.size();
+  if (k == "rhs" || k == "dx" || k == "x") return S_->n;
+  if (k == "vols") return nT;
+  if (k == "dX") return (long)S_->dX.size();
+  if (k == "positions") return (long)S_->Vcur.size();
+  if (k == "x_tilde") return (long)S_->Vcur.size();
+  return -1;
+}
+
+int oracle_sim_get(void* sim, const char* name, double* out) {
+  OracleSim* S_ = (OracleSim*)sim;
+  std::string k(name);
+  int nT = S_->nT;
+  auto put6 = [&](const std::vector<Vec<6>>& v) {
+    for (int e = 0; e < nT; ++e) for (int i = 0; i < 6; ++i) out[6 * e + i] = v[e][i];
+  };
+  if (k == "R") { for (int e = 0; e < nT; ++e) for (int i = 0; i < 9; ++i) out[9 * e + i] = S_->R[e][i]; return 0; }
+  if (k == "S") { put6(S_->S); return 0; }
+  if (k == "g") { put6(S_->g); return 0; }
+  if (k == "gl") { put6(S_->gl); return 0; }
+  if (k == "s") { std::copy(S_->s.begin(), S_->s.end(), out); return 0; }
+  if (k == "lambda") { std::copy(S_->lam.begin(), S_->lam.end(), out); return 0; }
+  if (k == "ds") { std::copy(S_->ds.begin(), S_->ds.end(), out); return 0; }
+  if (k == "dSdF") { for (int e = 0; e < nT; ++e) for (int i = 0; i < 54; ++i) out[54 * e + i] = S_->dSdF[e][i]; return 0; }
+  if (k == "H") { for (int e = 0; e < nT; ++e) for (int i = 0; i < 36; ++i) out[36 * e + i] = S_->Hm[e][i]; return 0; }
+  if (k == "Hinv") { for (int e = 0; e < nT; ++e) for (int i = 0; i < 36; ++i) out[36 * e + i] = S_->Hinv[e][i]; return 0; }
+  if (k == "Aloc") { std::copy(S_->Aloc.begin(), S_->Aloc.end(), out); return 0; }
+  if (k == "Hplain") { std::copy(S_->Hp.begin(), S_->Hp.end(), out); return 0; }
+  if (k == "gplain") { std::copy(S_->gp.begin(), S_->gp.end(), out); return 0; }
+  if (k == "lhs") { std::copy(S_->lhs.begin(), S_->lhs.end(), out); return 0; }
+  if (k == "PMP") { std::copy(S_->PMP.begin(), S_->PMP.end(), out); return 0; }
+  if (k == "rhs") { std::copy(S_->rhs.begin(), S_->rhs.end(), out); return 0; }
+  if (k == "dx") { std::copy(S_->dx.begin(), S_->dx.end(), out); return 0; }
+  if (k == "x") { std::copy(S_->x.begin(), S_->x.end(), out); return 0; }
+  if (k == "vols") { std::copy(S_->vols.begin(), S_->vols.end(), out); return 0; }
+  if (k == "dX") { std::copy(S_->dX.begin(), S_->dX.end(), out); return 0; }
+  if (k == "positions") { std::copy(S_->Vcur.begin(), S_->Vcur.end(), out); return 0; }
+  if (k == "x_tilde") { auto v = x_tilde(S_); std::copy(v.begin(), v.end(), out); return 0; }
+  return -1;
+}
+
+int oracle_sim_get_int(void* sim, const char* name, int* out) {
+  OracleSim* S_ = (OracleSim*)sim;
+  std::string k(name);
+  if (k == "rowptr") { std::copy(S_->asmb.rowptr.begin(), S_->asmb.rowptr.end(), out); return 0; }
+  if (k == "colind") { std::copy(S_->asmb.colind.begin(), S_->asmb.colind.end(), out); return 0; }
+  if (k == "free_map") { std::copy(S_->free_map.begin(), S_->free_map.end(), out); return 0; }
+  if (k == "pcg_iters") { *out = S_->pcg_iters_last; return 0; }
+  return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+"""Pins for the FP64 CPU oracle (CPU only).
+
+Every oracle function is checked against something it does not compute
+itself: the reference's own known-answer test (dR/dF by central differences,
+reference tests/test_dsvd.cpp:204-238), central finite differences of
+energies, closed forms, invariants, numpy/LAPACK routines, brute-force dense
+assembly, and mathematically fixed whole-step behaviour (free fall, mixed ==
+plain at convergence).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mfem_amd import meshio, scenes
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+MU, LA = scenes.enu_to_lame(1e6, 0.45)
+S_REF = np.array([1.11, 1.2, 1.3, 0.2, 0.3, 0.4])          # tests/test_dsvd.cpp:119-120
+F_REF = np.array([[1.0, 0.1, 0.2], [0.1, 2.0, 0.4], [0.3, 0.4, 0.5]])  # :76-79
+
+
+def s_of(R, F):
+    S = R.T @ F
+    return np.array([S[0, 0], S[1, 1], S[2, 2], 0.5 * (S[1, 0] + S[0, 1]),
+                     0.5 * (S[2, 0] + S[0, 2]), 0.5 * (S[2, 1] + S[1, 2])])
+
+
+def rand_pd_F(rng):
+    while True:
+        F = np.eye(3) + 0.4 * rng.uniform(-1, 1, (3, 3))
+        if np.linalg.det(F) > 0.2:
+            return F
+
+
+# ---------------------------------------------------------------- procrustes
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dRdF_matches_central_differences(oracle, seed):
+    """Reference known-answer test: FD with alpha = 1e-5, error <= alpha. The
+    FD evaluations are converged to |g| < 1e-12 (the reference's used the
+    default 1e-6, which leaves stopping noise of the size of the bound)."""
+    rng = np.random.default_rng(seed)
+    F = rand_pd_F(rng) * 2.0
+    A = rand_pd_F(rng)  # det(A F^T) > 0: a unique, smooth maximiser
+    alpha = 1e-5
+    fd = np.zeros((9, 9))
+    for ii in range(3):
+        for jj in range(3):
+            P = np.zeros((3, 3))
+            P[ii, jj] = alpha
+            R0 = oracle.newton_procrustes(np.eye(3), A, F + P, False, 1e-12, 200)[0]
+            R1 = oracle.newton_procrustes(np.eye(3), A, F - P, False, 1e-12, 200)[0]
+            fd[:, ii + 3 * jj] = ((R0 - R1) / (2 * alpha)).T.reshape(-1)
+    R, dRdF, rc, it = oracle.newton_procrustes(np.eye(3), A, F, True, 1e-8, 200)
+    assert rc == 0
+    assert np.linalg.norm(fd - dRdF) <= alpha
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_procrustes_is_polar_rotation(oracle, seed):
+    """For det F > 0 the maximiser of tr(R^T F) is U V^T (SVD polar factor).
+    The reference iteration stalls once steps fall below rodrigues' 1e-8 angle
+    cut-off (and its E0 = -tr(R Y) comparison rejects tiny steps), so R is
+    pinned to 1e-7, not to machine precision."""
+    rng = np.random.default_rng(100 + seed)
+    F = rand_pd_F(rng)
+    R, _, rc, _ = oracle.newton_procrustes(np.eye(3), np.eye(3), F, False, 1e-12, 100)
+    U, _, Vt = np.linalg.svd(F)
+    Rp = U @ Vt
+    assert np.allclose(R.T @ R, np.eye(3), atol=1e-12)
+    assert abs(np.linalg.det(R) - 1) < 1e-12
+    assert np.abs(R - Rp).max() < 1e-7
+    S = R.T @ F
+    assert np.abs(S - S.T).max() < 1e-7
+
+
+def test_polar_svd3_fixed_input(oracle):
+    R, s, dsdF, rc, it = oracle.polar_svd3(F_REF)
+    U, _, Vt = np.linalg.svd(F_REF)
+    assert np.abs(R - U @ Vt).max() < 1e-6
+    assert np.allclose(s, s_of(R, F_REF), atol=0)
+    # dsdF against central differences of s(F), s taken from the exact SVD
+    # polar factor (LAPACK) so that the FD side carries no iteration noise
+    def s_exact(F):
+        U, _, Vt = np.linalg.svd(F)
+        return s_of(U @ Vt, F)
+    fd = np.zeros((6, 9))
+    eps = 1e-5
+    for c in range(9):
+        E = np.zeros(9)
+        E[c] = eps
+        Ep = E.reshape(3, 3).T
+        fd[:, c] = (s_exact(F_REF + Ep) - s_exact(F_REF - Ep)) / (2 * eps)
+    assert np.abs(fd - dsdF).max() < 1e-6
+
+
+# ---------------------------------------------------------------- materials
+
+def fd_grad_hess(fun, x, eps=1e-6):
+    n = x.size
+    g = np.zeros(n)
+    H = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = eps
+        Ep, gp = fun(x + e)[:2]
+        Em, gm = fun(x - e)[:2]
+        g[i] = (Ep - Em) / (2 * eps)
+        H[:, i] = (gp - gm) / (2 * eps)
+    return g, H
+
+
+@pytest.mark.parametrize("mtype", [0, 2, 3, 4])
+def test_material_s6_derivatives(oracle, mtype):
+    fun = lambda s: oracle.material_s6(mtype, MU, LA, s, psd_fix=False)
+    E, g, H = fun(S_REF)
+    fg, fH = fd_grad_hess(fun, S_REF)
+    assert np.abs(fg - g).max() <= 1e-7 * np.abs(g).max()
+    assert np.abs(fH - H).max() <= 1e-7 * np.abs(H).max()
+    assert np.allclose(H, H.T)
+
+
+def test_snh_closed_form_invariants(oracle):
+    """derivs.m:29 closed form, Psi(I) = 0, grad Psi(I) = 0, small-strain
+    limit mu|e|^2 + (la - mu)/2 tr(e)^2, rotation invariance, and equality of
+    the 6-vector and 9-vector forms on the symmetric polar factor."""
+    I6 = np.array([1, 1, 1, 0, 0, 0.0])
+    E, g, _ = oracle.material_s6(0, MU, LA, I6)
+    assert E == 0.0 and np.abs(g).max() == 0.0
+    rng = np.random.default_rng(7)
+    e = 1e-5 * rng.uniform(-1, 1, (3, 3))
+    e = 0.5 * (e + e.T)
+    Fe = np.eye(3) + e
+    Ef = oracle.snh_f9(MU, LA, Fe.T.reshape(-1))[0]
+    lin = MU * np.sum(e * e) + 0.5 * (LA - MU) * np.trace(e) ** 2
+    assert abs(Ef - lin) <= 1e-4 * abs(lin)
+    F = rand_pd_F(rng)
+    J = np.linalg.det(F)
+    closed = 0.5 * MU * (np.sum(F * F) - 3) - MU * (J - 1) + 0.5 * LA * (J - 1) ** 2
+    assert abs(oracle.snh_f9(MU, LA, F.T.reshape(-1))[0] - closed) <= 1e-12 * abs(closed)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    Q *= np.sign(np.linalg.det(Q))
+    assert abs(oracle.snh_f9(MU, LA, (Q @ F).T.reshape(-1))[0] - closed) <= 1e-10 * abs(closed)
+    R, s, *_ = oracle.polar_svd3(F)
+    E6 = oracle.material_s6(0, MU, LA, s, psd_fix=False)[0]
+    assert abs(E6 - closed) <= 1e-9 * abs(closed)
+
+
+def test_snh_f9_and_f4_derivatives(oracle):
+    rng = np.random.default_rng(3)
+    F = rand_pd_F(rng).T.reshape(-1)
+    fun9 = lambda f: oracle.snh_f9(MU, LA, f)
+    E, g, H = fun9(F)
+    fg, _ = fd_grad_hess(fun9, F)
+    assert np.abs(fg - g).max() <= 1e-7 * np.abs(g).max()
+    # H is PSD-projected with tol 0: compare to numpy-projected FD Hessian
+    _, fH_raw = fd_grad_hess(lambda f: (oracle.snh_f9(MU, LA, f)[0],
+                                        oracle.snh_f9(MU, LA, f)[1]), F)
+    w, Q = np.linalg.eigh(0.5 * (fH_raw + fH_raw.T))
+    proj = (Q * np.maximum(w, 0.0)) @ Q.T
+    assert np.abs(proj - H).max() <= 1e-6 * np.abs(H).max()
+    F4 = np.array([1.1, 0.05, -0.1, 0.9])
+    fun4 = lambda f: oracle.snh_f4(MU, LA, f)
+    E, g, H = fun4(F4)
+    fg, fH = fd_grad_hess(fun4, F4)
+    assert np.abs(fg - g).max() <= 1e-7 * np.abs(g).max()
+    assert np.abs(fH - H).max() <= 1e-7 * np.abs(H).max()
+
+
+def test_psd_fix_properties(oracle):
+    rng = np.random.default_rng(11)
+    A = rng.normal(size=(6, 6))
+    A = A @ A.T + np.eye(6)
+    assert np.array_equal(oracle.psd_fix(A), A)                    # PD: untouched
+    B = rng.normal(size=(6, 6))
+    B = B + B.T
+    P = oracle.psd_fix(B, 1e-6)
+    w, Q = np.linalg.eigh(B)
+    ref = (Q * np.maximum(w, 1e-6)) @ Q.T                          # LAPACK route
+    assert np.abs(P - ref).max() < 1e-10 * np.abs(B).max()
+    assert np.linalg.eigvalsh(P).min() >= 1e-6 * (1 - 1e-6) - 1e-12
+    C = rng.normal(size=(9, 9))
+    C = C + C.T
+    P9 = oracle.psd_fix(C, 0.0)
+    w, Q = np.linalg.eigh(C)
+    assert np.abs(P9 - (Q * np.maximum(w, 0)) @ Q.T).max() < 1e-10 * np.abs(C).max()
+
+
+# ---------------------------------------------------------------- geometry
+
+def test_tet_geometry_fixtures(oracle):
+    for name in ("tet.mesh", "two_tets.mesh"):
+        V, T = meshio.read_mesh(os.path.join(GOLDEN, name))
+        vols, dX = oracle.tet_geometry(V, T)
+        for e in range(T.shape[0]):
+            X = V[T[e]]
+            Dm = (X[1:] - X[0]).T
+            assert abs(vols[e] - abs(np.linalg.det(Dm)) / 6) < 1e-15
+            F = sum(np.outer(X[k], dX[e, k]) for k in range(4))  # F(X_ref) = I
+            assert np.allclose(F, np.eye(3), atol=1e-12)
+            assert np.allclose(dX[e].sum(axis=0), 0, atol=1e-12)
+
+
+def test_kuhn_beam_counts_and_clamp(oracle):
+    V, T = scenes.kuhn_beam(40, 10, 10, 0.1, 0.1, seed=0)
+    assert V.shape == (4961, 3) and T.shape == (24000, 4)
+    vols, _ = oracle.tet_geometry(V, T)
+    assert vols.min() > 0
+    sc = scenes.config_c2()
+    assert sc.fixed.sum() == 121
+    assert np.all(sc.V[sc.fixed == 1, 0] == 4.0)
+    ratio = 0.5 * 0.1 / 4.0 * 0.8
+    ob = oracle.bc_fixed(3, sc.V, 0, ratio)
+    assert np.array_equal(ob, sc.fixed)
+    for kind in ("hang", "stretch", "onepoint", "null"):
+        assert np.array_equal(oracle.bc_fixed(scenes.BCS[kind], sc.V, 0, 0.1),
+                              scenes.bc_fixed(kind, sc.V, 0, 0.1))
+
+
+# ---------------------------------------------------------------- assembly
+
+def _dense_from_csr(rowptr, colind, vals, n):
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, colind[rowptr[i]:rowptr[i + 1]]] = vals[rowptr[i]:rowptr[i + 1]]
+    return A
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_assembly_matches_bruteforce(oracle, mixed):
+    sc = scenes.beam_scene(3, 2, 2, jitter=0.1, seed=5, mixed=mixed, itr_tol=1e-12,
+                           max_iters=5000)
+    sim = oracle.Sim(sc)
+    sim.step()
+    sim.update_system()
+    sim.solve()
+    n = int(sim.get("rhs").size)
+    rp = sim.get_int("rowptr", n + 1)
+    ci = sim.get_int("colind", rp[-1])
+    K = _dense_from_csr(rp, ci, sim.get("lhs"), n)
+    free = np.cumsum(1 - sc.fixed) - 1
+    fm = np.where(sc.fixed == 1, -1, free)
+    vols, _ = oracle.tet_geometry(sc.V, sc.T)
+    blocks = (sim.get("Aloc") if mixed else sim.get("Hplain")).reshape(sc.nT, 12, 12)
+    Kb = np.zeros((n, n))
+    for e in range(sc.nT):
+        m = 1000.0 * vols[e] / 20.0
+        for a in range(4):
+            for b in range(4):
+                fa, fb = fm[sc.T[e, a]], fm[sc.T[e, b]]
+                if fa < 0 or fb < 0:
+                    continue
+                blk = blocks[e].T[3 * a:3 * a + 3, 3 * b:3 * b + 3]   # column-major store
+                Kb[3 * fa:3 * fa + 3, 3 * fb:3 * fb + 3] += blk + m * (2 if a == b else 1) * np.eye(3)
+    assert np.abs(K - Kb).max() <= 1e-12 * np.abs(Kb).max()
+    assert np.abs(K - K.T).max() <= 1e-10 * np.abs(K).max()
+    dx = sim.get("dx")
+    rhs = sim.get("rhs")
+    assert np.linalg.norm(K @ dx - rhs) <= 1e-11 * np.linalg.norm(rhs)
+
+
+# ---------------------------------------------------------------- whole step
+
+def test_free_fall_closed_form(oracle):
+    """No pins, no deformation: BDF1 gives x_n = x_0 + h^2 g n(n+1)/2."""
+    sc = scenes.beam_scene(2, 1, 1, jitter=0.0, mixed=True, solver="eigen-llt",
+                           max_newton=5, newton_tol=1e-12)
+    sc.fixed[:] = 0
+    sim = oracle.Sim(sc)
+    h = sc.dt
+    g = np.array([0, np.float32(-9.8), 0], dtype=np.float64)
+    for n in range(1, 5):
+        sim.step()
+        expect = sc.V + h * h * g * n * (n + 1) / 2
+        assert np.abs(sim.positions() - expect).max() < 1e-10
+
+
+def test_mixed_equals_plain_at_convergence(oracle):
+    """SNH depends on F only through invariants of S, so a converged mixed
+    step equals a converged displacement-only step."""
+    kw = dict(itr_tol=1e-12, max_iters=20000, max_newton=30, newton_tol=1e-9)
+    a = oracle.Sim(scenes.beam_scene(4, 2, 2, seed=1, mixed=True, **kw))
+    b = oracle.Sim(scenes.beam_scene(4, 2, 2, seed=1, mixed=False, **kw))
+    for _ in range(4):
+        a.step()
+        b.step()
+        disp = np.abs(b.positions() - b.scene.V).max()
+        assert disp > 1e-3
+        assert np.abs(a.positions() - b.positions()).max() < 1e-6 * disp
+
+
+def test_line_search_decreases_energy(oracle):
+    sc = scenes.beam_scene(4, 2, 2, seed=2, mixed=True, itr_tol=1e-8, max_iters=5000)
+    sim = oracle.Sim(sc)
+    sim.update_system()
+    sim.solve()
+    e0 = sim.energy(0.0)
+    alpha = 1.0
+    while sim.energy(alpha) >= e0 and alpha > 1e-6:
+        alpha *= 0.5
+    assert sim.energy(alpha) < e0
+
+
+def test_pcg_matches_direct(oracle):
+    kw = dict(seed=3, mixed=True, max_newton=1)
+    a = oracle.Sim(scenes.beam_scene(4, 2, 2, solver="eigen-llt", **kw))
+    b = oracle.Sim(scenes.beam_scene(4, 2, 2, solver="eigen-pcg-diag", itr_tol=1e-13,
+                                     max_iters=10000, **kw))
+    for s in (a, b):
+        s.update_system()
+        s.solve()
+    da, db = a.get("dx"), b.get("dx")
+    assert np.abs(da - db).max() <= 1e-9 * np.abs(da).max()
+
+
+# ---------------------------------------------------------------- golden fixtures
+
+def test_golden_trajectories(oracle):
+    """Regression against the committed oracle trajectories
+    (tests/golden/make_golden.py)."""
+    from tests.golden import make_golden
+    data = np.load(os.path.join(GOLDEN, "trajectories.npz"))
+    for name, fn in make_golden.CASES.items():
+        sc, steps = fn()
+        sim = oracle.Sim(sc)
+        for k in range(steps):
+            sim.step()
+            ref = data[f"{name}_x{k}"]
+            assert np.abs(sim.positions() - ref).max() <= 1e-11 * np.abs(ref).max(), (name, k)
