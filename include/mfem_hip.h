@@ -1,0 +1,126 @@
+/*
+ * C ABI of the MI355X (gfx950) Newton-step library for the mixed variational
+ * FEM of tytrusty/mfem. Plain C types only: host pointers are borrowed for the
+ * duration of a call, device pointers are the caller's, every function
+ * returns 0 on success or a nonzero code with a message in
+ * mfem_last_error() (the reference prints and exits instead).
+ *
+ * The reference has no C ABI; its boundary is the C++ interface set of
+ * src/variables/, src/energies/, src/linear_solvers/, src/optimizers/ and
+ * src/simulation_state.cpp, selected by name from scene JSON. Each entry point
+ * below names the reference interface it stands in for.
+ */
+#ifndef MFEM_HIP_H
+#define MFEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfem_sim mfem_sim;
+
+/* Scene given as arrays (stands in for SimState<3>::load,
+ * reference src/simulation_state.cpp:92-341, for one tetrahedral object with
+ * per-element materials as with "material_ids", reference tet_mesh.h:11-15). */
+typedef struct {
+  int nV, nT;
+  const double* V;        /* nV x 3, row per vertex */
+  const int* T;           /* nT x 4 */
+  const double* mu;       /* nT Lame mu (reference config.h:13-16) */
+  const double* la;       /* nT Lame lambda */
+  const int* mat;         /* nT material id: 0 SNH, 1 NH, 2 Corot, 3 FCR, 4 ARAP, 5 Fung */
+  const int* fixed;       /* nV pin mask (1 = pinned), or NULL for bc_type/axis/ratio */
+  const double* v0;       /* nV x 3 initial velocity, or NULL */
+  double density;
+  double dt;              /* "dt" */
+  double gravity[3];      /* "body_force" (stored as float, as the reference does) */
+  int bdf_order;          /* "time_integrator": BDF1..BDF6 */
+  int mixed;              /* 1: "mixed_variables": ["mixed-stretch"]; 0: "variables": ["stretch"] */
+  const char* linear_solver; /* "linear_solver" name, e.g. "eigen-pcg-diag" */
+  double itr_tol;         /* "iterative_solver_tolerance" */
+  int max_iters;          /* "max_iterative_solver_iters" */
+  int max_newton;         /* "max_newton_iterations" */
+  double newton_tol;      /* SimConfig::newton_tol (no JSON key in the reference) */
+  int ls_iters;           /* "max_linesearch_iterations" */
+  int precision;          /* 0: FP64, 1: mixed FP32/FP64 */
+  int bc_type, bc_axis;   /* boundary condition when fixed == NULL (reference config.h:70-87) */
+  double bc_ratio;
+} mfem_sim_params;
+
+typedef struct {
+  int newton_iters;       /* Newton iterations of the last timestep */
+  int pcg_iters;          /* CG iterations summed over them */
+  int refinements;        /* defect-correction sweeps summed over them */
+  int ls_iters;           /* line-search halvings summed over them */
+  int ls_fail;            /* exhausted line searches */
+  double decrement;       /* last "Newton dec" (max |dx|, |ds|) */
+  double energy;          /* last E(0) */
+  double alpha;           /* last accepted step */
+  double res;             /* last energy residual */
+  double rel_residual;    /* |b - K dx| / |b| of the last solve */
+  double update_ms, solve_ms, ls_ms, step_ms; /* device time per phase (hipEvents) */
+  double spmv_ms;         /* summed device time of the CG SpMV kernel (when profiled) */
+  int spmv_count;
+} mfem_step_stats;
+
+const char* mfem_last_error(void);
+int mfem_version(void);
+
+/* SimState<3>::load(json) + OptimizerFactory::create("newton") + reset()
+ * (reference simulation_state.cpp:72-89, optimizer_factory.cpp:10-16,
+ * newton_optimizer.cpp:127-140). */
+int mfem_sim_create_json(const char* json_path, mfem_sim** out);
+/* Same from arrays. */
+int mfem_sim_create(const mfem_sim_params* p, mfem_sim** out);
+int mfem_sim_destroy(mfem_sim* sim);
+
+/* NewtonOptimizer<3>::step() (reference newton_optimizer.cpp:11-89). */
+int mfem_sim_step(mfem_sim* sim);
+/* Phases of step(): x_tilde + update_system() (:92-110), substep() (:113-124)
+ * and the line-search energy E(alpha) (:34-49). */
+int mfem_sim_update_system(mfem_sim* sim);
+int mfem_sim_solve(mfem_sim* sim, double* decrement);
+int mfem_sim_energy(mfem_sim* sim, double alpha, double* energy);
+int mfem_sim_stats(mfem_sim* sim, mfem_step_stats* out);
+/* Current vertex positions (nV x 3), i.e. Mesh::V_ after post_solve. */
+int mfem_sim_get_positions(mfem_sim* sim, double* out);
+/* Internal arrays by name, element-major, for parity checks: "R" (9/tet),
+ * "S", "s", "lambda", "ds", "g", "gl" (6/tet), "dsdF" (54/tet, ds/dvecF
+ * column-major), "H", "Hinv" (36/tet; H is Syminv Hpsi Syminv / vol as the
+ * reference's H_), "blocks" (90/tet), "fe" (12/tet), "lhs" (scalar CSR
+ * values in the reference's row order), "rhs", "dx" (3/free vertex),
+ * "x" (3/vertex), "x_tilde", "mass" (per nonzero block). */
+long mfem_sim_size(mfem_sim* sim, const char* name);
+int mfem_sim_get(mfem_sim* sim, const char* name, double* out);
+/* "rowptr", "colind" (scalar CSR pattern), "free_map", "fixed",
+ * "bsr_rowptr", "bsr_col", "slot_ptr", "slots" */
+int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out);
+/* bit 0: time every CG SpMV launch with hipEvents (summed in the stats) */
+int mfem_sim_set_profiling(mfem_sim* sim, int flags);
+/* Sizes used by the roofline model of the CG SpMV: free vertices, nonzero
+ * 3x3 blocks, bytes per stored value. */
+int mfem_sim_matrix_info(mfem_sim* sim, int* nfree, int* nnzb, int* value_bytes);
+/* Time `reps` back-to-back launches of the SpMV kernel on the current K and a
+ * device vector, returning the mean device time per launch (ms). */
+int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch);
+
+/* Batched MaterialModel evaluation on the 6-vector s (reference
+ * energies/material_model.h:29-38): device pointers, n points, s n x 6,
+ * E n, g n x 6, H n x 36 column-major; psd_fix as hessian(s, psd_fix). */
+int mfem_material_s6_batch(int n, const int* type, const double* mu, const double* la,
+                           const double* s, int psd_fix, double* E, double* g, double* H,
+                           void* stream);
+/* Batched plain SNH on vec(F) (reference stable_neohookean.cpp:97-234). */
+int mfem_snh_f9_batch(int n, const double* mu, const double* la, const double* F, double* E,
+                      double* g, double* H, void* stream);
+/* Batched polar fit: newton_procrustes(R, I, F, tol 1e-6, 100 iterations)
+ * warm-started from R (in/out), s = sym(R^T F), ds/dvecF (n x 54), iteration
+ * counts (reference svd/newton_procrustes.h:108-189, mixed_stretch.cpp:14-42).
+ * dsdF and iters may be NULL. */
+int mfem_polar_batch(int n, const double* F, double* R, double* s, double* dsdF, int* iters,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFEM_HIP_H */

@@ -1,0 +1,371 @@
+// extern "C" boundary of the device Newton step (declared in
+// include/mfem_hip.h). Exceptions never cross it: every entry point returns 0
+// or a nonzero code and leaves a message for mfem_last_error().
+#include "../../include/mfem_hip.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host/sim.h"
+#include "kernels.h"
+
+using namespace mfem;
+
+struct mfem_sim {
+  std::unique_ptr<NewtonOptimizer<3>> opt;
+  SimState<3>& state() { return opt->state(); }
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "unknown error";
+  }
+  return 1;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// structure-of-arrays (stride nT) -> element-major
+std::vector<double> soa_to_elem(const std::vector<double>& a, int nT, int k) {
+  std::vector<double> o((size_t)nT * k);
+  for (int e = 0; e < nT; ++e)
+    for (int i = 0; i < k; ++i) o[(size_t)e * k + i] = a[(size_t)i * nT + e];
+  return o;
+}
+
+int sym6(int i, int j) {
+  if (i > j) std::swap(i, j);
+  return i * 6 - (i * (i - 1)) / 2 + (j - i);
+}
+
+std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
+  auto& S = sim->state();
+  auto& m = *S.mesh_;
+  hipStream_t st = S.ctx_->stream;
+  const int nT = m.nT();
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+  MixedStretch<3>* ms = S.mixed_vars_.empty() ? nullptr
+                                               : dynamic_cast<MixedStretch<3>*>(S.mixed_vars_[0].get());
+  auto need_ms = [&]() {
+    if (!ms) throw std::runtime_error("'" + k + "' needs a mixed-stretch variable");
+    return ms;
+  };
+  if (k == "R") return soa_to_elem(need_ms()->R_.to_host(st), nT, 9);
+  if (k == "S") return soa_to_elem(need_ms()->S_.to_host(st), nT, 6);
+  if (k == "s") return soa_to_elem(need_ms()->s_.to_host(st), nT, 6);
+  if (k == "lambda") return soa_to_elem(need_ms()->la_.to_host(st), nT, 6);
+  if (k == "ds") return soa_to_elem(need_ms()->ds_.to_host(st), nT, 6);
+  if (k == "g") return soa_to_elem(need_ms()->g_.to_host(st), nT, 6);
+  if (k == "gl") return soa_to_elem(need_ms()->gl_.to_host(st), nT, 6);
+  if (k == "dsdF") return soa_to_elem(need_ms()->dsdF_.to_host_double(st), nT, 54);
+  if (k == "H" || k == "Hinv") {
+    auto P = soa_to_elem(k == "H" ? need_ms()->Hpsi_.to_host_double(st)
+                                  : need_ms()->Hinv_.to_host_double(st), nT, 21);
+    std::vector<double> o((size_t)nT * 36);
+    const double syminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+    for (int e = 0; e < nT; ++e)
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+          double v = P[(size_t)e * 21 + sym6(i, j)];
+          if (k == "H") v = syminv[i] * v * syminv[j] / m.vols_[e];
+          o[(size_t)e * 36 + i + 6 * j] = v;
+        }
+    return o;
+  }
+  if (k == "blocks") return S.ctx_->blocks.to_host_double(st);
+  if (k == "fe") return S.ctx_->fe.to_host(st);
+  if (k == "x") return S.x_->value().to_host(st);
+  if (k == "x_tilde") return S.ctx_->x_tilde.to_host(st);
+  if (k == "dx") return S.x_->delta().to_host(st);
+  auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
+  if (k == "rhs" || k == "lhs") {
+    if (!pcg) throw std::runtime_error("no assembled system");
+    if (k == "rhs") return pcg->system().rhs.to_host(st);
+    auto Kb = pcg->system().K.to_host_double(st);
+    std::vector<double> o;
+    o.reserve(Kb.size());
+    for (int i = 0; i < m.nfree(); ++i)
+      for (int r = 0; r < 3; ++r)
+        for (int b = m.rowptr_[i]; b < m.rowptr_[i + 1]; ++b)
+          for (int c = 0; c < 3; ++c) o.push_back(Kb[(size_t)9 * b + 3 * r + c]);
+    return o;
+  }
+  if (k == "mass") return m.d_mass.to_host(st);
+  throw std::runtime_error("unknown array '" + k + "'");
+}
+
+std::vector<int> fetch_int(mfem_sim* sim, const std::string& k) {
+  auto& m = *sim->state().mesh_;
+  hipStream_t st = sim->state().ctx_->stream;
+  if (k == "free_map") return m.free_map_;
+  if (k == "fixed") return m.is_fixed_;
+  if (k == "bsr_rowptr") return m.rowptr_;
+  if (k == "bsr_col") return m.col_;
+  if (k == "slot_ptr") return m.d_slot_ptr.to_host(st);
+  if (k == "slots") return m.d_slots.to_host(st);
+  if (k == "rowptr" || k == "colind") {
+    std::vector<int> rp(1, 0), ci;
+    for (int i = 0; i < m.nfree(); ++i)
+      for (int r = 0; r < 3; ++r) {
+        for (int b = m.rowptr_[i]; b < m.rowptr_[i + 1]; ++b)
+          for (int c = 0; c < 3; ++c) ci.push_back(3 * m.col_[b] + c);
+        rp.push_back((int)ci.size());
+      }
+    return k == "rowptr" ? rp : ci;
+  }
+  throw std::runtime_error("unknown array '" + k + "'");
+}
+
+void finish_create(std::unique_ptr<SimState<3>> state, mfem_sim** out) {
+  OptimizerFactory<3> of;
+  if (state->config_->optimizer != OPTIMIZER_NEWTON)
+    throw std::runtime_error("only the \"newton\" optimizer is implemented");
+  auto base = of.create(OPTIMIZER_NEWTON, std::move(state));
+  auto* raw = dynamic_cast<NewtonOptimizer<3>*>(base.get());
+  auto sim = std::make_unique<mfem_sim>();
+  sim->opt.reset(raw);
+  base.release();
+  sim->opt->reset();
+  *out = sim.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mfem_last_error(void) { return g_last_error.c_str(); }
+
+int mfem_version(void) { return 1; }
+
+int mfem_sim_create_json(const char* json_path, mfem_sim** out) {
+  return guarded([&] {
+    if (!out) throw std::runtime_error("null output handle");
+    auto state = std::make_unique<SimState<3>>();
+    if (!state->load(std::string(json_path))) throw std::runtime_error("scene load failed");
+    finish_create(std::move(state), out);
+  });
+}
+
+int mfem_sim_create(const mfem_sim_params* p, mfem_sim** out) {
+  return guarded([&] {
+    if (!p || !out) throw std::runtime_error("null argument");
+    if (p->nV <= 0 || p->nT <= 0) throw std::runtime_error("empty mesh");
+    SceneArrays sc;
+    sc.V.assign(p->V, p->V + 3L * p->nV);
+    sc.T.assign(p->T, p->T + 4L * p->nT);
+    for (int v : sc.T)
+      if (v < 0 || v >= p->nV) throw std::runtime_error("tet vertex index out of range");
+    sc.mu.assign(p->mu, p->mu + p->nT);
+    sc.la.assign(p->la, p->la + p->nT);
+    sc.mat.assign(p->mat, p->mat + p->nT);
+    if (p->fixed) sc.fixed.assign(p->fixed, p->fixed + p->nV);
+    if (p->v0) sc.v0.assign(p->v0, p->v0 + 3L * p->nV);
+    sc.density = p->density;
+    sc.bc.type = (BCScriptType)p->bc_type;
+    sc.bc.axis = p->bc_axis;
+    sc.bc.ratio = p->bc_ratio;
+    auto cfg = std::make_shared<SimConfig>();
+    cfg->h = p->dt;
+    for (int i = 0; i < 3; ++i) cfg->ext[i] = (float)p->gravity[i];
+    if (p->bdf_order < 1 || p->bdf_order > 6) throw std::runtime_error("bdf_order must be 1..6");
+    cfg->ti_type = (TimeIntegratorType)(p->bdf_order - 1);
+    LinearSolverFactory<3> lf;
+    const std::string sname = p->linear_solver ? p->linear_solver : "eigen-pcg-diag";
+    if (!lf.has_name(sname)) throw std::runtime_error("unknown linear solver '" + sname + "'");
+    cfg->solver_type = lf.type_by_name(sname);
+    cfg->itr_tol = p->itr_tol;
+    cfg->max_iterative_solver_iters = p->max_iters;
+    cfg->outer_steps = p->max_newton;
+    cfg->newton_tol = p->newton_tol;
+    cfg->ls_iters = p->ls_iters;
+    cfg->precision = p->precision ? Precision::MIXED : Precision::FP64;
+    cfg->mixed_variables.clear();
+    cfg->variables.clear();
+    auto state = std::make_unique<SimState<3>>();
+    std::vector<std::string> mixed, plain;
+    if (p->mixed) mixed.push_back("mixed-stretch");
+    else plain.push_back("stretch");
+    state->load_arrays(sc, cfg, mixed, plain);
+    finish_create(std::move(state), out);
+  });
+}
+
+int mfem_sim_destroy(mfem_sim* sim) {
+  return guarded([&] { delete sim; });
+}
+
+int mfem_sim_step(mfem_sim* sim) {
+  return guarded([&] { sim->opt->step(); });
+}
+
+int mfem_sim_update_system(mfem_sim* sim) {
+  return guarded([&] {
+    sim->state().x_->pre_solve();
+    sim->opt->update_system();
+    MFEM_HIP_CHECK(hipStreamSynchronize(sim->state().ctx_->stream));
+  });
+}
+
+int mfem_sim_solve(mfem_sim* sim, double* decrement) {
+  return guarded([&] {
+    double d = 0;
+    sim->opt->substep(d);
+    if (decrement) *decrement = d;
+  });
+}
+
+int mfem_sim_energy(mfem_sim* sim, double alpha, double* energy) {
+  return guarded([&] { *energy = sim->opt->energy(alpha); });
+}
+
+int mfem_sim_stats(mfem_sim* sim, mfem_step_stats* o) {
+  return guarded([&] {
+    const auto& s = sim->opt->last_stats();
+    o->newton_iters = s.newton_iters;
+    o->pcg_iters = s.pcg_iters;
+    o->refinements = s.refinements;
+    o->ls_iters = s.ls_iters;
+    o->ls_fail = s.ls_fail;
+    o->decrement = s.decrement;
+    o->energy = s.energy;
+    o->alpha = s.alpha;
+    o->res = s.res;
+    o->rel_residual = s.rel_residual;
+    o->update_ms = s.update_ms;
+    o->solve_ms = s.solve_ms;
+    o->ls_ms = s.ls_ms;
+    o->step_ms = s.step_ms;
+    o->spmv_ms = s.spmv_ms;
+    o->spmv_count = s.spmv_count;
+  });
+}
+
+int mfem_sim_get_positions(mfem_sim* sim, double* out) {
+  return guarded([&] {
+    auto v = sim->state().x_->value().to_host(sim->state().ctx_->stream);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+long mfem_sim_size(mfem_sim* sim, const char* name) {
+  long n = -1;
+  int rc = guarded([&] {
+    const std::string k(name);
+    if (k == "rowptr" || k == "colind" || k == "free_map" || k == "fixed" || k == "bsr_rowptr" ||
+        k == "bsr_col" || k == "slot_ptr" || k == "slots")
+      n = (long)fetch_int(sim, k).size();
+    else
+      n = (long)fetch(sim, k).size();
+  });
+  return rc == 0 ? n : -1;
+}
+
+int mfem_sim_get(mfem_sim* sim, const char* name, double* out) {
+  return guarded([&] {
+    auto v = fetch(sim, name);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out) {
+  return guarded([&] {
+    auto v = fetch_int(sim, name);
+    std::memcpy(out, v.data(), v.size() * sizeof(int));
+  });
+}
+
+int mfem_sim_set_profiling(mfem_sim* sim, int flags) {
+  return guarded([&] { sim->opt->solver()->profile_spmv = (flags & 1) != 0; });
+}
+
+int mfem_sim_matrix_info(mfem_sim* sim, int* nfree, int* nnzb, int* value_bytes) {
+  return guarded([&] {
+    auto& m = *sim->state().mesh_;
+    *nfree = m.nfree();
+    *nnzb = m.nnzb();
+    *value_bytes = sim->state().ctx_->fp32() ? 4 : 8;
+  });
+}
+
+int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch) {
+  return guarded([&] {
+    auto& S = sim->state();
+    auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
+    if (!pcg || pcg->system().K.bytes() == 0) throw std::runtime_error("no assembled system");
+    auto& m = *S.mesh_;
+    hipStream_t st = S.ctx_->stream;
+    const int n = 3 * m.nfree();
+    hipEvent_t a, b;
+    MFEM_HIP_CHECK(hipEventCreate(&a));
+    MFEM_HIP_CHECK(hipEventCreate(&b));
+    if (S.ctx_->fp32()) {
+      DeviceArray<float> x(n), y(n);
+      std::vector<float> ones(n, 1.0f);
+      x.upload(ones, st);
+      mfemhip::launch_spmv<float, float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
+                                         pcg->system().K.f32(), x.data(), y.data(), st);
+      MFEM_HIP_CHECK(hipEventRecord(a, st));
+      for (int r = 0; r < reps; ++r)
+        mfemhip::launch_spmv<float, float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
+                                           pcg->system().K.f32(), x.data(), y.data(), st);
+      MFEM_HIP_CHECK(hipEventRecord(b, st));
+      MFEM_HIP_CHECK(hipEventSynchronize(b));
+    } else {
+      DeviceArray<double> x(n), y(n);
+      std::vector<double> ones(n, 1.0);
+      x.upload(ones, st);
+      mfemhip::launch_spmv<double, double>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
+                                           pcg->system().K.f64(), x.data(), y.data(), st);
+      MFEM_HIP_CHECK(hipEventRecord(a, st));
+      for (int r = 0; r < reps; ++r)
+        mfemhip::launch_spmv<double, double>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
+                                             pcg->system().K.f64(), x.data(), y.data(), st);
+      MFEM_HIP_CHECK(hipEventRecord(b, st));
+      MFEM_HIP_CHECK(hipEventSynchronize(b));
+    }
+    float ms = 0;
+    MFEM_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    *ms_per_launch = reps > 0 ? ms / reps : 0.0;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  });
+}
+
+int mfem_material_s6_batch(int n, const int* type, const double* mu, const double* la,
+                           const double* s, int psd_fix, double* E, double* g, double* H,
+                           void* stream) {
+  return guarded([&] {
+    mfemhip::launch_batch_material(n, type, mu, la, s, psd_fix, E, g, H, as_stream(stream));
+    MFEM_HIP_CHECK(hipGetLastError());
+  });
+}
+
+int mfem_snh_f9_batch(int n, const double* mu, const double* la, const double* F, double* E,
+                      double* g, double* H, void* stream) {
+  return guarded([&] {
+    mfemhip::launch_batch_snh_f9(n, mu, la, F, E, g, H, as_stream(stream));
+    MFEM_HIP_CHECK(hipGetLastError());
+  });
+}
+
+int mfem_polar_batch(int n, const double* F, double* R, double* s, double* dsdF, int* iters,
+                     void* stream) {
+  return guarded([&] {
+    mfemhip::launch_batch_polar(n, F, R, s, dsdF, iters, as_stream(stream));
+    MFEM_HIP_CHECK(hipGetLastError());
+  });
+}
+
+}  // extern "C"

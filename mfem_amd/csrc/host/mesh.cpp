@@ -1,0 +1,265 @@
+#include "mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+namespace mfem {
+
+Mesh::Mesh(const std::vector<double>& V, const std::vector<int>& T,
+           const std::vector<int>& material_ids,
+           const std::vector<std::shared_ptr<MaterialModel>>& materials)
+    : V_(V), Vref_(V), Vinit_(V), T_(T) {
+  if (materials.empty()) throw std::runtime_error("Mesh: no materials");
+  material_ = materials[0];
+  nV_ = (int)(V.size() / 3);
+  nT_ = (int)(T.size() / 4);
+  if ((int)material_ids.size() != nT_) throw std::runtime_error("Mesh: material_ids size");
+  for (int i = 0; i < nT_; ++i) {
+    int id = material_ids[i];
+    if (id < 0 || id >= (int)materials.size()) throw std::runtime_error("Mesh: bad material id");
+    elements_.emplace_back(materials[id]);
+  }
+  initial_velocity_.assign(V.size(), 0.0);
+}
+
+Mesh::Mesh(const std::vector<double>& V, const std::vector<int>& T,
+           std::shared_ptr<MaterialModel> material)
+    : V_(V), Vref_(V), Vinit_(V), T_(T), material_(material) {
+  nV_ = (int)(V.size() / 3);
+  nT_ = (int)(T.size() / 4);
+  for (int i = 0; i < nT_; ++i) elements_.emplace_back(material);
+  initial_velocity_.assign(V.size(), 0.0);
+}
+
+Mesh::Mesh(const std::vector<double>& V, const std::vector<int>& T,
+           std::shared_ptr<MaterialModel> material, const std::vector<double>& mu,
+           const std::vector<double>& la, const std::vector<int>& mat)
+    : V_(V), Vref_(V), Vinit_(V), T_(T), material_(material), elem_mu_(mu), elem_la_(la),
+      elem_mat_(mat) {
+  nV_ = (int)(V.size() / 3);
+  nT_ = (int)(T.size() / 4);
+  if ((int)mu.size() != nT_ || (int)la.size() != nT_ || (int)mat.size() != nT_)
+    throw std::runtime_error("Mesh: per-element parameter arrays must have nT entries");
+  initial_velocity_.assign(V.size(), 0.0);
+}
+
+std::vector<int> boundary_fixed(const BoundaryConditionConfig& bc, const std::vector<double>& V,
+                                int dim) {
+  const int nV = (int)(V.size() / dim);
+  std::vector<int> fixed(nV, 0);
+  if (nV == 0) return fixed;
+  const int ax = bc.axis;
+  double mn = V[ax], mx = V[ax];
+  for (int i = 0; i < nV; ++i) {
+    mn = std::min(mn, V[i * dim + ax]);
+    mx = std::max(mx, V[i * dim + ax]);
+  }
+  const double range = mx - mn;
+  std::vector<int> g0, g1;
+  for (int i = 0; i < nV; ++i) {
+    double v = V[i * dim + ax];
+    if (v < mn + range * bc.ratio) g0.push_back(i);
+    else if (v > mx - range * bc.ratio) g1.push_back(i);
+  }
+  switch (bc.type) {
+    case BC_HANG:
+      if (!g0.empty()) fixed[g0.back()] = 1;
+      if (!g1.empty()) fixed[g1.back()] = 1;
+      break;
+    case BC_HANGENDS:
+      for (int i : g1) fixed[i] = 1;
+      break;
+    case BC_STRETCH:
+    case BC_BEND:
+    case BC_TWIST:
+      for (int i : g0) fixed[i] = 1;
+      for (int i : g1) fixed[i] = 1;
+      break;
+    case BC_ONEPOINT:
+      fixed[0] = 1;
+      break;
+    default:
+      break;
+  }
+  return fixed;
+}
+
+void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
+  V_ = Vinit_;
+  for (int v : T_)
+    if (v < 0 || v >= nV_) throw std::runtime_error("Mesh: tet vertex index out of range");
+  is_fixed_ = fixed_override.empty() ? boundary_fixed(bc_config_, V_, 3) : fixed_override;
+  if ((int)is_fixed_.size() != nV_) throw std::runtime_error("Mesh: pin mask size");
+  free_map_.assign(nV_, -1);
+  nfree_ = 0;
+  for (int i = 0; i < nV_; ++i)
+    if (!is_fixed_[i]) free_map_[i] = nfree_++;
+  // volumes |det Dm| / 6 and dphi/dX = rows of Dm^{-1} (phi_0 = -sum)
+  vols_.assign(nT_, 0.0);
+  dX_.assign((size_t)nT_ * 12, 0.0);
+  for (int e = 0; e < nT_; ++e) {
+    const double* X0 = &Vref_[3 * T_[4 * e]];
+    double D[9];
+    for (int k = 1; k < 4; ++k) {
+      const double* Xk = &Vref_[3 * T_[4 * e + k]];
+      for (int r = 0; r < 3; ++r) D[r + 3 * (k - 1)] = Xk[r] - X0[r];
+    }
+    double c00 = D[4] * D[8] - D[7] * D[5], c01 = D[7] * D[2] - D[1] * D[8],
+           c02 = D[1] * D[5] - D[4] * D[2];
+    double det = D[0] * c00 + D[3] * c01 + D[6] * c02;
+    if (det == 0.0) throw std::runtime_error("Mesh: degenerate tetrahedron " + std::to_string(e));
+    vols_[e] = std::fabs(det) / 6.0;
+    // inverse (row i of Dm^{-1} is grad phi_{i+1})
+    double inv[9];
+    inv[0] = c00 / det;
+    inv[3] = (D[6] * D[5] - D[3] * D[8]) / det;
+    inv[6] = (D[3] * D[7] - D[6] * D[4]) / det;
+    inv[1] = c01 / det;
+    inv[4] = (D[0] * D[8] - D[6] * D[2]) / det;
+    inv[7] = (D[6] * D[1] - D[0] * D[7]) / det;
+    inv[2] = c02 / det;
+    inv[5] = (D[3] * D[2] - D[0] * D[5]) / det;
+    inv[8] = (D[0] * D[4] - D[3] * D[1]) / det;
+    for (int c = 0; c < 3; ++c) {
+      double s = 0;
+      for (int k = 1; k < 4; ++k) {
+        double v = inv[(k - 1) + 3 * c];
+        dX_[(size_t)e * 12 + 3 * k + c] = v;
+        s += v;
+      }
+      dX_[(size_t)e * 12 + c] = -s;
+    }
+  }
+  build_maps();
+  // upload
+  std::vector<int4> T4(nT_);
+  for (int e = 0; e < nT_; ++e) T4[e] = make_int4(T_[4 * e], T_[4 * e + 1], T_[4 * e + 2], T_[4 * e + 3]);
+  d_T.upload(T4, st);
+  std::vector<double> dXs((size_t)nT_ * 12);
+  for (int e = 0; e < nT_; ++e)
+    for (int i = 0; i < 12; ++i) dXs[(size_t)i * nT_ + e] = dX_[(size_t)e * 12 + i];
+  d_dX.upload(dXs, st);
+  d_vol.upload(vols_, st);
+  std::vector<double> mu(nT_), la(nT_);
+  std::vector<int> mat(nT_);
+  for (int e = 0; e < nT_; ++e) {
+    MaterialModelType t;
+    if (!elem_mat_.empty()) {
+      t = (MaterialModelType)elem_mat_[e];
+      mu[e] = elem_mu_[e];
+      la[e] = elem_la_[e];
+    } else {
+      const auto& c = elements_[e].material_->config();
+      t = c->material_model;
+      mu[e] = c->mu;
+      la[e] = c->la;
+    }
+    if (!material_supported_on_device(t))
+      throw std::runtime_error("material '" + MaterialModelFactory().name_by_type(t) +
+                               "' has no device kernel");
+    mat[e] = (int)t;
+  }
+  d_mu.upload(mu, st);
+  d_la.upload(la, st);
+  d_mat.upload(mat, st);
+  d_free_map.upload(free_map_, st);
+  d_rowptr.upload(rowptr_, st);
+  d_col.upload(col_, st);
+  d_slot_ptr.upload(slot_ptr_, st);
+  d_slots.upload(slots_, st);
+  d_diag_block.upload(diag_block_, st);
+  d_inc_ptr.upload(inc_ptr_, st);
+  d_inc.upload(inc_, st);
+  d_mass.upload(mass_, st);
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// Scatter maps (reference sparse_utils.cpp:19-138): free-free vertex pairs of
+// every tet, sorted by (row, col) with ties in (tet, local pair) order.
+void Mesh::build_maps() {
+  // vertex -> (tet, corner) incidences of free vertices, in tet order
+  inc_ptr_.assign(nfree_ + 1, 0);
+  for (int e = 0; e < nT_; ++e)
+    for (int a = 0; a < 4; ++a) {
+      int f = free_map_[T_[4 * e + a]];
+      if (f >= 0) inc_ptr_[f + 1]++;
+    }
+  for (int i = 0; i < nfree_; ++i) inc_ptr_[i + 1] += inc_ptr_[i];
+  inc_.assign(inc_ptr_[nfree_], 0);
+  {
+    std::vector<int> fill(inc_ptr_.begin(), inc_ptr_.end() - 1);
+    for (int e = 0; e < nT_; ++e)
+      for (int a = 0; a < 4; ++a) {
+        int f = free_map_[T_[4 * e + a]];
+        if (f >= 0) inc_[fill[f]++] = (e << 2) | a;
+      }
+  }
+  if (nT_ >= (1 << 27)) throw std::runtime_error("Mesh: too many tets for 32-bit slot ids");
+  const double denom = 20.0;
+  const double rho = material_->config()->density;
+  rowptr_.assign(nfree_ + 1, 0);
+  col_.clear();
+  slot_ptr_.assign(1, 0);
+  slots_.clear();
+  mass_.clear();
+  diag_block_.assign(nfree_, -1);
+  struct Ent { int col, e, b; };
+  std::vector<Ent> ents;
+  for (int i = 0; i < nfree_; ++i) {
+    ents.clear();
+    for (int p = inc_ptr_[i]; p < inc_ptr_[i + 1]; ++p) {
+      int e = inc_[p] >> 2;
+      for (int b = 0; b < 4; ++b) {
+        int fb = free_map_[T_[4 * e + b]];
+        if (fb >= 0) ents.push_back({fb, e, b});
+      }
+    }
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent& x, const Ent& y) {
+      if (x.col != y.col) return x.col < y.col;
+      if (x.e != y.e) return x.e < y.e;
+      return x.b < y.b;
+    });
+    size_t k = 0;
+    while (k < ents.size()) {
+      int c = ents[k].col;
+      double m = 0;
+      if (c == i) diag_block_[i] = (int)col_.size();
+      while (k < ents.size() && ents[k].col == c) {
+        int e = ents[k].e;
+        // local index of row vertex i in tet e
+        int a = 0;
+        while (free_map_[T_[4 * e + a]] != i) ++a;
+        slots_.push_back((e << 4) | (a << 2) | ents[k].b);
+        m += rho * vols_[e] / denom * (a == ents[k].b ? 2.0 : 1.0);
+        ++k;
+      }
+      col_.push_back(c);
+      mass_.push_back(m);
+      slot_ptr_.push_back((int)slots_.size());
+    }
+    rowptr_[i + 1] = (int)col_.size();
+    if (diag_block_[i] < 0)
+      throw std::runtime_error("Mesh: free vertex not referenced by any tetrahedron (singular system)");
+  }
+  nnzb_ = (int)col_.size();
+}
+
+mfemhip::TetMeshDev Mesh::dev() const {
+  mfemhip::TetMeshDev m;
+  m.nV = nV_;
+  m.nT = nT_;
+  m.nfree = nfree_;
+  m.T = d_T.data();
+  m.dX = d_dX.data();
+  m.vol = d_vol.data();
+  m.mu = d_mu.data();
+  m.la = d_la.data();
+  m.mat = d_mat.data();
+  m.free_map = d_free_map.data();
+  m.density = material_->config()->density;
+  return m;
+}
+
+}  // namespace mfem

@@ -1,0 +1,82 @@
+// Tetrahedral mesh: setup-time quantities of the reference's Mesh /
+// TetrahedralMesh / Assembler (src/mesh/mesh.cpp:44-95, tet_mesh.cpp:16-80,
+// src/utils/sparse_utils.cpp:19-138), built once on the host and uploaded.
+//
+// Device layout (structure of arrays, stride nT): T as int4, dX as 12 rows of
+// dphi_k/dX_c, vol / mu / la / material id per tet; free_map per vertex.
+// Assembly maps: BSR(3x3) pattern over free vertices; for every nonzero block
+// its sorted (tet, local pair) slot list and its consistent-mass coefficient;
+// for every free vertex its sorted (tet, corner) incidence list.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "../kernels.h"
+#include "config.h"
+#include "device.h"
+#include "material_model.h"
+
+namespace mfem {
+
+struct Element {
+  explicit Element(std::shared_ptr<MaterialModel> m) : material_(std::move(m)) {}
+  std::shared_ptr<MaterialModel> material_;
+};
+
+class Mesh {
+ public:
+  // V: nV x 3 row-major, T: nT x 4 row-major.
+  Mesh(const std::vector<double>& V, const std::vector<int>& T, const std::vector<int>& material_ids,
+       const std::vector<std::shared_ptr<MaterialModel>>& materials);
+  Mesh(const std::vector<double>& V, const std::vector<int>& T,
+       std::shared_ptr<MaterialModel> material);
+  // Per-element Lame parameters and model ids given directly (one material
+  // per tet, as the reference's material_ids constructor with one model per
+  // id); `material` supplies the density (reference tet_mesh.cpp:36-41).
+  Mesh(const std::vector<double>& V, const std::vector<int>& T, std::shared_ptr<MaterialModel> material,
+       const std::vector<double>& mu, const std::vector<double>& la, const std::vector<int>& mat);
+
+  // BCs, volumes, shape gradients, assembly maps, upload (reference
+  // Mesh::init, mesh.cpp:44-79). `fixed_override`, when non-empty, replaces
+  // the boundary-condition pin set (one flag per vertex).
+  void init(hipStream_t st, const std::vector<int>& fixed_override = {});
+
+  mfemhip::TetMeshDev dev() const;
+  int nV() const { return nV_; }
+  int nT() const { return nT_; }
+  int nfree() const { return nfree_; }
+  int nnzb() const { return nnzb_; }
+
+  // host state (reference Mesh public members)
+  std::vector<double> V_, Vref_, Vinit_, initial_velocity_;
+  std::vector<int> T_;
+  std::vector<int> is_fixed_, free_map_;
+  BoundaryConditionConfig bc_config_;
+  std::vector<Element> elements_;
+  std::shared_ptr<MaterialModel> material_;
+  std::vector<double> vols_, dX_;  // host copies (dX_[e*12 + 3k + c])
+  std::vector<double> elem_mu_, elem_la_;  // per-element parameters (when given directly)
+  std::vector<int> elem_mat_;
+
+  // device state
+  DeviceArray<int4> d_T;
+  DeviceArray<double> d_dX, d_vol, d_mu, d_la;
+  DeviceArray<int> d_mat, d_free_map;
+  DeviceArray<int> d_rowptr, d_col, d_slot_ptr, d_slots, d_diag_block, d_inc_ptr, d_inc;
+  DeviceArray<double> d_mass;
+  std::vector<int> rowptr_, col_;  // host copy of the BSR pattern
+
+ private:
+  void build_maps();
+  int nV_ = 0, nT_ = 0, nfree_ = 0, nnzb_ = 0;
+  std::vector<int> slot_ptr_, slots_, diag_block_, inc_ptr_, inc_;
+  std::vector<double> mass_;
+};
+
+// Pinned set of the reference's boundary conditions
+// (boundary_condition.cpp:5-21, fixed_boundary_conditions.h:75-166,
+// bend/stretch/twist_boundary_condition.h).
+std::vector<int> boundary_fixed(const BoundaryConditionConfig& bc, const std::vector<double>& V,
+                                int dim);
+
+}  // namespace mfem

@@ -1,0 +1,961 @@
+#include "sim.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+namespace mfem {
+
+using namespace mfemhip;
+
+// ============================================================ Reducer
+void Reducer::init(long max_items) {
+  // Largest grid of any reducing kernel: element kernels (n/128), BSR SpMV
+  // with 8 lanes per block row (n/32 rows), vector kernels (n/256); the
+  // two-value CG reduction stores 2 partials per block.
+  long g = std::max<long>(reduce_grid(max_items), max_items / 32 + 1);
+  stride_ = std::max<long>(2 * g + 64, 4096);
+  partials_.resize((size_t)kSlots * stride_);
+  results_.resize(kSlots);
+  counters_.resize(kSlots);
+  counters_.zero();
+  results_.zero();
+  pinned_ = std::make_unique<PinnedBuffer>(256);
+  MFEM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+ReduceSlot Reducer::slot(int k) {
+  ReduceSlot r;
+  r.partials = partials_.data() + (size_t)k * stride_;
+  r.counter = counters_.data() + k;
+  r.result = results_.data() + k;
+  return r;
+}
+
+double Reducer::read(int k, hipStream_t st) {
+  MFEM_HIP_CHECK(hipMemcpyAsync(pinned_->as<double>(), results_.data() + k, sizeof(double),
+                                hipMemcpyDeviceToHost, st));
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+  return pinned_->as<double>()[0];
+}
+
+Inertia DeviceContext::inertia() const {
+  Inertia in;
+  in.x_tilde = x_tilde.data();
+  for (int i = 0; i < 3; ++i) in.h2g[i] = h2g[i];
+  return in;
+}
+
+DeviceContext::~DeviceContext() {
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+// reduction slot ids
+enum { RS_ENERGY = 0, RS_MAXDS = 1, RS_MAXDX = 2, RS_RES = 3, RS_PCG0 = 4, RS_PCG1 = 5, RS_PCG2 = 6 };
+
+// ============================================================ BDF
+template <int I>
+std::array<double, I> BDF<I>::alphas() {
+  // reference src/time_integrators/BDF.cpp:45-75
+  if constexpr (I == 1) return {1.0};
+  if constexpr (I == 2) return {4.0 / 3.0, -1.0 / 3.0};
+  if constexpr (I == 3) return {18.0 / 11.0, -9.0 / 11.0, 2.0 / 11.0};
+  if constexpr (I == 4) return {48.0 / 25.0, -36.0 / 25.0, 16.0 / 25.0, -3.0 / 25.0};
+  if constexpr (I == 5)
+    return {300.0 / 137.0, -300.0 / 137.0, 200.0 / 137.0, -75.0 / 137.0, 12.0 / 137.0};
+  if constexpr (I == 6)
+    return {360.0 / 147.0, -450.0 / 147.0, 400.0 / 147.0, -225.0 / 147.0, 72.0 / 147.0,
+            -10.0 / 147.0};
+}
+
+template <int I>
+double BDF<I>::beta() {
+  // reference src/time_integrators/BDF.cpp:77-93
+  switch (I) {
+    case 1: return 1.0;
+    case 2: return 2.0 / 3.0;
+    case 3: return 6.0 / 11.0;
+    case 4: return 12.0 / 25.0;
+    case 5: return 60.0 / 137.0;
+    default: return 60.0 / 147.0;
+  }
+}
+
+template <int I>
+BDF<I>::BDF(const std::vector<double>& x0, const std::vector<double>& v0, double h, hipStream_t st)
+    : ImplicitIntegrator(h) {
+  n_ = (int)x0.size();
+  for (int j = 0; j < I; ++j) {  // I copies of (x0, v0) (reference BDF.h:19-26)
+    xs_[j].upload(x0, st);
+    vs_[j].upload(v0, st);
+  }
+  head_ = 0;
+}
+
+template <int I>
+void BDF<I>::x_tilde(double* out, hipStream_t st) const {
+  const double* xh[6];
+  const double* vh[6];
+  for (int j = 0; j < I; ++j) {
+    xh[j] = xs_[(head_ + j) % I].data();
+    vh[j] = vs_[(head_ + j) % I].data();
+  }
+  auto a = alphas();
+  launch_bdf_xtilde(n_, I, xh, vh, a.data(), dt(), out, st);
+}
+
+template <int I>
+void BDF<I>::update(const double* x, hipStream_t st) {
+  const double* xh[6];
+  for (int j = 0; j < I; ++j) xh[j] = xs_[(head_ + j) % I].data();
+  const int oldest = (head_ + I - 1) % I;
+  auto a = alphas();
+  launch_bdf_update(n_, I, xh, a.data(), dt(), x, xs_[oldest].data(), vs_[oldest].data(), st);
+  head_ = oldest;
+}
+
+IntegratorFactory::IntegratorFactory() {
+#define MFEM_REG_BDF(k)                                                                    \
+  register_type(TI_BDF##k, BDF<k>::name(),                                                 \
+                [](const std::vector<double>& x0, const std::vector<double>& v0, double h, \
+                   hipStream_t st) -> std::unique_ptr<ImplicitIntegrator> {                \
+                  return std::make_unique<BDF<k>>(x0, v0, h, st);                          \
+                });
+  MFEM_REG_BDF(1) MFEM_REG_BDF(2) MFEM_REG_BDF(3) MFEM_REG_BDF(4) MFEM_REG_BDF(5) MFEM_REG_BDF(6)
+#undef MFEM_REG_BDF
+}
+
+// ============================================================ Displacement
+template <int DIM>
+Displacement<DIM>::Displacement(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config,
+                                DeviceContext* ctx)
+    : Variable<DIM>(std::move(mesh), ctx), config_(std::move(config)) {}
+
+// reference displacement.cpp:76-98
+template <int DIM>
+void Displacement<DIM>::reset() {
+  auto& m = *this->mesh_;
+  hipStream_t st = this->ctx_->stream;
+  x_full_.upload(m.V_, st);
+  IntegratorFactory f;
+  integrator_ = f.create(config_->ti_type, m.V_, m.initial_velocity_, config_->h, st);
+  dx_.resize(3 * m.nfree());
+  dx_.zero(st);
+  this->ctx_->x_tilde.resize(m.V_.size());
+  const double h = integrator_->dt();
+  for (int i = 0; i < 3; ++i)  // body force is stored as float (reference config.h:93)
+    this->ctx_->h2g[i] = h * h * (double)config_->ext[i];
+}
+
+template <int DIM>
+void Displacement<DIM>::pre_solve() {
+  integrator_->x_tilde(this->ctx_->x_tilde.data(), this->ctx_->stream);
+}
+
+// reference displacement.cpp:33-50 (pinned values are static here)
+template <int DIM>
+void Displacement<DIM>::post_solve() {
+  integrator_->update(x_full_.data(), this->ctx_->stream);
+}
+
+template <int DIM>
+double Displacement<DIM>::max_abs_delta() {
+  auto& R = this->ctx_->reducer;
+  launch_reduce_maxabs((int)dx_.size(), dx_.data(), R.slot(RS_MAXDX), this->ctx_->stream);
+  return R.read(RS_MAXDX, this->ctx_->stream);
+}
+
+// ============================================================ Stretch
+template <int DIM>
+void Stretch<DIM>::update(const double* x_full, double dt) {
+  h2_ = dt * dt;
+  auto* c = this->ctx_;
+  auto m = this->mesh_->dev();
+  if (c->fp32())
+    launch_plain_update<float>(m, x_full, c->inertia(), h2_, c->blocks.f32(), c->fe.data(),
+                               c->stream);
+  else
+    launch_plain_update<double>(m, x_full, c->inertia(), h2_, c->blocks.f64(),
+                                c->fe.data(), c->stream);
+}
+
+template <int DIM>
+double Stretch<DIM>::energy(const double* x_full, const double* dx, double alpha) {
+  auto* c = this->ctx_;
+  launch_plain_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), h2_,
+                      c->reducer.slot(RS_ENERGY), c->stream);
+  return c->reducer.read(RS_ENERGY, c->stream);
+}
+
+// ============================================================ MixedStretch
+// reference mixed_stretch.cpp:262-304
+template <int DIM>
+void MixedStretch<DIM>::reset() {
+  auto* c = this->ctx_;
+  const int nT = this->mesh_->nT();
+  hipStream_t st = c->stream;
+  R_.resize(9 * (size_t)nT);
+  S_.resize(6 * (size_t)nT);
+  s_.resize(6 * (size_t)nT);
+  la_.resize(6 * (size_t)nT);
+  ds_.resize(6 * (size_t)nT);
+  g_.resize(6 * (size_t)nT);
+  gl_.resize(6 * (size_t)nT);
+  dsdF_.resize(54 * (size_t)nT, c->fp32());
+  Hpsi_.resize(21 * (size_t)nT, c->fp32());
+  Hinv_.resize(21 * (size_t)nT, c->fp32());
+  la_.zero(st);
+  ds_.zero(st);
+  s_.zero(st);
+  // R = I, then fit at the rest configuration and set s = S
+  std::vector<double> Rh(9 * (size_t)nT, 0.0);
+  for (int e = 0; e < nT; ++e) { Rh[0 * nT + e] = 1; Rh[4 * nT + e] = 1; Rh[8 * nT + e] = 1; }
+  R_.upload(Rh, st);
+  DeviceArray<double> x0;
+  x0.upload(this->mesh_->V_, st);
+  auto m = this->mesh_->dev();
+  if (c->fp32())
+    launch_mixed_polar_material<float>(m, x0.data(), R_.data(), S_.data(), s_.data(),
+                                       dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
+                                       g_.data(), gl_.data(), 0.0, st);
+  else
+    launch_mixed_polar_material<double>(m, x0.data(), R_.data(), S_.data(), s_.data(),
+                                        dsdF_.f64(), Hpsi_.f64(),
+                                        Hinv_.f64(), g_.data(), gl_.data(), 0.0, st);
+  MFEM_HIP_CHECK(hipGetLastError());
+  MFEM_HIP_CHECK(hipMemcpyAsync(s_.data(), S_.data(), S_.bytes(), hipMemcpyDeviceToDevice, st));
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// update_rotations + update_derivatives + rhs (reference mixed_stretch.cpp:
+// 126-226), fused into two element kernels.
+template <int DIM>
+void MixedStretch<DIM>::update(const double* x_full, double dt) {
+  h2_ = dt * dt;
+  auto* c = this->ctx_;
+  auto m = this->mesh_->dev();
+  if (c->fp32()) {
+    launch_mixed_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
+                                       dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
+                                       g_.data(), gl_.data(), h2_, c->stream);
+    launch_mixed_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
+                                   gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
+  } else {
+    launch_mixed_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
+                                        dsdF_.f64(), Hpsi_.f64(),
+                                        Hinv_.f64(), g_.data(), gl_.data(), h2_, c->stream);
+    launch_mixed_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(),
+                                    Hpsi_.f64(), gl_.data(), c->blocks.f64(),
+                                    c->fe.data(), c->stream);
+  }
+}
+
+template <int DIM>
+void MixedStretch<DIM>::solve(const double* dx) {
+  auto* c = this->ctx_;
+  auto m = this->mesh_->dev();
+  if (c->fp32())
+    launch_mixed_local_solve<float>(m, dx, dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
+                                    g_.data(), gl_.data(), la_.data(), ds_.data(),
+                                    c->reducer.slot(RS_MAXDS), c->stream);
+  else
+    launch_mixed_local_solve<double>(m, dx, dsdF_.f64(), Hpsi_.f64(),
+                                     Hinv_.f64(), g_.data(), gl_.data(), la_.data(),
+                                     ds_.data(), c->reducer.slot(RS_MAXDS), c->stream);
+}
+
+template <int DIM>
+double MixedStretch<DIM>::max_abs_delta() {
+  return this->ctx_->reducer.read(RS_MAXDS, this->ctx_->stream);
+}
+
+template <int DIM>
+void MixedStretch<DIM>::post_solve() {
+  la_.zero(this->ctx_->stream);
+}
+
+template <int DIM>
+double MixedStretch<DIM>::energy(const double* x_full, const double* dx, double alpha) {
+  auto* c = this->ctx_;
+  launch_mixed_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), R_.data(), s_.data(),
+                      ds_.data(), la_.data(), h2_, c->reducer.slot(RS_ENERGY), c->stream);
+  return c->reducer.read(RS_ENERGY, c->stream);
+}
+
+template <int DIM>
+VariableFactory<DIM>::VariableFactory() {
+  this->register_type(VAR_STRETCH, Stretch<DIM>::name(),
+                      [](std::shared_ptr<Mesh> m, std::shared_ptr<SimConfig> c,
+                         DeviceContext* ctx) -> std::unique_ptr<Variable<DIM>> {
+                        return std::make_unique<Stretch<DIM>>(m, c, ctx);
+                      });
+}
+
+template <int DIM>
+MixedVariableFactory<DIM>::MixedVariableFactory() {
+  this->register_type(VAR_MIXED_STRETCH, MixedStretch<DIM>::name(),
+                      [](std::shared_ptr<Mesh> m, std::shared_ptr<SimConfig> c,
+                         DeviceContext* ctx) -> std::unique_ptr<MixedVariable<DIM>> {
+                        return std::make_unique<MixedStretch<DIM>>(m, c, ctx);
+                      });
+}
+
+// ============================================================ linear system
+// reference linear_system.h:18-40: lhs = PMP + sum lhs, rhs = sum rhs; the
+// element kernels already folded the inertia residual into fe.
+template <int DIM>
+void SystemMatrixPD<DIM>::pre_solve(SimState<DIM>* state) {
+  auto* c = state->ctx_.get();
+  auto& m = *state->mesh_;
+  const bool f32 = c->fp32();
+  K.resize((size_t)m.nnzb() * 9, f32);
+  invdiag.resize(3 * (size_t)m.nfree(), f32);
+  rhs.resize(3 * (size_t)m.nfree());
+  if (f32) {
+    launch_assemble_bsr<float>(m.nnzb(), m.d_slot_ptr.data(), m.d_slots.data(), m.d_mass.data(),
+                               c->blocks.f32(), K.f32(), c->stream);
+    launch_jacobi_diag<float, float>(m.nfree(), m.d_diag_block.data(), K.f32(),
+                                     invdiag.f32(), c->stream);
+  } else {
+    launch_assemble_bsr<double>(m.nnzb(), m.d_slot_ptr.data(), m.d_slots.data(), m.d_mass.data(),
+                                c->blocks.f64(), K.f64(), c->stream);
+    launch_jacobi_diag<double, double>(m.nfree(), m.d_diag_block.data(), K.f64(),
+                                       invdiag.f64(), c->stream);
+  }
+  launch_assemble_rhs(m.nfree(), m.d_inc_ptr.data(), m.d_inc.data(), c->fe.data(), rhs.data(),
+                      c->stream);
+}
+
+template <int DIM>
+void SystemMatrixPD<DIM>::post_solve(SimState<DIM>* state, const double* dx) {
+  auto& d = state->x_->delta();
+  if (d.data() != dx)
+    MFEM_HIP_CHECK(hipMemcpyAsync(d.data(), dx, d.bytes(), hipMemcpyDeviceToDevice,
+                                  state->ctx_->stream));
+  for (auto& var : state->mixed_vars_) var->solve(d.data());
+}
+
+// ============================================================ CG
+template <int DIM>
+PcgSolver<DIM>::PcgSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
+  pinned_ = std::make_unique<PinnedBuffer>(sizeof(PcgCtrl) + 64);
+  ctrl_.resize(1);
+}
+
+template <int DIM>
+PcgSolver<DIM>::~PcgSolver() {
+  for (auto e : ev_) (void)hipEventDestroy(e);
+}
+
+template <int DIM>
+template <typename T>
+int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters) {
+  auto* c = this->state_->ctx_.get();
+  auto& m = *this->state_->mesh_;
+  hipStream_t st = c->stream;
+  PcgBuffers<T, T> B;
+  B.n = 3 * m.nfree();
+  B.rowptr = m.d_rowptr.data();
+  B.col = m.d_col.data();
+  B.K = system_.K.template as<T>();
+  B.invdiag = system_.invdiag.template as<T>();
+  B.b = b;
+  B.x = x;
+  B.r = r_.template as<T>();
+  B.z = z_.template as<T>();
+  B.p = p_.template as<T>();
+  B.q = q_.template as<T>();
+  B.ctrl = ctrl_.data();
+  for (int k = 0; k < 3; ++k) B.red[k] = c->reducer.slot(RS_PCG0 + k);
+  launch_pcg_init(B, tol, max_iters, st);
+  const int poll = std::max(1, this->state_->config_->pcg_poll);
+  if (this->profile_spmv && (int)ev_.size() < 2 * poll) {
+    for (auto e : ev_) (void)hipEventDestroy(e);
+    ev_.assign(2 * poll, nullptr);
+    for (auto& e : ev_) MFEM_HIP_CHECK(hipEventCreate(&e));
+  }
+  auto* h = pinned_->as<PcgCtrl>();
+  int launched = 0;
+  while (true) {
+    int chunk = std::min(poll, std::max(1, max_iters + 1 - launched));
+    for (int j = 0; j < chunk; ++j) {
+      if (this->profile_spmv)
+        launch_pcg_iteration(B, st, ev_[2 * j], ev_[2 * j + 1]);
+      else
+        launch_pcg_iteration(B, st);
+    }
+    launched += chunk;
+    MFEM_HIP_CHECK(hipGetLastError());
+    MFEM_HIP_CHECK(hipMemcpyAsync(h, ctrl_.data(), sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));
+    if (this->profile_spmv) {
+      // iterations that ran their SpMV: those before convergence was detected
+      int ran = std::min(chunk, std::max(0, h->iter + (h->done ? 1 : 0) - (launched - chunk)));
+      for (int j = 0; j < ran; ++j) {
+        float ms = 0;
+        MFEM_HIP_CHECK(hipEventElapsedTime(&ms, ev_[2 * j], ev_[2 * j + 1]));
+        this->stats_.spmv_ms += ms;
+        this->stats_.spmv_count += 1;
+      }
+    }
+    if (h->done) break;
+    if (launched > max_iters + 1) throw std::runtime_error("CG control block did not terminate");
+  }
+  return h->iter;
+}
+
+// reference eigen_iterative_solver.h:20-29 (compute + solve, x0 = 0)
+template <int DIM>
+void PcgSolver<DIM>::solve() {
+  auto* c = this->state_->ctx_.get();
+  auto& m = *this->state_->mesh_;
+  auto& cfg = *this->state_->config_;
+  hipStream_t st = c->stream;
+  system_.pre_solve(this->state_);
+  const int n = 3 * m.nfree();
+  const bool f32 = c->fp32();
+  x_.resize(n);
+  r_.resize(n, f32);
+  z_.resize(n, f32);
+  p_.resize(n, f32);
+  q_.resize(n, f32);
+  this->stats_ = SolveStats();
+  auto& R = c->reducer;
+  if (!f32) {
+    this->stats_.iterations = run_cg<double>(system_.rhs.data(), x_.data(), cfg.itr_tol,
+                                             cfg.max_iterative_solver_iters);
+    this->stats_.refinements = 1;
+    auto* h = pinned_->as<PcgCtrl>();
+    this->stats_.rel_residual = h->b2 > 0 ? std::sqrt(h->rr / h->b2) : 0.0;
+  } else {
+    // FP32 CG sweeps inside FP64 defect correction on the true residual.
+    r64_.resize(n);
+    dx64_.resize(n);
+    x_.zero(st);
+    MFEM_HIP_CHECK(hipMemcpyAsync(r64_.data(), system_.rhs.data(), (size_t)n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, st));
+    // |b|^2
+    launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
+                           x_.data(), system_.rhs.data(), r64_.data(), R.slot(RS_RES), st);
+    const double b2 = R.read(RS_RES, st);
+    double r2 = b2;
+    const double thr = std::max(cfg.itr_tol * cfg.itr_tol * b2, 2.2250738585072014e-308);
+    int budget = cfg.max_iterative_solver_iters;
+    int sweeps = 0;
+    while (r2 >= thr && budget > 0 && sweeps < cfg.max_refinements && b2 > 0) {
+      const double inner = std::max(cfg.inner_tol, std::sqrt(thr / r2) * 0.5);
+      int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget);
+      budget -= std::max(it, 1);
+      this->stats_.iterations += it;
+      launch_axpy(n, 1.0, dx64_.data(), x_.data(), st);
+      launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
+                             x_.data(), system_.rhs.data(), r64_.data(), R.slot(RS_RES), st);
+      r2 = R.read(RS_RES, st);
+      ++sweeps;
+    }
+    this->stats_.refinements = sweeps;
+    this->stats_.rel_residual = b2 > 0 ? std::sqrt(r2 / b2) : 0.0;
+  }
+  system_.post_solve(this->state_, x_.data());
+}
+
+template <int DIM>
+LinearSolverFactory<DIM>::LinearSolverFactory() {
+  auto unsupported = [](SimState<DIM>*) -> std::unique_ptr<LinearSolver<DIM>> {
+    throw std::runtime_error(
+        "direct sparse factorisations (eigen-llt/ldlt/lu, cholmod) have no device "
+        "implementation; use \"eigen-pcg-diag\"");
+  };
+  this->register_type(SOLVER_EIGEN_LLT, "eigen-llt", unsupported);
+  this->register_type(SOLVER_EIGEN_LDLT, "eigen-ldlt", unsupported);
+  this->register_type(SOLVER_EIGEN_LU, "eigen-lu", unsupported);
+  this->register_type(SOLVER_CHOLMOD, "cholmod", unsupported);
+  this->register_type(SOLVER_EIGEN_CG_DIAG, "eigen-pcg-diag",
+                      [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
+                        return std::make_unique<PcgSolver<DIM>>(s);
+                      });
+}
+
+// ============================================================ optimizer
+template <int DIM>
+void Optimizer<DIM>::reset() {}
+
+template <int DIM>
+NewtonOptimizer<DIM>::~NewtonOptimizer() {
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+// reference newton_optimizer.cpp:127-140
+template <int DIM>
+void NewtonOptimizer<DIM>::reset() {
+  Optimizer<DIM>::reset();
+  auto& S = *this->state_;
+  S.x_->reset();
+  for (auto& v : S.vars_) v->reset();
+  for (auto& v : S.mixed_vars_) v->reset();
+  LinearSolverFactory<DIM> f;
+  solver_ = f.create(S.config_->solver_type, &S);
+  if (!solver_) throw std::runtime_error("unknown linear solver");
+  for (auto& e : ev_)
+    if (!e) MFEM_HIP_CHECK(hipEventCreate(&e));
+}
+
+// reference newton_optimizer.cpp:92-110
+template <int DIM>
+void NewtonOptimizer<DIM>::update_system() {
+  auto& S = *this->state_;
+  const double dt = S.x_->integrator()->dt();
+  const double* x = S.x_->value().data();
+  for (auto& v : S.vars_) v->update(x, dt);
+  for (auto& v : S.mixed_vars_) v->update(x, dt);
+  if (S.vars_.empty() && S.mixed_vars_.empty()) {
+    // no elastic variable: inertia-only residual and zero stiffness blocks
+    auto* c = S.ctx_.get();
+    if (c->fp32())
+      launch_plain_update<float>(S.mesh_->dev(), x, c->inertia(), 0.0, c->blocks.f32(),
+                                 c->fe.data(), c->stream);
+    else
+      launch_plain_update<double>(S.mesh_->dev(), x, c->inertia(), 0.0, c->blocks.f64(),
+                                  c->fe.data(), c->stream);
+  }
+}
+
+// reference newton_optimizer.cpp:113-124
+template <int DIM>
+void NewtonOptimizer<DIM>::substep(double& decrement) {
+  auto& S = *this->state_;
+  solver_->solve();
+  decrement = S.x_->max_abs_delta();
+  for (auto& v : S.mixed_vars_) decrement = std::max(decrement, v->max_abs_delta());
+}
+
+// reference newton_optimizer.cpp:34-49: E(a) = inertia + sum of variable
+// energies at (x + a dx, s + a ds); the element kernels include the inertia.
+template <int DIM>
+double NewtonOptimizer<DIM>::energy(double alpha) {
+  auto& S = *this->state_;
+  const double* x = S.x_->value().data();
+  const double* dx = S.x_->delta().data();
+  double val = 0;
+  bool inertia_done = false;
+  for (auto& v : S.mixed_vars_) { val += v->energy(x, dx, alpha); inertia_done = true; }
+  for (auto& v : S.vars_) { val += v->energy(x, dx, alpha); inertia_done = true; }
+  if (!inertia_done) {
+    auto* c = S.ctx_.get();
+    launch_plain_energy(S.mesh_->dev(), x, dx, alpha, c->inertia(), 0.0,
+                        c->reducer.slot(RS_ENERGY), c->stream);
+    val = c->reducer.read(RS_ENERGY, c->stream);
+  }
+  return val;
+}
+
+// reference newton_optimizer.cpp:11-89 with linesearch_backtracking
+// (linesearch.h:78-131, called with c = 0 and p = 0.5: accept on strict
+// decrease; an exhausted search leaves the state unchanged).
+template <int DIM>
+void NewtonOptimizer<DIM>::step() {
+  auto& S = *this->state_;
+  auto* c = S.ctx_.get();
+  hipStream_t st = c->stream;
+  stats_ = StepStats();
+  S.x_->pre_solve();
+  for (auto& v : S.mixed_vars_) v->pre_solve();
+  for (auto& v : S.vars_) v->pre_solve();
+  int i = 0;
+  double grad_norm = 0, E = 0, E_prev = 0, res = 0;
+  const bool has_mixed = !S.mixed_vars_.empty();
+  do {
+    MFEM_HIP_CHECK(hipEventRecord(ev_[0], st));
+    update_system();
+    MFEM_HIP_CHECK(hipEventRecord(ev_[1], st));
+    substep(grad_norm);
+    MFEM_HIP_CHECK(hipEventRecord(ev_[2], st));
+    const auto& ss = solver_->stats();
+    stats_.pcg_iters += ss.iterations;
+    stats_.refinements += ss.refinements;
+    stats_.rel_residual = ss.rel_residual;
+    stats_.spmv_ms += ss.spmv_ms;
+    stats_.spmv_count += ss.spmv_count;
+    double alpha = 1.0;
+    E = energy(0.0);
+    res = std::fabs((E - E_prev) / (E + 1e-6));
+    E_prev = E;
+    // fx0 = energy(0) equals E (same state, deterministic kernels)
+    const double fx0 = E;
+    int iter = 0;
+    while (iter < S.config_->ls_iters) {
+      double fx = energy(alpha);
+      if (fx < fx0) break;
+      alpha *= 0.5;
+      ++iter;
+    }
+    if (iter < S.config_->ls_iters) {
+      double* s = has_mixed ? S.mixed_vars_[0]->value().data() : nullptr;
+      const double* ds = has_mixed ? S.mixed_vars_[0]->delta().data() : nullptr;
+      launch_accept_step(S.mesh_->dev(), S.x_->value().data(), S.x_->delta().data(), s, ds, alpha,
+                         has_mixed ? 1 : 0, st);
+    } else {
+      stats_.ls_fail++;
+    }
+    stats_.ls_iters += iter;
+    MFEM_HIP_CHECK(hipEventRecord(ev_[3], st));
+    MFEM_HIP_CHECK(hipEventSynchronize(ev_[3]));
+    float t01 = 0, t12 = 0, t23 = 0;
+    MFEM_HIP_CHECK(hipEventElapsedTime(&t01, ev_[0], ev_[1]));
+    MFEM_HIP_CHECK(hipEventElapsedTime(&t12, ev_[1], ev_[2]));
+    MFEM_HIP_CHECK(hipEventElapsedTime(&t23, ev_[2], ev_[3]));
+    stats_.update_ms += t01;
+    stats_.solve_ms += t12;
+    stats_.ls_ms += t23;
+    stats_.alpha = alpha;
+    ++i;
+    this->callback(S);
+  } while (i < S.config_->outer_steps && grad_norm > S.config_->newton_tol && res > 1e-12);
+  stats_.newton_iters = i;
+  stats_.decrement = grad_norm;
+  stats_.energy = E;
+  stats_.res = res;
+  stats_.step_ms = stats_.update_ms + stats_.solve_ms + stats_.ls_ms;
+  // post-solve updates (boundary conditions are static here)
+  S.x_->post_solve();
+  for (auto& v : S.mixed_vars_) v->post_solve();
+  for (auto& v : S.vars_) v->post_solve();
+  MFEM_HIP_CHECK(hipGetLastError());
+}
+
+template <int DIM>
+OptimizerFactory<DIM>::OptimizerFactory() {
+  this->register_type(OPTIMIZER_NEWTON, NewtonOptimizer<DIM>::name(),
+                      [](std::unique_ptr<SimState<DIM>> s) -> std::unique_ptr<Optimizer<DIM>> {
+                        return std::make_unique<NewtonOptimizer<DIM>>(std::move(s));
+                      });
+}
+
+// ============================================================ SimState
+template <int DIM>
+void SimState<DIM>::load_arrays(const SceneArrays& sc, std::shared_ptr<SimConfig> config,
+                                const std::vector<std::string>& mixed_variables,
+                                const std::vector<std::string>& variables) {
+  config_ = std::move(config);
+  const int nT = (int)(sc.T.size() / 4);
+  // per-element Lame parameters (one model per tet, as with per-tet
+  // material ids); materials[0] supplies the density
+  MaterialModelFactory mf;
+  auto cfg0 = std::make_shared<MaterialConfig>();
+  cfg0->density = sc.density;
+  cfg0->material_model = nT > 0 ? (MaterialModelType)sc.mat[0] : MATERIAL_SNH;
+  material_models_ = {std::shared_ptr<MaterialModel>(mf.create(cfg0->material_model, cfg0))};
+  if (!material_models_[0]) throw std::runtime_error("unknown material type");
+  if ((int)sc.mu.size() != nT || (int)sc.la.size() != nT || (int)sc.mat.size() != nT)
+    throw std::runtime_error("per-element arrays must have nT entries");
+  mesh_ = std::make_shared<Mesh>(sc.V, sc.T, material_models_[0], sc.mu, sc.la, sc.mat);
+  mesh_->bc_config_ = sc.bc;
+  if (!sc.v0.empty()) mesh_->initial_velocity_ = sc.v0;
+  fixed_override_ = sc.fixed;
+  MixedVariableFactory<DIM> mvf;
+  VariableFactory<DIM> vf;
+  ctx_ = std::make_unique<DeviceContext>();
+  ctx_->precision = config_->precision;
+  mixed_vars_.clear();
+  vars_.clear();
+  for (const auto& n : mixed_variables) {
+    if (!mvf.has_name(n)) throw std::runtime_error("unknown mixed variable '" + n + "'");
+    config_->mixed_variables.insert(mvf.type_by_name(n));
+  }
+  for (const auto& n : variables) {
+    if (!vf.has_name(n)) throw std::runtime_error("unknown variable '" + n + "'");
+    config_->variables.insert(vf.type_by_name(n));
+  }
+  finish_setup();
+}
+
+template <int DIM>
+void SimState<DIM>::finish_setup() {
+  MixedVariableFactory<DIM> mvf;
+  VariableFactory<DIM> vf;
+  if (!ctx_) {
+    ctx_ = std::make_unique<DeviceContext>();
+    ctx_->precision = config_->precision;
+  }
+  if (!ctx_->stream) {
+    MFEM_HIP_CHECK(hipStreamCreateWithFlags(&ctx_->stream, hipStreamNonBlocking));
+    ctx_->own_stream = true;
+  }
+  mesh_->init(ctx_->stream, fixed_override_);
+  const int nT = mesh_->nT();
+  ctx_->reducer.init(std::max<long>(std::max(nT, 3 * mesh_->nV()), 1));
+  ctx_->blocks.resize(90 * (size_t)nT, ctx_->fp32());
+  ctx_->fe.resize(12 * (size_t)nT);
+  x_ = std::make_unique<Displacement<DIM>>(mesh_, config_, ctx_.get());
+  mixed_vars_.clear();
+  vars_.clear();
+  for (VariableType t : config_->mixed_variables) {
+    auto v = mvf.create(t, mesh_, config_, ctx_.get());
+    if (v) mixed_vars_.push_back(std::move(v));
+  }
+  for (VariableType t : config_->variables) {
+    auto v = vf.create(t, mesh_, config_, ctx_.get());
+    if (v) vars_.push_back(std::move(v));
+  }
+  if (mixed_vars_.size() + vars_.size() > 1)
+    throw std::runtime_error("the device backend supports one elastic variable per simulation");
+}
+
+template <int DIM>
+bool SimState<DIM>::load(const std::string& json_file) {
+  if (std::filesystem::path(json_file).extension() != ".json") {
+    std::cerr << "File: " << json_file << " needs to be json" << std::endl;
+    return false;
+  }
+  Json args;
+  try {
+    args = Json::parse_file(json_file);
+  } catch (const std::exception& e) {
+    std::cerr << e.what() << std::endl;
+    return false;
+  }
+  return load(args, std::filesystem::path(json_file).parent_path().string());
+}
+
+// reference simulation_state.cpp:344-366
+template <int DIM>
+void SimState<DIM>::load_params(const Json& a) {
+  if (a.has("body_force")) {
+    auto v = a["body_force"].as_numbers();
+    for (int i = 0; i < 3 && i < (int)v.size(); ++i) config_->ext[i] = (float)v[i];
+  }
+  if (a.has("dt")) config_->h = a["dt"].as_number();
+  if (a.has("print_timing")) config_->show_timing = a["print_timing"].as_bool();
+  if (a.has("print_stats")) config_->show_data = a["print_stats"].as_bool();
+  if (a.has("enable_ccd")) config_->enable_ccd = a["enable_ccd"].as_bool();
+  if (a.has("dhat")) config_->dhat = a["dhat"].as_number();
+  if (a.has("kappa")) config_->kappa = a["kappa"].as_number();
+  if (a.has("max_newton_iterations")) config_->outer_steps = a["max_newton_iterations"].as_int();
+  if (a.has("max_linesearch_iterations")) config_->ls_iters = a["max_linesearch_iterations"].as_int();
+  if (a.has("max_iterative_solver_iters"))
+    config_->max_iterative_solver_iters = a["max_iterative_solver_iters"].as_int();
+  if (a.has("iterative_solver_tolerance")) config_->itr_tol = a["iterative_solver_tolerance"].as_number();
+  // device options (no counterpart in the reference)
+  if (a.has("newton_tol")) config_->newton_tol = a["newton_tol"].as_number();
+  if (a.has("precision"))
+    config_->precision = a["precision"].as_string() == "mixed" ? Precision::MIXED : Precision::FP64;
+  if (a.has("pcg_poll")) config_->pcg_poll = a["pcg_poll"].as_int();
+}
+
+// reference simulation_state.cpp:92-341 (3D branch)
+template <int DIM>
+bool SimState<DIM>::load(const Json& args, const std::string& base_dir) {
+  config_ = std::make_shared<SimConfig>();
+  load_params(args);
+  if (args.has("optimizer")) {
+    OptimizerFactory<DIM> of;
+    config_->optimizer = of.type_by_name(args["optimizer"].as_string());
+  }
+  if (args.has("time_integrator")) {
+    IntegratorFactory itf;
+    config_->ti_type = itf.type_by_name(args["time_integrator"].as_string());
+  }
+  MaterialModelFactory mf;
+  std::vector<std::shared_ptr<MaterialModel>> materials;
+  if (args.has("material_models")) {
+    for (const auto& obj : args["material_models"].arr) {
+      auto cfg = std::make_shared<MaterialConfig>();
+      if (obj.has("youngs_modulus")) cfg->ym = obj["youngs_modulus"].as_number();
+      if (obj.has("poissons_ratio")) cfg->pr = obj["poissons_ratio"].as_number();
+      if (obj.has("density")) cfg->density = obj["density"].as_number();
+      if (obj.has("energy")) cfg->material_model = mf.type_by_name(obj["energy"].as_string());
+      Enu_to_lame(cfg->ym, cfg->pr, cfg->la, cfg->mu);
+      materials.push_back(std::shared_ptr<MaterialModel>(mf.create(cfg->material_model, cfg)));
+    }
+  } else {
+    auto cfg = std::make_shared<MaterialConfig>();
+    materials.push_back(std::shared_ptr<MaterialModel>(mf.create(cfg->material_model, cfg)));
+  }
+  material_models_ = materials;
+  auto resolve = [&](const std::string& p) {
+    std::filesystem::path q(p);
+    return q.is_absolute() ? p : (std::filesystem::path(base_dir) / q).string();
+  };
+  // objects: the device backend takes one object per simulation
+  std::vector<double> V;
+  std::vector<int> T;
+  std::vector<int> mat_ids;
+  BoundaryConditionConfig bc;
+  std::vector<double> v0;
+  if (!args.has("objects") || args["objects"].size() != 1) {
+    std::cerr << "the device backend loads exactly one object per scene" << std::endl;
+    return false;
+  }
+  const Json& obj = args["objects"][0];
+  if (!obj.has("path")) {
+    std::cerr << "Object missing path!" << std::endl;
+    return false;
+  }
+  read_mesh_file(resolve(obj["path"].as_string()), V, T);
+  // 3D meshes are scaled by 1/max coefficient (reference :62-68)
+  double fac = *std::max_element(V.begin(), V.end());
+  for (auto& v : V) v /= fac;
+  std::vector<double> offset = {0, 0, 0};
+  if (obj.has("offset")) offset = obj["offset"].as_numbers();
+  const int nV = (int)(V.size() / 3);
+  for (int i = 0; i < nV; ++i)
+    for (int d = 0; d < 3; ++d) V[3 * i + d] += offset[d];
+  if (obj.has("transformation")) {
+    auto tr = obj["transformation"].as_numbers();  // column-major DIM x DIM
+    double cen[3] = {0, 0, 0};
+    for (int i = 0; i < nV; ++i)
+      for (int d = 0; d < 3; ++d) cen[d] += V[3 * i + d] / nV;
+    for (int i = 0; i < nV; ++i) {
+      double p[3];
+      for (int d = 0; d < 3; ++d) p[d] = V[3 * i + d] - cen[d];
+      for (int r = 0; r < 3; ++r) {
+        double a = 0;
+        for (int c = 0; c < 3; ++c) a += tr[r + 3 * c] * p[c];
+        V[3 * i + r] = a + cen[r];
+      }
+    }
+  }
+  const int nT = (int)(T.size() / 4);
+  if (obj.has("material_ids")) {
+    int rows, cols;
+    auto ids = read_dmat(resolve(obj["material_ids"].as_string()), rows, cols);
+    mat_ids.resize(nT);
+    for (int e = 0; e < nT; ++e) mat_ids[e] = (int)ids.at(e);
+  } else {
+    int idx = obj.has("material_index") ? obj["material_index"].as_int() : 0;
+    mat_ids.assign(nT, idx);
+  }
+  if (obj.has("initial_velocity")) {
+    auto v = obj["initial_velocity"].as_numbers();
+    v0.resize(V.size());
+    for (int i = 0; i < nV; ++i)
+      for (int d = 0; d < 3; ++d) v0[3 * i + d] = v[d];
+  }
+  if (obj.has("boundary_condition")) {
+    const Json& b = obj["boundary_condition"];
+    static const std::pair<const char*, BCScriptType> names[] = {
+        {"null", BC_NULL}, {"scale", BC_SCALEF}, {"randomize", BC_RANDOM},
+        {"onepoint", BC_ONEPOINT}, {"hang", BC_HANG}, {"hangends", BC_HANGENDS},
+        {"bend", BC_BEND}, {"stretch", BC_STRETCH}, {"twist", BC_TWIST}};
+    if (b.has("type")) {
+      std::string t = b["type"].as_string();
+      bc.type = BC_NULL;
+      bool found = false;
+      for (auto& nm : names)
+        if (t == nm.first) { bc.type = nm.second; found = true; }
+      if (!found) std::cerr << "Factory type_by_name: type not found" << std::endl;
+    }
+    if (b.has("axis")) bc.axis = b["axis"].as_int();
+    if (b.has("ratio")) bc.ratio = b["ratio"].as_number();
+    if (b.has("velocity")) bc.velocity = b["velocity"].as_number();
+  }
+  mesh_ = std::make_shared<Mesh>(V, T, mat_ids, materials);
+  mesh_->bc_config_ = bc;
+  if (!v0.empty()) mesh_->initial_velocity_ = v0;
+  if (args.has("initial_state")) {
+    const Json& is = args["initial_state"];
+    int rows, cols;
+    if (is.has("x_path")) {
+      auto x = read_dmat(resolve(is["x_path"].as_string()), rows, cols);
+      if (rows != nV || cols != 3) throw std::runtime_error("initial_state x_path has wrong shape");
+      for (int i = 0; i < nV; ++i)
+        for (int d = 0; d < 3; ++d) mesh_->Vinit_[3 * i + d] = mesh_->V_[3 * i + d] = x[i + nV * d];
+    }
+    if (is.has("v_path")) {
+      auto v = read_dmat(resolve(is["v_path"].as_string()), rows, cols);
+      if (rows != nV || cols != 3) throw std::runtime_error("initial_state v_path has wrong shape");
+      for (int i = 0; i < nV; ++i)
+        for (int d = 0; d < 3; ++d) mesh_->initial_velocity_[3 * i + d] = v[i + nV * d];
+    }
+  }
+  MixedVariableFactory<DIM> mvf;
+  VariableFactory<DIM> vf;
+  config_->mixed_variables.clear();
+  config_->variables.clear();
+  if (args.has("mixed_variables"))
+    for (const auto& n : args["mixed_variables"].as_strings())
+      config_->mixed_variables.insert(mvf.type_by_name(n));
+  if (args.has("variables"))
+    for (const auto& n : args["variables"].as_strings()) config_->variables.insert(vf.type_by_name(n));
+  if (args.has("linear_solver")) {
+    LinearSolverFactory<DIM> lf;
+    config_->solver_type = lf.type_by_name(args["linear_solver"].as_string());
+  }
+  fixed_override_.clear();
+  ctx_.reset();
+  finish_setup();
+  return true;
+}
+
+// ============================================================ file formats
+void read_mesh_file(const std::string& path, std::vector<double>& V, std::vector<int>& T) {
+  std::ifstream f(path);
+  if (!f.good()) throw std::runtime_error("Unable to open mesh: " + path);
+  std::vector<std::string> tok;
+  std::string line;
+  while (std::getline(f, line)) {
+    auto h = line.find('#');
+    if (h != std::string::npos) line = line.substr(0, h);
+    std::istringstream ss(line);
+    std::string t;
+    while (ss >> t) tok.push_back(t);
+  }
+  V.clear();
+  T.clear();
+  for (size_t i = 0; i < tok.size();) {
+    if (tok[i] == "Vertices") {
+      long n = std::stol(tok[i + 1]);
+      for (long k = 0; k < n; ++k)
+        for (int d = 0; d < 3; ++d) V.push_back(std::stod(tok[i + 2 + 4 * k + d]));
+      i += 2 + 4 * n;
+    } else if (tok[i] == "Tetrahedra") {
+      long n = std::stol(tok[i + 1]);
+      for (long k = 0; k < n; ++k)
+        for (int d = 0; d < 4; ++d) T.push_back(std::stoi(tok[i + 2 + 5 * k + d]) - 1);
+      i += 2 + 5 * n;
+    } else if (tok[i] == "Triangles") {
+      i += 2 + 4 * std::stol(tok[i + 1]);
+    } else if (tok[i] == "Edges") {
+      i += 2 + 3 * std::stol(tok[i + 1]);
+    } else {
+      ++i;
+    }
+  }
+  if (V.empty() || T.empty()) throw std::runtime_error(path + ": missing Vertices or Tetrahedra");
+}
+
+std::vector<double> read_dmat(const std::string& path, int& rows, int& cols) {
+  std::ifstream f(path);
+  if (!f.good()) throw std::runtime_error("Unable to open dmat: " + path);
+  f >> cols >> rows;
+  std::vector<double> v((size_t)rows * cols);
+  for (auto& x : v) f >> x;
+  if (!f) throw std::runtime_error(path + ": short dmat");
+  return v;  // column-major rows x cols
+}
+
+template class BDF<1>;
+template class BDF<2>;
+template class BDF<3>;
+template class BDF<4>;
+template class BDF<5>;
+template class BDF<6>;
+template class Displacement<3>;
+template class Stretch<3>;
+template class MixedStretch<3>;
+template class VariableFactory<3>;
+template class MixedVariableFactory<3>;
+template class SystemMatrixPD<3>;
+template class PcgSolver<3>;
+template class LinearSolverFactory<3>;
+template class Optimizer<3>;
+template class NewtonOptimizer<3>;
+template class OptimizerFactory<3>;
+template struct SimState<3>;
+
+}  // namespace mfem

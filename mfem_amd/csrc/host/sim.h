@@ -1,0 +1,388 @@
+// Host framework of the device Newton step, mirroring the reference's
+// class structure and names:
+//   ImplicitIntegrator / BDF<I>           reference src/time_integrators/
+//   Variable / MixedVariable              reference src/variables/variable.h,
+//   Displacement / Stretch / MixedStretch              mixed_variable.h, ...
+//   LinearSolver / SystemMatrixPD / CG    reference src/linear_solvers/
+//   Optimizer / NewtonOptimizer           reference src/optimizers/
+//   SimState                              reference src/simulation_state.h
+// Vectors and per-element state live in device memory; the host keeps the
+// control flow (Newton loop, line search, termination tests), exactly as the
+// reference's NewtonOptimizer does.
+#pragma once
+#include <array>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../kernels.h"
+#include "config.h"
+#include "device.h"
+#include "factory.h"
+#include "json.h"
+#include "material_model.h"
+#include "mesh.h"
+
+namespace mfem {
+
+// Pool of deterministic reduction slots (see reduce.h) with pinned read-back.
+class Reducer {
+ public:
+  static constexpr int kSlots = 8;
+  void init(long max_items);
+  mfemhip::ReduceSlot slot(int k);
+  double read(int k, hipStream_t st);
+
+ private:
+  long stride_ = 0;
+  DeviceArray<double> partials_, results_;
+  DeviceArray<unsigned> counters_;
+  std::unique_ptr<PinnedBuffer> pinned_;
+};
+
+// Device resources shared by the variables and solvers of one simulation.
+struct DeviceContext {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  Precision precision = Precision::FP64;
+  bool fp32() const { return precision == Precision::MIXED; }
+  Reducer reducer;
+  RealArray blocks;           // element stiffness blocks, 90 per tet
+  DeviceArray<double> fe;     // element residual vectors, 12 per tet
+  DeviceArray<double> x_tilde;
+  double h2g[3] = {0, 0, 0};
+  mfemhip::Inertia inertia() const;
+  ~DeviceContext();
+};
+
+// ---------------------------------------------------------------------------
+class ImplicitIntegrator {
+ public:
+  ImplicitIntegrator(double h) : h_(h) {}
+  virtual ~ImplicitIntegrator() = default;
+  virtual void x_tilde(double* out, hipStream_t st) const = 0;  // device out
+  virtual double dt() const = 0;
+  virtual void update(const double* x, hipStream_t st) = 0;      // device x
+  virtual const double* x_prev() const = 0;
+  virtual const double* v_prev() const = 0;
+
+ protected:
+  double h_;
+};
+
+template <int I>
+class BDF : public ImplicitIntegrator {
+ public:
+  static std::string name() { return "BDF" + std::to_string(I); }
+  BDF(const std::vector<double>& x0, const std::vector<double>& v0, double h, hipStream_t st);
+  void x_tilde(double* out, hipStream_t st) const override;
+  double dt() const override { return beta() * h_; }
+  void update(const double* x, hipStream_t st) override;
+  const double* x_prev() const override { return xs_[head_].data(); }
+  const double* v_prev() const override { return vs_[head_].data(); }
+
+ private:
+  static std::array<double, I> alphas();
+  static double beta();
+  std::array<DeviceArray<double>, I> xs_, vs_;
+  int head_ = 0;
+  int n_ = 0;
+};
+
+class IntegratorFactory
+    : public Factory<TimeIntegratorType, ImplicitIntegrator, const std::vector<double>&,
+                     const std::vector<double>&, double, hipStream_t> {
+ public:
+  IntegratorFactory();
+};
+
+// ---------------------------------------------------------------------------
+template <int DIM>
+class Variable {
+ public:
+  Variable(std::shared_ptr<Mesh> mesh, DeviceContext* ctx) : mesh_(std::move(mesh)), ctx_(ctx) {}
+  virtual ~Variable() = default;
+  // element phase at the full configuration x (device, 3 nV) for time step dt
+  virtual void update(const double* x_full, double dt) = 0;
+  virtual void reset() = 0;
+  virtual void pre_solve() {}
+  virtual void post_solve() {}
+  virtual int size() const = 0;
+  // energy of this variable at (x + alpha dx, s + alpha ds), inertia included
+  // when the variable owns the element residual (device backend detail)
+  virtual double energy(const double* x_full, const double* dx, double alpha) { return 0.0; }
+  std::shared_ptr<Mesh> mesh() const { return mesh_; }
+
+ protected:
+  std::shared_ptr<Mesh> mesh_;
+  DeviceContext* ctx_;
+};
+
+template <int DIM>
+class MixedVariable : public Variable<DIM> {
+ public:
+  using Variable<DIM>::Variable;
+  // local solve for (lambda, ds) given the displacement step dx (device)
+  virtual void solve(const double* dx) = 0;
+  virtual DeviceArray<double>& value() = 0;   // s
+  virtual DeviceArray<double>& delta() = 0;   // ds
+  virtual DeviceArray<double>& lambda() = 0;
+  virtual int size_dual() const = 0;
+  virtual double max_abs_delta() = 0;
+};
+
+template <int DIM>
+class Displacement : public Variable<DIM> {
+ public:
+  Displacement(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config, DeviceContext* ctx);
+  void update(const double*, double) override {}
+  void reset() override;
+  void pre_solve() override;   // x_tilde for this timestep
+  void post_solve() override;  // BDF update, mesh positions
+  int size() const override { return 3 * this->mesh_->nfree(); }
+  DeviceArray<double>& value() { return x_full_; }  // full configuration (3 nV)
+  DeviceArray<double>& delta() { return dx_; }      // free-DOF step (3 nfree)
+  const std::shared_ptr<ImplicitIntegrator>& integrator() const { return integrator_; }
+  double max_abs_delta();
+
+ private:
+  std::shared_ptr<SimConfig> config_;
+  std::shared_ptr<ImplicitIntegrator> integrator_;
+  DeviceArray<double> x_full_, dx_;
+};
+
+template <int DIM>
+class Stretch : public Variable<DIM> {
+ public:
+  static std::string name() { return "stretch"; }
+  Stretch(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config, DeviceContext* ctx)
+      : Variable<DIM>(std::move(mesh), ctx) {}
+  void update(const double* x_full, double dt) override;
+  void reset() override {}
+  int size() const override { return 0; }
+  double energy(const double* x_full, const double* dx, double alpha) override;
+
+ private:
+  double h2_ = 0;
+};
+
+template <int DIM>
+class MixedStretch : public MixedVariable<DIM> {
+ public:
+  static std::string name() { return "mixed-stretch"; }
+  MixedStretch(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config, DeviceContext* ctx)
+      : MixedVariable<DIM>(std::move(mesh), ctx) {}
+  void update(const double* x_full, double dt) override;
+  void reset() override;
+  void post_solve() override;  // lambda <- 0 (reference mixed_stretch.cpp:307-310)
+  int size() const override { return 6 * this->mesh_->nT(); }
+  int size_dual() const override { return 6 * this->mesh_->nT(); }
+  void solve(const double* dx) override;
+  double energy(const double* x_full, const double* dx, double alpha) override;
+  DeviceArray<double>& value() override { return s_; }
+  DeviceArray<double>& delta() override { return ds_; }
+  DeviceArray<double>& lambda() override { return la_; }
+  double max_abs_delta() override;
+
+  // element state (structure of arrays, stride nT)
+  DeviceArray<double> R_, S_, s_, la_, ds_, g_, gl_;
+  RealArray dsdF_, Hpsi_, Hinv_;
+
+ private:
+  double h2_ = 0;
+  double last_max_ds_ = 0;
+};
+
+template <int DIM>
+class VariableFactory
+    : public Factory<VariableType, Variable<DIM>, std::shared_ptr<Mesh>,
+                     std::shared_ptr<SimConfig>, DeviceContext*> {
+ public:
+  VariableFactory();
+};
+
+template <int DIM>
+class MixedVariableFactory
+    : public Factory<VariableType, MixedVariable<DIM>, std::shared_ptr<Mesh>,
+                     std::shared_ptr<SimConfig>, DeviceContext*> {
+ public:
+  MixedVariableFactory();
+};
+
+// ---------------------------------------------------------------------------
+template <int DIM>
+struct SimState;
+
+struct SolveStats {
+  int iterations = 0;       // CG iterations (all refinement sweeps)
+  int refinements = 0;      // defect-correction sweeps (MIXED)
+  double rel_residual = 0;  // |b - K x| / |b| at exit (FP64)
+  double spmv_ms = 0;       // summed device time of the CG SpMV kernel, when profiled
+  int spmv_count = 0;
+};
+
+// K = PMP + sum of element blocks on the BSR pattern, rhs from element
+// residuals (reference linear_solvers/linear_system.h:9-58).
+template <int DIM>
+class SystemMatrixPD {
+ public:
+  void pre_solve(SimState<DIM>* state);
+  void post_solve(SimState<DIM>* state, const double* dx);
+  RealArray K;                // nnzb * 9
+  RealArray invdiag;          // 3 nfree
+  DeviceArray<double> rhs;    // 3 nfree
+};
+
+template <int DIM>
+class LinearSolver {
+ public:
+  explicit LinearSolver(SimState<DIM>* state) : state_(state) {}
+  virtual ~LinearSolver() = default;
+  virtual void solve() = 0;
+  const SolveStats& stats() const { return stats_; }
+  bool profile_spmv = false;
+
+ protected:
+  SimState<DIM>* state_;
+  SolveStats stats_;
+};
+
+// "eigen-pcg-diag": Jacobi-preconditioned CG with Eigen's stopping rule.
+// FP64: one CG solve on the FP64 system. MIXED: FP32 CG sweeps inside an FP64
+// defect-correction loop that stops on the true FP64 residual.
+template <int DIM>
+class PcgSolver : public LinearSolver<DIM> {
+ public:
+  explicit PcgSolver(SimState<DIM>* state);
+  ~PcgSolver() override;
+  void solve() override;
+  SystemMatrixPD<DIM>& system() { return system_; }
+
+ private:
+  template <typename T>
+  int run_cg(const double* b, double* x, double tol, int max_iters);
+  SystemMatrixPD<DIM> system_;
+  DeviceArray<double> x_, r64_, dx64_;
+  RealArray r_, z_, p_, q_;
+  DeviceArray<mfemhip::PcgCtrl> ctrl_;
+  std::unique_ptr<PinnedBuffer> pinned_;
+  std::vector<hipEvent_t> ev_;
+};
+
+template <int DIM>
+class LinearSolverFactory
+    : public Factory<LinearSolverType, LinearSolver<DIM>, SimState<DIM>*> {
+ public:
+  LinearSolverFactory();
+};
+
+// ---------------------------------------------------------------------------
+struct StepStats {
+  int newton_iters = 0;
+  int pcg_iters = 0;
+  int refinements = 0;
+  int ls_iters = 0;
+  int ls_fail = 0;
+  double decrement = 0;
+  double energy = 0;
+  double alpha = 0;
+  double res = 0;
+  double rel_residual = 0;
+  double update_ms = 0, solve_ms = 0, ls_ms = 0, step_ms = 0;
+  double spmv_ms = 0;
+  int spmv_count = 0;
+};
+
+template <int DIM>
+class Optimizer {
+ public:
+  // Takes ownership of the state (the reference moves it, optimizer.h:19).
+  explicit Optimizer(std::unique_ptr<SimState<DIM>> state) : state_(std::move(state)) {}
+  virtual ~Optimizer() = default;
+  static std::string name() { return "base"; }
+  virtual void reset();
+  virtual void step() = 0;
+  SimState<DIM>& state() { return *state_; }
+  std::function<void(const SimState<DIM>&)> callback = [](const SimState<DIM>&) {};
+
+ protected:
+  std::unique_ptr<SimState<DIM>> state_;
+};
+
+template <int DIM>
+class NewtonOptimizer : public Optimizer<DIM> {
+ public:
+  static std::string name() { return "newton"; }
+  using Optimizer<DIM>::Optimizer;
+  ~NewtonOptimizer() override;
+  void reset() override;
+  void step() override;
+  // pieces of step(), exposed for phase-level checks
+  void update_system();
+  void substep(double& decrement);
+  double energy(double alpha);
+  const StepStats& last_stats() const { return stats_; }
+  LinearSolver<DIM>* solver() { return solver_.get(); }
+
+ private:
+  std::unique_ptr<LinearSolver<DIM>> solver_;
+  StepStats stats_;
+  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+template <int DIM>
+class OptimizerFactory
+    : public Factory<OptimizerType, Optimizer<DIM>, std::unique_ptr<SimState<DIM>>> {
+ public:
+  OptimizerFactory();
+};
+
+// ---------------------------------------------------------------------------
+// Scene description accepted from the C ABI (arrays) or built by load(json).
+struct SceneArrays {
+  std::vector<double> V;      // nV x 3
+  std::vector<int> T;         // nT x 4
+  std::vector<double> mu, la;
+  std::vector<int> mat;
+  std::vector<int> fixed;     // optional pin mask
+  std::vector<double> v0;     // optional initial velocity
+  double density = 1000.0;
+  BoundaryConditionConfig bc;  // used when `fixed` is empty
+};
+
+template <int DIM>
+struct SimState {
+  std::shared_ptr<Mesh> mesh_;
+  std::shared_ptr<SimConfig> config_;
+  std::vector<std::shared_ptr<MaterialModel>> material_models_;
+  std::unique_ptr<Displacement<DIM>> x_;
+  std::vector<std::unique_ptr<MixedVariable<DIM>>> mixed_vars_;
+  std::vector<std::shared_ptr<Variable<DIM>>> vars_;
+  std::unique_ptr<DeviceContext> ctx_;
+
+  int size() const {
+    int n = x_->size();
+    for (const auto& v : mixed_vars_) n += v->size() + v->size_dual();
+    return n;
+  }
+  // Scene JSON (reference simulation_state.cpp:72-366 key set).
+  bool load(const std::string& json_file);
+  bool load(const Json& args, const std::string& base_dir = ".");
+  // Arrays (per-element materials as in the material_ids constructor,
+  // reference tet_mesh.h:11-15); names of variables / solver as in JSON.
+  void load_arrays(const SceneArrays& sc, std::shared_ptr<SimConfig> config,
+                   const std::vector<std::string>& mixed_variables,
+                   const std::vector<std::string>& variables);
+  std::vector<int> fixed_override_;
+
+ private:
+  void finish_setup();
+  void load_params(const Json& args);
+};
+
+// Medit .mesh and libigl .dmat readers used by the scene loader.
+void read_mesh_file(const std::string& path, std::vector<double>& V, std::vector<int>& T);
+std::vector<double> read_dmat(const std::string& path, int& rows, int& cols);
+
+}  // namespace mfem

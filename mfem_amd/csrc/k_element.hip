@@ -1,0 +1,700 @@
+// Element kernels of the Newton step: one tetrahedron per lane, structure-of-
+// arrays loads (stride nT) so that each load instruction of a wavefront is
+// coalesced; vertex positions are gathered through L2 (xyz interleaved).
+#include <hip/hip_runtime.h>
+#include "device_math.h"
+#include "kernels.h"
+#include "reduce.h"
+
+namespace mfemhip {
+
+namespace {
+
+constexpr int kElemBlock = 128;
+
+// Gather the 4 corner positions of tet e and form F = sum_k x_k (x) dphi_k.
+__device__ __forceinline__ void gather_F(const TetMeshDev& m, int e, const double* __restrict__ x,
+                                         int v[4], double xc[12], double F[9]) {
+  int4 t = m.T[e];
+  v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) xc[3 * k + r] = x[3 * v[k] + r];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double d = m.dX[(3 * k + c) * m.nT + e];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) F[r + 3 * c] += xc[3 * k + r] * d;
+    }
+}
+
+__device__ __forceinline__ void unpack_sym6(const double H[21], double A[36]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i + 6 * j] = H[sym6(i, j)];
+}
+
+// General 6x6 inverse by LU with partial pivoting (the reference inverts the
+// projected Hessian with Eigen's inverse(), mixed_stretch.cpp:162).
+template <typename Real>
+__device__ __forceinline__ void lu_inverse6(const Real* Ain, Real* X) {
+  Real A[36];
+  int piv[6];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) A[i] = Ain[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) piv[i] = i;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int p = k;
+    Real best = tabs(A[k + 6 * k]);
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i)
+      if (tabs(A[i + 6 * k]) > best) { best = tabs(A[i + 6 * k]); p = i; }
+    if (p != k) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) { Real t = A[k + 6 * j]; A[k + 6 * j] = A[p + 6 * j]; A[p + 6 * j] = t; }
+      int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+    }
+    Real id = Real(1) / A[k + 6 * k];
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      A[i + 6 * k] *= id;
+      Real l = A[i + 6 * k];
+#pragma unroll
+      for (int j = k + 1; j < 6; ++j) A[i + 6 * j] -= l * A[k + 6 * j];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    Real y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      Real s = (piv[i] == c) ? Real(1) : Real(0);
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= A[i + 6 * k] * y[k];
+      y[i] = s;
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+      Real s = y[i];
+#pragma unroll
+      for (int k = i + 1; k < 6; ++k) s -= A[i + 6 * k] * X[k + 6 * c];
+      X[i + 6 * c] = s / A[i + 6 * i];
+    }
+  }
+}
+
+// Projected, h^2-scaled material Hessian (full 6x6, Real) from s.
+template <typename Real>
+__device__ __forceinline__ void projected_hessian(int type, double mu, double la, const double s[6],
+                                                  double h2, double g[6], Real Hs[36]) {
+  double gg[6], HH[21];
+  material_grad_hess(type, mu, la, s, gg, HH);
+  Real A[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i + 6 * j] = (Real)HH[sym6(i, j)];
+  if (material_needs_psd(type) && !chol_pd<Real, 6>(A, Real(1e-6)))
+    jacobi_psd_project<Real, 6>(A, Real(1e-6));
+#pragma unroll
+  for (int i = 0; i < 36; ++i) Hs[i] = (Real)h2 * A[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) g[i] = h2 * gg[i];
+}
+
+__device__ __forceinline__ constexpr int blk_index(int k, int l) {
+  // upper pair (k <= l) of the 4 tet corners -> 0..9
+  return k * 4 - (k * (k - 1)) / 2 + (l - k);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed_polar_material(
+    TetMeshDev m, const double* __restrict__ x, double* __restrict__ R, double* __restrict__ S,
+    const double* __restrict__ s, Real* __restrict__ dsdF, Real* __restrict__ Hpsi,
+    Real* __restrict__ Hinv, double* __restrict__ g, double* __restrict__ gl, double h2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  int v[4];
+  double xc[12], F[9];
+  gather_F(m, e, x, v, xc, F);
+  double Rl[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Rl[i] = R[i * nT + e];
+  procrustes(Rl, F, 1e-6, 100);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i * nT + e] = Rl[i];
+  double Sm[9], sv[6];
+  stretch_of(Rl, F, sv, Sm);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) S[i * nT + e] = sv[i];
+  {
+    Real d[54];
+    dsdF_of<Real>(Rl, F, d);
+#pragma unroll
+    for (int i = 0; i < 54; ++i) dsdF[i * nT + e] = d[i];
+  }
+  double se[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) se[i] = s[i * nT + e];
+  double gg[6];
+  Real Hs[36];
+  projected_hessian<Real>(m.mat[e], m.mu[e], m.la[e], se, h2, gg, Hs);
+  Real Hi[36];
+  lu_inverse6<Real>(Hs, Hi);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      Hpsi[sym6(i, j) * nT + e] = Hs[i + 6 * j];
+      Hinv[sym6(i, j) * nT + e] = (Real)(0.5 * ((double)Hi[i + 6 * j] + (double)Hi[j + 6 * i]));
+    }
+  const double syminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double a = gg[i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a += (double)Hs[i + 6 * j] * (sv[j] - se[j]);
+    g[i * nT + e] = gg[i];
+    gl[i * nT + e] = syminv[i] * a;
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
+    TetMeshDev m, const double* __restrict__ x, Inertia in, const Real* __restrict__ dsdF,
+    const Real* __restrict__ Hpsi, const double* __restrict__ gl, Real* __restrict__ blocks,
+    double* __restrict__ fe) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  const double vol = m.vol[e];
+  // B = dsdF * Jloc (6 x 12): B[i][3k+r] = sum_c dsdF[i][r+3c] dX(k,c)
+  Real B[6 * 12];
+  {
+    Real d[54];
+#pragma unroll
+    for (int i = 0; i < 54; ++i) d[i] = dsdF[i * nT + e];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      Real dx0 = (Real)m.dX[(3 * k + 0) * nT + e];
+      Real dx1 = (Real)m.dX[(3 * k + 1) * nT + e];
+      Real dx2 = (Real)m.dX[(3 * k + 2) * nT + e];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+          B[i + 6 * (3 * k + r)] = d[i + 6 * (r + 0)] * dx0 + d[i + 6 * (r + 3)] * dx1 +
+                                   d[i + 6 * (r + 6)] * dx2;
+    }
+  }
+  Real H[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      Real h = Hpsi[sym6(i, j) * nT + e];
+      H[i + 6 * j] = h;
+      H[j + 6 * i] = h;
+    }
+  Real* out = blocks + (size_t)e * 90;
+  const Real rv = (Real)vol;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    Real C[18];  // H * B_l (6 x 3)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        Real a = 0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a += H[i + 6 * j] * B[j + 6 * (3 * l + q)];
+        C[i + 6 * q] = a;
+      }
+#pragma unroll
+    for (int k = 0; k <= l; ++k) {
+      Real* o = out + 9 * blk_index(k, l);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          Real a = 0;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) a += B[i + 6 * (3 * k + r)] * C[i + 6 * q];
+          o[3 * r + q] = rv * a;
+        }
+    }
+  }
+  // element residual: -vol B^T (Sym gl) - M_e (x - x_tilde - h^2 g)
+  double w[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) w[i] = (i < 3 ? 1.0 : 2.0) * gl[i * nT + e];
+  int4 t = m.T[e];
+  int v[4] = {t.x, t.y, t.z, t.w};
+  double d[12], dsum[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double dd = x[3 * v[k] + r] - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+      d[3 * k + r] = dd;
+      dsum[r] += dd;
+    }
+  const double mc = m.density * vol / 20.0;
+  double* fo = fe + (size_t)e * 12;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    double a = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) a += (double)B[i + 6 * j] * w[i];
+    fo[j] = -vol * a - mc * (d[j] + dsum[j % 3]);
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
+    TetMeshDev m, const double* __restrict__ dx, const Real* __restrict__ dsdF,
+    const Real* __restrict__ Hpsi, const Real* __restrict__ Hinv, const double* __restrict__ g,
+    const double* __restrict__ gl, double* __restrict__ lam, double* __restrict__ ds,
+    ReduceSlot max_ds) {
+  __shared__ double sm[kElemBlock / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double local_max = 0.0;
+  if (e < m.nT) {
+    const int nT = m.nT;
+    int4 t = m.T[e];
+    int v[4] = {t.x, t.y, t.z, t.w};
+    double dc[12];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int f = m.free_map[v[k]];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) dc[3 * k + r] = f >= 0 ? dx[3 * f + r] : 0.0;
+    }
+    double Fd[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fd[i] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double dd = m.dX[(3 * k + c) * nT + e];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Fd[r + 3 * c] += dc[3 * k + r] * dd;
+      }
+    double y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) a += (double)dsdF[(i + 6 * c) * nT + e] * Fd[c];
+      y[i] = a;
+    }
+    const double syminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+    const double sym[6] = {1, 1, 1, 2, 2, 2};
+    double l[6], rr[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) a += (double)Hpsi[sym6(i, j) * nT + e] * y[j];
+      l[i] = -gl[i * nT + e] - syminv[i] * a;
+      rr[i] = sym[i] * l[i] + g[i * nT + e];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) a += (double)Hinv[sym6(i, j) * nT + e] * rr[j];
+      lam[i * nT + e] = l[i];
+      ds[i * nT + e] = -a;
+      local_max = fmax(local_max, fabs(a));
+    }
+  }
+  block_reduce_finalize<kElemBlock, 1>(local_max, max_ds, sm);
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_mixed_energy(
+    TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha,
+    Inertia in, const double* __restrict__ R, const double* __restrict__ s,
+    const double* __restrict__ ds, const double* __restrict__ lam, double h2, ReduceSlot out) {
+  __shared__ double sm[kElemBlock / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double en = 0.0;
+  if (e < m.nT) {
+    const int nT = m.nT;
+    int4 t = m.T[e];
+    int v[4] = {t.x, t.y, t.z, t.w};
+    double q[12], dsum[3] = {0, 0, 0}, dsq = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int f = m.free_map[v[k]];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double qq = x[3 * v[k] + r] + (f >= 0 ? alpha * dx[3 * f + r] : 0.0);
+        q[3 * k + r] = qq;
+        double d = qq - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+        dsum[r] += d;
+        dsq += d * d;
+      }
+    }
+    const double vol = m.vol[e];
+    double e_in = 0.5 * (m.density * vol / 20.0) *
+                  (dsq + dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2]);
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double d = m.dX[(3 * k + c) * nT + e];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) F[r + 3 * c] += q[3 * k + r] * d;
+      }
+    double Rl[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Rl[i] = R[i * nT + e];
+    procrustes(Rl, F, 1e-6, 100);
+    double Sm[9], sv[6], sa[6];
+    stretch_of(Rl, F, sv, Sm);
+    double ec = 0;
+    const double sym[6] = {1, 1, 1, 2, 2, 2};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      sa[i] = s[i * nT + e] + alpha * ds[i * nT + e];
+      ec += lam[i * nT + e] * sym[i] * (sv[i] - sa[i]);
+    }
+    double psi = material_energy(m.mat[e], m.mu[e], m.la[e], sa);
+    en = e_in + h2 * vol * psi - vol * ec;
+  }
+  block_reduce_finalize<kElemBlock, 0>(en, out, sm);
+}
+
+// ---------------------------------------------------------------------------
+// Plain SNH on F (reference stable_neohookean.cpp:97-234): energy, gradient,
+// and the 9x9 Hessian mu I + la c c^T + (la (J-1) - mu) d2J/dF2, projected
+// with tolerance 0 (:232).
+__device__ __forceinline__ int eps3(int i, int j, int k) {
+  return (i == j || j == k || i == k) ? 0 : (((j - i + 3) % 3 == 1) ? 1 : -1);
+}
+
+__device__ __forceinline__ void snh_f9_cof(const double F[9], double& J, double c[9]) {
+  c[0] = F[4] * F[8] - F[7] * F[5];
+  c[3] = F[7] * F[2] - F[1] * F[8];
+  c[6] = F[1] * F[5] - F[4] * F[2];
+  c[1] = F[6] * F[5] - F[3] * F[8];
+  c[4] = F[0] * F[8] - F[6] * F[2];
+  c[7] = F[3] * F[2] - F[0] * F[5];
+  c[2] = F[3] * F[7] - F[6] * F[4];
+  c[5] = F[6] * F[1] - F[0] * F[7];
+  c[8] = F[0] * F[4] - F[3] * F[1];
+  J = F[0] * c[0] + F[3] * c[3] + F[6] * c[6];
+}
+
+__device__ __forceinline__ double snh_f9_energy(double mu, double la, const double F[9]) {
+  double J, c[9];
+  snh_f9_cof(F, J, c);
+  double n2 = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) n2 += F[i] * F[i];
+  return 0.5 * mu * (n2 - 3.0) - mu * (J - 1.0) + 0.5 * la * (J - 1.0) * (J - 1.0);
+}
+
+template <typename Real>
+__device__ __forceinline__ void snh_f9_grad_hess(double mu, double la, const double F[9],
+                                                 double g[9], Real H[81]) {
+  double J, c[9];
+  snh_f9_cof(F, J, c);
+  double k = la * (J - 1.0) - mu;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) g[i] = mu * F[i] + k * c[i];
+#pragma unroll
+  for (int p = 0; p < 9; ++p)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      int i = p % 3, j = p / 3, kk = q % 3, l = q / 3;
+      double v = (p == q ? mu : 0.0) + la * c[p] * c[q];
+      if (i != kk && j != l) {
+        int mm = 3 - i - kk, n = 3 - j - l;
+        v += k * (double)(eps3(i, kk, mm) * eps3(j, l, n)) * F[mm + 3 * n];
+      }
+      H[p + 9 * q] = (Real)v;
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_plain_update(
+    TetMeshDev m, const double* __restrict__ x, Inertia in, double h2, Real* __restrict__ blocks,
+    double* __restrict__ fe) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  int v[4];
+  double xc[12], F[9];
+  gather_F(m, e, x, v, xc, F);
+  double g[9];
+  Real H[81];
+  snh_f9_grad_hess<Real>(m.mu[e], m.la[e], F, g, H);
+  if (!chol_pd<Real, 9>(H, Real(0))) jacobi_psd_project<Real, 9>(H, Real(0));
+  const double vol = m.vol[e];
+  const Real sc = (Real)(vol * h2);
+  Real dxr[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) dxr[i] = (Real)m.dX[i * nT + e];
+  Real* out = blocks + (size_t)e * 90;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    // C = H Jloc_l (9 x 3): C[p][q] = sum_c2 H[p][q + 3 c2] dX(l, c2)
+    Real C[27];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+        C[p + 9 * q] = H[p + 9 * (q + 0)] * dxr[3 * l + 0] + H[p + 9 * (q + 3)] * dxr[3 * l + 1] +
+                       H[p + 9 * (q + 6)] * dxr[3 * l + 2];
+#pragma unroll
+    for (int k = 0; k <= l; ++k) {
+      Real* o = out + 9 * blk_index(k, l);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          o[3 * r + q] = sc * (dxr[3 * k + 0] * C[(r + 0) + 9 * q] + dxr[3 * k + 1] * C[(r + 3) + 9 * q] +
+                               dxr[3 * k + 2] * C[(r + 6) + 9 * q]);
+    }
+  }
+  double d[12], dsum[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double dd = xc[3 * k + r] - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+      d[3 * k + r] = dd;
+      dsum[r] += dd;
+    }
+  const double mc = m.density * vol / 20.0;
+  double* fo = fe + (size_t)e * 12;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double a = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a += m.dX[(3 * k + c) * nT + e] * g[r + 3 * c];
+      fo[3 * k + r] = -vol * h2 * a - mc * (d[3 * k + r] + dsum[r]);
+    }
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_plain_energy(
+    TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha,
+    Inertia in, double h2, ReduceSlot out) {
+  __shared__ double sm[kElemBlock / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double en = 0.0;
+  if (e < m.nT) {
+    const int nT = m.nT;
+    int4 t = m.T[e];
+    int v[4] = {t.x, t.y, t.z, t.w};
+    double q[12], dsum[3] = {0, 0, 0}, dsq = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int f = m.free_map[v[k]];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double qq = x[3 * v[k] + r] + (f >= 0 ? alpha * dx[3 * f + r] : 0.0);
+        q[3 * k + r] = qq;
+        double d = qq - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+        dsum[r] += d;
+        dsq += d * d;
+      }
+    }
+    const double vol = m.vol[e];
+    double e_in = 0.5 * (m.density * vol / 20.0) *
+                  (dsq + dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2]);
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double d = m.dX[(3 * k + c) * nT + e];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) F[r + 3 * c] += q[3 * k + r] * d;
+      }
+    en = e_in + h2 * vol * snh_f9_energy(m.mu[e], m.la[e], F);
+  }
+  block_reduce_finalize<kElemBlock, 0>(en, out, sm);
+}
+
+// ---------------------------------------------------------------------------
+// Batched element evaluations (points in AoS, column-major per point).
+__global__ void k_batch_polar(int n, const double* __restrict__ F, double* __restrict__ R,
+                              double* __restrict__ s, double* __restrict__ dsdF,
+                              int* __restrict__ iters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double Fl[9], Rl[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) { Fl[k] = F[9 * i + k]; Rl[k] = R[9 * i + k]; }
+  int it = procrustes(Rl, Fl, 1e-6, 100);
+  double Sm[9], sv[6];
+  stretch_of(Rl, Fl, sv, Sm);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[9 * i + k] = Rl[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[6 * i + k] = sv[k];
+  if (dsdF) {
+    double d[54];
+    dsdF_of<double>(Rl, Fl, d);
+#pragma unroll
+    for (int k = 0; k < 54; ++k) dsdF[54 * i + k] = d[k];
+  }
+  if (iters) iters[i] = it;
+}
+
+__global__ void k_batch_material(int n, const int* __restrict__ type, const double* __restrict__ mu,
+                                 const double* __restrict__ la, const double* __restrict__ s,
+                                 int psd, double* __restrict__ E, double* __restrict__ g,
+                                 double* __restrict__ H) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sv[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sv[k] = s[6 * i + k];
+  if (E) E[i] = material_energy(type[i], mu[i], la[i], sv);
+  double gg[6], HH[21];
+  material_grad_hess(type[i], mu[i], la[i], sv, gg, HH);
+  double A[36];
+  unpack_sym6(HH, A);
+  if (psd && material_needs_psd(type[i]) && !chol_pd<double, 6>(A, 1e-6))
+    jacobi_psd_project<double, 6>(A, 1e-6);
+  if (g)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[6 * i + k] = gg[k];
+  if (H)
+#pragma unroll
+    for (int k = 0; k < 36; ++k) H[36 * i + k] = A[k];
+}
+
+__global__ void k_batch_snh_f9(int n, const double* __restrict__ mu, const double* __restrict__ la,
+                               const double* __restrict__ F, double* __restrict__ E,
+                               double* __restrict__ g, double* __restrict__ H) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double Fl[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fl[k] = F[9 * i + k];
+  if (E) E[i] = snh_f9_energy(mu[i], la[i], Fl);
+  double gg[9], HH[81];
+  snh_f9_grad_hess<double>(mu[i], la[i], Fl, gg, HH);
+  if (!chol_pd<double, 9>(HH, 0.0)) jacobi_psd_project<double, 9>(HH, 0.0);
+  if (g)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[9 * i + k] = gg[k];
+  if (H)
+#pragma unroll
+    for (int k = 0; k < 81; ++k) H[81 * i + k] = HH[k];
+}
+
+inline int grid_for(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+int reduce_grid(long n) { return n > 0 ? grid_for(n, kElemBlock) : 1; }
+
+template <typename Real>
+void launch_mixed_polar_material(const TetMeshDev& m, const double* x_full, double* R, double* S,
+                                 const double* s, Real* dsdF, Real* Hpsi, Real* Hinv, double* g,
+                                 double* gl, double h2, hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_mixed_polar_material<Real>, dim3(grid_for(m.nT, kElemBlock)),
+                     dim3(kElemBlock), 0, st, m, x_full, R, S, s, dsdF, Hpsi, Hinv, g, gl, h2);
+}
+
+template <typename Real>
+void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                             const Real* dsdF, const Real* Hpsi, const double* gl, Real* blocks,
+                             double* fe, hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_mixed_blocks_rhs<Real>, dim3(grid_for(m.nT, kElemBlock)), dim3(kElemBlock),
+                     0, st, m, x_full, in, dsdF, Hpsi, gl, blocks, fe);
+}
+
+template <typename Real>
+void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
+                              const Real* Hpsi, const Real* Hinv, const double* g,
+                              const double* gl, double* lam, double* ds, ReduceSlot max_ds,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_mixed_local_solve<Real>, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st,
+                     m, dx, dsdF, Hpsi, Hinv, g, gl, lam, ds, max_ds);
+}
+
+void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double* dx, double alpha,
+                         const Inertia& in, const double* R, const double* s, const double* ds,
+                         const double* lam, double h2, ReduceSlot out, hipStream_t st) {
+  hipLaunchKernelGGL(k_mixed_energy, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st, m, x_full,
+                     dx, alpha, in, R, s, ds, lam, h2, out);
+}
+
+template <typename Real>
+void launch_plain_update(const TetMeshDev& m, const double* x_full, const Inertia& in, double h2,
+                         Real* blocks, double* fe, hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_plain_update<Real>, dim3(grid_for(m.nT, kElemBlock)), dim3(kElemBlock), 0,
+                     st, m, x_full, in, h2, blocks, fe);
+}
+
+void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double* dx, double alpha,
+                         const Inertia& in, double h2, ReduceSlot out, hipStream_t st) {
+  hipLaunchKernelGGL(k_plain_energy, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st, m, x_full,
+                     dx, alpha, in, h2, out);
+}
+
+void launch_batch_polar(int n, const double* F, double* R, double* s, double* dsdF, int* iters,
+                        hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_batch_polar, dim3(grid_for(n, 64)), dim3(64), 0, st, n, F, R, s, dsdF, iters);
+}
+
+void launch_batch_material(int n, const int* type, const double* mu, const double* la,
+                           const double* s, int psd, double* E, double* g, double* H,
+                           hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_batch_material, dim3(grid_for(n, 64)), dim3(64), 0, st, n, type, mu, la, s,
+                     psd, E, g, H);
+}
+
+void launch_batch_snh_f9(int n, const double* mu, const double* la, const double* F, double* E,
+                         double* g, double* H, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_batch_snh_f9, dim3(grid_for(n, 64)), dim3(64), 0, st, n, mu, la, F, E, g, H);
+}
+
+#define MFEM_INST(Real)                                                                          \
+  template void launch_mixed_polar_material<Real>(const TetMeshDev&, const double*, double*,    \
+                                                  double*, const double*, Real*, Real*, Real*,  \
+                                                  double*, double*, double, hipStream_t);       \
+  template void launch_mixed_blocks_rhs<Real>(const TetMeshDev&, const double*, const Inertia&, \
+                                              const Real*, const Real*, const double*, Real*,   \
+                                              double*, hipStream_t);                            \
+  template void launch_mixed_local_solve<Real>(const TetMeshDev&, const double*, const Real*,   \
+                                               const Real*, const Real*, const double*,         \
+                                               const double*, double*, double*, ReduceSlot,     \
+                                               hipStream_t);                                    \
+  template void launch_plain_update<Real>(const TetMeshDev&, const double*, const Inertia&,     \
+                                          double, Real*, double*, hipStream_t);
+MFEM_INST(double)
+MFEM_INST(float)
+#undef MFEM_INST
+
+}  // namespace mfemhip

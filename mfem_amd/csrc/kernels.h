@@ -1,0 +1,148 @@
+// Launchers for the gfx950 kernels of the Newton-step hot path. Host code
+// (the variable / solver / optimizer classes and the C ABI) calls these; all
+// pointers are device pointers unless stated, all launches are asynchronous
+// on `stream`.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mfemhip {
+
+// Mesh data on the device, structure-of-arrays with stride nT.
+struct TetMeshDev {
+  int nV = 0, nT = 0, nfree = 0;
+  const int4* T = nullptr;         // [nT] vertex ids
+  const double* dX = nullptr;      // [12][nT]: dphi_k/dX_c at (3k+c)*nT + e
+  const double* vol = nullptr;     // [nT]
+  const double* mu = nullptr;      // [nT]
+  const double* la = nullptr;      // [nT]
+  const int* mat = nullptr;        // [nT]
+  const int* free_map = nullptr;   // [nV] free index or -1
+  double density = 1000.0;
+};
+
+// Deterministic reduction target: per-block partials plus an arrival
+// counter; the last block to arrive reduces the partials in index order.
+struct ReduceSlot {
+  double* partials = nullptr;   // >= grid size
+  unsigned* counter = nullptr;  // zero-initialised, reset by the last block
+  double* result = nullptr;     // 1 value
+};
+
+struct Inertia {
+  const double* x_tilde = nullptr;  // [3nV]
+  double h2g[3] = {0, 0, 0};        // h^2 * gravity
+};
+
+// --- mixed stretch (reference src/variables/mixed_stretch.cpp) ------------
+// Phase 1: polar fit (warm start R), S, ds/dF, material gradient g, PSD-
+// projected Hessian Hpsi (= h^2 Hessian), its inverse, and gl.
+template <typename Real>
+void launch_mixed_polar_material(const TetMeshDev& m, const double* x_full, double* R,
+                                 double* S, const double* s, Real* dsdF, Real* Hpsi,
+                                 Real* Hinv, double* g, double* gl, double h2,
+                                 hipStream_t st);
+// Phase 2: element stiffness blocks vol * B^T Hpsi B (10 unique 3x3 blocks
+// per tet) and the element residual f_e (12 per tet).
+template <typename Real>
+void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                             const Real* dsdF, const Real* Hpsi, const double* gl,
+                             Real* blocks, double* fe, hipStream_t st);
+// Local solve (reference mixed_stretch.cpp:244-259): lambda and ds from dx.
+template <typename Real>
+void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
+                              const Real* Hpsi, const Real* Hinv, const double* g,
+                              const double* gl, double* lam, double* ds, ReduceSlot max_ds,
+                              hipStream_t st);
+// Line-search energy E(alpha) of the mixed formulation (reference
+// newton_optimizer.cpp:34-49 with mixed_stretch.cpp:83-123 and
+// displacement.cpp:21-31), summed per element.
+void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double* dx,
+                         double alpha, const Inertia& in, const double* R, const double* s,
+                         const double* ds, const double* lam, double h2, ReduceSlot out,
+                         hipStream_t st);
+
+// --- plain stretch (reference src/variables/stretch.cpp) -------------------
+template <typename Real>
+void launch_plain_update(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                         double h2, Real* blocks, double* fe, hipStream_t st);
+void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double* dx,
+                         double alpha, const Inertia& in, double h2, ReduceSlot out,
+                         hipStream_t st);
+
+// --- assembly (reference src/utils/sparse_utils.cpp:141-203) ---------------
+// K block b = mass[b] * I + sum over its slot list of element blocks.
+// slot = (e << 4) | (a << 2) | bb  (local vertex pair a, bb of tet e)
+template <typename Real>
+void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const double* mass,
+                         const Real* blocks, Real* K, hipStream_t st);
+// invdiag[3i+r] = 1 / K_(3i+r),(3i+r) from the diagonal blocks (Jacobi).
+template <typename Real, typename VT>
+void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* invdiag,
+                        hipStream_t st);
+// rhs at free vertex i = sum over its (tet, corner) incidences of f_e[corner]
+void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const double* fe,
+                         double* rhs, hipStream_t st);
+
+// --- BSR 3x3 SpMV / Jacobi PCG (reference Eigen ConjugateGradient semantics)
+struct PcgCtrl {
+  double rz, rz_new, pq, rr, b2, threshold, alpha, beta;
+  int iter, done, max_iters, pad;
+};
+template <typename MT, typename VT>
+void launch_spmv(int nrows, const int* rowptr, const int* col, const MT* val, const VT* x,
+                 VT* y, hipStream_t st);
+// y = b - K x in FP64 (defect correction residual)
+template <typename MT>
+void launch_residual(int nrows, const int* rowptr, const int* col, const MT* val,
+                     const double* x, const double* b, double* r, ReduceSlot r2,
+                     hipStream_t st);
+template <typename MT, typename VT>
+struct PcgBuffers {
+  int n = 0;  // scalar rows (3 * nfree)
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const MT* K = nullptr;
+  const VT* invdiag = nullptr;
+  const double* b = nullptr;  // right-hand side
+  double* x = nullptr;        // solution (FP64 accumulation)
+  VT* r = nullptr;
+  VT* z = nullptr;
+  VT* p = nullptr;
+  VT* q = nullptr;
+  PcgCtrl* ctrl = nullptr;
+  ReduceSlot red[3];  // independent reduction slots
+};
+template <typename MT, typename VT>
+void launch_pcg_init(const PcgBuffers<MT, VT>& B, double tol, int max_iters, hipStream_t st);
+// One CG iteration (3 kernels). When ev0/ev1 are given they are recorded
+// immediately before and after the SpMV kernel (device-side timing).
+template <typename MT, typename VT>
+void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, hipStream_t st, hipEvent_t ev0 = nullptr,
+                          hipEvent_t ev1 = nullptr);
+
+// --- vectors ---------------------------------------------------------------
+// x_full[free] += alpha * dx ; s += alpha * ds (line-search acceptance,
+// reference linesearch.h:120-126)
+void launch_accept_step(const TetMeshDev& m, double* x_full, const double* dx, double* s,
+                        const double* ds, double alpha, int with_s, hipStream_t st);
+void launch_bdf_xtilde(int n, int order, const double* const* xh, const double* const* vh,
+                       const double* a, double dt, double* x_tilde, hipStream_t st);
+void launch_bdf_update(int n, int order, const double* const* xh, const double* a, double dt,
+                       const double* x_full, double* x_new, double* v_new, hipStream_t st);
+void launch_reduce_maxabs(int n, const double* v, ReduceSlot out, hipStream_t st);
+void launch_fill(double* p, double v, long n, hipStream_t st);
+void launch_axpy(long n, double alpha, const double* x, double* y, hipStream_t st);  // y += a x
+
+// --- batched element entry points (MaterialModel / polar on many points) ---
+void launch_batch_polar(int n, const double* F, double* R, double* s, double* dsdF,
+                        int* iters, hipStream_t st);
+void launch_batch_material(int n, const int* type, const double* mu, const double* la,
+                           const double* s, int psd, double* E, double* g, double* H,
+                           hipStream_t st);
+void launch_batch_snh_f9(int n, const double* mu, const double* la, const double* F,
+                         double* E, double* g, double* H, hipStream_t st);
+
+int reduce_grid(long n);  // grid size used by the reductions for n items
+
+}  // namespace mfemhip

@@ -45,6 +45,18 @@ def read_mesh(path: str):
     return V, T
 
 
+def write_mesh(path: str, V: np.ndarray, T: np.ndarray) -> None:
+    """Write a Medit .mesh file (1-based indices, reference zero labels)."""
+    with open(path, "w") as f:
+        f.write("MeshVersionFormatted 1\nDimension 3\nVertices\n%d\n" % V.shape[0])
+        for v in V:
+            f.write("%.17g %.17g %.17g 0\n" % tuple(v))
+        f.write("Triangles\n0\nTetrahedra\n%d\n" % T.shape[0])
+        for t in T:
+            f.write("%d %d %d %d 0\n" % tuple(int(i) + 1 for i in t))
+        f.write("Edges\n0\nEnd\n")
+
+
 def read_dmat(path: str) -> np.ndarray:
     """Read a libigl ASCII .dmat file into a (rows x cols) array."""
     with open(path) as f:

@@ -1,0 +1,275 @@
+"""ctypes binding of libmfem_hip.so (the C ABI in include/mfem_hip.h).
+
+The library must have been built (python -m mfem_amd.build); loading fails
+loudly otherwise -- there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmfem_hip.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+EXPORTED = [
+    "mfem_last_error", "mfem_version", "mfem_sim_create_json", "mfem_sim_create",
+    "mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system", "mfem_sim_solve",
+    "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
+    "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
+    "mfem_sim_time_spmv", "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
+]
+
+
+class SimParams(C.Structure):
+    _fields_ = [
+        ("nV", C.c_int), ("nT", C.c_int),
+        ("V", dp), ("T", ip), ("mu", dp), ("la", dp), ("mat", ip), ("fixed", ip), ("v0", dp),
+        ("density", C.c_double), ("dt", C.c_double), ("gravity", C.c_double * 3),
+        ("bdf_order", C.c_int), ("mixed", C.c_int), ("linear_solver", C.c_char_p),
+        ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
+        ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("precision", C.c_int),
+        ("bc_type", C.c_int), ("bc_axis", C.c_int), ("bc_ratio", C.c_double),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("newton_iters", C.c_int), ("pcg_iters", C.c_int), ("refinements", C.c_int),
+        ("ls_iters", C.c_int), ("ls_fail", C.c_int),
+        ("decrement", C.c_double), ("energy", C.c_double), ("alpha", C.c_double),
+        ("res", C.c_double), ("rel_residual", C.c_double),
+        ("update_ms", C.c_double), ("solve_ms", C.c_double), ("ls_ms", C.c_double),
+        ("step_ms", C.c_double), ("spmv_ms", C.c_double), ("spmv_count", C.c_int),
+    ]
+
+
+class MfemError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MfemError(f"{LIB_PATH} is missing: build it with `python -m mfem_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.mfem_last_error.restype = C.c_char_p
+        L.mfem_sim_create_json.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.mfem_sim_create.argtypes = [C.POINTER(SimParams), C.POINTER(vp)]
+        for f in ("mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system"):
+            getattr(L, f).argtypes = [vp]
+        L.mfem_sim_solve.argtypes = [vp, dp]
+        L.mfem_sim_energy.argtypes = [vp, C.c_double, dp]
+        L.mfem_sim_stats.argtypes = [vp, C.POINTER(StepStats)]
+        L.mfem_sim_get_positions.argtypes = [vp, dp]
+        L.mfem_sim_size.argtypes = [vp, C.c_char_p]
+        L.mfem_sim_size.restype = C.c_long
+        L.mfem_sim_get.argtypes = [vp, C.c_char_p, dp]
+        L.mfem_sim_get_int.argtypes = [vp, C.c_char_p, ip]
+        L.mfem_sim_set_profiling.argtypes = [vp, C.c_int]
+        L.mfem_sim_matrix_info.argtypes = [vp, ip, ip, ip]
+        L.mfem_sim_time_spmv.argtypes = [vp, C.c_int, dp]
+        L.mfem_material_s6_batch.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
+        L.mfem_snh_f9_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.mfem_polar_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise MfemError(lib().mfem_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Sim:
+    """One simulation on the GPU: SimState<3> + NewtonOptimizer<3> behind the C ABI."""
+
+    def __init__(self, scene=None, precision: str = "fp64", json_path: str | None = None):
+        self.h = C.c_void_p()
+        self.scene = scene
+        if json_path is not None:
+            check(lib().mfem_sim_create_json(json_path.encode(), C.byref(self.h)))
+            self.nV = int(lib().mfem_sim_size(self.h, b"x") // 3)
+            return
+        sc = scene
+        if sc.dim != 3:
+            raise MfemError("the device backend is 3D (tetrahedra)")
+        keep = dict(V=_f64(sc.V), T=_i32(sc.T), mu=_f64(sc.mu), la=_f64(sc.la), mat=_i32(sc.mat),
+                    fixed=_i32(sc.fixed))
+        p = SimParams()
+        p.nV, p.nT = sc.nV, sc.nT
+        p.V = keep["V"].ctypes.data_as(dp)
+        p.T = keep["T"].ctypes.data_as(ip)
+        p.mu = keep["mu"].ctypes.data_as(dp)
+        p.la = keep["la"].ctypes.data_as(dp)
+        p.mat = keep["mat"].ctypes.data_as(ip)
+        p.fixed = keep["fixed"].ctypes.data_as(ip)
+        if sc.v0 is not None:
+            keep["v0"] = _f64(sc.v0)
+            p.v0 = keep["v0"].ctypes.data_as(dp)
+        else:
+            p.v0 = dp()
+        p.density, p.dt = sc.density, sc.dt
+        g = list(sc.gravity) + [0.0] * 3
+        p.gravity = (C.c_double * 3)(*g[:3])
+        p.bdf_order = sc.bdf_order
+        p.mixed = int(sc.mixed)
+        p.linear_solver = sc.solver.encode()
+        p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
+        p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
+        p.precision = {"fp64": 0, "mixed": 1}[precision]
+        self._keep = keep
+        check(lib().mfem_sim_create(C.byref(p), C.byref(self.h)))
+        self.nV = sc.nV
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().mfem_sim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self):
+        check(lib().mfem_sim_step(self.h))
+
+    def update_system(self):
+        check(lib().mfem_sim_update_system(self.h))
+
+    def solve(self):
+        d = C.c_double(0)
+        check(lib().mfem_sim_solve(self.h, C.byref(d)))
+        return d.value
+
+    def energy(self, alpha):
+        e = C.c_double(0)
+        check(lib().mfem_sim_energy(self.h, float(alpha), C.byref(e)))
+        return e.value
+
+    def stats(self):
+        st = StepStats()
+        check(lib().mfem_sim_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in StepStats._fields_}
+
+    def positions(self):
+        out = np.zeros(self.nV * 3)
+        check(lib().mfem_sim_get_positions(self.h, out.ctypes.data_as(dp)))
+        return out.reshape(-1, 3)
+
+    def get(self, name):
+        n = lib().mfem_sim_size(self.h, name.encode())
+        if n < 0:
+            raise MfemError(lib().mfem_last_error().decode())
+        out = np.zeros(n)
+        check(lib().mfem_sim_get(self.h, name.encode(), out.ctypes.data_as(dp)))
+        return out
+
+    def get_int(self, name):
+        n = lib().mfem_sim_size(self.h, name.encode())
+        if n < 0:
+            raise MfemError(lib().mfem_last_error().decode())
+        out = np.zeros(n, dtype=np.int32)
+        check(lib().mfem_sim_get_int(self.h, name.encode(), out.ctypes.data_as(ip)))
+        return out
+
+    def set_profiling(self, flags=1):
+        check(lib().mfem_sim_set_profiling(self.h, int(flags)))
+
+    def matrix_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mfem_sim_matrix_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def time_spmv(self, reps=50):
+        ms = C.c_double()
+        check(lib().mfem_sim_time_spmv(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
+
+# ---- batched element entry points (torch tensors as device memory) ---------
+
+def _dev(a, dtype):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).cuda()
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def polar_batch(F, R0=None):
+    """F: (n,3,3). Returns R (n,3,3), s (n,6), dsdF (n,6,9), iterations (n)."""
+    import torch
+    F = np.asarray(F, dtype=np.float64)
+    n = F.shape[0]
+    Fc = _dev(np.transpose(F, (0, 2, 1)).reshape(n, 9), torch.float64)
+    R = np.tile(np.eye(3), (n, 1, 1)) if R0 is None else np.asarray(R0, dtype=np.float64)
+    Rc = _dev(np.transpose(R, (0, 2, 1)).reshape(n, 9), torch.float64)
+    s = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    d = torch.zeros((n, 54), dtype=torch.float64, device="cuda")
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    check(lib().mfem_polar_batch(n, C.c_void_p(Fc.data_ptr()), C.c_void_p(Rc.data_ptr()),
+                                 C.c_void_p(s.data_ptr()), C.c_void_p(d.data_ptr()),
+                                 C.c_void_p(it.data_ptr()), _stream()))
+    torch.cuda.synchronize()
+    Rh = Rc.cpu().numpy().reshape(n, 3, 3).transpose(0, 2, 1)
+    dsdF = d.cpu().numpy().reshape(n, 9, 6).transpose(0, 2, 1)
+    return Rh, s.cpu().numpy(), dsdF, it.cpu().numpy()
+
+
+def material_s6_batch(mtype, mu, la, s, psd_fix=True):
+    """s: (n,6). Returns E (n), g (n,6), H (n,6,6)."""
+    import torch
+    s = np.asarray(s, dtype=np.float64)
+    n = s.shape[0]
+    t = _dev(np.broadcast_to(np.asarray(mtype, dtype=np.int32), (n,)), torch.int32)
+    m = _dev(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,)), torch.float64)
+    l = _dev(np.broadcast_to(np.asarray(la, dtype=np.float64), (n,)), torch.float64)
+    sd = _dev(s, torch.float64)
+    E = torch.zeros(n, dtype=torch.float64, device="cuda")
+    g = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    H = torch.zeros((n, 36), dtype=torch.float64, device="cuda")
+    check(lib().mfem_material_s6_batch(n, C.c_void_p(t.data_ptr()), C.c_void_p(m.data_ptr()),
+                                       C.c_void_p(l.data_ptr()), C.c_void_p(sd.data_ptr()),
+                                       int(psd_fix), C.c_void_p(E.data_ptr()),
+                                       C.c_void_p(g.data_ptr()), C.c_void_p(H.data_ptr()),
+                                       _stream()))
+    torch.cuda.synchronize()
+    return E.cpu().numpy(), g.cpu().numpy(), H.cpu().numpy().reshape(n, 6, 6).transpose(0, 2, 1)
+
+
+def snh_f9_batch(mu, la, F9):
+    """F9: (n,9) column-major vec(F). Returns E (n), g (n,9), H (n,9,9) PSD-projected."""
+    import torch
+    F9 = np.asarray(F9, dtype=np.float64)
+    n = F9.shape[0]
+    m = _dev(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,)), torch.float64)
+    l = _dev(np.broadcast_to(np.asarray(la, dtype=np.float64), (n,)), torch.float64)
+    Fd = _dev(F9, torch.float64)
+    E = torch.zeros(n, dtype=torch.float64, device="cuda")
+    g = torch.zeros((n, 9), dtype=torch.float64, device="cuda")
+    H = torch.zeros((n, 81), dtype=torch.float64, device="cuda")
+    check(lib().mfem_snh_f9_batch(n, C.c_void_p(m.data_ptr()), C.c_void_p(l.data_ptr()),
+                                  C.c_void_p(Fd.data_ptr()), C.c_void_p(E.data_ptr()),
+                                  C.c_void_p(g.data_ptr()), C.c_void_p(H.data_ptr()), _stream()))
+    torch.cuda.synchronize()
+    return E.cpu().numpy(), g.cpu().numpy(), H.cpu().numpy().reshape(n, 9, 9).transpose(0, 2, 1)

@@ -212,6 +212,41 @@ def rect2d_scene(nx=40, ny=10, h=0.1, E=1e5, nu=0.45, **kw) -> Scene:
                  name=f"rect{nx}x{ny}", **kw)
 
 
+def scene_from_json(path: str) -> Scene:
+    """Arrays of a one-object 3D scene JSON, loaded with the reference's rules
+    (reference src/simulation_state.cpp:92-366): mesh scaled by 1/max
+    coefficient, offset, material_index, boundary_condition, solver
+    parameters."""
+    import json
+    import os
+
+    from . import meshio
+    d = json.load(open(path))
+    base = os.path.dirname(os.path.abspath(path))
+    obj = d["objects"][0]
+    V, T = meshio.read_mesh(os.path.join(base, obj["path"]))
+    V = V / V.max()
+    V = V + np.asarray(obj.get("offset", [0, 0, 0]), dtype=np.float64)[None, :]
+    mats = d.get("material_models", [{}])
+    m = mats[obj.get("material_index", 0)]
+    E, nu = m.get("youngs_modulus", 1e6), m.get("poissons_ratio", 0.45)
+    mu, la = enu_to_lame(np.full(T.shape[0], E), nu)
+    mat = np.full(T.shape[0], MATERIALS[m.get("energy", "Stable-Neohookean")], dtype=np.int32)
+    bc = obj.get("boundary_condition", {})
+    fixed = bc_fixed(bc.get("type", "null"), V, bc.get("axis", 0), bc.get("ratio", 0.1))
+    g = d.get("body_force", [0, -9.8, 0])
+    return Scene(V=V, T=T, mu=mu, la=la, mat=mat, fixed=fixed,
+                 density=mats[0].get("density", 1000.0), dt=d.get("dt", 0.034),
+                 gravity=tuple(float(np.float32(x)) for x in g),
+                 bdf_order=int(d.get("time_integrator", "BDF1")[3:]),
+                 mixed="mixed-stretch" in d.get("mixed_variables", []),
+                 solver=d.get("linear_solver", "eigen-llt"),
+                 itr_tol=d.get("iterative_solver_tolerance", 1e-4),
+                 max_iters=d.get("max_iterative_solver_iters", 500),
+                 max_newton=d.get("max_newton_iterations", 5),
+                 ls_iters=d.get("max_linesearch_iterations", 20), name=os.path.basename(path))
+
+
 # --- the benchmark configurations --------------------------------------------
 
 def config_c1():

@@ -1,0 +1,221 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle (needs an MI355X).
+
+Tolerances (FP64 device path vs FP64 oracle): element quantities 1e-9
+relative to the array's scale (different but equivalent evaluation orders,
+e.g. the closed-form ds/dF vs the reference's Kronecker products); integer
+maps bit-exact; positions after whole timesteps 1e-9 relative with tight
+solver tolerances, 1e-6 relative for the mixed FP32/FP64 mode.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from mfem_amd import scenes
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def native():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    from mfem_amd import native as N
+    N.lib()
+    return N
+
+
+def relerr(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def rand_pd_F(rng, n, amp=0.4):
+    out = []
+    while len(out) < n:
+        F = np.eye(3) + amp * rng.uniform(-1, 1, (3, 3))
+        if np.linalg.det(F) > 0.2:
+            out.append(F)
+    return np.array(out)
+
+
+# ------------------------------------------------------------------ element level
+
+def test_polar_batch_matches_oracle(native, oracle):
+    rng = np.random.default_rng(0)
+    F = rand_pd_F(rng, 256)
+    R, s, dsdF, it = native.polar_batch(F)
+    for i in range(F.shape[0]):
+        Ro, so, do, rc, ito = oracle.polar_svd3(F[i])
+        assert it[i] == ito
+        assert np.abs(R[i] - Ro).max() < 1e-12
+        assert np.abs(s[i] - so).max() < 1e-12
+        assert np.abs(dsdF[i] - do).max() < 1e-10 * np.abs(do).max()
+
+
+def test_material_batch_matches_oracle(native, oracle):
+    mu, la = scenes.enu_to_lame(1e6, 0.45)
+    rng = np.random.default_rng(1)
+    S = np.concatenate([np.array([1.11, 1.2, 1.3, 0.2, 0.3, 0.4])[None],
+                        np.tile([1, 1, 1, 0, 0, 0.0], (8, 1)) + 0.3 * rng.uniform(-1, 1, (8, 6)),
+                        np.tile([0.5, 0.6, 0.4, 0, 0, 0.0], (8, 1)) + 0.05 * rng.uniform(-1, 1, (8, 6))])
+    for mtype in (0, 2, 3, 4):
+        for psd in (False, True):
+            E, g, H = native.material_s6_batch(mtype, mu, la, S, psd)
+            for i in range(S.shape[0]):
+                Eo, go, Ho = oracle.material_s6(mtype, mu, la, S[i], psd)
+                assert abs(E[i] - Eo) <= 1e-12 * max(abs(Eo), mu)
+                assert np.abs(g[i] - go).max() <= 1e-12 * max(np.abs(go).max(), mu)
+                assert np.abs(H[i] - Ho).max() <= 1e-9 * np.abs(Ho).max(), (mtype, psd, i)
+
+
+def test_snh_f9_batch_matches_oracle(native, oracle):
+    mu, la = scenes.enu_to_lame(1e6, 0.45)
+    rng = np.random.default_rng(2)
+    F = np.concatenate([rand_pd_F(rng, 16, 0.3), rand_pd_F(rng, 16, 0.8)])
+    F9 = np.transpose(F, (0, 2, 1)).reshape(-1, 9)
+    E, g, H = native.snh_f9_batch(mu, la, F9)
+    for i in range(F9.shape[0]):
+        Eo, go, Ho = oracle.snh_f9(mu, la, F9[i])
+        assert abs(E[i] - Eo) <= 1e-12 * max(abs(Eo), mu)
+        assert np.abs(g[i] - go).max() <= 1e-12 * np.abs(go).max()
+        assert np.abs(H[i] - Ho).max() <= 1e-9 * np.abs(Ho).max()
+
+
+# ------------------------------------------------------------------ phase level
+
+def _pair(scene, precision="fp64"):
+    from oracle import oracle as O
+    from mfem_amd import native as N
+    return O.Sim(scene), N.Sim(scene, precision=precision)
+
+
+def test_setup_maps_bit_exact(native):
+    sc = scenes.beam_scene(5, 3, 2, seed=4, mixed=True)
+    o, g = _pair(sc)
+    o.step()  # materialises the oracle's pattern
+    n = int(o.get("rhs").size)
+    assert np.array_equal(g.get_int("fixed"), sc.fixed)
+    assert np.array_equal(g.get_int("free_map"), o.get_int("free_map", sc.nV))
+    rp = o.get_int("rowptr", n + 1)
+    assert np.array_equal(g.get_int("rowptr"), rp)
+    assert np.array_equal(g.get_int("colind"), o.get_int("colind", rp[-1]))
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_update_solve_energy_phases(native, mixed):
+    sc = scenes.beam_scene(5, 3, 2, seed=4, mixed=mixed, itr_tol=1e-13, max_iters=20000)
+    o, g = _pair(sc)
+    for k in range(2):  # phases of the second Newton iteration of a later step
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < 1e-11
+    o.update_system()
+    g.update_system()
+    if mixed:
+        for name in ("R", "S"):
+            assert relerr(g.get(name), o.get(name)) < 1e-11, name
+        d_o = o.get("dSdF").reshape(-1, 6, 9)  # dSdF column-major 9x6 -> [e][i][c]
+        d_g = g.get("dsdF").reshape(-1, 9, 6).transpose(0, 2, 1)  # [e][c][i] -> [e][i][c]
+        sym = np.array([1, 1, 1, 2, 2, 2.0])
+        assert relerr(d_g * sym[None, :, None], d_o) < 1e-9
+        assert relerr(g.get("g"), o.get("g")) < 1e-10
+        assert relerr(g.get("H"), o.get("H")) < 1e-10
+        assert relerr(g.get("Hinv"), o.get("Hinv")) < 1e-9
+    e0o, e0g = o.energy(0.0), g.energy(0.0)
+    o.solve()
+    dec = g.solve()
+    assert relerr(g.get("lhs"), o.get("lhs")) < 1e-10
+    assert relerr(g.get("rhs"), o.get("rhs")) < 1e-9
+    assert relerr(g.get("dx"), o.get("dx")) < 1e-8
+    if mixed:
+        assert relerr(g.get("gl"), o.get("gl")) < 1e-9
+        assert relerr(g.get("lambda"), o.get("lambda")) < 1e-8
+        assert relerr(g.get("ds"), o.get("ds")) < 1e-7
+    for a in (0.0, 0.5, 1.0):
+        eo, eg = o.energy(a), g.energy(a)
+        assert abs(eg - eo) <= 1e-11 * abs(eo), a
+    assert dec > 0
+
+
+# ------------------------------------------------------------------ whole steps
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_trajectory_fp64(native, mixed):
+    sc = scenes.beam_scene(8, 3, 3, seed=7, mixed=mixed, E_range=(1e5, 1e6), itr_tol=1e-12,
+                           max_iters=20000, max_newton=10, newton_tol=1e-9)
+    o, g = _pair(sc)
+    for k in range(5):
+        o.step()
+        g.step()
+        err = relerr(g.positions(), o.positions())
+        assert err < 1e-9, (k, err)
+        so, sg = o.stats(), g.stats()
+        assert sg["newton_iters"] == so["newton_iters"], (k, so, sg)
+
+
+def test_trajectory_default_tolerances(native):
+    """Reference defaults (CG tol 1e-4, 500 iterations, 5 Newton iterations)."""
+    sc = scenes.beam_scene(10, 3, 3, seed=8, mixed=True)
+    o, g = _pair(sc)
+    for k in range(4):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < 1e-7, k
+
+
+def test_mixed_precision_within_1e6(native):
+    sc = scenes.beam_scene(10, 4, 4, seed=9, mixed=True, E_range=(1e5, 1e6), itr_tol=1e-10,
+                           max_iters=20000, max_newton=20, newton_tol=1e-8)
+    o, g = _pair(sc, precision="mixed")
+    for k in range(4):
+        o.step()
+        g.step()
+        x_o = o.positions()
+        err = np.linalg.norm(g.positions() - x_o) / np.linalg.norm(x_o)
+        assert err < 1e-6, (k, err)
+
+
+def test_golden_trajectories_gpu(native):
+    from tests.golden import make_golden
+    data = np.load(os.path.join(GOLDEN, "trajectories.npz"))
+    for name in ("beam_mixed_pcg",):
+        sc, steps = make_golden.CASES[name]()
+        g = native.Sim(sc)
+        for k in range(steps):
+            g.step()
+            ref = data[f"{name}_x{k}"]
+            assert relerr(g.positions(), ref) < 1e-9, (name, k)
+
+
+def test_deterministic_reruns(native):
+    sc = scenes.beam_scene(6, 3, 3, seed=10, mixed=True)
+    a = native.Sim(sc)
+    b = native.Sim(sc)
+    for _ in range(2):
+        a.step()
+        b.step()
+    assert np.array_equal(a.positions(), b.positions())
+
+
+def test_json_scene_matches_array_scene(native, oracle):
+    path = os.path.join(GOLDEN, "beam_scene.json")
+    g = native.Sim(json_path=path)
+    sc = scenes.scene_from_json(path)
+    o = oracle.Sim(sc)
+    for k in range(3):
+        g.step()
+        o.step()
+        assert relerr(g.positions(), o.positions()) < 1e-9, k
+
+
+def test_errors_are_reported(native):
+    sc = scenes.beam_scene(2, 1, 1, mixed=True, solver="eigen-llt")
+    with pytest.raises(native.MfemError, match="no device implementation"):
+        native.Sim(sc)
+    bad = scenes.beam_scene(2, 1, 1, mixed=True)
+    bad.T = bad.T.copy()
+    bad.T[0, 0] = 10 ** 6
+    with pytest.raises(native.MfemError, match="out of range"):
+        native.Sim(bad)
