@@ -1,0 +1,175 @@
+"""Benchmark of the per-Newton-step mixed-FEM hot path on one MI355X.
+
+Workload (N=1): the precision config of the benchmark set -- a jittered
+100x40x40x6 Kuhn beam (960,000 tets), per-tet Young's modulus log-uniform in
+[1e5, 1e6], nu = 0.45, Stable Neo-Hookean, mixed-stretch variables, BDF1
+dt = 1/30, gravity, clamped end, Jacobi CG with the reference defaults
+(tolerance 1e-4, 500 iterations, 5 Newton iterations). A bench "step" is one
+timestep (NewtonOptimizer::step); the metric counts Newton iterations x tets
+per second over the timed timesteps.
+
+The mixed FP32/FP64 mode is the reported `value`; the FP64 device mode runs
+the same timesteps for the comparison; the FP64 CPU oracle is timed on a
+bounded sample (one Newton iteration of the same scene) on the host cores.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nfree, nnzb, vb):
+    """Algorithmic bytes of one BSR(3x3) SpMV: values + block column ids +
+    row pointers + x read once + y written once."""
+    n = 3 * nfree
+    return nnzb * (9 * vb + 4) + 4 * (nfree + 1) + 2 * n * vb
+
+
+def run_gpu(scene, precision, steps, warmup, profile=True):
+    from mfem_amd import native
+    sim = native.Sim(scene, precision=precision)
+    for _ in range(warmup):
+        sim.step()
+    if profile:
+        sim.set_profiling(1)
+    newton = pcg = 0
+    spmv_ms = 0.0
+    spmv_n = 0
+    t0 = time.perf_counter()
+    per_step = []
+    for k in range(steps):
+        ts = time.perf_counter()
+        sim.step()
+        st = sim.stats()
+        per_step.append(time.perf_counter() - ts)
+        print(f"[bench] {precision} step {k}: {per_step[-1] * 1e3:.1f} ms, newton {st['newton_iters']}, "
+              f"cg {st['pcg_iters']}", file=sys.stderr, flush=True)
+        newton += st["newton_iters"]
+        pcg += st["pcg_iters"]
+        spmv_ms += st["spmv_ms"]
+        spmv_n += st["spmv_count"]
+    elapsed = time.perf_counter() - t0
+    nfree, nnzb, vb = sim.matrix_info()
+    x = sim.positions()
+    sim.close()
+    return dict(elapsed=elapsed, newton=newton, pcg=pcg, spmv_ms=spmv_ms, spmv_n=spmv_n,
+                nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st)
+
+
+def run_cpu_sample(scene, threads):
+    """FP64 oracle on one Newton iteration of the same scene (bounded sample)."""
+    from oracle import oracle as O
+    import copy
+    sc = copy.copy(scene)
+    sc.max_newton = 1
+    sim = O.Sim(sc, threads=threads)
+    print(f"[bench] CPU oracle sample on {threads} threads ...", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sim.step()
+    dt = time.perf_counter() - t0
+    st = sim.stats()
+    print(f"[bench] CPU oracle sample: {dt:.2f} s", file=sys.stderr, flush=True)
+    return dt, st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nx", type=int, default=100)
+    ap.add_argument("--ny", type=int, default=40)
+    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mfem_amd import scenes
+    scene = scenes.config_c3(args.nx, args.ny, args.nz)
+    nT = scene.nT
+
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup)
+    fp64 = None if args.no_fp64 else run_gpu(scene, "fp64", args.steps, args.warmup)
+
+    value = mixed["newton"] * nT / mixed["elapsed"]
+    ms_per_step = 1000.0 * mixed["elapsed"] / args.steps
+    bytes_spmv = spmv_bytes(mixed["nfree"], mixed["nnzb"], mixed["vb"])
+    avg_spmv_ms = mixed["spmv_ms"] / max(mixed["spmv_n"], 1)
+    achieved = bytes_spmv / (avg_spmv_ms * 1e-3) / 1e9 if avg_spmv_ms > 0 else 0.0
+
+    extra = {}
+    if fp64 is not None:
+        v64 = fp64["newton"] * nT / fp64["elapsed"]
+        x64 = fp64["positions"]
+        err = float(np.linalg.norm(mixed["positions"] - x64) / np.linalg.norm(x64))
+        avg64 = fp64["spmv_ms"] / max(fp64["spmv_n"], 1)
+        b64 = spmv_bytes(fp64["nfree"], fp64["nnzb"], fp64["vb"])
+        extra["fp64_gpu"] = {
+            "value": v64, "ms_per_step": 1000.0 * fp64["elapsed"] / args.steps,
+            "newton_iters": fp64["newton"], "pcg_iters": fp64["pcg"],
+            "spmv_gbs": b64 / (avg64 * 1e-3) / 1e9 if avg64 > 0 else 0.0,
+            "spmv_ms": avg64}
+        extra["mixed_over_fp64"] = value / v64 if v64 > 0 else None
+        extra["mixed_vs_fp64_rel_pos_err"] = err
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        dt_cpu, st_cpu = run_cpu_sample(scene, threads)
+        cpu_value = st_cpu["newton_iters"] * nT / dt_cpu
+        cpu = {"value": cpu_value, "unit": "Newton-steps*tets/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle, 1 Newton iteration (1 timestep, max_newton=1) of the same "
+                         f"{nT}-tet scene: {dt_cpu:.2f} s, {st_cpu['pcg_iters_total']} CG iterations"}
+        extra["gpu_over_cpu"] = value / cpu_value if cpu_value > 0 else None
+
+    out = {
+        "metric": "Newton-steps*tets/s at 1M tets, FP32-mixed vs FP64 GPU; PCG HBM GB/s",
+        "value": value,
+        "unit": "Newton-steps*tets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-stretch SNH jittered Kuhn beam {args.nx}x{args.ny}x{args.nz}x6 "
+                               f"({nT} tets), per-tet E log-uniform [1e5,1e6], nu 0.45, BDF1 dt 1/30, "
+                               f"Jacobi CG tol 1e-4 / 500 its, 5 Newton its",
+                   "tets": nT, "free_dofs": 3 * mixed["nfree"], "bsr_blocks": mixed["nnzb"],
+                   "precision": "mixed"},
+        "newton_iters": mixed["newton"],
+        "pcg_iters": mixed["pcg"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pcg_spmv (BSR 3x3 SpMV + p.q)", "bytes_per_launch": bytes_spmv,
+                     "avg_launch_ms": avg_spmv_ms, "launches": mixed["spmv_n"]},
+        "cpu_baseline": cpu,
+        "phase_ms_last_step": {k: mixed["last"][k] for k in ("update_ms", "solve_ms", "ls_ms")},
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

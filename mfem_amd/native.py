@@ -58,6 +58,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise MfemError(f"{LIB_PATH} is missing: build it with `python -m mfem_amd.build`")
+        # One HIP runtime per process: torch bundles its own libamdhip64 (same
+        # soname). Loading torch first lets this library bind to that instance
+        # by soname; loading it second would bring up a second runtime, and
+        # whichever of the two initialises later finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.mfem_last_error.restype = C.c_char_p
