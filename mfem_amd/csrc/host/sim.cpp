@@ -21,8 +21,6 @@ void Reducer::init(long max_items) {
   stride_ = std::max<long>(2 * g + 64, 4096);
   partials_.resize((size_t)kSlots * stride_);
   results_.resize(kSlots);
-  counters_.resize(kSlots);
-  counters_.zero();
   results_.zero();
   pinned_ = std::make_unique<PinnedBuffer>(256);
   MFEM_HIP_CHECK(hipDeviceSynchronize());
@@ -31,7 +29,6 @@ void Reducer::init(long max_items) {
 ReduceSlot Reducer::slot(int k) {
   ReduceSlot r;
   r.partials = partials_.data() + (size_t)k * stride_;
-  r.counter = counters_.data() + k;
   r.result = results_.data() + k;
   return r;
 }
@@ -367,10 +364,12 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   B.x = x;
   B.r = r_.template as<T>();
   B.z = z_.template as<T>();
-  B.p = p_.template as<T>();
+  B.p[0] = p_.template as<T>();
+  B.p[1] = p2_.template as<T>();
   B.q = q_.template as<T>();
   B.ctrl = ctrl_.data();
-  for (int k = 0; k < 3; ++k) B.red[k] = c->reducer.slot(RS_PCG0 + k);
+  B.parts_pq = c->reducer.slot(RS_PCG0).partials;
+  B.parts_rz = c->reducer.slot(RS_PCG1).partials;
   launch_pcg_init(B, tol, max_iters, st);
   const int poll = std::max(1, this->state_->config_->pcg_poll);
   if (this->profile_spmv && (int)ev_.size() < 2 * poll) {
@@ -383,18 +382,19 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   while (true) {
     int chunk = std::min(poll, std::max(1, max_iters + 1 - launched));
     for (int j = 0; j < chunk; ++j) {
+      const int it = launched + j;
       if (this->profile_spmv)
-        launch_pcg_iteration(B, st, ev_[2 * j], ev_[2 * j + 1]);
+        launch_pcg_iteration(B, it, st, ev_[2 * j], ev_[2 * j + 1]);
       else
-        launch_pcg_iteration(B, st);
+        launch_pcg_iteration(B, it, st);
     }
     launched += chunk;
     MFEM_HIP_CHECK(hipGetLastError());
     MFEM_HIP_CHECK(hipMemcpyAsync(h, ctrl_.data(), sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
     MFEM_HIP_CHECK(hipStreamSynchronize(st));
     if (this->profile_spmv) {
-      // iterations that ran their SpMV: those before convergence was detected
-      int ran = std::min(chunk, std::max(0, h->iter + (h->done ? 1 : 0) - (launched - chunk)));
+      // SpMV launches that did work: those before the stopping test fired
+      int ran = h->done ? std::min(chunk, std::max(0, h->stop_it - (launched - chunk))) : chunk;
       for (int j = 0; j < ran; ++j) {
         float ms = 0;
         MFEM_HIP_CHECK(hipEventElapsedTime(&ms, ev_[2 * j], ev_[2 * j + 1]));
@@ -422,6 +422,7 @@ void PcgSolver<DIM>::solve() {
   r_.resize(n, f32);
   z_.resize(n, f32);
   p_.resize(n, f32);
+  p2_.resize(n, f32);
   q_.resize(n, f32);
   this->stats_ = SolveStats();
   auto& R = c->reducer;

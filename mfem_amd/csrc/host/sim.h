@@ -37,7 +37,6 @@ class Reducer {
  private:
   long stride_ = 0;
   DeviceArray<double> partials_, results_;
-  DeviceArray<unsigned> counters_;
   std::unique_ptr<PinnedBuffer> pinned_;
 };
 
@@ -264,7 +263,7 @@ class PcgSolver : public LinearSolver<DIM> {
   int run_cg(const double* b, double* x, double tol, int max_iters);
   SystemMatrixPD<DIM> system_;
   DeviceArray<double> x_, r64_, dx64_;
-  RealArray r_, z_, p_, q_;
+  RealArray r_, z_, p_, p2_, q_;
   DeviceArray<mfemhip::PcgCtrl> ctrl_;
   std::unique_ptr<PinnedBuffer> pinned_;
   std::vector<hipEvent_t> ev_;

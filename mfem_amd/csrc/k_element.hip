@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
       local_max = fmax(local_max, fabs(a));
     }
   }
-  block_reduce_finalize<kElemBlock, 1>(local_max, max_ds, sm);
+  block_partial<kElemBlock, 1>(local_max, max_ds.partials, sm);
 }
 
 __global__ __launch_bounds__(kElemBlock) void k_mixed_energy(
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_energy(
     double psi = material_energy(m.mat[e], m.mu[e], m.la[e], sa);
     en = e_in + h2 * vol * psi - vol * ec;
   }
-  block_reduce_finalize<kElemBlock, 0>(en, out, sm);
+  block_partial<kElemBlock, 0>(en, out.partials, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kElemBlock) void k_plain_energy(
       }
     en = e_in + h2 * vol * snh_f9_energy(m.mu[e], m.la[e], F);
   }
-  block_reduce_finalize<kElemBlock, 0>(en, out, sm);
+  block_partial<kElemBlock, 0>(en, out.partials, sm);
 }
 
 // ---------------------------------------------------------------------------
@@ -637,6 +637,7 @@ void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real*
                               hipStream_t st) {
   hipLaunchKernelGGL(k_mixed_local_solve<Real>, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st,
                      m, dx, dsdF, Hpsi, Hinv, g, gl, lam, ds, max_ds);
+  launch_finalize(max_ds.partials, reduce_grid(m.nT), 1, max_ds.result, st);
 }
 
 void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double* dx, double alpha,
@@ -644,6 +645,7 @@ void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double
                          const double* lam, double h2, ReduceSlot out, hipStream_t st) {
   hipLaunchKernelGGL(k_mixed_energy, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st, m, x_full,
                      dx, alpha, in, R, s, ds, lam, h2, out);
+  launch_finalize(out.partials, reduce_grid(m.nT), 0, out.result, st);
 }
 
 template <typename Real>
@@ -658,6 +660,7 @@ void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double
                          const Inertia& in, double h2, ReduceSlot out, hipStream_t st) {
   hipLaunchKernelGGL(k_plain_energy, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st, m, x_full,
                      dx, alpha, in, h2, out);
+  launch_finalize(out.partials, reduce_grid(m.nT), 0, out.result, st);
 }
 
 void launch_batch_polar(int n, const double* F, double* R, double* s, double* dsdF, int* iters,

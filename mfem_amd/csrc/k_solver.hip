@@ -13,6 +13,7 @@
 // is three kernels that exit at once when the solve has converged, so the
 // host only polls the control block every few iterations.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cfloat>
 #include "kernels.h"
 #include "reduce.h"
@@ -94,21 +95,23 @@ __global__ void k_assemble_rhs(int nfree, const int* __restrict__ inc_ptr, const
   rhs[3 * i + 2] = a2;
 }
 
+
 // ---------------------------------------------------------------------------
 // BSR 3x3 SpMV: G lanes cooperate on one block row; lane l takes blocks
 // l, l+G, ... of the row, then the 3 partial row sums are combined with
-// cross-lane shuffles. Consecutive lanes read consecutive 3x3 blocks.
-template <int G, typename MT, typename VT>
+// cross-lane shuffles. Consecutive lanes read consecutive 3x3 blocks, so a
+// wavefront streams the value array contiguously. `xv(c, k)` supplies
+// component k of the input at block column c.
+template <int G, typename MT, typename XF>
 __device__ __forceinline__ void bsr_row(int row, int lane, const int* __restrict__ rowptr,
                                         const int* __restrict__ col, const MT* __restrict__ val,
-                                        const VT* __restrict__ x, double& y0, double& y1,
-                                        double& y2) {
+                                        XF xv, double& y0, double& y1, double& y2) {
   double a0 = 0, a1 = 0, a2 = 0;
   const int p1 = rowptr[row + 1];
   for (int p = rowptr[row] + lane; p < p1; p += G) {
     const int c = col[p];
     const MT* v = val + (size_t)9 * p;
-    const double x0 = (double)x[3 * c], x1 = (double)x[3 * c + 1], x2 = (double)x[3 * c + 2];
+    const double x0 = xv(c, 0), x1 = xv(c, 1), x2 = xv(c, 2);
     a0 += (double)v[0] * x0 + (double)v[1] * x1 + (double)v[2] * x2;
     a1 += (double)v[3] * x0 + (double)v[4] * x1 + (double)v[5] * x2;
     a2 += (double)v[6] * x0 + (double)v[7] * x1 + (double)v[8] * x2;
@@ -132,7 +135,8 @@ __global__ __launch_bounds__(kBS) void k_spmv(int nrows, const int* __restrict__
   const int lane = threadIdx.x & (G - 1);
   if (row >= nrows) return;
   double y0, y1, y2;
-  bsr_row<G>(row, lane, rowptr, col, val, x, y0, y1, y2);
+  bsr_row<G>(row, lane, rowptr, col, val,
+             [&](int c, int k) { return (double)x[3 * c + k]; }, y0, y1, y2);
   if (lane == 0) {
     y[3 * row] = (VT)y0;
     y[3 * row + 1] = (VT)y1;
@@ -140,20 +144,22 @@ __global__ __launch_bounds__(kBS) void k_spmv(int nrows, const int* __restrict__
   }
 }
 
+// r = b - K x (FP64 accumulation) with per-block partials of |r|^2.
 template <int G, typename MT>
 __global__ __launch_bounds__(kBS) void k_residual(int nrows, const int* __restrict__ rowptr,
                                                   const int* __restrict__ col,
                                                   const MT* __restrict__ val,
                                                   const double* __restrict__ x,
                                                   const double* __restrict__ b,
-                                                  double* __restrict__ r, ReduceSlot r2) {
+                                                  double* __restrict__ r, double* partials) {
   __shared__ double sm[kBS / kWave];
   const int row = (blockIdx.x * kBS + threadIdx.x) / G;
   const int lane = threadIdx.x & (G - 1);
   double acc = 0;
   if (row < nrows) {
     double y0, y1, y2;
-    bsr_row<G>(row, lane, rowptr, col, val, x, y0, y1, y2);
+    bsr_row<G>(row, lane, rowptr, col, val, [&](int c, int k) { return x[3 * c + k]; }, y0, y1,
+               y2);
     if (lane == 0) {
       double r0 = b[3 * row] - y0, r1 = b[3 * row + 1] - y1, rr2 = b[3 * row + 2] - y2;
       r[3 * row] = r0;
@@ -162,163 +168,172 @@ __global__ __launch_bounds__(kBS) void k_residual(int nrows, const int* __restri
       acc = r0 * r0 + r1 * r1 + rr2 * rr2;
     }
   }
-  block_reduce_finalize<kBS, 0>(acc, r2, sm);
+  block_partial<kBS, 0>(acc, partials, sm);
 }
 
-// q = K p, and alpha = rz / (p . q) finalised by the last block.
+__global__ __launch_bounds__(kBS) void k_finalize(const double* __restrict__ partials, int n,
+                                                  int op, double* result) {
+  __shared__ double sm[kBS / kWave];
+  double r = op == 0 ? reduce_array<kBS, 0>(partials, n, 1, 0, sm)
+                     : reduce_array<kBS, 1>(partials, n, 1, 0, sm);
+  if (threadIdx.x == 0) *result = r;
+}
+
+// ---------------------------------------------------------------------------
+// CG, two kernels per iteration `it` (p_old = p[it & 1], p_new = p[(it+1) & 1]):
+//   k_cg_spmv:  reduce the (r.r, r.z) partials of the previous update;
+//               stop if |r|^2 < threshold or it == max_iters (Eigen's loop
+//               test); beta = rz / rz_prev (0 at it == 0);
+//               p_new = z + beta p_old (formed on the fly for the gathered
+//               columns, stored by the row owner); q = K p_new; partial p.q
+//   k_cg_update: alpha = rz / (p.q); x += alpha p; r -= alpha q; z = D^-1 r;
+//               partials of r.r and r.z
+// Every block of a kernel reduces the previous kernel's partials itself, in
+// the same fixed order, so all blocks take the same branch without any
+// inter-block communication inside a kernel.
 template <int G, typename MT, typename VT>
-__global__ __launch_bounds__(kBS) void k_pcg_spmv(int nrows, const int* __restrict__ rowptr,
-                                                  const int* __restrict__ col,
-                                                  const MT* __restrict__ val,
-                                                  const VT* __restrict__ p, VT* __restrict__ q,
-                                                  PcgCtrl* ctrl, ReduceSlot red) {
+__global__ __launch_bounds__(kBS) void k_cg_spmv(int it, int nrows, const int* __restrict__ rowptr,
+                                                 const int* __restrict__ col,
+                                                 const MT* __restrict__ val,
+                                                 const VT* __restrict__ z,
+                                                 const VT* __restrict__ p_old,
+                                                 VT* __restrict__ p_new, VT* __restrict__ q,
+                                                 PcgCtrl* ctrl, const double* __restrict__ parts_rz,
+                                                 int np_rz, double* __restrict__ parts_pq) {
   __shared__ double sm[kBS / kWave];
   if (ctrl->done) return;
-  const int row = (blockIdx.x * kBS + threadIdx.x) / G;
+  const double rr = reduce_array<kBS, 0>(parts_rz, np_rz, 2, 0, sm);
+  const double rz = reduce_array<kBS, 0>(parts_rz, np_rz, 2, 1, sm);
+  const bool converged = rr < ctrl->threshold;
+  const bool exhausted = it >= ctrl->max_iters;
+  if (converged || exhausted) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctrl->rr = rr;
+      ctrl->iter = converged ? (it > 0 ? it - 1 : 0) : it;
+      ctrl->stop_it = it;  // SpMV launches 0 .. it-1 did work
+      ctrl->done = 1;
+    }
+    return;
+  }
+  const double beta = it == 0 ? 0.0 : rz / ctrl->rz[(it - 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctrl->rz[it & 1] = rz;
+    ctrl->rr = rr;
+  }
   const int lane = threadIdx.x & (G - 1);
+  const int rows_per_pass = gridDim.x * (kBS / G);
   double acc = 0;
-  if (row < nrows) {
+  for (int row = (blockIdx.x * kBS + threadIdx.x) / G; row < nrows; row += rows_per_pass) {
     double y0, y1, y2;
-    bsr_row<G>(row, lane, rowptr, col, val, p, y0, y1, y2);
+    bsr_row<G>(row, lane, rowptr, col, val,
+               [&](int c, int k) {
+                 return (double)(VT)((double)z[3 * c + k] + beta * (double)p_old[3 * c + k]);
+               },
+               y0, y1, y2);
     if (lane == 0) {
+      const VT pn0 = (VT)((double)z[3 * row] + beta * (double)p_old[3 * row]);
+      const VT pn1 = (VT)((double)z[3 * row + 1] + beta * (double)p_old[3 * row + 1]);
+      const VT pn2 = (VT)((double)z[3 * row + 2] + beta * (double)p_old[3 * row + 2]);
+      p_new[3 * row] = pn0;
+      p_new[3 * row + 1] = pn1;
+      p_new[3 * row + 2] = pn2;
       const VT q0 = (VT)y0, q1 = (VT)y1, q2 = (VT)y2;
       q[3 * row] = q0;
       q[3 * row + 1] = q1;
       q[3 * row + 2] = q2;
-      acc = (double)p[3 * row] * (double)q0 + (double)p[3 * row + 1] * (double)q1 +
-            (double)p[3 * row + 2] * (double)q2;
+      acc += (double)pn0 * (double)q0 + (double)pn1 * (double)q1 + (double)pn2 * (double)q2;
     }
   }
-  if (block_reduce_finalize<kBS, 0>(acc, red, sm) && threadIdx.x == 0) {
-    ctrl->pq = *red.result;
-    ctrl->alpha = ctrl->rz / ctrl->pq;
-  }
+  block_partial<kBS, 0>(acc, parts_pq, sm);
 }
 
-// Two-value deterministic reduction finalised by one block (rr and rz).
-template <int BS>
-__device__ __forceinline__ bool block_reduce2_finalize(double v0, double v1, double* partials,
-                                                       unsigned* counter, double* sm,
-                                                       double& t0, double& t1) {
-  __shared__ int is_last;
-  double a = block_reduce<BS, 0>(v0, sm);
-  double b = block_reduce<BS, 0>(v1, sm);
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = a;
-    partials[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    unsigned ticket = atomicAdd(counter, 1u);
-    is_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double s0 = 0, s1 = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-    s0 += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s1 += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  t0 = block_reduce<BS, 0>(s0, sm);
-  t1 = block_reduce<BS, 0>(s1, sm);
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
-}
-
-// x += alpha p; r -= alpha q; z = D^{-1} r; then (last block) the stopping
-// test on |r|^2, beta = rz_new / rz and the iteration count.
 template <typename VT>
-__global__ __launch_bounds__(kBS) void k_pcg_update(int n, const VT* __restrict__ p,
-                                                    const VT* __restrict__ q,
-                                                    const VT* __restrict__ invdiag,
-                                                    double* __restrict__ x, VT* __restrict__ r,
-                                                    VT* __restrict__ z, PcgCtrl* ctrl,
-                                                    ReduceSlot red) {
+__global__ __launch_bounds__(kBS) void k_cg_update(int it, int n, const VT* __restrict__ p,
+                                                   const VT* __restrict__ q,
+                                                   const VT* __restrict__ invdiag,
+                                                   double* __restrict__ x, VT* __restrict__ r,
+                                                   VT* __restrict__ z, const PcgCtrl* ctrl,
+                                                   const double* __restrict__ parts_pq, int np_pq,
+                                                   double* __restrict__ parts_rz) {
   __shared__ double sm[kBS / kWave];
   if (ctrl->done) return;
-  const double alpha = ctrl->alpha;
-  const int i = blockIdx.x * kBS + threadIdx.x;
+  const double pq = reduce_array<kBS, 0>(parts_pq, np_pq, 1, 0, sm);
+  const double alpha = ctrl->rz[it & 1] / pq;
   double rr = 0, rz = 0;
-  if (i < n) {
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
     x[i] += alpha * (double)p[i];
     const VT ri = (VT)((double)r[i] - alpha * (double)q[i]);
     r[i] = ri;
     const VT zi = invdiag[i] * ri;
     z[i] = zi;
-    rr = (double)ri * (double)ri;
-    rz = (double)ri * (double)zi;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)zi;
   }
-  double t0, t1;
-  if (block_reduce2_finalize<kBS>(rr, rz, red.partials, red.counter, sm, t0, t1) &&
-      threadIdx.x == 0) {
-    ctrl->rr = t0;
-    if (t0 < ctrl->threshold) {
-      ctrl->done = 1;
-    } else {
-      ctrl->rz_new = t1;
-      ctrl->beta = t1 / ctrl->rz;
-      ctrl->rz = t1;
-      ctrl->iter += 1;
-      if (ctrl->iter >= ctrl->max_iters) ctrl->done = 1;
-    }
+  const double a = block_reduce_all<kBS, 0>(rr, sm);
+  const double b = block_reduce_all<kBS, 0>(rz, sm);
+  if (threadIdx.x == 0) {
+    parts_rz[2 * blockIdx.x] = a;
+    parts_rz[2 * blockIdx.x + 1] = b;
   }
 }
 
-// p = z + beta p (skipped when the stopping test fired or the iteration
-// budget ran out; in the latter case Eigen also leaves p unused).
+// r = b, x = 0, z = D^-1 b, p_old = 0; partials of (b.b, r.z).
 template <typename VT>
-__global__ __launch_bounds__(kBS) void k_pcg_dir(int n, const VT* __restrict__ z, VT* __restrict__ p,
-                                                 const PcgCtrl* ctrl) {
-  if (ctrl->done) return;
-  const double beta = ctrl->beta;
-  const int i = blockIdx.x * kBS + threadIdx.x;
-  if (i < n) p[i] = (VT)((double)z[i] + beta * (double)p[i]);
-}
-
-// r = b, x = 0, z = p = D^{-1} b; reductions of r.z, r.r (= b.b).
-template <typename VT>
-__global__ __launch_bounds__(kBS) void k_pcg_init(int n, const double* __restrict__ b,
-                                                  const VT* __restrict__ invdiag,
-                                                  double* __restrict__ x, VT* __restrict__ r,
-                                                  VT* __restrict__ z, VT* __restrict__ p,
-                                                  ReduceSlot red_rr, ReduceSlot red_rz) {
+__global__ __launch_bounds__(kBS) void k_cg_init(int n, const double* __restrict__ b,
+                                                 const VT* __restrict__ invdiag,
+                                                 double* __restrict__ x, VT* __restrict__ r,
+                                                 VT* __restrict__ z, VT* __restrict__ p0,
+                                                 double* __restrict__ parts_rz) {
   __shared__ double sm[kBS / kWave];
-  const int i = blockIdx.x * kBS + threadIdx.x;
   double rr = 0, rz = 0;
-  if (i < n) {
+  for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
     const double bi = b[i];
     x[i] = 0.0;
     const VT ri = (VT)bi;
     r[i] = ri;
     const VT zi = invdiag[i] * ri;
     z[i] = zi;
-    p[i] = zi;
-    rr = bi * bi;
-    rz = (double)ri * (double)zi;
+    p0[i] = (VT)0;
+    rr += bi * bi;
+    rz += (double)ri * (double)zi;
   }
-  block_reduce_finalize<kBS, 0>(rr, red_rr, sm);
-  block_reduce_finalize<kBS, 0>(rz, red_rz, sm);
+  const double a = block_reduce_all<kBS, 0>(rr, sm);
+  const double c = block_reduce_all<kBS, 0>(rz, sm);
+  if (threadIdx.x == 0) {
+    parts_rz[2 * blockIdx.x] = a;
+    parts_rz[2 * blockIdx.x + 1] = c;
+  }
 }
 
-__global__ void k_pcg_init_ctrl(PcgCtrl* ctrl, const double* rr, const double* rz, double tol,
-                                int max_iters) {
-  const double b2 = *rr;
-  ctrl->b2 = b2;
-  ctrl->rr = b2;
-  ctrl->rz = *rz;
-  ctrl->threshold = fmax(tol * tol * b2, DBL_MIN);
-  ctrl->iter = 0;
-  ctrl->max_iters = max_iters;
-  ctrl->alpha = 0;
-  ctrl->beta = 0;
-  ctrl->pq = 0;
-  ctrl->done = (b2 == 0.0 || b2 < ctrl->threshold || max_iters <= 0) ? 1 : 0;
+// |b|^2 and the stopping threshold max(tol^2 |b|^2, DBL_MIN).
+__global__ __launch_bounds__(kBS) void k_cg_setup(PcgCtrl* ctrl, const double* __restrict__ parts_rz,
+                                                  int np, double tol, int max_iters) {
+  __shared__ double sm[kBS / kWave];
+  const double b2 = reduce_array<kBS, 0>(parts_rz, np, 2, 0, sm);
+  if (threadIdx.x == 0) {
+    ctrl->b2 = b2;
+    ctrl->rr = b2;
+    ctrl->threshold = fmax(tol * tol * b2, DBL_MIN);
+    ctrl->iter = 0;
+    ctrl->max_iters = max_iters;
+    ctrl->rz[0] = ctrl->rz[1] = 0.0;
+    ctrl->stop_it = 0;
+    ctrl->done = (b2 == 0.0) ? 1 : 0;  // x = 0 (Eigen returns at once)
+  }
 }
 
-inline int grid(long n, int bs) { return (int)((n + bs - 1) / bs); }
+inline int grid(long n, int bs) { return n > 0 ? (int)((n + bs - 1) / bs) : 1; }
 
 constexpr int kG = 8;  // lanes per BSR block row
 
 }  // namespace
+
+int pcg_spmv_blocks(int nrows) { return std::min(grid((long)nrows * kG, kBS), 2048); }
+int pcg_vec_blocks(int n) { return std::min(grid(n, kBS), 1024); }
+
+void launch_finalize(const double* partials, int n, int op, double* result, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, st, partials, n, op, result);
+}
 
 template <typename Real>
 void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const double* mass,
@@ -354,30 +369,33 @@ void launch_spmv(int nrows, const int* rowptr, const int* col, const MT* val, co
 template <typename MT>
 void launch_residual(int nrows, const int* rowptr, const int* col, const MT* val, const double* x,
                      const double* b, double* r, ReduceSlot r2, hipStream_t st) {
-  hipLaunchKernelGGL((k_residual<kG, MT>), dim3(grid((long)nrows * kG, kBS) > 0 ? grid((long)nrows * kG, kBS) : 1),
-                     dim3(kBS), 0, st, nrows, rowptr, col, val, x, b, r, r2);
+  const int g = grid((long)nrows * kG, kBS);
+  hipLaunchKernelGGL((k_residual<kG, MT>), dim3(g), dim3(kBS), 0, st, nrows, rowptr, col, val, x, b,
+                     r, r2.partials);
+  launch_finalize(r2.partials, g, 0, r2.result, st);
 }
 
 template <typename MT, typename VT>
 void launch_pcg_init(const PcgBuffers<MT, VT>& B, double tol, int max_iters, hipStream_t st) {
-  int g = B.n > 0 ? grid(B.n, kBS) : 1;
-  hipLaunchKernelGGL(k_pcg_init<VT>, dim3(g), dim3(kBS), 0, st, B.n, B.b, B.invdiag, B.x, B.r, B.z,
-                     B.p, B.red[0], B.red[1]);
-  hipLaunchKernelGGL(k_pcg_init_ctrl, dim3(1), dim3(1), 0, st, B.ctrl, B.red[0].result,
-                     B.red[1].result, tol, max_iters);
+  const int gv = pcg_vec_blocks(B.n);
+  hipLaunchKernelGGL(k_cg_init<VT>, dim3(gv), dim3(kBS), 0, st, B.n, B.b, B.invdiag, B.x, B.r, B.z,
+                     B.p[0], B.parts_rz);
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kBS), 0, st, B.ctrl, B.parts_rz, gv, tol, max_iters);
 }
 
 template <typename MT, typename VT>
-void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, hipStream_t st, hipEvent_t ev0,
+void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, int it, hipStream_t st, hipEvent_t ev0,
                           hipEvent_t ev1) {
   const int nrows = B.n / 3;
+  const int gs = pcg_spmv_blocks(nrows), gv = pcg_vec_blocks(B.n);
+  VT* p_old = B.p[it & 1];
+  VT* p_new = B.p[(it + 1) & 1];
   if (ev0) (void)hipEventRecord(ev0, st);
-  hipLaunchKernelGGL((k_pcg_spmv<kG, MT, VT>), dim3(grid((long)nrows * kG, kBS)), dim3(kBS), 0, st,
-                     nrows, B.rowptr, B.col, B.K, B.p, B.q, B.ctrl, B.red[0]);
+  hipLaunchKernelGGL((k_cg_spmv<kG, MT, VT>), dim3(gs), dim3(kBS), 0, st, it, nrows, B.rowptr, B.col,
+                     B.K, B.z, p_old, p_new, B.q, B.ctrl, B.parts_rz, gv, B.parts_pq);
   if (ev1) (void)hipEventRecord(ev1, st);
-  hipLaunchKernelGGL(k_pcg_update<VT>, dim3(grid(B.n, kBS)), dim3(kBS), 0, st, B.n, B.p, B.q,
-                     B.invdiag, B.x, B.r, B.z, B.ctrl, B.red[1]);
-  hipLaunchKernelGGL(k_pcg_dir<VT>, dim3(grid(B.n, kBS)), dim3(kBS), 0, st, B.n, B.z, B.p, B.ctrl);
+  hipLaunchKernelGGL(k_cg_update<VT>, dim3(gv), dim3(kBS), 0, st, it, B.n, p_new, B.q, B.invdiag,
+                     B.x, B.r, B.z, B.ctrl, B.parts_pq, gs, B.parts_rz);
 }
 
 template void launch_assemble_bsr<double>(int, const int*, const int*, const double*, const double*,
@@ -398,9 +416,9 @@ template void launch_pcg_init<double, double>(const PcgBuffers<double, double>&,
                                               hipStream_t);
 template void launch_pcg_init<float, float>(const PcgBuffers<float, float>&, double, int,
                                             hipStream_t);
-template void launch_pcg_iteration<double, double>(const PcgBuffers<double, double>&, hipStream_t,
-                                                   hipEvent_t, hipEvent_t);
-template void launch_pcg_iteration<float, float>(const PcgBuffers<float, float>&, hipStream_t,
+template void launch_pcg_iteration<double, double>(const PcgBuffers<double, double>&, int,
+                                                   hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_pcg_iteration<float, float>(const PcgBuffers<float, float>&, int, hipStream_t,
                                                  hipEvent_t, hipEvent_t);
 
 }  // namespace mfemhip

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBS) void k_maxabs(int n, const double* __restrict_
   __shared__ double sm[kBS / kWave];
   double m = 0;
   for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) m = fmax(m, fabs(v[i]));
-  block_reduce_finalize<kBS, 1>(m, out, sm);
+  block_partial<kBS, 1>(m, out.partials, sm);
 }
 
 __global__ void k_fill(double* p, double v, long n) {
@@ -113,6 +113,7 @@ void launch_reduce_maxabs(int n, const double* v, ReduceSlot out, hipStream_t st
   int g = grid(n, kBS);
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(kBS), 0, st, n, v, out);
+  launch_finalize(out.partials, g, 1, out.result, st);
 }
 
 void launch_fill(double* p, double v, long n, hipStream_t st) {

@@ -21,12 +21,11 @@ struct TetMeshDev {
   double density = 1000.0;
 };
 
-// Deterministic reduction target: per-block partials plus an arrival
-// counter; the last block to arrive reduces the partials in index order.
+// Deterministic reduction target: per-block partials, summed in index order
+// by a one-block finalize kernel into `result`.
 struct ReduceSlot {
-  double* partials = nullptr;   // >= grid size
-  unsigned* counter = nullptr;  // zero-initialised, reset by the last block
-  double* result = nullptr;     // 1 value
+  double* partials = nullptr;  // >= grid size of the reducing kernel
+  double* result = nullptr;    // 1 value
 };
 
 struct Inertia {
@@ -85,9 +84,13 @@ void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const do
                          double* rhs, hipStream_t st);
 
 // --- BSR 3x3 SpMV / Jacobi PCG (reference Eigen ConjugateGradient semantics)
+// Control block of one CG solve. rz[it & 1] holds r.z of iteration it
+// (double-buffered so that a kernel can read the previous value while its
+// block 0 stores the new one).
 struct PcgCtrl {
-  double rz, rz_new, pq, rr, b2, threshold, alpha, beta;
-  int iter, done, max_iters, pad;
+  double rz[2];
+  double rr, b2, threshold;
+  int iter, done, max_iters, stop_it;
 };
 template <typename MT, typename VT>
 void launch_spmv(int nrows, const int* rowptr, const int* col, const MT* val, const VT* x,
@@ -108,18 +111,22 @@ struct PcgBuffers {
   double* x = nullptr;        // solution (FP64 accumulation)
   VT* r = nullptr;
   VT* z = nullptr;
-  VT* p = nullptr;
+  VT* p[2] = {nullptr, nullptr};  // double-buffered search direction
   VT* q = nullptr;
   PcgCtrl* ctrl = nullptr;
-  ReduceSlot red[3];  // independent reduction slots
+  double* parts_pq = nullptr;  // >= pcg_spmv_blocks(n / 3)
+  double* parts_rz = nullptr;  // >= 2 * pcg_vec_blocks(n)
 };
+int pcg_spmv_blocks(int nrows);
+int pcg_vec_blocks(int n);
 template <typename MT, typename VT>
 void launch_pcg_init(const PcgBuffers<MT, VT>& B, double tol, int max_iters, hipStream_t st);
-// One CG iteration (3 kernels). When ev0/ev1 are given they are recorded
-// immediately before and after the SpMV kernel (device-side timing).
+// CG iteration `it` (2 kernels): stopping test and p = z + beta p fused into
+// the SpMV q = K p, then the x / r / z update. When ev0/ev1 are given they
+// are recorded immediately before and after the SpMV kernel.
 template <typename MT, typename VT>
-void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, hipStream_t st, hipEvent_t ev0 = nullptr,
-                          hipEvent_t ev1 = nullptr);
+void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, int it, hipStream_t st,
+                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // --- vectors ---------------------------------------------------------------
 // x_full[free] += alpha * dx ; s += alpha * ds (line-search acceptance,

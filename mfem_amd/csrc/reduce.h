@@ -1,10 +1,11 @@
-// Deterministic block reductions for gfx950 (64-lane wavefronts).
+// Deterministic reductions for gfx950 (64-lane wavefronts), without atomics
+// or device-scope fences.
 //
-// Each block reduces its values in a fixed tree, writes one partial, and
-// counts itself in with an integer atomic. The last block to arrive sums all
-// partials in index order (each lane a fixed strided subset, then the same
-// fixed tree), so the result does not depend on block scheduling. No float
-// atomics are used anywhere.
+// A reducing kernel writes one partial per block (fixed tree inside the
+// block); the kernel boundary makes the partials visible, and either a
+// one-block finalize kernel or every block of the consuming kernel sums them
+// in index order (each lane a fixed strided subset, then the same fixed
+// tree). The result therefore does not depend on block scheduling.
 #pragma once
 #include <hip/hip_runtime.h>
 #include "kernels.h"
@@ -24,49 +25,42 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// op: 0 = sum, 1 = max. Must be called by every thread of the block.
+// op: 0 = sum, 1 = max. Called by every thread of the block; the result is
+// returned to every thread.
 template <int BS, int OP>
-__device__ __forceinline__ double block_reduce(double v, double* sm) {
+__device__ __forceinline__ double block_reduce_all(double v, double* sm) {
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   v = OP == 0 ? wave_sum(v) : wave_max(v);
+  __syncthreads();  // sm may still be read by a previous call
   if (lane == 0) sm[wid] = v;
   __syncthreads();
-  double r = 0;
-  if (threadIdx.x == 0) {
-    r = sm[0];
+  double r = sm[0];
 #pragma unroll
-    for (int w = 1; w < BS / kWave; ++w) r = OP == 0 ? r + sm[w] : fmax(r, sm[w]);
-  }
-  __syncthreads();
-  return r;  // valid in thread 0
+  for (int w = 1; w < BS / kWave; ++w) r = OP == 0 ? r + sm[w] : fmax(r, sm[w]);
+  return r;
 }
 
-// Writes this block's partial; the last block reduces all partials into
-// *out.result. Returns true in every thread of the block that finalised.
+// Block partial: partials[blockIdx.x] = reduction of v over the block.
 template <int BS, int OP>
-__device__ __forceinline__ bool block_reduce_finalize(double v, ReduceSlot out, double* sm) {
-  __shared__ int is_last;
-  double r = block_reduce<BS, OP>(v, sm);
-  if (threadIdx.x == 0) {
-    out.partials[blockIdx.x] = r;
-    __threadfence();
-    unsigned ticket = atomicAdd(out.counter, 1u);
-    is_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double a = OP == 0 ? 0.0 : -INFINITY;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-    double p = __hip_atomic_load(&out.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    a = OP == 0 ? a + p : fmax(a, p);
-  }
-  double tot = block_reduce<BS, OP>(a, sm);
-  if (threadIdx.x == 0) {
-    *out.result = tot;
-    *out.counter = 0u;
-  }
-  return true;
+__device__ __forceinline__ void block_partial(double v, double* partials, double* sm) {
+  double r = block_reduce_all<BS, OP>(v, sm);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
+
+// Fixed-order reduction of n values (stride `stride`, offset `off`) by one
+// block; result returned to every thread.
+template <int BS, int OP>
+__device__ __forceinline__ double reduce_array(const double* a, int n, int stride, int off,
+                                               double* sm) {
+  double acc = OP == 0 ? 0.0 : -INFINITY;
+  for (int i = threadIdx.x; i < n; i += BS) {
+    double v = a[(size_t)i * stride + off];
+    acc = OP == 0 ? acc + v : fmax(acc, v);
+  }
+  return block_reduce_all<BS, OP>(acc, sm);
+}
+
+// Launch the one-block finalize of `n` partials into *result.
+void launch_finalize(const double* partials, int n, int op, double* result, hipStream_t st);
 
 }  // namespace mfemhip
