@@ -63,9 +63,13 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
     elapsed = time.perf_counter() - t0
     nfree, nnzb, vb = sim.matrix_info()
     x = sim.positions()
+    times, chosen, shape = sim.spmv_tuning()
     sim.close()
+    tuning = {"shape_ms": [round(float(t), 5) for t in times], "chosen": chosen,
+              "lanes_per_row": shape[0], "unroll": shape[1], "nontemporal": shape[2]}
     return dict(elapsed=elapsed, newton=newton, pcg=pcg, spmv_ms=spmv_ms, spmv_n=spmv_n,
-                nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st)
+                nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st,
+                tuning=tuning)
 
 
 def run_cpu_sample(scene, threads):
@@ -166,6 +170,7 @@ def main():
                      "avg_launch_ms": avg_spmv_ms, "launches": mixed["spmv_n"]},
         "cpu_baseline": cpu,
         "phase_ms_last_step": {k: mixed["last"][k] for k in ("update_ms", "solve_ms", "ls_ms")},
+        "spmv_autotune": mixed["tuning"],
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

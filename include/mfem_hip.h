@@ -100,9 +100,15 @@ int mfem_sim_set_profiling(mfem_sim* sim, int flags);
 /* Sizes used by the roofline model of the CG SpMV: free vertices, nonzero
  * 3x3 blocks, bytes per stored value. */
 int mfem_sim_matrix_info(mfem_sim* sim, int* nfree, int* nnzb, int* value_bytes);
-/* Time `reps` back-to-back launches of the SpMV kernel on the current K and a
- * device vector, returning the mean device time per launch (ms). */
+/* Time `reps` back-to-back launches of the SpMV kernel (the autotuned shape)
+ * on the current K and a device vector, returning the mean device time per
+ * launch (ms). */
 int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch);
+/* SpMV autotuner result (no counterpart in the reference): mean device time
+ * per launch of every shape (times[count], ms; pass NULL to query count),
+ * the chosen shape, and its lanes-per-row / unroll / non-temporal settings. */
+int mfem_sim_spmv_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
+                         int* unroll, int* nontemporal);
 
 /* Batched MaterialModel evaluation on the 6-vector s (reference
  * energies/material_model.h:29-38): device pointers, n points, s n x 6,

@@ -307,6 +307,7 @@ int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch) {
     auto& m = *S.mesh_;
     hipStream_t st = S.ctx_->stream;
     const int n = 3 * m.nfree();
+    const int var = std::max(pcg->spmv_variant(), 0);
     hipEvent_t a, b;
     MFEM_HIP_CHECK(hipEventCreate(&a));
     MFEM_HIP_CHECK(hipEventCreate(&b));
@@ -315,11 +316,11 @@ int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch) {
       std::vector<float> ones(n, 1.0f);
       x.upload(ones, st);
       mfemhip::launch_spmv<float, float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
-                                         pcg->system().K.f32(), x.data(), y.data(), st);
+                                         pcg->system().K.f32(), x.data(), y.data(), st, var);
       MFEM_HIP_CHECK(hipEventRecord(a, st));
       for (int r = 0; r < reps; ++r)
         mfemhip::launch_spmv<float, float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
-                                           pcg->system().K.f32(), x.data(), y.data(), st);
+                                           pcg->system().K.f32(), x.data(), y.data(), st, var);
       MFEM_HIP_CHECK(hipEventRecord(b, st));
       MFEM_HIP_CHECK(hipEventSynchronize(b));
     } else {
@@ -327,11 +328,11 @@ int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch) {
       std::vector<double> ones(n, 1.0);
       x.upload(ones, st);
       mfemhip::launch_spmv<double, double>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
-                                           pcg->system().K.f64(), x.data(), y.data(), st);
+                                           pcg->system().K.f64(), x.data(), y.data(), st, var);
       MFEM_HIP_CHECK(hipEventRecord(a, st));
       for (int r = 0; r < reps; ++r)
         mfemhip::launch_spmv<double, double>(m.nfree(), m.d_rowptr.data(), m.d_col.data(),
-                                             pcg->system().K.f64(), x.data(), y.data(), st);
+                                             pcg->system().K.f64(), x.data(), y.data(), st, var);
       MFEM_HIP_CHECK(hipEventRecord(b, st));
       MFEM_HIP_CHECK(hipEventSynchronize(b));
     }
@@ -340,6 +341,26 @@ int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch) {
     *ms_per_launch = reps > 0 ? ms / reps : 0.0;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+  });
+}
+
+int mfem_sim_spmv_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
+                         int* unroll, int* nontemporal) {
+  return guarded([&] {
+    auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
+    if (!pcg) throw std::runtime_error("no CG solver");
+    const auto& t = pcg->spmv_tuning_ms();
+    if (count) *count = mfemhip::spmv_variant_count();
+    if (times)
+      for (int i = 0; i < mfemhip::spmv_variant_count(); ++i)
+        times[i] = i < (int)t.size() ? t[i] : 0.0;
+    const int v = pcg->spmv_variant();
+    if (chosen) *chosen = v;
+    int G = 0, U = 0, NT = 0;
+    mfemhip::spmv_variant_shape(std::max(v, 0), &G, &U, &NT);
+    if (lanes) *lanes = G;
+    if (unroll) *unroll = U;
+    if (nontemporal) *nontemporal = NT;
   });
 }
 

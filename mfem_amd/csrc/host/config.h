@@ -116,6 +116,7 @@ struct SimConfig {
   int pcg_poll = 16;         // CG iterations launched between host polls
   double inner_tol = 1e-4;   // relative tolerance of each FP32 inner CG solve (MIXED)
   int max_refinements = 50;  // defect-correction sweeps (MIXED)
+  int spmv_variant = -1;     // BSR SpMV shape; -1: autotune on the first solve
 };
 
 // reference src/config.h:127-135

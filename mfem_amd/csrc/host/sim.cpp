@@ -363,13 +363,13 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   B.b = b;
   B.x = x;
   B.r = r_.template as<T>();
-  B.z = z_.template as<T>();
-  B.p[0] = p_.template as<T>();
-  B.p[1] = p2_.template as<T>();
+  B.zp[0] = p_.template as<T>();
+  B.zp[1] = p2_.template as<T>();
   B.q = q_.template as<T>();
   B.ctrl = ctrl_.data();
   B.parts_pq = c->reducer.slot(RS_PCG0).partials;
   B.parts_rz = c->reducer.slot(RS_PCG1).partials;
+  B.variant = std::max(variant_, 0);
   launch_pcg_init(B, tol, max_iters, st);
   const int poll = std::max(1, this->state_->config_->pcg_poll);
   if (this->profile_spmv && (int)ev_.size() < 2 * poll) {
@@ -408,6 +408,46 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   return h->iter;
 }
 
+// Autotuner: time every SpMV shape on the assembled matrix (device events
+// around `reps` back-to-back launches) and keep the fastest. Runs once per
+// solver, on the first solve; a fixed SimConfig::spmv_variant skips it.
+template <int DIM>
+template <typename T>
+void PcgSolver<DIM>::tune_spmv() {
+  auto* c = this->state_->ctx_.get();
+  auto& m = *this->state_->mesh_;
+  hipStream_t st = c->stream;
+  const int nv = spmv_variant_count();
+  const int forced = this->state_->config_->spmv_variant;
+  if (forced >= 0 && forced < nv) {
+    variant_ = forced;
+    return;
+  }
+  const int reps = 20;
+  hipEvent_t a, b;
+  MFEM_HIP_CHECK(hipEventCreate(&a));
+  MFEM_HIP_CHECK(hipEventCreate(&b));
+  tune_ms_.assign(nv, 0.0);
+  const T* x = system_.invdiag.template as<T>();
+  T* y = q_.template as<T>();
+  for (int v = 0; v < nv; ++v) {
+    launch_spmv<T, T>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.template as<T>(), x,
+                      y, st, v);  // warm-up
+    MFEM_HIP_CHECK(hipEventRecord(a, st));
+    for (int r = 0; r < reps; ++r)
+      launch_spmv<T, T>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.template as<T>(),
+                        x, y, st, v);
+    MFEM_HIP_CHECK(hipEventRecord(b, st));
+    MFEM_HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    MFEM_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    tune_ms_[v] = ms / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  variant_ = (int)(std::min_element(tune_ms_.begin(), tune_ms_.end()) - tune_ms_.begin());
+}
+
 // reference eigen_iterative_solver.h:20-29 (compute + solve, x0 = 0)
 template <int DIM>
 void PcgSolver<DIM>::solve() {
@@ -420,11 +460,14 @@ void PcgSolver<DIM>::solve() {
   const bool f32 = c->fp32();
   x_.resize(n);
   r_.resize(n, f32);
-  z_.resize(n, f32);
-  p_.resize(n, f32);
-  p2_.resize(n, f32);
+  p_.resize(2 * (size_t)n, f32);   // {z, p} pairs, buffer 0
+  p2_.resize(2 * (size_t)n, f32);  // {z, p} pairs, buffer 1
   q_.resize(n, f32);
   this->stats_ = SolveStats();
+  if (variant_ < 0) {
+    if (f32) tune_spmv<float>();
+    else tune_spmv<double>();
+  }
   auto& R = c->reducer;
   if (!f32) {
     this->stats_.iterations = run_cg<double>(system_.rhs.data(), x_.data(), cfg.itr_tol,

@@ -257,13 +257,21 @@ class PcgSolver : public LinearSolver<DIM> {
   ~PcgSolver() override;
   void solve() override;
   SystemMatrixPD<DIM>& system() { return system_; }
+  // SpMV shape chosen by the autotuner (-1 before the first solve) and the
+  // mean device time of every shape on this matrix.
+  int spmv_variant() const { return variant_; }
+  const std::vector<double>& spmv_tuning_ms() const { return tune_ms_; }
 
  private:
   template <typename T>
   int run_cg(const double* b, double* x, double tol, int max_iters);
+  template <typename T>
+  void tune_spmv();
+  int variant_ = -1;
+  std::vector<double> tune_ms_;
   SystemMatrixPD<DIM> system_;
   DeviceArray<double> x_, r64_, dx64_;
-  RealArray r_, z_, p_, p2_, q_;
+  RealArray r_, p_, p2_, q_;  // p_, p2_: double-buffered {z, p} pairs
   DeviceArray<mfemhip::PcgCtrl> ctrl_;
   std::unique_ptr<PinnedBuffer> pinned_;
   std::vector<hipEvent_t> ev_;

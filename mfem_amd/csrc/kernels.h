@@ -92,9 +92,13 @@ struct PcgCtrl {
   double rr, b2, threshold;
   int iter, done, max_iters, stop_it;
 };
+// SpMV shapes (lanes per block row G, blocks per lane per trip U,
+// non-temporal value loads NT), chosen per matrix by the host autotuner.
+int spmv_variant_count();
+void spmv_variant_shape(int variant, int* G, int* U, int* NT);
 template <typename MT, typename VT>
 void launch_spmv(int nrows, const int* rowptr, const int* col, const MT* val, const VT* x,
-                 VT* y, hipStream_t st);
+                 VT* y, hipStream_t st, int variant = 0);
 // y = b - K x in FP64 (defect correction residual)
 template <typename MT>
 void launch_residual(int nrows, const int* rowptr, const int* col, const MT* val,
@@ -110,12 +114,12 @@ struct PcgBuffers {
   const double* b = nullptr;  // right-hand side
   double* x = nullptr;        // solution (FP64 accumulation)
   VT* r = nullptr;
-  VT* z = nullptr;
-  VT* p[2] = {nullptr, nullptr};  // double-buffered search direction
+  VT* zp[2] = {nullptr, nullptr};  // double-buffered {z, p} pairs (2n each)
   VT* q = nullptr;
   PcgCtrl* ctrl = nullptr;
   double* parts_pq = nullptr;  // >= pcg_spmv_blocks(n / 3)
   double* parts_rz = nullptr;  // >= 2 * pcg_vec_blocks(n)
+  int variant = 0;             // SpMV shape
 };
 int pcg_spmv_blocks(int nrows);
 int pcg_vec_blocks(int n);

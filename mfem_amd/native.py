@@ -22,7 +22,8 @@ EXPORTED = [
     "mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system", "mfem_sim_solve",
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
-    "mfem_sim_time_spmv", "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
+    "mfem_sim_time_spmv", "mfem_sim_spmv_tuning", "mfem_material_s6_batch", "mfem_snh_f9_batch",
+    "mfem_polar_batch",
 ]
 
 
@@ -84,6 +85,7 @@ def lib():
         L.mfem_sim_set_profiling.argtypes = [vp, C.c_int]
         L.mfem_sim_matrix_info.argtypes = [vp, ip, ip, ip]
         L.mfem_sim_time_spmv.argtypes = [vp, C.c_int, dp]
+        L.mfem_sim_spmv_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
         L.mfem_material_s6_batch.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mfem_snh_f9_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.mfem_polar_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
@@ -210,6 +212,16 @@ class Sim:
         ms = C.c_double()
         check(lib().mfem_sim_time_spmv(self.h, int(reps), C.byref(ms)))
         return ms.value
+
+    def spmv_tuning(self):
+        """(per-shape mean ms, chosen shape index, (lanes, unroll, nontemporal))."""
+        n = C.c_int()
+        check(lib().mfem_sim_spmv_tuning(self.h, dp(), C.byref(n), None, None, None, None))
+        t = np.zeros(n.value)
+        ch, G, U, NT = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().mfem_sim_spmv_tuning(self.h, t.ctypes.data_as(dp), C.byref(n), C.byref(ch),
+                                         C.byref(G), C.byref(U), C.byref(NT)))
+        return t, ch.value, (G.value, U.value, NT.value)
 
 
 # ---- batched element entry points (torch tensors as device memory) ---------
