@@ -30,11 +30,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(nfree, nnzb, vb):
-    """Algorithmic bytes of one BSR(3x3) SpMV: values + block column ids +
-    row pointers + x read once + y written once."""
+def cg_bytes(nfree, nnzb, vb):
+    """Algorithmic bytes of one CG iteration (one k_cg_iter launch): the BSR
+    3x3 matrix (values + block column ids + row pointers) read once, the
+    per-row state {r, q, p, D^-1} read once and written once, and the FP64
+    solution read and written once."""
     n = 3 * nfree
-    return nnzb * (9 * vb + 4) + 4 * (nfree + 1) + 2 * n * vb
+    return nnzb * (9 * vb + 4) + 4 * (nfree + 1) + n * (2 * 4 * vb + 16)
 
 
 def run_gpu(scene, precision, steps, warmup, profile=True):
@@ -55,7 +57,7 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
         st = sim.stats()
         per_step.append(time.perf_counter() - ts)
         print(f"[bench] {precision} step {k}: {per_step[-1] * 1e3:.1f} ms, newton {st['newton_iters']}, "
-              f"cg {st['pcg_iters']}", file=sys.stderr, flush=True)
+              f"cg {st['pcg_iters']}, sweeps {st['refinements']}", file=sys.stderr, flush=True)
         newton += st["newton_iters"]
         pcg += st["pcg_iters"]
         spmv_ms += st["spmv_ms"]
@@ -63,10 +65,12 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
     elapsed = time.perf_counter() - t0
     nfree, nnzb, vb = sim.matrix_info()
     x = sim.positions()
-    times, chosen, shape = sim.spmv_tuning()
+    times, chosen, shape = sim.cg_tuning()
+    spmv_alone_ms = sim.time_spmv(50)
     sim.close()
-    tuning = {"shape_ms": [round(float(t), 5) for t in times], "chosen": chosen,
-              "lanes_per_row": shape[0], "unroll": shape[1], "nontemporal": shape[2]}
+    tuning = {"ms_per_iteration": [round(float(t), 5) for t in times], "chosen": chosen,
+              "lanes_per_row": shape[0], "unroll": shape[1], "blocks": shape[2],
+              "plain_spmv_ms": spmv_alone_ms}
     return dict(elapsed=elapsed, newton=newton, pcg=pcg, spmv_ms=spmv_ms, spmv_n=spmv_n,
                 nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st,
                 tuning=tuning)
@@ -115,7 +119,7 @@ def main():
 
     value = mixed["newton"] * nT / mixed["elapsed"]
     ms_per_step = 1000.0 * mixed["elapsed"] / args.steps
-    bytes_spmv = spmv_bytes(mixed["nfree"], mixed["nnzb"], mixed["vb"])
+    bytes_spmv = cg_bytes(mixed["nfree"], mixed["nnzb"], mixed["vb"])
     avg_spmv_ms = mixed["spmv_ms"] / max(mixed["spmv_n"], 1)
     achieved = bytes_spmv / (avg_spmv_ms * 1e-3) / 1e9 if avg_spmv_ms > 0 else 0.0
 
@@ -125,12 +129,12 @@ def main():
         x64 = fp64["positions"]
         err = float(np.linalg.norm(mixed["positions"] - x64) / np.linalg.norm(x64))
         avg64 = fp64["spmv_ms"] / max(fp64["spmv_n"], 1)
-        b64 = spmv_bytes(fp64["nfree"], fp64["nnzb"], fp64["vb"])
+        b64 = cg_bytes(fp64["nfree"], fp64["nnzb"], fp64["vb"])
         extra["fp64_gpu"] = {
             "value": v64, "ms_per_step": 1000.0 * fp64["elapsed"] / args.steps,
             "newton_iters": fp64["newton"], "pcg_iters": fp64["pcg"],
-            "spmv_gbs": b64 / (avg64 * 1e-3) / 1e9 if avg64 > 0 else 0.0,
-            "spmv_ms": avg64}
+            "cg_kernel_gbs": b64 / (avg64 * 1e-3) / 1e9 if avg64 > 0 else 0.0,
+            "cg_kernel_ms": avg64}
         extra["mixed_over_fp64"] = value / v64 if v64 > 0 else None
         extra["mixed_vs_fp64_rel_pos_err"] = err
 
@@ -166,11 +170,14 @@ def main():
         "pcg_iters": mixed["pcg"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pcg_spmv (BSR 3x3 SpMV + p.q)", "bytes_per_launch": bytes_spmv,
-                     "avg_launch_ms": avg_spmv_ms, "launches": mixed["spmv_n"]},
+                     "kernel": "k_cg_iter (one CG iteration: BSR 3x3 SpMV, x/r/z/p updates, "
+                               "5 dot-product partials)",
+                     "bytes_per_launch": bytes_spmv, "avg_launch_ms": avg_spmv_ms,
+                     "launches": mixed["spmv_n"],
+                     "timing": "per-workgroup device wall-clock stamps, span of each launch"},
         "cpu_baseline": cpu,
         "phase_ms_last_step": {k: mixed["last"][k] for k in ("update_ms", "solve_ms", "ls_ms")},
-        "spmv_autotune": mixed["tuning"],
+        "cg_autotune": mixed["tuning"],
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

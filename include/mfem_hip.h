@@ -59,7 +59,7 @@ typedef struct {
   double res;             /* last energy residual */
   double rel_residual;    /* |b - K dx| / |b| of the last solve */
   double update_ms, solve_ms, ls_ms, step_ms; /* device time per phase (hipEvents) */
-  double spmv_ms;         /* summed device time of the CG SpMV kernel (when profiled) */
+  double spmv_ms;         /* summed device time of the CG kernels (when profiled) */
   int spmv_count;
 } mfem_step_stats;
 
@@ -93,9 +93,11 @@ int mfem_sim_get_positions(mfem_sim* sim, double* out);
 long mfem_sim_size(mfem_sim* sim, const char* name);
 int mfem_sim_get(mfem_sim* sim, const char* name, double* out);
 /* "rowptr", "colind" (scalar CSR pattern), "free_map", "fixed",
- * "bsr_rowptr", "bsr_col", "slot_ptr", "slots" */
+ * "bsr_rowptr", "bsr_col", "slot_ptr", "slots", "pcg_iters" (CG iterations
+ * of the last solve) */
 int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out);
-/* bit 0: time every CG SpMV launch with hipEvents (summed in the stats) */
+/* bit 0: time every CG kernel launch with per-workgroup device clock
+ * stamps (span of the launch, summed in the stats as spmv_ms/spmv_count) */
 int mfem_sim_set_profiling(mfem_sim* sim, int flags);
 /* Sizes used by the roofline model of the CG SpMV: free vertices, nonzero
  * 3x3 blocks, bytes per stored value. */
@@ -109,6 +111,18 @@ int mfem_sim_time_spmv(mfem_sim* sim, int reps, double* ms_per_launch);
  * the chosen shape, and its lanes-per-row / unroll / non-temporal settings. */
 int mfem_sim_spmv_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
                          int* unroll, int* nontemporal);
+/* CG kernel shape (no counterpart in the reference; the iteration itself
+ * follows Eigen's ConjugateGradient as linear_solver_factory.cpp:144-154
+ * configures it). Force shape `variant` (0 .. count/grids - 1) on a grid of
+ * `blocks` workgroups (a multiple of 8 in [8, 4096], or 0 for the default);
+ * variant -1 restores autotuning on the next solve. */
+int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks);
+/* CG kernel autotuner result: mean device time per iteration of every
+ * (shape, grid) pair (times[count], ms, index shape * grids + grid; NULL to
+ * query count), the chosen index (-1 when untuned or forced off-table), and
+ * the chosen lanes-per-row / unroll / grid. */
+int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
+                       int* unroll, int* blocks);
 
 /* Batched MaterialModel evaluation on the 6-vector s (reference
  * energies/material_model.h:29-38): device pointers, n points, s n x 6,

@@ -115,6 +115,7 @@ std::vector<int> fetch_int(mfem_sim* sim, const std::string& k) {
   if (k == "bsr_col") return m.col_;
   if (k == "slot_ptr") return m.d_slot_ptr.to_host(st);
   if (k == "slots") return m.d_slots.to_host(st);
+  if (k == "pcg_iters") return {sim->opt->solver()->stats().iterations};
   if (k == "rowptr" || k == "colind") {
     std::vector<int> rp(1, 0), ci;
     for (int i = 0; i < m.nfree(); ++i)
@@ -264,7 +265,7 @@ long mfem_sim_size(mfem_sim* sim, const char* name) {
   int rc = guarded([&] {
     const std::string k(name);
     if (k == "rowptr" || k == "colind" || k == "free_map" || k == "fixed" || k == "bsr_rowptr" ||
-        k == "bsr_col" || k == "slot_ptr" || k == "slots")
+        k == "bsr_col" || k == "slot_ptr" || k == "slots" || k == "pcg_iters")
       n = (long)fetch_int(sim, k).size();
     else
       n = (long)fetch(sim, k).size();
@@ -361,6 +362,39 @@ int mfem_sim_spmv_tuning(mfem_sim* sim, double* times, int* count, int* chosen, 
     if (lanes) *lanes = G;
     if (unroll) *unroll = U;
     if (nontemporal) *nontemporal = NT;
+  });
+}
+
+int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks) {
+  return guarded([&] {
+    auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
+    if (!pcg) throw std::runtime_error("no CG solver");
+    pcg->force_cg_shape(variant, blocks);
+  });
+}
+
+int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
+                       int* unroll, int* blocks) {
+  return guarded([&] {
+    auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
+    if (!pcg) throw std::runtime_error("no CG solver");
+    const auto& grids = PcgSolver<3>::cg_grid_options();
+    const int ng = (int)grids.size();
+    const int total = mfemhip::cg_variant_count() * ng;
+    const auto& t = pcg->cg_tuning_ms();
+    if (count) *count = total;
+    if (times)
+      for (int i = 0; i < total; ++i) times[i] = i < (int)t.size() ? t[i] : 0.0;
+    const int v = pcg->cg_variant();
+    int g = -1;
+    for (int i = 0; i < ng; ++i)
+      if (grids[i] == pcg->cg_blocks()) g = i;
+    if (chosen) *chosen = (v >= 0 && g >= 0) ? v * ng + g : -1;
+    int G = 0, U = 0;
+    mfemhip::cg_variant_shape(std::max(v, 0), &G, &U);
+    if (lanes) *lanes = G;
+    if (unroll) *unroll = U;
+    if (blocks) *blocks = pcg->cg_blocks();
   });
 }
 

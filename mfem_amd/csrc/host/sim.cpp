@@ -339,22 +339,44 @@ void SystemMatrixPD<DIM>::post_solve(SimState<DIM>* state, const double* dx) {
 // ============================================================ CG
 template <int DIM>
 PcgSolver<DIM>::PcgSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
-  pinned_ = std::make_unique<PinnedBuffer>(sizeof(PcgCtrl) + 64);
+  pinned_ = std::make_unique<PinnedBuffer>(2 * sizeof(PcgCtrl) + 64);
   ctrl_.resize(1);
+  parts_.resize(2 * (size_t)kCgParts * kCgMaxBlocks);
+  stamps_.resize(2 * 2 * (size_t)kCgMaxBlocks);
+  for (auto& e : poll_ev_) MFEM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int dev = 0, khz = 0;
+  MFEM_HIP_CHECK(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+    wall_khz_ = khz;
 }
 
 template <int DIM>
 PcgSolver<DIM>::~PcgSolver() {
-  for (auto e : ev_) (void)hipEventDestroy(e);
+  for (auto e : poll_ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+template <int DIM>
+const std::vector<int>& PcgSolver<DIM>::cg_grid_options() {
+  static const std::vector<int> g = {256, 512, 1024, 2048};
+  return g;
+}
+
+template <int DIM>
+void PcgSolver<DIM>::force_cg_shape(int variant, int blocks) {
+  if (variant < -1 || variant >= cg_variant_count())
+    throw std::invalid_argument("CG kernel shape out of range");
+  if (blocks != 0 && (blocks < 8 || blocks > kCgMaxBlocks || blocks % 8 != 0))
+    throw std::invalid_argument("CG grid must be 0 or a multiple of 8 in [8, 4096]");
+  cg_variant_ = variant;
+  cg_blocks_ = variant < 0 ? 0 : blocks;
 }
 
 template <int DIM>
 template <typename T>
-int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters) {
-  auto* c = this->state_->ctx_.get();
+PcgBuffers<T> PcgSolver<DIM>::buffers(const double* b, double* x) {
   auto& m = *this->state_->mesh_;
-  hipStream_t st = c->stream;
-  PcgBuffers<T, T> B;
+  PcgBuffers<T> B;
   B.n = 3 * m.nfree();
   B.rowptr = m.d_rowptr.data();
   B.col = m.d_col.data();
@@ -362,55 +384,66 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   B.invdiag = system_.invdiag.template as<T>();
   B.b = b;
   B.x = x;
-  B.r = r_.template as<T>();
-  B.zp[0] = p_.template as<T>();
-  B.zp[1] = p2_.template as<T>();
-  B.q = q_.template as<T>();
+  B.state[0] = state0_.template as<T>();
+  B.state[1] = state1_.template as<T>();
   B.ctrl = ctrl_.data();
-  B.parts_pq = c->reducer.slot(RS_PCG0).partials;
-  B.parts_rz = c->reducer.slot(RS_PCG1).partials;
-  B.variant = std::max(variant_, 0);
-  launch_pcg_init(B, tol, max_iters, st);
-  const int poll = std::max(1, this->state_->config_->pcg_poll);
-  if (this->profile_spmv && (int)ev_.size() < 2 * poll) {
-    for (auto e : ev_) (void)hipEventDestroy(e);
-    ev_.assign(2 * poll, nullptr);
-    for (auto& e : ev_) MFEM_HIP_CHECK(hipEventCreate(&e));
+  B.parts[0] = parts_.data();
+  B.parts[1] = parts_.data() + (size_t)kCgParts * kCgMaxBlocks;
+  if (this->profile_spmv) {
+    B.stamps[0] = stamps_.data();
+    B.stamps[1] = stamps_.data() + 2 * (size_t)kCgMaxBlocks;
   }
-  auto* h = pinned_->as<PcgCtrl>();
-  int launched = 0;
-  while (true) {
-    int chunk = std::min(poll, std::max(1, max_iters + 1 - launched));
-    for (int j = 0; j < chunk; ++j) {
-      const int it = launched + j;
-      if (this->profile_spmv)
-        launch_pcg_iteration(B, it, st, ev_[2 * j], ev_[2 * j + 1]);
-      else
-        launch_pcg_iteration(B, it, st);
-    }
-    launched += chunk;
-    MFEM_HIP_CHECK(hipGetLastError());
-    MFEM_HIP_CHECK(hipMemcpyAsync(h, ctrl_.data(), sizeof(PcgCtrl), hipMemcpyDeviceToHost, st));
-    MFEM_HIP_CHECK(hipStreamSynchronize(st));
-    if (this->profile_spmv) {
-      // SpMV launches that did work: those before the stopping test fired
-      int ran = h->done ? std::min(chunk, std::max(0, h->stop_it - (launched - chunk))) : chunk;
-      for (int j = 0; j < ran; ++j) {
-        float ms = 0;
-        MFEM_HIP_CHECK(hipEventElapsedTime(&ms, ev_[2 * j], ev_[2 * j + 1]));
-        this->stats_.spmv_ms += ms;
-        this->stats_.spmv_count += 1;
-      }
-    }
-    if (h->done) break;
-    if (launched > max_iters + 1) throw std::runtime_error("CG control block did not terminate");
-  }
-  return h->iter;
+  B.variant = std::max(cg_variant_, 0);
+  B.blocks = cg_blocks_ > 0 ? cg_blocks_ : pcg_default_blocks(B.n);
+  return B;
 }
 
-// Autotuner: time every SpMV shape on the assembled matrix (device events
-// around `reps` back-to-back launches) and keep the fastest. Runs once per
-// solver, on the first solve; a fixed SimConfig::spmv_variant skips it.
+template <int DIM>
+template <typename T>
+int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters) {
+  hipStream_t st = this->state_->ctx_->stream;
+  const PcgBuffers<T> B = buffers<T>(b, x);
+  launch_pcg_init(B, tol, max_iters, st);
+  const int poll = std::max(1, this->state_->config_->pcg_poll);
+  const long limit = (long)max_iters + 2;  // kernels 0 .. max_iters + 1
+  auto* snap = pinned_->as<PcgCtrl>();     // two control-block snapshots
+  long launched = 0;
+  int issued = 0, waited = 0;  // snapshots; snapshot k uses slot k & 1
+  auto issue = [&] {
+    const long chunk = std::min<long>(poll, limit - launched);
+    for (long j = 0; j < chunk; ++j) launch_pcg_iteration(B, (int)(launched + j), st);
+    launched += chunk;
+    MFEM_HIP_CHECK(hipGetLastError());
+    const int slot = issued & 1;
+    MFEM_HIP_CHECK(hipMemcpyAsync(snap + slot, ctrl_.data(), sizeof(PcgCtrl),
+                                  hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[slot], st));
+    ++issued;
+  };
+  // One chunk stays queued behind the one being polled, so the device never
+  // waits for the host; kernels queued after the stop exit at once.
+  issue();
+  while (true) {
+    if (launched < limit && issued - waited < 2) issue();
+    const int slot = waited & 1;
+    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[slot]));
+    ++waited;
+    if (snap[slot].done) {
+      last_ = snap[slot];
+      break;
+    }
+    if (waited == issued) throw std::runtime_error("CG control block did not terminate");
+  }
+  if (this->profile_spmv) {
+    this->stats_.spmv_ms += last_.kern_ticks / wall_khz_;
+    this->stats_.spmv_count += last_.kern_launches;
+  }
+  return last_.iter;
+}
+
+// Autotuner: time every plain SpMV shape on the assembled matrix (device
+// events around `reps` back-to-back launches) and keep the fastest. Runs once
+// per solver, on the first solve; a fixed SimConfig::spmv_variant skips it.
 template <int DIM>
 template <typename T>
 void PcgSolver<DIM>::tune_spmv() {
@@ -448,6 +481,55 @@ void PcgSolver<DIM>::tune_spmv() {
   variant_ = (int)(std::min_element(tune_ms_.begin(), tune_ms_.end()) - tune_ms_.begin());
 }
 
+// CG kernel autotuner: runs a few iterations of a CG solve on the current
+// system (tolerance 0, so it does not stop) with every shape and grid, timing
+// back-to-back iterations with device events, and keeps the fastest pair.
+// Systems below kTuneMinRows block rows keep shape 0 and the default grid.
+template <int DIM>
+template <typename T>
+void PcgSolver<DIM>::tune_cg() {
+  constexpr int kTuneMinRows = 20000;
+  auto& m = *this->state_->mesh_;
+  hipStream_t st = this->state_->ctx_->stream;
+  const auto& grids = cg_grid_options();
+  const int nv = cg_variant_count(), ng = (int)grids.size();
+  cg_tune_ms_.assign((size_t)nv * ng, 0.0);
+  cg_variant_ = 0;
+  cg_blocks_ = 0;
+  if (m.nfree() < kTuneMinRows) return;
+  const int reps = 10;
+  hipEvent_t a, b;
+  MFEM_HIP_CHECK(hipEventCreate(&a));
+  MFEM_HIP_CHECK(hipEventCreate(&b));
+  PcgBuffers<T> B = buffers<T>(system_.rhs.data(), x_.data());
+  B.stamps[0] = B.stamps[1] = nullptr;
+  double best = 1e300;
+  for (int v = 0; v < nv; ++v) {
+    for (int g = 0; g < ng; ++g) {
+      B.variant = v;
+      B.blocks = grids[g];
+      launch_pcg_init(B, 0.0, 1 << 30, st);
+      launch_pcg_iteration(B, 0, st);
+      launch_pcg_iteration(B, 1, st);
+      MFEM_HIP_CHECK(hipEventRecord(a, st));
+      for (int it = 2; it < 2 + reps; ++it) launch_pcg_iteration(B, it, st);
+      MFEM_HIP_CHECK(hipEventRecord(b, st));
+      MFEM_HIP_CHECK(hipEventSynchronize(b));
+      float ms = 0;
+      MFEM_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+      const double t = ms / reps;
+      cg_tune_ms_[(size_t)v * ng + g] = t;
+      if (t < best) {
+        best = t;
+        cg_variant_ = v;
+        cg_blocks_ = grids[g];
+      }
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+}
+
 // reference eigen_iterative_solver.h:20-29 (compute + solve, x0 = 0)
 template <int DIM>
 void PcgSolver<DIM>::solve() {
@@ -459,22 +541,24 @@ void PcgSolver<DIM>::solve() {
   const int n = 3 * m.nfree();
   const bool f32 = c->fp32();
   x_.resize(n);
-  r_.resize(n, f32);
-  p_.resize(2 * (size_t)n, f32);   // {z, p} pairs, buffer 0
-  p2_.resize(2 * (size_t)n, f32);  // {z, p} pairs, buffer 1
   q_.resize(n, f32);
+  state0_.resize(4 * (size_t)n, f32);
+  state1_.resize(4 * (size_t)n, f32);
   this->stats_ = SolveStats();
   if (variant_ < 0) {
     if (f32) tune_spmv<float>();
     else tune_spmv<double>();
+  }
+  if (cg_variant_ < 0) {
+    if (f32) tune_cg<float>();
+    else tune_cg<double>();
   }
   auto& R = c->reducer;
   if (!f32) {
     this->stats_.iterations = run_cg<double>(system_.rhs.data(), x_.data(), cfg.itr_tol,
                                              cfg.max_iterative_solver_iters);
     this->stats_.refinements = 1;
-    auto* h = pinned_->as<PcgCtrl>();
-    this->stats_.rel_residual = h->b2 > 0 ? std::sqrt(h->rr / h->b2) : 0.0;
+    this->stats_.rel_residual = last_.b2 > 0 ? std::sqrt(last_.rr / last_.b2) : 0.0;
   } else {
     // FP32 CG sweeps inside FP64 defect correction on the true residual.
     r64_.resize(n);

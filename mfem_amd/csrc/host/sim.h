@@ -257,24 +257,44 @@ class PcgSolver : public LinearSolver<DIM> {
   ~PcgSolver() override;
   void solve() override;
   SystemMatrixPD<DIM>& system() { return system_; }
-  // SpMV shape chosen by the autotuner (-1 before the first solve) and the
-  // mean device time of every shape on this matrix.
+  // Plain SpMV shape chosen by its autotuner (-1 before the first solve) and
+  // the mean device time of every shape on this matrix.
   int spmv_variant() const { return variant_; }
   const std::vector<double>& spmv_tuning_ms() const { return tune_ms_; }
+  // CG kernel shape and grid: picked by timing every (shape, grid) pair on
+  // the first solve of a large system (index v * grids + g in the table), or
+  // forced with force_cg_shape (variant -1 re-enables the tuner; blocks 0
+  // takes the default grid).
+  int cg_variant() const { return cg_variant_; }
+  int cg_blocks() const { return cg_blocks_; }
+  const std::vector<double>& cg_tuning_ms() const { return cg_tune_ms_; }
+  static const std::vector<int>& cg_grid_options();
+  void force_cg_shape(int variant, int blocks);
 
  private:
+  template <typename T>
+  mfemhip::PcgBuffers<T> buffers(const double* b, double* x);
   template <typename T>
   int run_cg(const double* b, double* x, double tol, int max_iters);
   template <typename T>
   void tune_spmv();
+  template <typename T>
+  void tune_cg();
   int variant_ = -1;
   std::vector<double> tune_ms_;
+  int cg_variant_ = -1, cg_blocks_ = 0;
+  std::vector<double> cg_tune_ms_;
   SystemMatrixPD<DIM> system_;
   DeviceArray<double> x_, r64_, dx64_;
-  RealArray r_, p_, p2_, q_;  // p_, p2_: double-buffered {z, p} pairs
+  RealArray q_;                  // plain SpMV output (autotuner)
+  RealArray state0_, state1_;    // CG state {r, q, p, D^-1}, 4 n values each
+  DeviceArray<double> parts_;    // 2 x kCgParts x kCgMaxBlocks partial sums
+  DeviceArray<unsigned long long> stamps_;  // 2 x 2 x kCgMaxBlocks clock stamps
   DeviceArray<mfemhip::PcgCtrl> ctrl_;
-  std::unique_ptr<PinnedBuffer> pinned_;
-  std::vector<hipEvent_t> ev_;
+  mfemhip::PcgCtrl last_{};      // control block at the end of the last CG solve
+  std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots
+  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  double wall_khz_ = 100000.0;   // device wall-clock rate
 };
 
 template <int DIM>

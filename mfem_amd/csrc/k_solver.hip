@@ -10,11 +10,12 @@
 // (linear_solver_factory.cpp:144-154, eigen_iterative_solver.h:20-29):
 // x0 = 0, Jacobi preconditioner, stop when |r|^2 < max(tol^2 |b|^2, DBL_MIN)
 // or after max_iters iterations. Scalars live on the device; every iteration
-// is three kernels that exit at once when the solve has converged, so the
-// host only polls the control block every few iterations.
+// is one kernel that exits at once when the solve has stopped, so the host
+// only polls the control block every few iterations.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cfloat>
+#include <stdexcept>
 #include <utility>
 #include <hip/hip_ext.h>
 #include "kernels.h"
@@ -232,176 +233,192 @@ __global__ __launch_bounds__(kBS) void k_finalize(const double* __restrict__ par
 }
 
 // ---------------------------------------------------------------------------
-// CG, two kernels per iteration `it`. z and p are stored interleaved as
-// pairs zp = {z, p}, double-buffered: iteration `it` reads zp[it & 1] and
-// builds zp[(it+1) & 1].
-//   k_cg_spmv:  reduce the (r.r, r.z) partials of the previous update;
-//               stop if |r|^2 < threshold or it == max_iters (Eigen's loop
-//               test); beta = rz / rz_prev (0 at it == 0);
-//               p_new = z + beta p_old, formed on the fly for the gathered
-//               columns (one paired load gives z and p_old) and stored by
-//               the row owner into zp_next.p; q = K p_new; partial p.q
-//   k_cg_update: alpha = rz / (p.q); x += alpha p; r -= alpha q;
-//               zp_next.z = D^-1 r; partials of r.r and r.z
-// Every block of a kernel reduces the previous kernel's partials itself, in
-// the same fixed order, so all blocks take the same branch without any
-// inter-block communication inside a kernel.
+// CG, one kernel per iteration.
+//
+// Eigen's loop (x0 = 0, r0 = b, z = D^-1 r, p0 = z0):
+//   q = K p; alpha = rz / p.q; x += alpha p; r -= alpha q;
+//   stop if |r|^2 < threshold; z = D^-1 r; beta = rz_new / rz; p = z + beta p.
+// Kernel `it` reads the state {r, q, p, D^-1} of iteration it-1 and the five
+// partial sums its predecessor wrote, then
+//   1. reduces the partials (every block, in the same fixed order) to p.q,
+//      z.q, q.D^-1 q, r.r and r.z of iteration it-1;
+//   2. applies Eigen's stopping test to r_{it-1} (one kernel late: x already
+//      holds x_{it-1}), or stops when it-1 == max_iters;
+//   3. alpha = rz / p.q and beta from the r.z of the coming residual,
+//      (r - alpha q).D^-1(r - alpha q) = rz - 2 alpha z.q + alpha^2 q.D^-1 q;
+//   4. forms r = r - alpha q, z = D^-1 r, p = z + beta p on the fly for every
+//      gathered block column and computes q = K p; the row owner applies
+//      x += alpha p_{it-1} and stores the new state;
+//   5. writes the five partial sums of the new state.
+// Step 3 starts from the exact r.z of r_{it-1} (partial 4), so the scalar
+// recurrence carries the rounding of one step only. Kernel 0 runs the same
+// code with alpha = beta = 0 on the initial state {b, 0, 0, D^-1}.
+//
+// Rows are split into 8 contiguous ranges, one per XCD (workgroups are
+// dispatched round-robin over the 8 XCDs), so the block columns a range
+// gathers stay in that XCD's L2.
+template <typename T>
+struct alignas(4 * sizeof(T)) CgDof {
+  T r, q, p, d;
+};
 
-template <typename VT> struct Pair;
-template <> struct Pair<float> { using T = float2; };
-template <> struct Pair<double> { using T = double2; };
+struct CgShape {
+  int G, U;
+};
+constexpr CgShape kCgShapes[] = {{8, 1}, {8, 2}, {16, 1}, {4, 1}, {4, 2}, {16, 2}};
+constexpr int kNumCgShapes = sizeof(kCgShapes) / sizeof(kCgShapes[0]);
 
-// z + beta p, one rounding (the same expression for gathered and owned rows)
-template <typename VT>
-__device__ __forceinline__ VT pdir(VT z, VT beta, VT p) {
-  return fma(beta, p, z);
+// r = r - alpha q, z = D^-1 r, p = z + beta p (identical for owned and
+// gathered rows)
+template <typename T>
+__device__ __forceinline__ void cg_form(const CgDof<T>& s, T alpha, T beta, T& r, T& z, T& p) {
+  r = fma(-alpha, s.q, s.r);
+  z = s.d * r;
+  p = fma(beta, s.p, z);
 }
 
-// Sums of the two halves [a_0..a_{n-1}, b_0..b_{n-1}] in one pass.
-template <int BS>
-__device__ __forceinline__ void reduce_pair(const double* __restrict__ parts, int n, double& a,
-                                            double& b, double* sm) {
-  double s0 = 0, s1 = 0;
-  for (int i = threadIdx.x; i < n; i += BS) {
-    s0 += parts[i];
-    s1 += parts[n + i];
+// Profiling: the span of the previous CG kernel, from the per-block
+// (start, end) wall-clock stamps it wrote. Called by all threads of block 0.
+__device__ __forceinline__ void cg_account(const unsigned long long* __restrict__ stamps, int nb,
+                                           PcgCtrl* ctrl, double* sm) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = threadIdx.x; i < nb; i += kBS) {
+    lo = fmin(lo, (double)stamps[2 * i]);
+    hi = fmax(hi, (double)stamps[2 * i + 1]);
   }
-  a = block_reduce_all<BS, 0>(s0, sm);
-  b = block_reduce_all<BS, 0>(s1, sm);
-}
-
-template <int G, int U, bool NT, typename MT, typename VT>
-__global__ __launch_bounds__(kBS) void k_cg_spmv(
-    int it, int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
-    const MT* __restrict__ val, const typename Pair<VT>::T* __restrict__ zp,
-    VT* __restrict__ zp_next, VT* __restrict__ q, PcgCtrl* ctrl,
-    const double* __restrict__ parts_rz, int np_rz, double* __restrict__ parts_pq) {
-  __shared__ double sm[kBS / kWave];
-  if (ctrl->done) return;
-  double rr, rz;
-  reduce_pair<kBS>(parts_rz, np_rz, rr, rz, sm);
-  const bool converged = rr < ctrl->threshold;
-  const bool exhausted = it >= ctrl->max_iters;
-  if (converged || exhausted) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      ctrl->rr = rr;
-      ctrl->iter = converged ? (it > 0 ? it - 1 : 0) : it;
-      ctrl->stop_it = it;  // SpMV launches 0 .. it-1 did work
-      ctrl->done = 1;
-    }
-    return;
-  }
-  const double beta = it == 0 ? 0.0 : rz / ctrl->rz[(it - 1) & 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctrl->rz[it & 1] = rz;
-    ctrl->rr = rr;
-  }
-  const VT bv = (VT)beta;
-  const int lane = threadIdx.x & (G - 1);
-  const int rows_per_pass = gridDim.x * (kBS / G);
-  double acc = 0;
-  for (int row = (blockIdx.x * kBS + threadIdx.x) / G; row < nrows; row += rows_per_pass) {
-    MT y0, y1, y2;
-    bsr_row<G, U, NT>(row, lane, rowptr, col, val,
-                      [&](int c, int k) {
-                        const auto v = zp[3 * c + k];
-                        return (MT)pdir(v.x, bv, v.y);
-                      },
-                      y0, y1, y2);
-    if (lane == 0) {
-      const auto v0 = zp[3 * row], v1 = zp[3 * row + 1], v2 = zp[3 * row + 2];
-      const VT pn0 = pdir(v0.x, bv, v0.y), pn1 = pdir(v1.x, bv, v1.y), pn2 = pdir(v2.x, bv, v2.y);
-      // p component of the next pair (the z component is written by the update)
-      zp_next[2 * (3 * row) + 1] = pn0;
-      zp_next[2 * (3 * row + 1) + 1] = pn1;
-      zp_next[2 * (3 * row + 2) + 1] = pn2;
-      const VT q0 = (VT)y0, q1 = (VT)y1, q2 = (VT)y2;
-      q[3 * row] = q0;
-      q[3 * row + 1] = q1;
-      q[3 * row + 2] = q2;
-      acc += (double)pn0 * (double)q0 + (double)pn1 * (double)q1 + (double)pn2 * (double)q2;
-    }
-  }
-  block_partial<kBS, 0>(acc, parts_pq, sm);
-}
-
-template <typename VT>
-__global__ __launch_bounds__(kBS) void k_cg_update(int it, int n,
-                                                   typename Pair<VT>::T* __restrict__ zp_next,
-                                                   const VT* __restrict__ q,
-                                                   const VT* __restrict__ invdiag,
-                                                   double* __restrict__ x, VT* __restrict__ r,
-                                                   const PcgCtrl* ctrl,
-                                                   const double* __restrict__ parts_pq, int np_pq,
-                                                   double* __restrict__ parts_rz) {
-  __shared__ double sm[kBS / kWave];
-  if (ctrl->done) return;
-  const double pq = reduce_array<kBS, 0>(parts_pq, np_pq, 1, 0, sm);
-  const double alpha = ctrl->rz[it & 1] / pq;
-  const VT av = (VT)alpha;
-  double rr = 0, rz = 0;
-  for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
-    auto v = zp_next[i];  // v.y = p_new
-    x[i] += alpha * (double)v.y;
-    const VT ri = r[i] - av * q[i];
-    r[i] = ri;
-    const VT zi = invdiag[i] * ri;
-    v.x = zi;
-    zp_next[i] = v;
-    rr += (double)ri * (double)ri;
-    rz += (double)ri * (double)zi;
-  }
-  const double a = block_reduce_all<kBS, 0>(rr, sm);
-  const double b = block_reduce_all<kBS, 0>(rz, sm);
+  lo = -block_reduce_all<kBS, 1>(-lo, sm);
+  hi = block_reduce_all<kBS, 1>(hi, sm);
   if (threadIdx.x == 0) {
-    parts_rz[blockIdx.x] = a;
-    parts_rz[gridDim.x + blockIdx.x] = b;
+    ctrl->kern_ticks += hi - lo;
+    ctrl->kern_launches += 1;
   }
 }
 
-// r = b, x = 0, zp[0] = {D^-1 b, 0}; partials of (b.b, r.z).
-template <typename VT>
+template <int G, int U, typename T>
+__global__ __launch_bounds__(kBS) void k_cg_iter(
+    int it, int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const T* __restrict__ val, const CgDof<T>* __restrict__ s_old, CgDof<T>* __restrict__ s_new,
+    double* __restrict__ x, PcgCtrl* ctrl, const double* __restrict__ parts_old,
+    double* __restrict__ parts_new, const unsigned long long* __restrict__ stamps_old,
+    unsigned long long* __restrict__ stamps_new) {
+  __shared__ double sm[kCgParts * (kBS / kWave)];
+  const unsigned long long t_start = (unsigned long long)wall_clock64();
+  if (ctrl->done) return;
+  const int nb = gridDim.x;
+  double alpha = 0.0, beta = 0.0;
+  if (it > 0) {
+    double v[kCgParts];
+#pragma unroll
+    for (int k = 0; k < kCgParts; ++k) v[k] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kBS) {
+#pragma unroll
+      for (int k = 0; k < kCgParts; ++k) v[k] += parts_old[k * nb + i];
+    }
+    block_sum_n<kBS, kCgParts>(v, sm);
+    if (stamps_old && blockIdx.x == 0) cg_account(stamps_old, nb, ctrl, sm);
+    const double pq = v[0], zq = v[1], qdq = v[2], rr = v[3], rz = v[4];
+    const int j = it - 1;  // iteration of the partials
+    const bool converged = rr < ctrl->threshold;
+    if (converged || j >= ctrl->max_iters) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->rr = rr;
+        ctrl->iter = converged ? (j > 0 ? j - 1 : 0) : j;
+        ctrl->stop_it = it;
+        ctrl->done = 1;
+      }
+      return;
+    }
+    alpha = rz / pq;
+    beta = (rz - 2.0 * alpha * zq + alpha * alpha * qdq) / rz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->rr = rr;
+  }
+  const T a = (T)alpha, bt = (T)beta;
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int kRowsPerBlock = kBS / G;
+  const int xcd = blockIdx.x & 7, nbx = nb >> 3;
+  const int r0 = (int)((long long)nrows * xcd / 8);
+  const int r1 = (int)((long long)nrows * (xcd + 1) / 8);
+  double acc[kCgParts];
+#pragma unroll
+  for (int k = 0; k < kCgParts; ++k) acc[k] = 0.0;
+  for (int row = r0 + (blockIdx.x >> 3) * kRowsPerBlock + (int)threadIdx.x / G; row < r1;
+       row += nbx * kRowsPerBlock) {
+    T y0, y1, y2;
+    bsr_row<G, U, false>(row, lane, rowptr, col, val,
+                         [&](int c, int k) {
+                           T r, z, p;
+                           cg_form(s_old[3 * c + k], a, bt, r, z, p);
+                           return p;
+                         },
+                         y0, y1, y2);
+    if (lane < 3) {
+      const int i = 3 * row + lane;
+      const CgDof<T> s = s_old[i];
+      T r, z, p;
+      cg_form(s, a, bt, r, z, p);
+      x[i] += alpha * (double)s.p;
+      const T q = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+      CgDof<T> o;
+      o.r = r;
+      o.q = q;
+      o.p = p;
+      o.d = s.d;
+      s_new[i] = o;
+      const double dq = (double)q, dz = (double)z, dr = (double)r;
+      acc[0] += (double)p * dq;
+      acc[1] += dz * dq;
+      acc[2] += dq * ((double)s.d * dq);
+      acc[3] += dr * dr;
+      acc[4] += dr * dz;
+    }
+  }
+  block_sum_n<kBS, kCgParts>(acc, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kCgParts; ++k) parts_new[k * nb + blockIdx.x] = acc[k];
+    if (stamps_new) {
+      stamps_new[2 * blockIdx.x] = t_start;
+      stamps_new[2 * blockIdx.x + 1] = (unsigned long long)wall_clock64();
+    }
+  }
+}
+
+// x = 0, state = {b, 0, 0, D^-1}; partials of b.b
+template <typename T>
 __global__ __launch_bounds__(kBS) void k_cg_init(int n, const double* __restrict__ b,
-                                                 const VT* __restrict__ invdiag,
-                                                 double* __restrict__ x, VT* __restrict__ r,
-                                                 typename Pair<VT>::T* __restrict__ zp0,
-                                                 double* __restrict__ parts_rz) {
+                                                 const T* __restrict__ invdiag,
+                                                 double* __restrict__ x, CgDof<T>* __restrict__ s0,
+                                                 double* __restrict__ parts) {
   __shared__ double sm[kBS / kWave];
-  double rr = 0, rz = 0;
+  double bb = 0;
   for (int i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) {
     const double bi = b[i];
     x[i] = 0.0;
-    const VT ri = (VT)bi;
-    r[i] = ri;
-    const VT zi = invdiag[i] * ri;
-    typename Pair<VT>::T v;
-    v.x = zi;
-    v.y = (VT)0;
-    zp0[i] = v;
-    rr += bi * bi;
-    rz += (double)ri * (double)zi;
+    CgDof<T> o;
+    o.r = (T)bi;
+    o.q = (T)0;
+    o.p = (T)0;
+    o.d = invdiag[i];
+    s0[i] = o;
+    bb += bi * bi;
   }
-  const double a = block_reduce_all<kBS, 0>(rr, sm);
-  const double c = block_reduce_all<kBS, 0>(rz, sm);
-  if (threadIdx.x == 0) {
-    parts_rz[blockIdx.x] = a;
-    parts_rz[gridDim.x + blockIdx.x] = c;
-  }
+  block_partial<kBS, 0>(bb, parts, sm);
 }
 
 // |b|^2 and the stopping threshold max(tol^2 |b|^2, DBL_MIN).
-__global__ __launch_bounds__(kBS) void k_cg_setup(PcgCtrl* ctrl, const double* __restrict__ parts_rz,
+__global__ __launch_bounds__(kBS) void k_cg_setup(PcgCtrl* ctrl, const double* __restrict__ parts,
                                                   int np, double tol, int max_iters) {
   __shared__ double sm[kBS / kWave];
-  const double b2 = reduce_array<kBS, 0>(parts_rz, np, 1, 0, sm);
+  const double b2 = reduce_array<kBS, 0>(parts, np, 1, 0, sm);
   if (threadIdx.x == 0) {
     ctrl->b2 = b2;
     ctrl->rr = b2;
     ctrl->threshold = fmax(tol * tol * b2, DBL_MIN);
+    ctrl->kern_ticks = 0.0;
     ctrl->iter = 0;
     ctrl->max_iters = max_iters;
-    ctrl->rz[0] = ctrl->rz[1] = 0.0;
     ctrl->stop_it = 0;
+    ctrl->kern_launches = 0;
     ctrl->done = (b2 == 0.0) ? 1 : 0;  // x = 0 (Eigen returns at once)
   }
 }
@@ -438,28 +455,6 @@ struct SpmvRun {
   };
 };
 
-template <typename MT, typename VT>
-struct CgSpmvRun {
-  template <int G, int U, bool NT>
-  struct R {
-    static void run(int it, int nrows, const PcgBuffers<MT, VT>& B, int gv, hipStream_t st,
-                    hipEvent_t ev0, hipEvent_t ev1) {
-      using P2 = typename Pair<VT>::T;
-      const int gs = std::min(grid((long)nrows * G, kBS), 2048);
-      const P2* zp = reinterpret_cast<const P2*>(B.zp[it & 1]);
-      VT* zp_next = B.zp[(it + 1) & 1];
-      if (ev0)
-        hipExtLaunchKernelGGL((k_cg_spmv<G, U, NT, MT, VT>), dim3(gs), dim3(kBS), 0, st, ev0, ev1, 0,
-                              it, nrows, B.rowptr, B.col, B.K, zp, zp_next, B.q, B.ctrl, B.parts_rz,
-                              gv, B.parts_pq);
-      else
-        hipLaunchKernelGGL((k_cg_spmv<G, U, NT, MT, VT>), dim3(gs), dim3(kBS), 0, st, it, nrows,
-                           B.rowptr, B.col, B.K, zp, zp_next, B.q, B.ctrl, B.parts_rz, gv,
-                           B.parts_pq);
-    }
-  };
-};
-
 }  // namespace
 
 int spmv_variant_count() { return kNumShapes; }
@@ -470,8 +465,17 @@ void spmv_variant_shape(int v, int* G, int* U, int* NT) {
   *NT = s.NT;
 }
 
-int pcg_spmv_blocks(int nrows) { return 2048; }  // upper bound over all shapes
-int pcg_vec_blocks(int n) { return std::min(grid(n, kBS), 256); }
+int cg_variant_count() { return kNumCgShapes; }
+void cg_variant_shape(int v, int* G, int* U) {
+  const CgShape s = (v >= 0 && v < kNumCgShapes) ? kCgShapes[v] : kCgShapes[0];
+  *G = s.G;
+  *U = s.U;
+}
+
+int pcg_default_blocks(int n) {
+  const int b = grid((long)(n / 3) * 8, kBS);
+  return std::min(std::max((b + 7) / 8 * 8, 8), 1024);
+}
 
 void launch_finalize(const double* partials, int n, int op, double* result, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, st, partials, n, op, result);
@@ -516,28 +520,36 @@ void launch_residual(int nrows, const int* rowptr, const int* col, const MT* val
   launch_finalize(r2.partials, g, 0, r2.result, st);
 }
 
-template <typename MT, typename VT>
-void launch_pcg_init(const PcgBuffers<MT, VT>& B, double tol, int max_iters, hipStream_t st) {
-  using P2 = typename Pair<VT>::T;
-  const int gv = pcg_vec_blocks(B.n);
-  hipLaunchKernelGGL(k_cg_init<VT>, dim3(gv), dim3(kBS), 0, st, B.n, B.b, B.invdiag, B.x, B.r,
-                     reinterpret_cast<P2*>(B.zp[0]), B.parts_rz);
-  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kBS), 0, st, B.ctrl, B.parts_rz, gv, tol, max_iters);
+template <typename T>
+void launch_pcg_init(const PcgBuffers<T>& B, double tol, int max_iters, hipStream_t st) {
+  const int gv = std::min(grid(B.n, kBS), 256);
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(gv), dim3(kBS), 0, st, B.n, B.b, B.invdiag, B.x,
+                     reinterpret_cast<CgDof<T>*>(B.state[0]), B.parts[1]);
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kBS), 0, st, B.ctrl, B.parts[1], gv, tol, max_iters);
 }
 
-template <typename MT, typename VT>
-void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, int it, hipStream_t st, hipEvent_t ev0,
-                          hipEvent_t ev1) {
-  using P2 = typename Pair<VT>::T;
+template <typename T>
+void launch_pcg_iteration(const PcgBuffers<T>& B, int it, hipStream_t st) {
+  if (B.blocks < 8 || B.blocks > kCgMaxBlocks || (B.blocks & 7))
+    throw std::invalid_argument("CG grid must be a multiple of 8 in [8, 4096]");
+  const auto* so = reinterpret_cast<const CgDof<T>*>(B.state[it & 1]);
+  auto* sn = reinterpret_cast<CgDof<T>*>(B.state[(it + 1) & 1]);
+  const double* po = B.parts[(it + 1) & 1];
+  double* pn = B.parts[it & 1];
+  const unsigned long long* to = B.stamps[(it + 1) & 1];
+  unsigned long long* tn = B.stamps[it & 1];
   const int nrows = B.n / 3;
-  const int gv = pcg_vec_blocks(B.n);
-  int G, U, NT;
-  spmv_variant_shape(B.variant, &G, &U, &NT);
-  const int gs = std::min(grid((long)nrows * G, kBS), 2048);
-  dispatch_shape<CgSpmvRun<MT, VT>::template R>(B.variant, it, nrows, B, gv, st, ev0, ev1);
-  hipLaunchKernelGGL(k_cg_update<VT>, dim3(gv), dim3(kBS), 0, st, it, B.n,
-                     reinterpret_cast<P2*>(B.zp[(it + 1) & 1]), B.q, B.invdiag, B.x, B.r, B.ctrl,
-                     B.parts_pq, gs, B.parts_rz);
+  switch (B.variant) {
+#define MFEM_CG(i)                                                                              \
+  case i:                                                                                       \
+    hipLaunchKernelGGL((k_cg_iter<kCgShapes[i].G, kCgShapes[i].U, T>), dim3(B.blocks), dim3(kBS), \
+                       0, st, it, nrows, B.rowptr, B.col, B.K, so, sn, B.x, B.ctrl, po, pn, to, tn); \
+    break;
+    MFEM_CG(0) MFEM_CG(1) MFEM_CG(2) MFEM_CG(3) MFEM_CG(4) MFEM_CG(5)
+#undef MFEM_CG
+    default:
+      throw std::invalid_argument("CG kernel shape out of range");
+  }
 }
 
 template void launch_assemble_bsr<double>(int, const int*, const int*, const double*, const double*,
@@ -554,13 +566,9 @@ template void launch_residual<double>(int, const int*, const int*, const double*
                                       const double*, double*, ReduceSlot, hipStream_t);
 template void launch_residual<float>(int, const int*, const int*, const float*, const double*,
                                      const double*, double*, ReduceSlot, hipStream_t);
-template void launch_pcg_init<double, double>(const PcgBuffers<double, double>&, double, int,
-                                              hipStream_t);
-template void launch_pcg_init<float, float>(const PcgBuffers<float, float>&, double, int,
-                                            hipStream_t);
-template void launch_pcg_iteration<double, double>(const PcgBuffers<double, double>&, int,
-                                                   hipStream_t, hipEvent_t, hipEvent_t);
-template void launch_pcg_iteration<float, float>(const PcgBuffers<float, float>&, int, hipStream_t,
-                                                 hipEvent_t, hipEvent_t);
+template void launch_pcg_init<double>(const PcgBuffers<double>&, double, int, hipStream_t);
+template void launch_pcg_init<float>(const PcgBuffers<float>&, double, int, hipStream_t);
+template void launch_pcg_iteration<double>(const PcgBuffers<double>&, int, hipStream_t);
+template void launch_pcg_iteration<float>(const PcgBuffers<float>&, int, hipStream_t);
 
 }  // namespace mfemhip

@@ -84,13 +84,13 @@ void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const do
                          double* rhs, hipStream_t st);
 
 // --- BSR 3x3 SpMV / Jacobi PCG (reference Eigen ConjugateGradient semantics)
-// Control block of one CG solve. rz[it & 1] holds r.z of iteration it
-// (double-buffered so that a kernel can read the previous value while its
-// block 0 stores the new one).
+// Control block of one CG solve (device memory, read by the host at polls).
 struct PcgCtrl {
-  double rz[2];
   double rr, b2, threshold;
+  double kern_ticks;  // profiling: summed wall-clock ticks of the CG kernels
   int iter, done, max_iters, stop_it;
+  int kern_launches;  // profiling: CG kernels accounted in kern_ticks
+  int pad_;
 };
 // SpMV shapes (lanes per block row G, blocks per lane per trip U,
 // non-temporal value loads NT), chosen per matrix by the host autotuner.
@@ -104,33 +104,40 @@ template <typename MT>
 void launch_residual(int nrows, const int* rowptr, const int* col, const MT* val,
                      const double* x, const double* b, double* r, ReduceSlot r2,
                      hipStream_t st);
-template <typename MT, typename VT>
+
+// One CG iteration is one kernel (see k_solver.hip). Per scalar row the
+// iteration state is the 4-tuple {r, q, p, D^-1} (16 B in FP32, 32 B in
+// FP64), double-buffered; every kernel writes 5 partial sums per block,
+// double-buffered as well.
+constexpr int kCgParts = 5;
+// (lanes per block row, blocks per lane per trip) of the CG kernel shapes
+int cg_variant_count();
+void cg_variant_shape(int variant, int* G, int* U);
+template <typename T>
 struct PcgBuffers {
   int n = 0;  // scalar rows (3 * nfree)
   const int* rowptr = nullptr;
   const int* col = nullptr;
-  const MT* K = nullptr;
-  const VT* invdiag = nullptr;
-  const double* b = nullptr;  // right-hand side
-  double* x = nullptr;        // solution (FP64 accumulation)
-  VT* r = nullptr;
-  VT* zp[2] = {nullptr, nullptr};  // double-buffered {z, p} pairs (2n each)
-  VT* q = nullptr;
+  const T* K = nullptr;
+  const T* invdiag = nullptr;
+  const double* b = nullptr;   // right-hand side
+  double* x = nullptr;         // solution (FP64 accumulation)
+  T* state[2] = {nullptr, nullptr};            // 4 n values each
   PcgCtrl* ctrl = nullptr;
-  double* parts_pq = nullptr;  // >= pcg_spmv_blocks(n / 3)
-  double* parts_rz = nullptr;  // >= 2 * pcg_vec_blocks(n)
-  int variant = 0;             // SpMV shape
+  double* parts[2] = {nullptr, nullptr};       // kCgParts * blocks each
+  unsigned long long* stamps[2] = {nullptr, nullptr};  // 2 * blocks each (profiling) or null
+  int variant = 0;  // CG kernel shape
+  int blocks = 0;   // grid of the CG kernel, a multiple of 8, <= kCgMaxBlocks
 };
-int pcg_spmv_blocks(int nrows);
-int pcg_vec_blocks(int n);
-template <typename MT, typename VT>
-void launch_pcg_init(const PcgBuffers<MT, VT>& B, double tol, int max_iters, hipStream_t st);
-// CG iteration `it` (2 kernels): stopping test and p = z + beta p fused into
-// the SpMV q = K p, then the x / r / z update. When ev0/ev1 are given they
-// are recorded immediately before and after the SpMV kernel.
-template <typename MT, typename VT>
-void launch_pcg_iteration(const PcgBuffers<MT, VT>& B, int it, hipStream_t st,
-                          hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+constexpr int kCgMaxBlocks = 4096;
+// Default grid of the CG kernel for n scalar rows.
+int pcg_default_blocks(int n);
+template <typename T>
+void launch_pcg_init(const PcgBuffers<T>& B, double tol, int max_iters, hipStream_t st);
+// CG kernel `it` (it = 0 forms p_0 = D^-1 b and q_0 = K p_0; kernel it >= 1
+// also applies the update of iteration it-1 and tests its residual).
+template <typename T>
+void launch_pcg_iteration(const PcgBuffers<T>& B, int it, hipStream_t st);
 
 // --- vectors ---------------------------------------------------------------
 // x_full[free] += alpha * dx ; s += alpha * ds (line-search acceptance,

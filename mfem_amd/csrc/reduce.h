@@ -40,6 +40,28 @@ __device__ __forceinline__ double block_reduce_all(double v, double* sm) {
   return r;
 }
 
+// Sums of N values over the block in one LDS round; sm holds N * BS / 64
+// doubles. Every thread gets the N sums.
+template <int BS, int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* sm) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) sm[k * (BS / kWave) + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double r = sm[k * (BS / kWave)];
+#pragma unroll
+    for (int w = 1; w < BS / kWave; ++w) r += sm[k * (BS / kWave) + w];
+    v[k] = r;
+  }
+}
+
 // Block partial: partials[blockIdx.x] = reduction of v over the block.
 template <int BS, int OP>
 __device__ __forceinline__ void block_partial(double v, double* partials, double* sm) {

@@ -22,8 +22,8 @@ EXPORTED = [
     "mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system", "mfem_sim_solve",
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
-    "mfem_sim_time_spmv", "mfem_sim_spmv_tuning", "mfem_material_s6_batch", "mfem_snh_f9_batch",
-    "mfem_polar_batch",
+    "mfem_sim_time_spmv", "mfem_sim_spmv_tuning", "mfem_sim_set_cg_shape", "mfem_sim_cg_tuning",
+    "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
 ]
 
 
@@ -86,6 +86,8 @@ def lib():
         L.mfem_sim_matrix_info.argtypes = [vp, ip, ip, ip]
         L.mfem_sim_time_spmv.argtypes = [vp, C.c_int, dp]
         L.mfem_sim_spmv_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
+        L.mfem_sim_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
+        L.mfem_sim_cg_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
         L.mfem_material_s6_batch.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mfem_snh_f9_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.mfem_polar_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
@@ -222,6 +224,21 @@ class Sim:
         check(lib().mfem_sim_spmv_tuning(self.h, t.ctypes.data_as(dp), C.byref(n), C.byref(ch),
                                          C.byref(G), C.byref(U), C.byref(NT)))
         return t, ch.value, (G.value, U.value, NT.value)
+
+    def set_cg_shape(self, variant, blocks=0):
+        """Force the CG kernel shape and grid (variant -1: autotune again)."""
+        check(lib().mfem_sim_set_cg_shape(self.h, int(variant), int(blocks)))
+
+    def cg_tuning(self):
+        """(mean ms per CG iteration of every (shape, grid) pair, chosen index,
+        (lanes, unroll, grid))."""
+        n = C.c_int()
+        check(lib().mfem_sim_cg_tuning(self.h, dp(), C.byref(n), None, None, None, None))
+        t = np.zeros(n.value)
+        ch, G, U, B = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().mfem_sim_cg_tuning(self.h, t.ctypes.data_as(dp), C.byref(n), C.byref(ch),
+                                       C.byref(G), C.byref(U), C.byref(B)))
+        return t, ch.value, (G.value, U.value, B.value)
 
 
 # ---- batched element entry points (torch tensors as device memory) ---------
