@@ -39,6 +39,21 @@ def cg_bytes(nfree, nnzb, vb):
     return nnzb * (9 * vb + 4) + 4 * (nfree + 1) + n * (2 * 4 * vb + 16)
 
 
+def measured_traffic(lanes, unroll, default_workload):
+    """HBM bytes per CG-kernel launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (copy-calibrated) on the default workload, for the shape the
+    tuner picked; None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "k_cg_iter_c3_mixed_traffic.json")
+    if not default_workload or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    per = d["per_shape_bytes"]
+    key = f"k_cg_iter<{lanes}, {unroll}, float>"
+    vals = per.get(key) or sorted(per.values(), key=lambda v: v["read"] + v["write"])[len(per) // 2]
+    return vals["read"] + vals["write"]
+
+
 def run_gpu(scene, precision, steps, warmup, profile=True):
     from mfem_amd import native
     sim = native.Sim(scene, precision=precision)
@@ -169,7 +184,11 @@ def main():
         "newton_iters": mixed["newton"],
         "pcg_iters": mixed["pcg"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(mixed["tuning"]["lanes_per_row"], mixed["tuning"]["unroll"],
+                                                 (args.nx, args.ny, args.nz) == (100, 40, 40)),
+                     "traffic_source": "profiles/k_cg_iter_c3_mixed_traffic.md (rocprofv3 FETCH_SIZE x 2.0, "
+                                       "WRITE_SIZE x 1.0 from a copy calibration)",
                      "kernel": "k_cg_iter (one CG iteration: BSR 3x3 SpMV, x/r/z/p updates, "
                                "5 dot-product partials)",
                      "bytes_per_launch": bytes_spmv, "avg_launch_ms": avg_spmv_ms,
