@@ -39,7 +39,7 @@ def cg_bytes(nfree, nnzb, vb):
     return nnzb * (9 * vb + 4) + 4 * (nfree + 1) + n * (2 * 4 * vb + 16)
 
 
-def measured_traffic(lanes, unroll, default_workload):
+def measured_traffic(lanes, unroll, workgroup, default_workload):
     """HBM bytes per CG-kernel launch measured with rocprofv3 FETCH_SIZE and
     WRITE_SIZE (copy-calibrated) on the default workload, for the shape the
     tuner picked; None for other workloads."""
@@ -49,7 +49,7 @@ def measured_traffic(lanes, unroll, default_workload):
     with open(path) as f:
         d = json.load(f)
     per = d["per_shape_bytes"]
-    key = f"k_cg_iter<{lanes}, {unroll}, float>"
+    key = f"k_cg_iter<{lanes}, {unroll}, {workgroup}, float>"
     vals = per.get(key) or sorted(per.values(), key=lambda v: v["read"] + v["write"])[len(per) // 2]
     return vals["read"] + vals["write"]
 
@@ -80,12 +80,12 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
     elapsed = time.perf_counter() - t0
     nfree, nnzb, vb = sim.matrix_info()
     x = sim.positions()
-    times, chosen, shape = sim.cg_tuning()
+    trials, chosen, shape = sim.cg_tuning()
     spmv_alone_ms = sim.time_spmv(50)
     sim.close()
-    tuning = {"ms_per_iteration": [round(float(t), 5) for t in times], "chosen": chosen,
-              "lanes_per_row": shape[0], "unroll": shape[1], "blocks": shape[2],
-              "plain_spmv_ms": spmv_alone_ms}
+    tuning = {"trials_variant_grid_ms": [[v, g, round(t, 5)] for v, g, t in trials],
+              "chosen": chosen, "lanes_per_row": shape[0], "blocks_in_flight": shape[1],
+              "workgroup": shape[2], "grid": shape[3], "plain_spmv_ms": spmv_alone_ms}
     return dict(elapsed=elapsed, newton=newton, pcg=pcg, spmv_ms=spmv_ms, spmv_n=spmv_n,
                 nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st,
                 tuning=tuning)
@@ -185,7 +185,9 @@ def main():
         "pcg_iters": mixed["pcg"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(mixed["tuning"]["lanes_per_row"], mixed["tuning"]["unroll"],
+                     "traffic": measured_traffic(mixed["tuning"]["lanes_per_row"],
+                                                 mixed["tuning"]["blocks_in_flight"],
+                                                 mixed["tuning"]["workgroup"],
                                                  (args.nx, args.ny, args.nz) == (100, 40, 40)),
                      "traffic_source": "profiles/k_cg_iter_c3_mixed_traffic.md (rocprofv3 FETCH_SIZE x 2.0, "
                                        "WRITE_SIZE x 1.0 from a copy calibration)",

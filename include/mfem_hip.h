@@ -89,7 +89,9 @@ int mfem_sim_get_positions(mfem_sim* sim, double* out);
  * column-major), "H", "Hinv" (36/tet; H is Syminv Hpsi Syminv / vol as the
  * reference's H_), "blocks" (90/tet), "fe" (12/tet), "lhs" (scalar CSR
  * values in the reference's row order), "rhs", "dx" (3/free vertex),
- * "x" (3/vertex), "x_tilde", "mass" (per nonzero block). */
+ * "x" (3/vertex), "x_tilde", "mass" (per nonzero block), "cg_stamps"
+ * (profiling: per-workgroup start/end of the last two CG launches, in us
+ * from the earliest stamp, [2][grid][2]). */
 long mfem_sim_size(mfem_sim* sim, const char* name);
 int mfem_sim_get(mfem_sim* sim, const char* name, double* out);
 /* "rowptr", "colind" (scalar CSR pattern), "free_map", "fixed",
@@ -113,16 +115,18 @@ int mfem_sim_spmv_tuning(mfem_sim* sim, double* times, int* count, int* chosen, 
                          int* unroll, int* nontemporal);
 /* CG kernel shape (no counterpart in the reference; the iteration itself
  * follows Eigen's ConjugateGradient as linear_solver_factory.cpp:144-154
- * configures it). Force shape `variant` (0 .. count/grids - 1) on a grid of
- * `blocks` workgroups (a multiple of 8 in [8, 4096], or 0 for the default);
- * variant -1 restores autotuning on the next solve. */
+ * configures it). Force shape `variant` (0-7: lanes per block row, blocks
+ * in flight per lane and workgroup size as listed in k_solver.hip) on a grid
+ * of `blocks` workgroups (a multiple of 8 in [8, 4096], or 0 for the
+ * default); variant -1 restores autotuning on the next solve. */
 int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks);
-/* CG kernel autotuner result: mean device time per iteration of every
- * (shape, grid) pair (times[count], ms, index shape * grids + grid; NULL to
- * query count), the chosen index (-1 when untuned or forced off-table), and
- * the chosen lanes-per-row / unroll / grid. */
-int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
-                       int* unroll, int* blocks);
+/* CG kernel autotuner result: every (shape, grid) pair timed on the first
+ * solve (times[count] ms per iteration, variants[count], grids[count]; pass
+ * NULL arrays to query count), the index of the chosen pair (-1 when untuned
+ * or forced), and the chosen lanes-per-row / blocks in flight per lane /
+ * workgroup size / grid. */
+int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* variants, int* grids, int* count,
+                       int* chosen, int* lanes, int* unroll, int* threads, int* blocks);
 
 /* Batched MaterialModel evaluation on the 6-vector s (reference
  * energies/material_model.h:29-38): device pointers, n points, s n x 6,

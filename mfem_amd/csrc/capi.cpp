@@ -3,6 +3,7 @@
 // or a nonzero code and leaves a message for mfem_last_error().
 #include "../../include/mfem_hip.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -103,6 +104,15 @@ std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
     return o;
   }
   if (k == "mass") return m.d_mass.to_host(st);
+  if (k == "cg_stamps") {
+    if (!pcg) throw std::runtime_error("no CG solver");
+    auto t = pcg->cg_stamps(st);
+    std::vector<double> o(t.size());
+    const double us_per_tick = 1e3 / pcg->wall_clock_khz();
+    const unsigned long long t0 = t.empty() ? 0 : *std::min_element(t.begin(), t.end());
+    for (size_t i = 0; i < t.size(); ++i) o[i] = (double)(t[i] - t0) * us_per_tick;
+    return o;
+  }
   throw std::runtime_error("unknown array '" + k + "'");
 }
 
@@ -373,27 +383,26 @@ int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks) {
   });
 }
 
-int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* count, int* chosen, int* lanes,
-                       int* unroll, int* blocks) {
+int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* variants, int* grids, int* count,
+                       int* chosen, int* lanes, int* unroll, int* threads, int* blocks) {
   return guarded([&] {
     auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
     if (!pcg) throw std::runtime_error("no CG solver");
-    const auto& grids = PcgSolver<3>::cg_grid_options();
-    const int ng = (int)grids.size();
-    const int total = mfemhip::cg_variant_count() * ng;
-    const auto& t = pcg->cg_tuning_ms();
-    if (count) *count = total;
-    if (times)
-      for (int i = 0; i < total; ++i) times[i] = i < (int)t.size() ? t[i] : 0.0;
-    const int v = pcg->cg_variant();
-    int g = -1;
-    for (int i = 0; i < ng; ++i)
-      if (grids[i] == pcg->cg_blocks()) g = i;
-    if (chosen) *chosen = (v >= 0 && g >= 0) ? v * ng + g : -1;
-    int G = 0, U = 0;
-    mfemhip::cg_variant_shape(std::max(v, 0), &G, &U);
+    const auto& t = pcg->cg_trials();
+    if (count) *count = (int)t.size();
+    int ch = -1;
+    for (size_t i = 0; i < t.size(); ++i) {
+      if (times) times[i] = t[i].ms;
+      if (variants) variants[i] = t[i].variant;
+      if (grids) grids[i] = t[i].blocks;
+      if (t[i].variant == pcg->cg_variant() && t[i].blocks == pcg->cg_blocks()) ch = (int)i;
+    }
+    if (chosen) *chosen = ch;
+    int G = 0, U = 0, BS = 0;
+    mfemhip::cg_variant_shape(std::max(pcg->cg_variant(), 0), &G, &U, &BS);
     if (lanes) *lanes = G;
     if (unroll) *unroll = U;
+    if (threads) *threads = BS;
     if (blocks) *blocks = pcg->cg_blocks();
   });
 }

@@ -357,12 +357,6 @@ PcgSolver<DIM>::~PcgSolver() {
 }
 
 template <int DIM>
-const std::vector<int>& PcgSolver<DIM>::cg_grid_options() {
-  static const std::vector<int> g = {256, 512, 1024, 2048};
-  return g;
-}
-
-template <int DIM>
 void PcgSolver<DIM>::force_cg_shape(int variant, int blocks) {
   if (variant < -1 || variant >= cg_variant_count())
     throw std::invalid_argument("CG kernel shape out of range");
@@ -370,6 +364,17 @@ void PcgSolver<DIM>::force_cg_shape(int variant, int blocks) {
     throw std::invalid_argument("CG grid must be 0 or a multiple of 8 in [8, 4096]");
   cg_variant_ = variant;
   cg_blocks_ = variant < 0 ? 0 : blocks;
+}
+
+template <int DIM>
+std::vector<unsigned long long> PcgSolver<DIM>::cg_stamps(hipStream_t st) const {
+  const int nb = cg_blocks_ > 0 ? cg_blocks_ : pcg_default_blocks(3 * this->state_->mesh_->nfree());
+  auto all = stamps_.to_host(st);
+  std::vector<unsigned long long> out;
+  for (int half = 0; half < 2; ++half)
+    out.insert(out.end(), all.begin() + (size_t)half * 2 * kCgMaxBlocks,
+               all.begin() + (size_t)half * 2 * kCgMaxBlocks + 2 * (size_t)nb);
+  return out;
 }
 
 template <int DIM>
@@ -482,21 +487,21 @@ void PcgSolver<DIM>::tune_spmv() {
 }
 
 // CG kernel autotuner: runs a few iterations of a CG solve on the current
-// system (tolerance 0, so it does not stop) with every shape and grid, timing
-// back-to-back iterations with device events, and keeps the fastest pair.
-// Systems below kTuneMinRows block rows keep shape 0 and the default grid.
+// system (tolerance 0, so it does not stop) with every shape on grids of 1,
+// 2, 3, 4 and 6 workgroups per CU, timing back-to-back iterations with device
+// events, and keeps the fastest pair. Systems below kTuneMinRows block rows
+// keep shape 0 and the default grid.
 template <int DIM>
 template <typename T>
 void PcgSolver<DIM>::tune_cg() {
   constexpr int kTuneMinRows = 20000;
   auto& m = *this->state_->mesh_;
   hipStream_t st = this->state_->ctx_->stream;
-  const auto& grids = cg_grid_options();
-  const int nv = cg_variant_count(), ng = (int)grids.size();
-  cg_tune_ms_.assign((size_t)nv * ng, 0.0);
+  cg_trials_.clear();
   cg_variant_ = 0;
   cg_blocks_ = 0;
   if (m.nfree() < kTuneMinRows) return;
+  const int n = 3 * m.nfree();
   const int reps = 10;
   hipEvent_t a, b;
   MFEM_HIP_CHECK(hipEventCreate(&a));
@@ -504,10 +509,15 @@ void PcgSolver<DIM>::tune_cg() {
   PcgBuffers<T> B = buffers<T>(system_.rhs.data(), x_.data());
   B.stamps[0] = B.stamps[1] = nullptr;
   double best = 1e300;
-  for (int v = 0; v < nv; ++v) {
-    for (int g = 0; g < ng; ++g) {
+  for (int v = 0; v < cg_variant_count(); ++v) {
+    std::vector<int> grids;
+    for (int per_cu : {1, 2, 3, 4, 6}) {
+      const int g = cg_blocks_per_cu(v, n, per_cu);
+      if (std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    }
+    for (int g : grids) {
       B.variant = v;
-      B.blocks = grids[g];
+      B.blocks = g;
       launch_pcg_init(B, 0.0, 1 << 30, st);
       launch_pcg_iteration(B, 0, st);
       launch_pcg_iteration(B, 1, st);
@@ -518,11 +528,11 @@ void PcgSolver<DIM>::tune_cg() {
       float ms = 0;
       MFEM_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
       const double t = ms / reps;
-      cg_tune_ms_[(size_t)v * ng + g] = t;
+      cg_trials_.push_back({v, g, t});
       if (t < best) {
         best = t;
         cg_variant_ = v;
-        cg_blocks_ = grids[g];
+        cg_blocks_ = g;
       }
     }
   }

@@ -261,15 +261,22 @@ class PcgSolver : public LinearSolver<DIM> {
   // the mean device time of every shape on this matrix.
   int spmv_variant() const { return variant_; }
   const std::vector<double>& spmv_tuning_ms() const { return tune_ms_; }
-  // CG kernel shape and grid: picked by timing every (shape, grid) pair on
-  // the first solve of a large system (index v * grids + g in the table), or
-  // forced with force_cg_shape (variant -1 re-enables the tuner; blocks 0
-  // takes the default grid).
+  // CG kernel shape and grid: picked by timing every shape on a few balanced
+  // grids on the first solve of a large system, or forced with
+  // force_cg_shape (variant -1 re-enables the tuner; blocks 0 takes the
+  // default grid).
+  struct CgTrial {
+    int variant, blocks;
+    double ms;  // mean device time per CG iteration
+  };
   int cg_variant() const { return cg_variant_; }
   int cg_blocks() const { return cg_blocks_; }
-  const std::vector<double>& cg_tuning_ms() const { return cg_tune_ms_; }
-  static const std::vector<int>& cg_grid_options();
+  const std::vector<CgTrial>& cg_trials() const { return cg_trials_; }
   void force_cg_shape(int variant, int blocks);
+  // Profiling stamps of the last two CG launches: [2][blocks][start, end]
+  // device wall-clock ticks, and the clock rate in kHz.
+  std::vector<unsigned long long> cg_stamps(hipStream_t st) const;
+  double wall_clock_khz() const { return wall_khz_; }
 
  private:
   template <typename T>
@@ -283,7 +290,7 @@ class PcgSolver : public LinearSolver<DIM> {
   int variant_ = -1;
   std::vector<double> tune_ms_;
   int cg_variant_ = -1, cg_blocks_ = 0;
-  std::vector<double> cg_tune_ms_;
+  std::vector<CgTrial> cg_trials_;
   SystemMatrixPD<DIM> system_;
   DeviceArray<double> x_, r64_, dx64_;
   RealArray q_;                  // plain SpMV output (autotuner)

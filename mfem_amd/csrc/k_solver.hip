@@ -167,6 +167,56 @@ __device__ __forceinline__ void bsr_row(int row, int lane, const int* __restrict
   y2 = a2;
 }
 
+// One block row with every lane's U blocks in flight at once: the U column
+// ids and the U 3x3 value blocks are loaded together, then the U gathers, so
+// a row of up to U*G blocks costs one column-id round trip and one gather
+// round trip. Lanes past the row end are masked off.
+template <int G, int U, typename MT, typename XF>
+__device__ __forceinline__ void bsr_row_batched(int row, int lane, const int* __restrict__ rowptr,
+                                                const int* __restrict__ col,
+                                                const MT* __restrict__ val, XF xv, MT& y0, MT& y1,
+                                                MT& y2) {
+  MT a0 = 0, a1 = 0, a2 = 0;
+  const int p0 = rowptr[row], p1 = rowptr[row + 1];
+  for (int p = p0 + lane; p < p1; p += U * G) {
+    int c[U];
+    MT v[U][9];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = p + u * G;
+      c[u] = q < p1 ? col[q] : -1;
+      if (q < p1) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[u][k] = val[(size_t)9 * q + k];
+      }
+    }
+    MT xs[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c[u] >= 0) {
+        xs[u][0] = xv(c[u], 0);
+        xs[u][1] = xv(c[u], 1);
+        xs[u][2] = xv(c[u], 2);
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (c[u] >= 0) {
+        a0 += v[u][0] * xs[u][0] + v[u][1] * xs[u][1] + v[u][2] * xs[u][2];
+        a1 += v[u][3] * xs[u][0] + v[u][4] * xs[u][1] + v[u][5] * xs[u][2];
+        a2 += v[u][6] * xs[u][0] + v[u][7] * xs[u][1] + v[u][8] * xs[u][2];
+      }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, G);
+    a1 += __shfl_xor(a1, off, G);
+    a2 += __shfl_xor(a2, off, G);
+  }
+  y0 = a0;
+  y1 = a1;
+  y2 = a2;
+}
+
 template <int G, int U, bool NT, typename MT, typename VT>
 __global__ __launch_bounds__(kBS) void k_spmv(int nrows, const int* __restrict__ rowptr,
                                               const int* __restrict__ col, const MT* __restrict__ val,
@@ -262,10 +312,13 @@ struct alignas(4 * sizeof(T)) CgDof {
   T r, q, p, d;
 };
 
+constexpr int kCusPerXcd = 32;  // MI355X: 8 XCDs x 32 CUs
+
 struct CgShape {
-  int G, U;
+  int G, U, BS;  // lanes per block row, blocks per lane in flight, workgroup size
 };
-constexpr CgShape kCgShapes[] = {{8, 1}, {8, 2}, {16, 1}, {4, 1}, {4, 2}, {16, 2}};
+constexpr CgShape kCgShapes[] = {{4, 4, 256},  {4, 4, 512},  {8, 2, 256},  {8, 2, 1024},
+                                 {4, 2, 256},  {4, 2, 1024}, {16, 1, 1024}, {8, 1, 256}};
 constexpr int kNumCgShapes = sizeof(kCgShapes) / sizeof(kCgShapes[0]);
 
 // r = r - alpha q, z = D^-1 r, p = z + beta p (identical for owned and
@@ -279,43 +332,62 @@ __device__ __forceinline__ void cg_form(const CgDof<T>& s, T alpha, T beta, T& r
 
 // Profiling: the span of the previous CG kernel, from the per-block
 // (start, end) wall-clock stamps it wrote. Called by all threads of block 0.
+template <int BS>
 __device__ __forceinline__ void cg_account(const unsigned long long* __restrict__ stamps, int nb,
                                            PcgCtrl* ctrl, double* sm) {
   double lo = INFINITY, hi = -INFINITY;
-  for (int i = threadIdx.x; i < nb; i += kBS) {
+  for (int i = threadIdx.x; i < nb; i += BS) {
     lo = fmin(lo, (double)stamps[2 * i]);
     hi = fmax(hi, (double)stamps[2 * i + 1]);
   }
-  lo = -block_reduce_all<kBS, 1>(-lo, sm);
-  hi = block_reduce_all<kBS, 1>(hi, sm);
+  lo = -block_reduce_all<BS, 1>(-lo, sm);
+  hi = block_reduce_all<BS, 1>(hi, sm);
   if (threadIdx.x == 0) {
     ctrl->kern_ticks += hi - lo;
     ctrl->kern_launches += 1;
   }
 }
 
-template <int G, int U, typename T>
-__global__ __launch_bounds__(kBS) void k_cg_iter(
+// Sum of the kCgParts partial arrays (layout [k][nb]) by one workgroup, with
+// all of a thread's loads issued before the first add.
+template <int BS>
+__device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts, int nb,
+                                                double (&v)[kCgParts], double* sm) {
+  constexpr int R = BS >= 1024 ? 1 : 4;  // loads in flight per thread (register budget)
+#pragma unroll
+  for (int k = 0; k < kCgParts; ++k) v[k] = 0.0;
+  for (int i0 = threadIdx.x; i0 < nb; i0 += R * BS) {
+    double t[R][kCgParts];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int i = i0 + j * BS;
+#pragma unroll
+      for (int k = 0; k < kCgParts; ++k) t[j][k] = i < nb ? parts[k * nb + i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+      for (int k = 0; k < kCgParts; ++k) v[k] += t[j][k];
+  }
+  block_sum_n<BS, kCgParts>(v, sm);
+}
+
+template <int G, int U, int BS, typename T>
+__global__ __launch_bounds__(BS) void k_cg_iter(
     int it, int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const CgDof<T>* __restrict__ s_old, CgDof<T>* __restrict__ s_new,
     double* __restrict__ x, PcgCtrl* ctrl, const double* __restrict__ parts_old,
     double* __restrict__ parts_new, const unsigned long long* __restrict__ stamps_old,
     unsigned long long* __restrict__ stamps_new) {
-  __shared__ double sm[kCgParts * (kBS / kWave)];
+  __shared__ double sm[kCgParts * (BS / kWave)];
   const unsigned long long t_start = (unsigned long long)wall_clock64();
   if (ctrl->done) return;
   const int nb = gridDim.x;
   double alpha = 0.0, beta = 0.0;
   if (it > 0) {
     double v[kCgParts];
-#pragma unroll
-    for (int k = 0; k < kCgParts; ++k) v[k] = 0.0;
-    for (int i = threadIdx.x; i < nb; i += kBS) {
-#pragma unroll
-      for (int k = 0; k < kCgParts; ++k) v[k] += parts_old[k * nb + i];
-    }
-    block_sum_n<kBS, kCgParts>(v, sm);
-    if (stamps_old && blockIdx.x == 0) cg_account(stamps_old, nb, ctrl, sm);
+    cg_reduce_parts<BS>(parts_old, nb, v, sm);
+    if (stamps_old && blockIdx.x == 0) cg_account<BS>(stamps_old, nb, ctrl, sm);
     const double pq = v[0], zq = v[1], qdq = v[2], rr = v[3], rz = v[4];
     const int j = it - 1;  // iteration of the partials
     const bool converged = rr < ctrl->threshold;
@@ -334,23 +406,24 @@ __global__ __launch_bounds__(kBS) void k_cg_iter(
   }
   const T a = (T)alpha, bt = (T)beta;
   const int lane = threadIdx.x & (G - 1);
-  constexpr int kRowsPerBlock = kBS / G;
-  const int xcd = blockIdx.x & 7, nbx = nb >> 3;
-  const int r0 = (int)((long long)nrows * xcd / 8);
-  const int r1 = (int)((long long)nrows * (xcd + 1) / 8);
+  // XCD x (= blockIdx % 8) owns rows [nrows x / 8, nrows (x+1) / 8); its nbx
+  // workgroups split that range into equal contiguous chunks.
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = nb >> 3;
+  const long long r0 = (long long)nrows * xcd / 8, r1 = (long long)nrows * (xcd + 1) / 8;
+  const int c0 = (int)(r0 + (r1 - r0) * lb / nbx);
+  const int c1 = (int)(r0 + (r1 - r0) * (lb + 1) / nbx);
   double acc[kCgParts];
 #pragma unroll
   for (int k = 0; k < kCgParts; ++k) acc[k] = 0.0;
-  for (int row = r0 + (blockIdx.x >> 3) * kRowsPerBlock + (int)threadIdx.x / G; row < r1;
-       row += nbx * kRowsPerBlock) {
+  for (int row = c0 + (int)threadIdx.x / G; row < c1; row += BS / G) {
     T y0, y1, y2;
-    bsr_row<G, U, false>(row, lane, rowptr, col, val,
-                         [&](int c, int k) {
-                           T r, z, p;
-                           cg_form(s_old[3 * c + k], a, bt, r, z, p);
-                           return p;
-                         },
-                         y0, y1, y2);
+    bsr_row_batched<G, U>(row, lane, rowptr, col, val,
+                          [&](int c, int k) {
+                            T r, z, p;
+                            cg_form(s_old[3 * c + k], a, bt, r, z, p);
+                            return p;
+                          },
+                          y0, y1, y2);
     if (lane < 3) {
       const int i = 3 * row + lane;
       const CgDof<T> s = s_old[i];
@@ -372,7 +445,7 @@ __global__ __launch_bounds__(kBS) void k_cg_iter(
       acc[4] += dr * dz;
     }
   }
-  block_sum_n<kBS, kCgParts>(acc, sm);
+  block_sum_n<BS, kCgParts>(acc, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < kCgParts; ++k) parts_new[k * nb + blockIdx.x] = acc[k];
@@ -466,16 +539,23 @@ void spmv_variant_shape(int v, int* G, int* U, int* NT) {
 }
 
 int cg_variant_count() { return kNumCgShapes; }
-void cg_variant_shape(int v, int* G, int* U) {
+void cg_variant_shape(int v, int* G, int* U, int* BS) {
   const CgShape s = (v >= 0 && v < kNumCgShapes) ? kCgShapes[v] : kCgShapes[0];
   *G = s.G;
   *U = s.U;
+  *BS = s.BS;
 }
 
-int pcg_default_blocks(int n) {
-  const int b = grid((long)(n / 3) * 8, kBS);
-  return std::min(std::max((b + 7) / 8 * 8, 8), 1024);
+int cg_blocks_per_cu(int variant, int n, int per_cu) {
+  const CgShape s = (variant >= 0 && variant < kNumCgShapes) ? kCgShapes[variant] : kCgShapes[0];
+  // no more workgroups than chunks of at least one full pass of rows
+  const long rows_per_xcd = (n / 3 + 7) / 8;
+  const long max_nbx = std::max<long>(1, rows_per_xcd / (s.BS / s.G));
+  const long nbx = std::min<long>((long)per_cu * kCusPerXcd, max_nbx);
+  return (int)std::min<long>(8 * nbx, kCgMaxBlocks);
 }
+
+int pcg_default_blocks(int n) { return cg_blocks_per_cu(0, n, 2); }
 
 void launch_finalize(const double* partials, int n, int op, double* result, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, st, partials, n, op, result);
@@ -540,12 +620,13 @@ void launch_pcg_iteration(const PcgBuffers<T>& B, int it, hipStream_t st) {
   unsigned long long* tn = B.stamps[it & 1];
   const int nrows = B.n / 3;
   switch (B.variant) {
-#define MFEM_CG(i)                                                                              \
-  case i:                                                                                       \
-    hipLaunchKernelGGL((k_cg_iter<kCgShapes[i].G, kCgShapes[i].U, T>), dim3(B.blocks), dim3(kBS), \
-                       0, st, it, nrows, B.rowptr, B.col, B.K, so, sn, B.x, B.ctrl, po, pn, to, tn); \
+#define MFEM_CG(i)                                                                             \
+  case i:                                                                                      \
+    hipLaunchKernelGGL((k_cg_iter<kCgShapes[i].G, kCgShapes[i].U, kCgShapes[i].BS, T>),          \
+                       dim3(B.blocks), dim3(kCgShapes[i].BS), 0, st, it, nrows, B.rowptr, B.col, \
+                       B.K, so, sn, B.x, B.ctrl, po, pn, to, tn);                              \
     break;
-    MFEM_CG(0) MFEM_CG(1) MFEM_CG(2) MFEM_CG(3) MFEM_CG(4) MFEM_CG(5)
+    MFEM_CG(0) MFEM_CG(1) MFEM_CG(2) MFEM_CG(3) MFEM_CG(4) MFEM_CG(5) MFEM_CG(6) MFEM_CG(7)
 #undef MFEM_CG
     default:
       throw std::invalid_argument("CG kernel shape out of range");

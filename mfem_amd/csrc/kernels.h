@@ -110,9 +110,13 @@ void launch_residual(int nrows, const int* rowptr, const int* col, const MT* val
 // FP64), double-buffered; every kernel writes 5 partial sums per block,
 // double-buffered as well.
 constexpr int kCgParts = 5;
-// (lanes per block row, blocks per lane per trip) of the CG kernel shapes
+// (lanes per block row, blocks per lane in flight, workgroup size) of the CG
+// kernel shapes
 int cg_variant_count();
-void cg_variant_shape(int variant, int* G, int* U);
+void cg_variant_shape(int variant, int* G, int* U, int* BS);
+// Grid for shape `variant` on n scalar rows with `per_cu` workgroups on
+// every CU (fewer when the rows do not fill that many workgroups).
+int cg_blocks_per_cu(int variant, int n, int per_cu);
 template <typename T>
 struct PcgBuffers {
   int n = 0;  // scalar rows (3 * nfree)

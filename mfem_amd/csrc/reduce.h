@@ -53,7 +53,9 @@ __device__ __forceinline__ void block_sum_n(double (&v)[N], double* sm) {
     for (int k = 0; k < N; ++k) sm[k * (BS / kWave) + wid] = v[k];
   }
   __syncthreads();
-#pragma unroll
+  // one quantity at a time, so large workgroups (16 waves) do not hold all
+  // N * waves values in registers at once
+#pragma unroll 1
   for (int k = 0; k < N; ++k) {
     double r = sm[k * (BS / kWave)];
 #pragma unroll

@@ -87,7 +87,7 @@ def lib():
         L.mfem_sim_time_spmv.argtypes = [vp, C.c_int, dp]
         L.mfem_sim_spmv_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
         L.mfem_sim_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
-        L.mfem_sim_cg_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
+        L.mfem_sim_cg_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip, ip, ip, ip]
         L.mfem_material_s6_batch.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mfem_snh_f9_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.mfem_polar_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
@@ -230,15 +230,21 @@ class Sim:
         check(lib().mfem_sim_set_cg_shape(self.h, int(variant), int(blocks)))
 
     def cg_tuning(self):
-        """(mean ms per CG iteration of every (shape, grid) pair, chosen index,
-        (lanes, unroll, grid))."""
+        """List of (variant, grid, ms per CG iteration) the tuner timed, the
+        chosen index, and the chosen (lanes, blocks in flight, workgroup size,
+        grid)."""
         n = C.c_int()
-        check(lib().mfem_sim_cg_tuning(self.h, dp(), C.byref(n), None, None, None, None))
+        check(lib().mfem_sim_cg_tuning(self.h, None, None, None, C.byref(n), None, None, None,
+                                       None, None))
         t = np.zeros(n.value)
-        ch, G, U, B = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(lib().mfem_sim_cg_tuning(self.h, t.ctypes.data_as(dp), C.byref(n), C.byref(ch),
-                                       C.byref(G), C.byref(U), C.byref(B)))
-        return t, ch.value, (G.value, U.value, B.value)
+        v = np.zeros(n.value, dtype=np.int32)
+        g = np.zeros(n.value, dtype=np.int32)
+        ch, G, U, BS, NB = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().mfem_sim_cg_tuning(self.h, t.ctypes.data_as(dp), v.ctypes.data_as(ip),
+                                       g.ctypes.data_as(ip), C.byref(n), C.byref(ch), C.byref(G),
+                                       C.byref(U), C.byref(BS), C.byref(NB)))
+        trials = [(int(a), int(b), float(c)) for a, b, c in zip(v, g, t)]
+        return trials, ch.value, (G.value, U.value, BS.value, NB.value)
 
 
 # ---- batched element entry points (torch tensors as device memory) ---------

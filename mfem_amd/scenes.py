@@ -271,3 +271,67 @@ def config_c3(nx=100, ny=40, nz=40, seed=0, **kw):
     kw.setdefault("max_iters", 500)
     return beam_scene(nx, ny, nz, 0.1, 0.1, seed, E_range=(1e5, 1e6), nu=0.45,
                       mixed=True, solver="eigen-pcg-diag", dt=1.0 / 30.0, **kw)
+
+
+def config_c5(nx=130, ny=50, nz=50, seed=0, **kw):
+    """Stiff, ill-conditioned mixed beam: 130x50x50x6 jittered beam (1,950,000
+    tets), per-tet E log-uniform in [1e4, 1e8], nu = 0.49, ARAP on one side
+    of a seeded random plane and Stable Neo-Hookean on the other (per-element
+    materials, as with the reference's material_ids, tet_mesh.h:11-15).
+
+    The plane's normal is uniform on the sphere (stream 3 of the seed) and it
+    passes through a uniform point of the beam's box (stream 4); a tet is ARAP
+    when its centroid lies on the positive side."""
+    kw.setdefault("itr_tol", 1e-4)
+    kw.setdefault("max_iters", 500)
+    h = 0.1
+    sc = beam_scene(nx, ny, nz, h, 0.1, seed, E_range=(1e4, 1e8), nu=0.49, mixed=True,
+                    solver="eigen-pcg-diag", dt=1.0 / 30.0, **kw)
+    u = splitmix64_uniform(seed, 3, 2)
+    z = 2.0 * u[0] - 1.0
+    phi = 2.0 * math.pi * u[1]
+    rxy = math.sqrt(max(0.0, 1.0 - z * z))
+    normal = np.array([rxy * math.cos(phi), rxy * math.sin(phi), z])
+    point = splitmix64_uniform(seed, 4, 3) * np.array([nx * h, ny * h, nz * h])
+    centroid = sc.V[sc.T].mean(axis=1)
+    arap = (centroid - point) @ normal > 0.0
+    sc.mat = np.where(arap, MATERIALS["ARAP"], MATERIALS["Stable-Neohookean"]).astype(np.int32)
+    sc.name = f"c5_{nx}x{ny}x{nz}"
+    sc.extra = {"plane_normal": normal.tolist(), "plane_point": point.tolist(),
+                "arap_fraction": float(arap.mean())}
+    return sc
+
+
+def c4_beam_sizes(count=256, seed=0, lo=2e3, hi=6e4):
+    """Cell counts (nx, ny, nz = ny) of the batching config: 6 nx ny^2 drawn
+    log-uniform in [lo, hi] (stream 5), the aspect nx / ny uniform in [2, 6]
+    (stream 6), nx clipped to [10, 60]; ny >= 2."""
+    t = np.exp(math.log(lo) + splitmix64_uniform(seed, 5, count) * (math.log(hi) - math.log(lo)))
+    aspect = 2.0 + 4.0 * splitmix64_uniform(seed, 6, count)
+    out = []
+    for k in range(count):
+        cells = t[k] / 6.0
+        ny = max(2, int(round((cells / aspect[k]) ** (1.0 / 3.0))))
+        nx = int(min(60, max(10, round(cells / (ny * ny)))))
+        out.append((nx, ny, ny))
+    return out
+
+
+def config_c4(count=256, seed=0, **kw):
+    """Batching config: `count` independent mixed-stretch beams whose tet
+    counts are log-uniform in [2e3, 6e4], each with its own E log-uniform in
+    [1e4, 1e7] (stream 7) and nu uniform in [0.3, 0.48] (stream 8); SNH, BDF1
+    dt = 1/30, CG tolerance 1e-4 / 500 iterations. Returns a list of Scenes
+    (beam k uses seed + k for its jitter)."""
+    kw.setdefault("itr_tol", 1e-4)
+    kw.setdefault("max_iters", 500)
+    sizes = c4_beam_sizes(count, seed)
+    E = np.exp(math.log(1e4) + splitmix64_uniform(seed, 7, count) * (math.log(1e7) - math.log(1e4)))
+    nu = 0.3 + 0.18 * splitmix64_uniform(seed, 8, count)
+    scenes_ = []
+    for k, (nx, ny, nz) in enumerate(sizes):
+        sc = beam_scene(nx, ny, nz, 0.1, 0.1, seed + k, E=float(E[k]), nu=float(nu[k]), mixed=True,
+                        solver="eigen-pcg-diag", dt=1.0 / 30.0, **kw)
+        sc.name = f"c4_{k}_{nx}x{ny}x{nz}"
+        scenes_.append(sc)
+    return scenes_

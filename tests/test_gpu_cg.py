@@ -43,7 +43,7 @@ def first_solve(native, oracle, sc, variant, blocks, precision="fp64"):
     return o, g
 
 
-SHAPES = range(6)
+SHAPES = range(8)  # every CG kernel shape
 
 
 @pytest.mark.parametrize("blocks", [8, 64])
