@@ -34,10 +34,17 @@ ReduceSlot Reducer::slot(int k) {
 }
 
 double Reducer::read(int k, hipStream_t st) {
-  MFEM_HIP_CHECK(hipMemcpyAsync(pinned_->as<double>(), results_.data() + k, sizeof(double),
+  double v = 0;
+  read_range(k, 1, &v, st);
+  return v;
+}
+
+void Reducer::read_range(int k0, int n, double* out, hipStream_t st) {
+  if (k0 < 0 || n < 0 || k0 + n > kSlots) throw std::logic_error("reducer slot out of range");
+  MFEM_HIP_CHECK(hipMemcpyAsync(pinned_->as<double>(), results_.data() + k0, n * sizeof(double),
                                 hipMemcpyDeviceToHost, st));
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
-  return pinned_->as<double>()[0];
+  for (int i = 0; i < n; ++i) out[i] = pinned_->as<double>()[i];
 }
 
 Inertia DeviceContext::inertia() const {
@@ -51,8 +58,16 @@ DeviceContext::~DeviceContext() {
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
-// reduction slot ids
-enum { RS_ENERGY = 0, RS_MAXDS = 1, RS_MAXDX = 2, RS_RES = 3, RS_PCG0 = 4, RS_PCG1 = 5, RS_PCG2 = 6 };
+// reduction slot ids: the decrement pair and the two line-search energies of
+// a Newton iteration are adjacent so that one read returns all of them
+enum {
+  RS_MAXDX = 0,
+  RS_MAXDS = 1,
+  RS_E0 = 2,      // E(0), 2 slots (one per elastic variable)
+  RS_E1 = 4,      // E(1), 2 slots
+  RS_ENERGY = 6,  // other E(alpha), 2 slots
+  RS_RES = 8
+};
 
 // ============================================================ BDF
 template <int I>
@@ -160,10 +175,10 @@ void Displacement<DIM>::post_solve() {
 }
 
 template <int DIM>
-double Displacement<DIM>::max_abs_delta() {
+int Displacement<DIM>::launch_max_abs_delta() {
   auto& R = this->ctx_->reducer;
   launch_reduce_maxabs((int)dx_.size(), dx_.data(), R.slot(RS_MAXDX), this->ctx_->stream);
-  return R.read(RS_MAXDX, this->ctx_->stream);
+  return RS_MAXDX;
 }
 
 // ============================================================ Stretch
@@ -181,11 +196,11 @@ void Stretch<DIM>::update(const double* x_full, double dt) {
 }
 
 template <int DIM>
-double Stretch<DIM>::energy(const double* x_full, const double* dx, double alpha) {
+bool Stretch<DIM>::energy_launch(const double* x_full, const double* dx, double alpha,
+                                 ReduceSlot out) {
   auto* c = this->ctx_;
-  launch_plain_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), h2_,
-                      c->reducer.slot(RS_ENERGY), c->stream);
-  return c->reducer.read(RS_ENERGY, c->stream);
+  launch_plain_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), h2_, out, c->stream);
+  return true;
 }
 
 // ============================================================ MixedStretch
@@ -266,8 +281,8 @@ void MixedStretch<DIM>::solve(const double* dx) {
 }
 
 template <int DIM>
-double MixedStretch<DIM>::max_abs_delta() {
-  return this->ctx_->reducer.read(RS_MAXDS, this->ctx_->stream);
+int MixedStretch<DIM>::max_abs_delta_slot() const {
+  return RS_MAXDS;
 }
 
 template <int DIM>
@@ -276,11 +291,12 @@ void MixedStretch<DIM>::post_solve() {
 }
 
 template <int DIM>
-double MixedStretch<DIM>::energy(const double* x_full, const double* dx, double alpha) {
+bool MixedStretch<DIM>::energy_launch(const double* x_full, const double* dx, double alpha,
+                                      ReduceSlot out) {
   auto* c = this->ctx_;
   launch_mixed_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), R_.data(), s_.data(),
-                      ds_.data(), la_.data(), h2_, c->reducer.slot(RS_ENERGY), c->stream);
-  return c->reducer.read(RS_ENERGY, c->stream);
+                      ds_.data(), la_.data(), h2_, out, c->stream);
+  return true;
 }
 
 template <int DIM>
@@ -570,29 +586,39 @@ void PcgSolver<DIM>::solve() {
     this->stats_.refinements = 1;
     this->stats_.rel_residual = last_.b2 > 0 ? std::sqrt(last_.rr / last_.b2) : 0.0;
   } else {
-    // FP32 CG sweeps inside FP64 defect correction on the true residual.
+    // FP32 CG sweeps inside FP64 defect correction on the true residual. The
+    // first sweep solves K x = b from x = 0 (its CG setup yields |b|^2); each
+    // further sweep solves K d = b - K x for the FP64 residual and adds d.
+    // The loop stops on Eigen's rule for the FP64 residual, on the iteration
+    // budget, or after max_refinements sweeps.
     r64_.resize(n);
     dx64_.resize(n);
-    x_.zero(st);
-    MFEM_HIP_CHECK(hipMemcpyAsync(r64_.data(), system_.rhs.data(), (size_t)n * sizeof(double),
-                                  hipMemcpyDeviceToDevice, st));
-    // |b|^2
-    launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
-                           x_.data(), system_.rhs.data(), r64_.data(), R.slot(RS_RES), st);
-    const double b2 = R.read(RS_RES, st);
-    double r2 = b2;
-    const double thr = std::max(cfg.itr_tol * cfg.itr_tol * b2, 2.2250738585072014e-308);
     int budget = cfg.max_iterative_solver_iters;
     int sweeps = 0;
-    while (r2 >= thr && budget > 0 && sweeps < cfg.max_refinements && b2 > 0) {
-      const double inner = std::max(cfg.inner_tol, std::sqrt(thr / r2) * 0.5);
-      int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget);
+    double b2 = 0, r2 = 0;
+    if (cfg.max_refinements > 0) {
+      // sqrt(threshold / |r|^2) = tol while r = b
+      const double inner = std::min(std::max(cfg.inner_tol, 0.5 * cfg.itr_tol), 0.5);
+      const int it = run_cg<float>(system_.rhs.data(), x_.data(), inner, budget);
       budget -= std::max(it, 1);
       this->stats_.iterations += it;
-      launch_axpy(n, 1.0, dx64_.data(), x_.data(), st);
+      b2 = last_.b2;
+      r2 = last_.rr;
+      ++sweeps;
+    } else {
+      x_.zero(st);
+    }
+    const double thr = std::max(cfg.itr_tol * cfg.itr_tol * b2, 2.2250738585072014e-308);
+    while (b2 > 0) {
       launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
                              x_.data(), system_.rhs.data(), r64_.data(), R.slot(RS_RES), st);
       r2 = R.read(RS_RES, st);
+      if (!(r2 >= thr) || budget <= 0 || sweeps >= cfg.max_refinements) break;
+      const double inner = std::max(cfg.inner_tol, std::sqrt(thr / r2) * 0.5);
+      const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget);
+      budget -= std::max(it, 1);
+      this->stats_.iterations += it;
+      launch_axpy(n, 1.0, dx64_.data(), x_.data(), st);
       ++sweeps;
     }
     this->stats_.refinements = sweeps;
@@ -668,33 +694,58 @@ template <int DIM>
 void NewtonOptimizer<DIM>::substep(double& decrement) {
   auto& S = *this->state_;
   solver_->solve();
-  decrement = S.x_->max_abs_delta();
-  for (auto& v : S.mixed_vars_) decrement = std::max(decrement, v->max_abs_delta());
+  double v[2] = {0, 0};
+  S.ctx_->reducer.read_range(RS_MAXDX, launch_decrement(), v, S.ctx_->stream);
+  decrement = std::max(v[0], v[1]);
+}
+
+template <int DIM>
+int NewtonOptimizer<DIM>::launch_decrement() {
+  auto& S = *this->state_;
+  S.x_->launch_max_abs_delta();
+  // the local solve already reduced max |ds| into its slot (RS_MAXDS)
+  for (auto& v : S.mixed_vars_)
+    if (v->max_abs_delta_slot() != RS_MAXDS) throw std::logic_error("unexpected max|ds| slot");
+  return S.mixed_vars_.empty() ? 1 : 2;
 }
 
 // reference newton_optimizer.cpp:34-49: E(a) = inertia + sum of variable
 // energies at (x + a dx, s + a ds); the element kernels include the inertia.
 template <int DIM>
-double NewtonOptimizer<DIM>::energy(double alpha) {
+int NewtonOptimizer<DIM>::launch_energy(double alpha, int base) {
   auto& S = *this->state_;
+  auto* c = S.ctx_.get();
   const double* x = S.x_->value().data();
   const double* dx = S.x_->delta().data();
-  double val = 0;
-  bool inertia_done = false;
-  for (auto& v : S.mixed_vars_) { val += v->energy(x, dx, alpha); inertia_done = true; }
-  for (auto& v : S.vars_) { val += v->energy(x, dx, alpha); inertia_done = true; }
-  if (!inertia_done) {
-    auto* c = S.ctx_.get();
-    launch_plain_energy(S.mesh_->dev(), x, dx, alpha, c->inertia(), 0.0,
-                        c->reducer.slot(RS_ENERGY), c->stream);
-    val = c->reducer.read(RS_ENERGY, c->stream);
+  int k = 0;
+  for (auto& v : S.mixed_vars_)
+    if (v->energy_launch(x, dx, alpha, c->reducer.slot(base + k))) ++k;
+  for (auto& v : S.vars_)
+    if (k < 2 && v->energy_launch(x, dx, alpha, c->reducer.slot(base + k))) ++k;
+  if (S.mixed_vars_.size() + S.vars_.size() > 2)
+    throw std::runtime_error("at most two elastic variables are supported");
+  if (k == 0) {  // no elastic variable: inertia only
+    launch_plain_energy(S.mesh_->dev(), x, dx, alpha, c->inertia(), 0.0, c->reducer.slot(base),
+                        c->stream);
+    k = 1;
   }
-  return val;
+  return k;
+}
+
+template <int DIM>
+double NewtonOptimizer<DIM>::energy(double alpha) {
+  auto& S = *this->state_;
+  double v[2] = {0, 0};
+  const int k = launch_energy(alpha, RS_ENERGY);
+  S.ctx_->reducer.read_range(RS_ENERGY, k, v, S.ctx_->stream);
+  return k == 2 ? v[0] + v[1] : v[0];
 }
 
 // reference newton_optimizer.cpp:11-89 with linesearch_backtracking
 // (linesearch.h:78-131, called with c = 0 and p = 0.5: accept on strict
-// decrease; an exhausted search leaves the state unchanged).
+// decrease; an exhausted search leaves the state unchanged). The decrement,
+// E(0) and the first trial E(1) are launched together and read with one
+// synchronisation.
 template <int DIM>
 void NewtonOptimizer<DIM>::step() {
   auto& S = *this->state_;
@@ -707,32 +758,41 @@ void NewtonOptimizer<DIM>::step() {
   int i = 0;
   double grad_norm = 0, E = 0, E_prev = 0, res = 0;
   const bool has_mixed = !S.mixed_vars_.empty();
+  const int ls_iters = S.config_->ls_iters;
   do {
     MFEM_HIP_CHECK(hipEventRecord(ev_[0], st));
     update_system();
     MFEM_HIP_CHECK(hipEventRecord(ev_[1], st));
-    substep(grad_norm);
+    solver_->solve();
+    const int nd = launch_decrement();
     MFEM_HIP_CHECK(hipEventRecord(ev_[2], st));
+    const int ne = launch_energy(0.0, RS_E0);
+    if (ls_iters > 0) launch_energy(1.0, RS_E1);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    c->reducer.read_range(RS_MAXDX, ls_iters > 0 ? RS_E1 + 2 : RS_E0 + 2, v, st);
+    grad_norm = nd == 2 ? std::max(v[0], v[1]) : v[0];
     const auto& ss = solver_->stats();
     stats_.pcg_iters += ss.iterations;
     stats_.refinements += ss.refinements;
     stats_.rel_residual = ss.rel_residual;
     stats_.spmv_ms += ss.spmv_ms;
     stats_.spmv_count += ss.spmv_count;
-    double alpha = 1.0;
-    E = energy(0.0);
+    E = ne == 2 ? v[RS_E0] + v[RS_E0 + 1] : v[RS_E0];
     res = std::fabs((E - E_prev) / (E + 1e-6));
     E_prev = E;
     // fx0 = energy(0) equals E (same state, deterministic kernels)
     const double fx0 = E;
+    double alpha = 1.0;
     int iter = 0;
-    while (iter < S.config_->ls_iters) {
-      double fx = energy(alpha);
-      if (fx < fx0) break;
-      alpha *= 0.5;
-      ++iter;
+    if (ls_iters > 0) {
+      double fx = ne == 2 ? v[RS_E1] + v[RS_E1 + 1] : v[RS_E1];
+      while (!(fx < fx0)) {
+        alpha *= 0.5;
+        if (++iter >= ls_iters) break;
+        fx = energy(alpha);
+      }
     }
-    if (iter < S.config_->ls_iters) {
+    if (iter < ls_iters) {
       double* s = has_mixed ? S.mixed_vars_[0]->value().data() : nullptr;
       const double* ds = has_mixed ? S.mixed_vars_[0]->delta().data() : nullptr;
       launch_accept_step(S.mesh_->dev(), S.x_->value().data(), S.x_->delta().data(), s, ds, alpha,

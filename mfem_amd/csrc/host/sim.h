@@ -29,10 +29,12 @@ namespace mfem {
 // Pool of deterministic reduction slots (see reduce.h) with pinned read-back.
 class Reducer {
  public:
-  static constexpr int kSlots = 8;
+  static constexpr int kSlots = 12;
   void init(long max_items);
   mfemhip::ReduceSlot slot(int k);
   double read(int k, hipStream_t st);
+  // results of slots k0 .. k0 + n - 1 with one copy and one synchronisation
+  void read_range(int k0, int n, double* out, hipStream_t st);
 
  private:
   long stride_ = 0;
@@ -109,8 +111,12 @@ class Variable {
   virtual void post_solve() {}
   virtual int size() const = 0;
   // energy of this variable at (x + alpha dx, s + alpha ds), inertia included
-  // when the variable owns the element residual (device backend detail)
-  virtual double energy(const double* x_full, const double* dx, double alpha) { return 0.0; }
+  // when the variable owns the element residual (device backend detail),
+  // launched into `out` on the context's stream; false if it has none
+  virtual bool energy_launch(const double* x_full, const double* dx, double alpha,
+                             mfemhip::ReduceSlot out) {
+    return false;
+  }
   std::shared_ptr<Mesh> mesh() const { return mesh_; }
 
  protected:
@@ -128,7 +134,8 @@ class MixedVariable : public Variable<DIM> {
   virtual DeviceArray<double>& delta() = 0;   // ds
   virtual DeviceArray<double>& lambda() = 0;
   virtual int size_dual() const = 0;
-  virtual double max_abs_delta() = 0;
+  // max |ds| of the last solve, in the reducer slot the local solve wrote
+  virtual int max_abs_delta_slot() const = 0;
 };
 
 template <int DIM>
@@ -143,7 +150,8 @@ class Displacement : public Variable<DIM> {
   DeviceArray<double>& value() { return x_full_; }  // full configuration (3 nV)
   DeviceArray<double>& delta() { return dx_; }      // free-DOF step (3 nfree)
   const std::shared_ptr<ImplicitIntegrator>& integrator() const { return integrator_; }
-  double max_abs_delta();
+  // launch max |dx| into the reducer; returns its slot
+  int launch_max_abs_delta();
 
  private:
   std::shared_ptr<SimConfig> config_;
@@ -160,7 +168,8 @@ class Stretch : public Variable<DIM> {
   void update(const double* x_full, double dt) override;
   void reset() override {}
   int size() const override { return 0; }
-  double energy(const double* x_full, const double* dx, double alpha) override;
+  bool energy_launch(const double* x_full, const double* dx, double alpha,
+                     mfemhip::ReduceSlot out) override;
 
  private:
   double h2_ = 0;
@@ -178,11 +187,12 @@ class MixedStretch : public MixedVariable<DIM> {
   int size() const override { return 6 * this->mesh_->nT(); }
   int size_dual() const override { return 6 * this->mesh_->nT(); }
   void solve(const double* dx) override;
-  double energy(const double* x_full, const double* dx, double alpha) override;
+  bool energy_launch(const double* x_full, const double* dx, double alpha,
+                     mfemhip::ReduceSlot out) override;
   DeviceArray<double>& value() override { return s_; }
   DeviceArray<double>& delta() override { return ds_; }
   DeviceArray<double>& lambda() override { return la_; }
-  double max_abs_delta() override;
+  int max_abs_delta_slot() const override;
 
   // element state (structure of arrays, stride nT)
   DeviceArray<double> R_, S_, s_, la_, ds_, g_, gl_;
@@ -190,7 +200,6 @@ class MixedStretch : public MixedVariable<DIM> {
 
  private:
   double h2_ = 0;
-  double last_max_ds_ = 0;
 };
 
 template <int DIM>
@@ -360,6 +369,12 @@ class NewtonOptimizer : public Optimizer<DIM> {
   LinearSolver<DIM>* solver() { return solver_.get(); }
 
  private:
+  // E(alpha) as one partial per elastic variable in reducer slots base, ...;
+  // returns the number of slots written (at most 2)
+  int launch_energy(double alpha, int base);
+  // the decrement's slots after a solve: max |dx| and, with a mixed
+  // variable, max |ds| (returns the number of leading slots to read)
+  int launch_decrement();
   std::unique_ptr<LinearSolver<DIM>> solver_;
   StepStats stats_;
   hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};

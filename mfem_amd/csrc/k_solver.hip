@@ -31,14 +31,30 @@ __device__ __forceinline__ constexpr int blk_index(int k, int l) {
   return k * 4 - (k * (k - 1)) / 2 + (l - k);
 }
 
+// Item of this thread when [0, n) is split into 8 contiguous ranges, one per
+// XCD (workgroups are dispatched round-robin over the XCDs, so workgroup b
+// runs on XCD b % 8), each walked in order by that XCD's workgroups; -1 past
+// the range. Rows that share tets then share one L2. The grid is
+// xcd_grid(n, bs).
+__device__ __forceinline__ int xcd_item(int n) {
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3;
+  const long long start = (long long)n * xcd / 8, end = (long long)n * (xcd + 1) / 8;
+  const long long i = start + (long long)lb * blockDim.x + threadIdx.x;
+  return i < end ? (int)i : -1;
+}
+inline int xcd_grid(int n, int bs) {
+  const int per_xcd = (n + 7) / 8;
+  return 8 * std::max(1, (per_xcd + bs - 1) / bs);
+}
+
 template <typename Real>
 __global__ __launch_bounds__(kBS) void k_assemble_bsr(int nnzb, const int* __restrict__ slot_ptr,
                                                       const int* __restrict__ slots,
                                                       const double* __restrict__ mass,
                                                       const Real* __restrict__ blocks,
                                                       Real* __restrict__ K) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nnzb) return;
+  const int b = xcd_item(nnzb);
+  if (b < 0) return;
   double acc[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) acc[i] = 0.0;
@@ -82,8 +98,8 @@ __global__ void k_jacobi_diag(int nfree, const int* __restrict__ diag_block, con
 
 __global__ void k_assemble_rhs(int nfree, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
                                const double* __restrict__ fe, double* __restrict__ rhs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nfree) return;
+  const int i = xcd_item(nfree);
+  if (i < 0) return;
   double a0 = 0, a1 = 0, a2 = 0;
   const int p1 = inc_ptr[i + 1];
   for (int p = inc_ptr[i]; p < p1; ++p) {
@@ -565,7 +581,7 @@ template <typename Real>
 void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const double* mass,
                          const Real* blocks, Real* K, hipStream_t st) {
   if (nnzb == 0) return;
-  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(grid(nnzb, kBS)), dim3(kBS), 0, st, nnzb, slot_ptr,
+  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(xcd_grid(nnzb, kBS)), dim3(kBS), 0, st, nnzb, slot_ptr,
                      slots, mass, blocks, K);
 }
 
@@ -580,7 +596,7 @@ void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* inv
 void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const double* fe,
                          double* rhs, hipStream_t st) {
   if (nfree == 0) return;
-  hipLaunchKernelGGL(k_assemble_rhs, dim3(grid(nfree, kBS)), dim3(kBS), 0, st, nfree, inc_ptr, inc,
+  hipLaunchKernelGGL(k_assemble_rhs, dim3(xcd_grid(nfree, kBS)), dim3(kBS), 0, st, nfree, inc_ptr, inc,
                      fe, rhs);
 }
 
