@@ -107,14 +107,30 @@ def run_cpu_sample(scene, threads):
     return dt, st
 
 
+def workload_name(config, nx, ny, nz, nT, scene):
+    beam = f"jittered Kuhn beam {nx}x{ny}x{nz}x6 ({nT} tets)"
+    if config == "c3":
+        return (f"mixed-stretch SNH {beam}, per-tet E log-uniform [1e5,1e6], nu 0.45, BDF1 dt 1/30, "
+                f"Jacobi CG tol 1e-4 / 500 its, 5 Newton its")
+    if config == "c2":
+        return (f"plain stretch SNH {beam}, E 1e6, nu 0.45, BDF1 dt 1/30, Jacobi CG rel-res 1e-10 / "
+                f"20000 its, 5 Newton its")
+    return (f"mixed-stretch {beam}, ARAP / SNH split by a seeded plane "
+            f"({100 * scene.extra['arap_fraction']:.1f}% ARAP), per-tet E log-uniform [1e4,1e8], "
+            f"nu 0.49, BDF1 dt 1/30, Jacobi CG tol 1e-4 / 500 its, 5 Newton its")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nx", type=int, default=100)
-    ap.add_argument("--ny", type=int, default=40)
-    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--config", choices=("c3", "c2", "c5"), default="c3",
+                    help="c3: the metric's 960k-tet mixed beam (default); c2: plain FP64 "
+                         "24k-tet beam; c5: 1.95M-tet stiff ARAP/SNH beam")
+    ap.add_argument("--nx", type=int, default=0)
+    ap.add_argument("--ny", type=int, default=0)
+    ap.add_argument("--nz", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -126,7 +142,11 @@ def main():
         sys.exit(2)
 
     from mfem_amd import scenes
-    scene = scenes.config_c3(args.nx, args.ny, args.nz)
+    dims = {"c3": (100, 40, 40), "c2": (40, 10, 10), "c5": (130, 50, 50)}[args.config]
+    dims = tuple(a or d for a, d in zip((args.nx, args.ny, args.nz), dims))
+    args.nx, args.ny, args.nz = dims
+    make = {"c3": scenes.config_c3, "c2": scenes.config_c2, "c5": scenes.config_c5}[args.config]
+    scene = make(*dims)
     nT = scene.nT
 
     mixed = run_gpu(scene, "mixed", args.steps, args.warmup)
@@ -176,9 +196,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64",
         "data": "synthetic",
-        "config": {"workload": f"mixed-stretch SNH jittered Kuhn beam {args.nx}x{args.ny}x{args.nz}x6 "
-                               f"({nT} tets), per-tet E log-uniform [1e5,1e6], nu 0.45, BDF1 dt 1/30, "
-                               f"Jacobi CG tol 1e-4 / 500 its, 5 Newton its",
+        "config": {"workload": workload_name(args.config, args.nx, args.ny, args.nz, nT, scene),
                    "tets": nT, "free_dofs": 3 * mixed["nfree"], "bsr_blocks": mixed["nnzb"],
                    "precision": "mixed"},
         "newton_iters": mixed["newton"],
@@ -188,7 +206,7 @@ def main():
                      "traffic": measured_traffic(mixed["tuning"]["lanes_per_row"],
                                                  mixed["tuning"]["blocks_in_flight"],
                                                  mixed["tuning"]["workgroup"],
-                                                 (args.nx, args.ny, args.nz) == (100, 40, 40)),
+                                                 args.config == "c3" and dims == (100, 40, 40)),
                      "traffic_source": "profiles/k_cg_iter_c3_mixed_traffic.md (rocprofv3 FETCH_SIZE x 2.0, "
                                        "WRITE_SIZE x 1.0 from a copy calibration)",
                      "kernel": "k_cg_iter (one CG iteration: BSR 3x3 SpMV, x/r/z/p updates, "

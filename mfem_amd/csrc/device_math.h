@@ -215,7 +215,12 @@ MFEM_DEV void det_s6(const double s[6], double& J, double dJ[6]) {
   dJ[5] = 2.0 * (d * e - a * f);
 }
 
-// energy only
+// Fung exponent constant (reference src/energies/fung.cpp:7)
+constexpr double kFungC = 4.0;
+
+// energy only (reference energies: stable_neohookean.cpp, neohookean.cpp,
+// corotational.cpp, fixed_corotational.cpp, arap.cpp, fung.cpp; closed forms
+// in scripts/derivs.m)
 MFEM_DEV double material_energy(int type, double mu, double la, const double s[6]) {
   double d0 = s[0] - 1.0, d1 = s[1] - 1.0, d2 = s[2] - 1.0;
   double off = 2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
@@ -228,6 +233,15 @@ MFEM_DEV double material_energy(int type, double mu, double la, const double s[6
   det_s6(s, J, dJ);
   if (type == MAT_FCR) return 0.5 * la * (J - 1.0) * (J - 1.0) + mu * (d0 * d0 + d1 * d1 + d2 * d2 + off);
   double trS2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + off;
+  if (type == MAT_NH) {
+    const double lJ = log(J);
+    return 0.5 * mu * (trS2 - 3.0) - mu * lJ + 0.5 * la * lJ * lJ;
+  }
+  if (type == MAT_FUNG) {
+    const double A = 1.0 + (mu + mu * kFungC) / la;
+    return 0.5 * mu * (trS2 - 3.0) + 0.5 * la * (J - A) * (J - A) +
+           0.5 * mu * (exp(0.5 * kFungC * (trS2 - 3.0)) - 1.0);
+  }
   return 0.5 * mu * (trS2 - 3.0) - mu * (J - 1.0) + 0.5 * la * (J - 1.0) * (J - 1.0);
 }
 
@@ -260,7 +274,8 @@ MFEM_DEV void material_grad_hess(int type, double mu, double la, const double s[
   }
   double J, dJ[6];
   det_s6(s, J, dJ);
-  double k;  // coefficient of d2J
+  double k;         // coefficient of d2J
+  double cjj = la;  // coefficient of dJ dJ^T
   if (type == MAT_FCR) {
     k = la * (J - 1.0);
 #pragma unroll
@@ -268,6 +283,34 @@ MFEM_DEV void material_grad_hess(int type, double mu, double la, const double s[
       g[i] = 2.0 * mu * vg[i] * (s[i] - (i < 3 ? 1.0 : 0.0)) + k * dJ[i];
       H[sym6(i, i)] = 2.0 * mu * vg[i];
     }
+  } else if (type == MAT_NH) {
+    // 0.5 mu (I2 - 3) - mu log J + 0.5 la (log J)^2 (symmetric closed form)
+    const double lJ = log(J);
+    k = (la * lJ - mu) / J;
+    cjj = (la + mu - la * lJ) / (J * J);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      g[i] = mu * vg[i] * s[i] + k * dJ[i];
+      H[sym6(i, i)] = mu * vg[i];
+    }
+  } else if (type == MAT_FUNG) {
+    // 0.5 mu (I2 - 3) + 0.5 la (J - A)^2 + 0.5 mu (exp(c (I2 - 3) / 2) - 1)
+    double trS2 = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) trS2 += vg[i] * s[i] * s[i];
+    const double ex = exp(0.5 * kFungC * (trS2 - 3.0));
+    const double A = 1.0 + (mu + mu * kFungC) / la;
+    const double a = mu + 0.5 * kFungC * mu * ex, b = 0.5 * kFungC * kFungC * mu * ex;
+    k = la * (J - A);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      g[i] = a * vg[i] * s[i] + k * dJ[i];
+      H[sym6(i, i)] = a * vg[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = i; j < 6; ++j) H[sym6(i, j)] += b * vg[i] * s[i] * vg[j] * s[j];
   } else {  // SNH
     k = la * (J - 1.0) - mu;
 #pragma unroll
@@ -279,7 +322,7 @@ MFEM_DEV void material_grad_hess(int type, double mu, double la, const double s[
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = i; j < 6; ++j) H[sym6(i, j)] += la * dJ[i] * dJ[j];
+    for (int j = i; j < 6; ++j) H[sym6(i, j)] += cjj * dJ[i] * dJ[j];
   // + k * d2J (d2J is linear in s)
   double a = s[0], b = s[1], c = s[2], d = s[3], e = s[4], f = s[5];
   H[sym6(0, 1)] += k * c;

@@ -50,7 +50,8 @@ MaterialModelFactory::MaterialModelFactory() {
 }
 
 bool material_supported_on_device(MaterialModelType t) {
-  return t == MATERIAL_SNH || t == MATERIAL_ARAP || t == MATERIAL_COROT || t == MATERIAL_FCR;
+  return t == MATERIAL_SNH || t == MATERIAL_NH || t == MATERIAL_ARAP || t == MATERIAL_COROT ||
+         t == MATERIAL_FCR || t == MATERIAL_FUNG;
 }
 
 }  // namespace mfem

@@ -178,6 +178,29 @@ void material_s6(int type, double mu, double la, const Vec<6>& s, bool psd_fix,
       if (psd_fix) simple_psd_fix<6>(HH);
       break;
     }
+    case MAT_NH: {  // derivs.m:50: 0.5 mu (I2 - 3) - mu log J + 0.5 la (log J)^2 (neohookean.cpp:8-101)
+      // The Hessian is the symmetric closed form; neohookean.cpp:61 (H(0,2))
+      // carries a typo (S2*3 for S2*S3) that makes the reference's matrix
+      // asymmetric, which this restatement does not reproduce.
+      const double lJ = std::log(J);
+      e = 0.5 * mu * (trS2 - 3.0) - mu * lJ + 0.5 * la * lJ * lJ;
+      const double k1 = (la * lJ - mu) / J, k2 = (la + mu - la * lJ) / (J * J);
+      gg = mu * dI + k1 * dJ;
+      HH = mu * D + k2 * (dJ * dJ.T()) + k1 * d2J;
+      if (psd_fix) simple_psd_fix<6>(HH);
+      break;
+    }
+    case MAT_FUNG: {  // derivs.m:41-42 with c = 4 (fung.cpp:7-97)
+      const double cf = 4.0;
+      const double ex = std::exp(0.5 * cf * (trS2 - 3.0));
+      const double A = 1.0 + (mu + mu * cf) / la;
+      e = 0.5 * mu * (trS2 - 3.0) + 0.5 * la * (J - A) * (J - A) + 0.5 * mu * (ex - 1.0);
+      const double a = mu + 0.5 * cf * mu * ex;
+      gg = a * dI + (la * (J - A)) * dJ;
+      HH = a * D + (0.5 * cf * cf * mu * ex) * (dI * dI.T()) + la * (dJ * dJ.T()) + (la * (J - A)) * d2J;
+      if (psd_fix) simple_psd_fix<6>(HH);
+      break;
+    }
     case MAT_ARAP: {  // arap.cpp:7-65 (no PSD fix; constant Hessian)
       Vec<6> d = s; d[0] -= 1; d[1] -= 1; d[2] -= 1;
       for (int i = 0; i < 6; ++i) e += kVoigt[i] * d[i] * d[i];

@@ -67,11 +67,12 @@ def test_material_batch_matches_oracle(native, oracle):
     S = np.concatenate([np.array([1.11, 1.2, 1.3, 0.2, 0.3, 0.4])[None],
                         np.tile([1, 1, 1, 0, 0, 0.0], (8, 1)) + 0.3 * rng.uniform(-1, 1, (8, 6)),
                         np.tile([0.5, 0.6, 0.4, 0, 0, 0.0], (8, 1)) + 0.05 * rng.uniform(-1, 1, (8, 6))])
-    for mtype in (0, 2, 3, 4):
+    for mtype in (0, 1, 2, 3, 4, 5):
         for psd in (False, True):
             E, g, H = native.material_s6_batch(mtype, mu, la, S, psd)
             for i in range(S.shape[0]):
                 Eo, go, Ho = oracle.material_s6(mtype, mu, la, S[i], psd)
+                assert np.isfinite(Eo), (mtype, i)
                 assert abs(E[i] - Eo) <= 1e-12 * max(abs(Eo), mu)
                 assert np.abs(g[i] - go).max() <= 1e-12 * max(np.abs(go).max(), mu)
                 assert np.abs(H[i] - Ho).max() <= 1e-9 * np.abs(Ho).max(), (mtype, psd, i)
@@ -225,3 +226,52 @@ def test_errors_are_reported(native):
     bad.T[0, 0] = 10 ** 6
     with pytest.raises(native.MfemError, match="out of range"):
         native.Sim(bad)
+
+
+def test_c5_mixed_materials_small(native, oracle):
+    """The stiff config at reduced size (13x5x5x6 = 1950 tets): ARAP and SNH
+    regions split by the seeded plane, E log-uniform in [1e4, 1e8], nu 0.49.
+    FP64 device vs oracle to POS_TOL, mixed device vs oracle below 1e-6."""
+    sc = scenes.config_c5(13, 5, 5, itr_tol=1e-10, max_iters=20000, max_newton=20,
+                          newton_tol=1e-8)
+    assert 0.1 < sc.extra["arap_fraction"] < 0.9
+    o = oracle.Sim(sc)
+    g = native.Sim(sc)
+    gm = native.Sim(sc, precision="mixed")
+    for k in range(2):
+        o.step()
+        g.step()
+        gm.step()
+        x_o = o.positions()
+        assert relerr(g.positions(), x_o) < POS_TOL, k
+        assert np.linalg.norm(gm.positions() - x_o) / np.linalg.norm(x_o) < 1e-6, k
+
+
+def test_c2_plain_small(native, oracle):
+    """The plain FP64 config at reduced size (10x3x3x6 = 540 tets), CG to
+    rel-res 1e-10: positions and Newton counts against the oracle."""
+    sc = scenes.config_c2(10, 3, 3)
+    o = oracle.Sim(sc)
+    g = native.Sim(sc)
+    for k in range(3):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < POS_TOL, k
+        assert g.stats()["newton_iters"] == o.stats()["newton_iters"], k
+
+
+@pytest.mark.parametrize("material", ["Neohookean", "Fung"])
+def test_mixed_nh_fung_trajectory(native, oracle, material):
+    """Mixed-stretch beams of the log-barrier Neo-Hookean and the exponential
+    Fung models (reference neohookean.cpp, fung.cpp) against the oracle.
+    These scenes exhaust the line search once near convergence (energy
+    differences at roundoff, c = 0), where the two sides' different
+    summation orders may decide differently; positions are compared, not
+    Newton counts."""
+    sc = scenes.beam_scene(6, 3, 3, seed=11, mixed=True, E=1e5, material=material,
+                           itr_tol=1e-12, max_iters=20000, max_newton=20, newton_tol=1e-9)
+    o, g = _pair(sc)
+    for k in range(3):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < POS_TOL, (material, k)

@@ -111,7 +111,7 @@ def fd_grad_hess(fun, x, eps=1e-6):
     return g, H
 
 
-@pytest.mark.parametrize("mtype", [0, 2, 3, 4])
+@pytest.mark.parametrize("mtype", [0, 1, 2, 3, 4, 5])
 def test_material_s6_derivatives(oracle, mtype):
     fun = lambda s: oracle.material_s6(mtype, MU, LA, s, psd_fix=False)
     E, g, H = fun(S_REF)
@@ -119,6 +119,34 @@ def test_material_s6_derivatives(oracle, mtype):
     assert np.abs(fg - g).max() <= 1e-7 * np.abs(g).max()
     assert np.abs(fH - H).max() <= 1e-7 * np.abs(H).max()
     assert np.allclose(H, H.T)
+
+
+def _S_of(s):
+    return np.array([[s[0], s[3], s[4]], [s[3], s[1], s[5]], [s[4], s[5], s[2]]])
+
+
+def test_nh_and_fung_closed_forms(oracle):
+    """Energies against derivs.m evaluated independently (I2 = tr S^T S and
+    I3 = det S by numpy): NH (derivs.m:50) is stress-free at S = I; the
+    reference's Fung (derivs.m:41-42, c = 4 as fung.cpp:7) is not: its
+    volumetric offset (mu + c mu) / la leaves grad Psi(I) = -2 mu on the
+    diagonal, which both the oracle and the device path keep."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        s = np.array([1, 1, 1, 0, 0, 0.0]) + 0.2 * rng.uniform(-1, 1, 6)
+        S = _S_of(s)
+        I2, I3 = np.trace(S.T @ S), np.linalg.det(S)
+        nh = 0.5 * MU * (I2 - 3) - MU * np.log(I3) + 0.5 * LA * np.log(I3) ** 2
+        c = 4.0
+        fung = (0.5 * MU * (I2 - 3) + 0.5 * LA * (I3 - 1 - (MU + MU * c) / LA) ** 2
+                + 0.5 * MU * (np.exp(c * 0.5 * (I2 - 3)) - 1))
+        assert abs(oracle.material_s6(1, MU, LA, s, psd_fix=False)[0] - nh) <= 1e-12 * max(1, abs(nh))
+        assert abs(oracle.material_s6(5, MU, LA, s, psd_fix=False)[0] - fung) <= 1e-12 * max(1, abs(fung))
+    I = np.array([1, 1, 1, 0, 0, 0.0])
+    E, g, H = oracle.material_s6(1, MU, LA, I, psd_fix=False)
+    assert abs(E) < 1e-12 * MU and np.abs(g).max() < 1e-12 * MU
+    E, g, H = oracle.material_s6(5, MU, LA, I, psd_fix=False)
+    assert np.allclose(g, [-2 * MU] * 3 + [0] * 3, rtol=1e-12, atol=1e-9 * MU)
 
 
 def test_snh_closed_form_invariants(oracle):
