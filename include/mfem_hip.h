@@ -128,6 +128,31 @@ int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks);
 int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* variants, int* grids, int* count,
                        int* chosen, int* lanes, int* unroll, int* threads, int* blocks);
 
+/* Batch of independent mixed-stretch simulations (no counterpart in the
+ * reference, where each would be its own SimState + NewtonOptimizer,
+ * reference newton_optimizer.cpp:11-89). The meshes are advanced together
+ * as one block-diagonal system, one launch per Newton phase for all of
+ * them, while each keeps its own CG scalars, line search and Newton
+ * termination. Time step, integrator, gravity, solver settings and
+ * precision must agree across `sims`; "mixed" must be 1. */
+typedef struct mfem_batch mfem_batch;
+int mfem_batch_create(const mfem_sim_params* sims, int count, mfem_batch** out);
+int mfem_batch_destroy(mfem_batch* batch);
+/* One time step of every simulation. */
+int mfem_batch_step(mfem_batch* batch);
+/* count, then per simulation vertex and tet counts (arrays of count; NULL
+ * allowed). */
+int mfem_batch_sizes(mfem_batch* batch, int* count, int* nV, int* nT);
+int mfem_batch_get_positions(mfem_batch* batch, int sim, double* out);
+/* Per-simulation stats of the last step (arrays of count; NULL allowed) and
+ * the step's wall time. */
+int mfem_batch_stats(mfem_batch* batch, int* newton_iters, int* pcg_iters, int* refinements,
+                     int* ls_fail, double* decrement, double* step_ms);
+/* Batched CG shape (as mfem_sim_set_cg_shape) and rows per workgroup;
+ * variant -1 restores autotuning on the next solve. */
+int mfem_batch_set_cg_shape(mfem_batch* batch, int variant, int rows_per_workgroup);
+int mfem_batch_cg_shape(mfem_batch* batch, int* variant, int* rows_per_workgroup);
+
 /* Batched MaterialModel evaluation on the 6-vector s (reference
  * energies/material_model.h:29-38): device pointers, n points, s n x 6,
  * E n, g n x 6, H n x 36 column-major; psd_fix as hessian(s, psd_fix). */

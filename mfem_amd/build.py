@@ -19,6 +19,7 @@ SOURCES = [
     "k_element.hip",
     "k_solver.hip",
     "k_vector.hip",
+    "host/batch.cpp",
     "host/json.cpp",
     "host/material_model.cpp",
     "host/mesh.cpp",

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "host/batch.h"
 #include "host/sim.h"
 #include "kernels.h"
 
@@ -17,6 +18,10 @@ using namespace mfem;
 struct mfem_sim {
   std::unique_ptr<NewtonOptimizer<3>> opt;
   SimState<3>& state() { return opt->state(); }
+};
+
+struct mfem_batch {
+  std::unique_ptr<BatchSim> sim;
 };
 
 namespace {
@@ -152,6 +157,47 @@ void finish_create(std::unique_ptr<SimState<3>> state, mfem_sim** out) {
   *out = sim.release();
 }
 
+SceneArrays scene_of(const mfem_sim_params& p) {
+  if (p.nV <= 0 || p.nT <= 0) throw std::runtime_error("empty mesh");
+  if (!p.V || !p.T || !p.mu || !p.la || !p.mat) throw std::runtime_error("null mesh array");
+  SceneArrays sc;
+  sc.V.assign(p.V, p.V + 3L * p.nV);
+  sc.T.assign(p.T, p.T + 4L * p.nT);
+  for (int v : sc.T)
+    if (v < 0 || v >= p.nV) throw std::runtime_error("tet vertex index out of range");
+  sc.mu.assign(p.mu, p.mu + p.nT);
+  sc.la.assign(p.la, p.la + p.nT);
+  sc.mat.assign(p.mat, p.mat + p.nT);
+  if (p.fixed) sc.fixed.assign(p.fixed, p.fixed + p.nV);
+  if (p.v0) sc.v0.assign(p.v0, p.v0 + 3L * p.nV);
+  sc.density = p.density;
+  sc.bc.type = (BCScriptType)p.bc_type;
+  sc.bc.axis = p.bc_axis;
+  sc.bc.ratio = p.bc_ratio;
+  return sc;
+}
+
+std::shared_ptr<SimConfig> config_of(const mfem_sim_params& p) {
+  auto cfg = std::make_shared<SimConfig>();
+  cfg->h = p.dt;
+  for (int i = 0; i < 3; ++i) cfg->ext[i] = (float)p.gravity[i];
+  if (p.bdf_order < 1 || p.bdf_order > 6) throw std::runtime_error("bdf_order must be 1..6");
+  cfg->ti_type = (TimeIntegratorType)(p.bdf_order - 1);
+  LinearSolverFactory<3> lf;
+  const std::string sname = p.linear_solver ? p.linear_solver : "eigen-pcg-diag";
+  if (!lf.has_name(sname)) throw std::runtime_error("unknown linear solver '" + sname + "'");
+  cfg->solver_type = lf.type_by_name(sname);
+  cfg->itr_tol = p.itr_tol;
+  cfg->max_iterative_solver_iters = p.max_iters;
+  cfg->outer_steps = p.max_newton;
+  cfg->newton_tol = p.newton_tol;
+  cfg->ls_iters = p.ls_iters;
+  cfg->precision = p.precision ? Precision::MIXED : Precision::FP64;
+  cfg->mixed_variables.clear();
+  cfg->variables.clear();
+  return cfg;
+}
+
 }  // namespace
 
 extern "C" {
@@ -172,38 +218,8 @@ int mfem_sim_create_json(const char* json_path, mfem_sim** out) {
 int mfem_sim_create(const mfem_sim_params* p, mfem_sim** out) {
   return guarded([&] {
     if (!p || !out) throw std::runtime_error("null argument");
-    if (p->nV <= 0 || p->nT <= 0) throw std::runtime_error("empty mesh");
-    SceneArrays sc;
-    sc.V.assign(p->V, p->V + 3L * p->nV);
-    sc.T.assign(p->T, p->T + 4L * p->nT);
-    for (int v : sc.T)
-      if (v < 0 || v >= p->nV) throw std::runtime_error("tet vertex index out of range");
-    sc.mu.assign(p->mu, p->mu + p->nT);
-    sc.la.assign(p->la, p->la + p->nT);
-    sc.mat.assign(p->mat, p->mat + p->nT);
-    if (p->fixed) sc.fixed.assign(p->fixed, p->fixed + p->nV);
-    if (p->v0) sc.v0.assign(p->v0, p->v0 + 3L * p->nV);
-    sc.density = p->density;
-    sc.bc.type = (BCScriptType)p->bc_type;
-    sc.bc.axis = p->bc_axis;
-    sc.bc.ratio = p->bc_ratio;
-    auto cfg = std::make_shared<SimConfig>();
-    cfg->h = p->dt;
-    for (int i = 0; i < 3; ++i) cfg->ext[i] = (float)p->gravity[i];
-    if (p->bdf_order < 1 || p->bdf_order > 6) throw std::runtime_error("bdf_order must be 1..6");
-    cfg->ti_type = (TimeIntegratorType)(p->bdf_order - 1);
-    LinearSolverFactory<3> lf;
-    const std::string sname = p->linear_solver ? p->linear_solver : "eigen-pcg-diag";
-    if (!lf.has_name(sname)) throw std::runtime_error("unknown linear solver '" + sname + "'");
-    cfg->solver_type = lf.type_by_name(sname);
-    cfg->itr_tol = p->itr_tol;
-    cfg->max_iterative_solver_iters = p->max_iters;
-    cfg->outer_steps = p->max_newton;
-    cfg->newton_tol = p->newton_tol;
-    cfg->ls_iters = p->ls_iters;
-    cfg->precision = p->precision ? Precision::MIXED : Precision::FP64;
-    cfg->mixed_variables.clear();
-    cfg->variables.clear();
+    SceneArrays sc = scene_of(*p);
+    auto cfg = config_of(*p);
     auto state = std::make_unique<SimState<3>>();
     std::vector<std::string> mixed, plain;
     if (p->mixed) mixed.push_back("mixed-stretch");
@@ -429,6 +445,84 @@ int mfem_polar_batch(int n, const double* F, double* R, double* s, double* dsdF,
   return guarded([&] {
     mfemhip::launch_batch_polar(n, F, R, s, dsdF, iters, as_stream(stream));
     MFEM_HIP_CHECK(hipGetLastError());
+  });
+}
+
+int mfem_batch_create(const mfem_sim_params* sims, int count, mfem_batch** out) {
+  return guarded([&] {
+    if (!sims || !out || count <= 0) throw std::runtime_error("null argument or empty batch");
+    const mfem_sim_params& p0 = sims[0];
+    std::vector<SceneArrays> scenes;
+    for (int k = 0; k < count; ++k) {
+      const mfem_sim_params& p = sims[k];
+      if (p.dt != p0.dt || p.bdf_order != p0.bdf_order || p.precision != p0.precision ||
+          p.itr_tol != p0.itr_tol || p.max_iters != p0.max_iters || p.max_newton != p0.max_newton ||
+          p.newton_tol != p0.newton_tol || p.ls_iters != p0.ls_iters ||
+          p.gravity[0] != p0.gravity[0] || p.gravity[1] != p0.gravity[1] ||
+          p.gravity[2] != p0.gravity[2])
+        throw std::runtime_error("batch: time step, integrator, gravity, solver settings and "
+                                 "precision must be the same for every simulation");
+      if (!p.mixed) throw std::runtime_error("batch: only mixed-stretch simulations are batched");
+      scenes.push_back(scene_of(p));
+    }
+    auto cfg = config_of(p0);
+    if (cfg->solver_type != SOLVER_EIGEN_CG_DIAG)
+      throw std::runtime_error("batch: the linear solver must be \"eigen-pcg-diag\"");
+    auto b = std::make_unique<mfem_batch>();
+    b->sim = std::make_unique<BatchSim>(scenes, cfg);
+    *out = b.release();
+  });
+}
+
+int mfem_batch_destroy(mfem_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int mfem_batch_step(mfem_batch* b) {
+  return guarded([&] { b->sim->step(); });
+}
+
+int mfem_batch_sizes(mfem_batch* b, int* count, int* nV, int* nT) {
+  return guarded([&] {
+    const int n = b->sim->count();
+    if (count) *count = n;
+    for (int k = 0; k < n; ++k) {
+      if (nV) nV[k] = b->sim->vertices(k);
+      if (nT) nT[k] = b->sim->tets(k);
+    }
+  });
+}
+
+int mfem_batch_get_positions(mfem_batch* b, int sim, double* out) {
+  return guarded([&] {
+    auto x = b->sim->positions(sim);
+    std::memcpy(out, x.data(), x.size() * sizeof(double));
+  });
+}
+
+int mfem_batch_stats(mfem_batch* b, int* newton_iters, int* pcg_iters, int* refinements,
+                     int* ls_fail, double* decrement, double* step_ms) {
+  return guarded([&] {
+    const auto& st = b->sim->stats();
+    for (size_t k = 0; k < st.size(); ++k) {
+      if (newton_iters) newton_iters[k] = st[k].newton_iters;
+      if (pcg_iters) pcg_iters[k] = st[k].pcg_iters;
+      if (refinements) refinements[k] = st[k].refinements;
+      if (ls_fail) ls_fail[k] = st[k].ls_fail;
+      if (decrement) decrement[k] = st[k].decrement;
+    }
+    if (step_ms) *step_ms = b->sim->last_step_ms();
+  });
+}
+
+int mfem_batch_set_cg_shape(mfem_batch* b, int variant, int rows_per_workgroup) {
+  return guarded([&] { b->sim->force_cg_shape(variant, rows_per_workgroup); });
+}
+
+int mfem_batch_cg_shape(mfem_batch* b, int* variant, int* rows_per_workgroup) {
+  return guarded([&] {
+    if (variant) *variant = b->sim->cg_variant();
+    if (rows_per_workgroup) *rows_per_workgroup = b->sim->rows_per_workgroup();
   });
 }
 

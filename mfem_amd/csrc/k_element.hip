@@ -322,6 +322,64 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
   block_partial<kElemBlock, 1>(local_max, max_ds.partials, sm);
 }
 
+// Line-search energy of tet e at (x + alpha dx, s + alpha ds): inertia share
+// (consistent mass), h^2 vol Psi(s) and the constraint term -vol lambda^T
+// Sym (S(x) - s), with S(x) from a polar fit warm-started from R (the stored
+// R is not changed).
+__device__ __forceinline__ double mixed_energy_tet(const TetMeshDev& m, int e,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ dx, double alpha,
+                                                   const Inertia& in, const double* __restrict__ R,
+                                                   const double* __restrict__ s,
+                                                   const double* __restrict__ ds,
+                                                   const double* __restrict__ lam, double h2) {
+  const int nT = m.nT;
+  int4 t = m.T[e];
+  int v[4] = {t.x, t.y, t.z, t.w};
+  double q[12], dsum[3] = {0, 0, 0}, dsq = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int f = m.free_map[v[k]];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double qq = x[3 * v[k] + r] + (f >= 0 ? alpha * dx[3 * f + r] : 0.0);
+      q[3 * k + r] = qq;
+      double d = qq - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+      dsum[r] += d;
+      dsq += d * d;
+    }
+  }
+  const double vol = m.vol[e];
+  double e_in = 0.5 * (m.density * vol / 20.0) *
+                (dsq + dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2]);
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double d = m.dX[(3 * k + c) * nT + e];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) F[r + 3 * c] += q[3 * k + r] * d;
+    }
+  double Rl[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Rl[i] = R[i * nT + e];
+  procrustes(Rl, F, 1e-6, 100);
+  double Sm[9], sv[6], sa[6];
+  stretch_of(Rl, F, sv, Sm);
+  double ec = 0;
+  const double sym[6] = {1, 1, 1, 2, 2, 2};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    sa[i] = s[i * nT + e] + alpha * ds[i * nT + e];
+    ec += lam[i * nT + e] * sym[i] * (sv[i] - sa[i]);
+  }
+  double psi = material_energy(m.mat[e], m.mu[e], m.la[e], sa);
+  return e_in + h2 * vol * psi - vol * ec;
+}
+
 __global__ __launch_bounds__(kElemBlock) void k_mixed_energy(
     TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha,
     Inertia in, const double* __restrict__ R, const double* __restrict__ s,
@@ -329,54 +387,22 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_energy(
   __shared__ double sm[kElemBlock / kWave];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double en = 0.0;
-  if (e < m.nT) {
-    const int nT = m.nT;
-    int4 t = m.T[e];
-    int v[4] = {t.x, t.y, t.z, t.w};
-    double q[12], dsum[3] = {0, 0, 0}, dsq = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      int f = m.free_map[v[k]];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        double qq = x[3 * v[k] + r] + (f >= 0 ? alpha * dx[3 * f + r] : 0.0);
-        q[3 * k + r] = qq;
-        double d = qq - in.x_tilde[3 * v[k] + r] - in.h2g[r];
-        dsum[r] += d;
-        dsq += d * d;
-      }
-    }
-    const double vol = m.vol[e];
-    double e_in = 0.5 * (m.density * vol / 20.0) *
-                  (dsq + dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2]);
-    double F[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) F[i] = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double d = m.dX[(3 * k + c) * nT + e];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) F[r + 3 * c] += q[3 * k + r] * d;
-      }
-    double Rl[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) Rl[i] = R[i * nT + e];
-    procrustes(Rl, F, 1e-6, 100);
-    double Sm[9], sv[6], sa[6];
-    stretch_of(Rl, F, sv, Sm);
-    double ec = 0;
-    const double sym[6] = {1, 1, 1, 2, 2, 2};
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      sa[i] = s[i * nT + e] + alpha * ds[i * nT + e];
-      ec += lam[i * nT + e] * sym[i] * (sv[i] - sa[i]);
-    }
-    double psi = material_energy(m.mat[e], m.mu[e], m.la[e], sa);
-    en = e_in + h2 * vol * psi - vol * ec;
-  }
+  if (e < m.nT) en = mixed_energy_tet(m, e, x, dx, alpha, in, R, s, ds, lam, h2);
   block_partial<kElemBlock, 0>(en, out.partials, sm);
+}
+
+// Batched simulations: energy of every tet with the step length of its
+// simulation (tets of simulations that are off get 0).
+__global__ __launch_bounds__(kElemBlock) void k_mixed_energy_batch(
+    TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx,
+    const int* __restrict__ tet_sim, const double* __restrict__ alpha_sim,
+    const int* __restrict__ sim_on, Inertia in, const double* __restrict__ R,
+    const double* __restrict__ s, const double* __restrict__ ds, const double* __restrict__ lam,
+    double h2, double* __restrict__ e_tet) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int b = tet_sim[e];
+  e_tet[e] = sim_on[b] ? mixed_energy_tet(m, e, x, dx, alpha_sim[b], in, R, s, ds, lam, h2) : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -638,6 +664,16 @@ void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real*
   hipLaunchKernelGGL(k_mixed_local_solve<Real>, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st,
                      m, dx, dsdF, Hpsi, Hinv, g, gl, lam, ds, max_ds);
   launch_finalize(max_ds.partials, reduce_grid(m.nT), 1, max_ds.result, st);
+}
+
+void launch_mixed_energy_batch(const TetMeshDev& m, const double* x_full, const double* dx,
+                               const int* tet_sim, const double* alpha_sim, const int* sim_on,
+                               const Inertia& in, const double* R, const double* s,
+                               const double* ds, const double* lam, double h2, double* e_tet,
+                               hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_mixed_energy_batch, dim3(grid_for(m.nT, kElemBlock)), dim3(kElemBlock), 0,
+                     st, m, x_full, dx, tet_sim, alpha_sim, sim_on, in, R, s, ds, lam, h2, e_tet);
 }
 
 void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double* dx, double alpha,

@@ -364,11 +364,12 @@ __device__ __forceinline__ void cg_account(const unsigned long long* __restrict_
   }
 }
 
-// Sum of the kCgParts partial arrays (layout [k][nb]) by one workgroup, with
-// all of a thread's loads issued before the first add.
+// Sum of the kCgParts partial arrays (partial i of quantity k at
+// parts[k * stride + i], i < nb) by one workgroup, with all of a thread's
+// loads issued before the first add.
 template <int BS>
-__device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts, int nb,
-                                                double (&v)[kCgParts], double* sm) {
+__device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts, int stride,
+                                                int nb, double (&v)[kCgParts], double* sm) {
   constexpr int R = BS >= 1024 ? 1 : 4;  // loads in flight per thread (register budget)
 #pragma unroll
   for (int k = 0; k < kCgParts; ++k) v[k] = 0.0;
@@ -378,7 +379,7 @@ __device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts
     for (int j = 0; j < R; ++j) {
       const int i = i0 + j * BS;
 #pragma unroll
-      for (int k = 0; k < kCgParts; ++k) t[j][k] = i < nb ? parts[k * nb + i] : 0.0;
+      for (int k = 0; k < kCgParts; ++k) t[j][k] = i < nb ? parts[(size_t)k * stride + i] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < R; ++j)
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
   double alpha = 0.0, beta = 0.0;
   if (it > 0) {
     double v[kCgParts];
-    cg_reduce_parts<BS>(parts_old, nb, v, sm);
+    cg_reduce_parts<BS>(parts_old, nb, nb, v, sm);
     if (stamps_old && blockIdx.x == 0) cg_account<BS>(stamps_old, nb, ctrl, sm);
     const double pq = v[0], zq = v[1], qdq = v[2], rr = v[3], rz = v[4];
     const int j = it - 1;  // iteration of the partials
@@ -470,6 +471,111 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
       stamps_new[2 * blockIdx.x + 1] = (unsigned long long)wall_clock64();
     }
   }
+}
+
+// Batched CG: many independent systems (the diagonal blocks of a block-
+// diagonal K, one per simulation) advance one iteration per launch. The
+// workgroups [wg0[b], wg0[b+1]) belong to system b and split its block rows
+// [row0[b], row0[b+1]) into equal chunks; each system has its own control
+// block, scalars and Eigen stopping test (the single-system kernel above,
+// with the partial-sum reduction restricted to the system's workgroups).
+template <int G, int U, int BS, typename T>
+__global__ __launch_bounds__(BS) void k_cg_iter_batch(
+    int it, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const T* __restrict__ val, const CgDof<T>* __restrict__ s_old, CgDof<T>* __restrict__ s_new,
+    double* __restrict__ x, PcgCtrl* ctrls, const double* __restrict__ parts_old,
+    double* __restrict__ parts_new, int nwg, const int* __restrict__ wg_sys,
+    const int* __restrict__ wg0, const int* __restrict__ row0) {
+  __shared__ double sm[kCgParts * (BS / kWave)];
+  const int b = wg_sys[blockIdx.x];
+  PcgCtrl* ctrl = ctrls + b;
+  if (ctrl->done) return;
+  const int w0 = wg0[b], nw = wg0[b + 1] - w0;
+  double alpha = 0.0, beta = 0.0;
+  if (it > 0) {
+    double v[kCgParts];
+    cg_reduce_parts<BS>(parts_old + w0, nwg, nw, v, sm);
+    const double pq = v[0], zq = v[1], qdq = v[2], rr = v[3], rz = v[4];
+    const int j = it - 1;
+    const bool converged = rr < ctrl->threshold;
+    if (converged || j >= ctrl->max_iters) {
+      if (blockIdx.x == w0 && threadIdx.x == 0) {
+        ctrl->rr = rr;
+        ctrl->iter = converged ? (j > 0 ? j - 1 : 0) : j;
+        ctrl->stop_it = it;
+        ctrl->done = 1;
+      }
+      return;
+    }
+    alpha = rz / pq;
+    beta = (rz - 2.0 * alpha * zq + alpha * alpha * qdq) / rz;
+    if (blockIdx.x == w0 && threadIdx.x == 0) ctrl->rr = rr;
+  }
+  const T a = (T)alpha, bt = (T)beta;
+  const int lane = threadIdx.x & (G - 1);
+  const int lb = blockIdx.x - w0;
+  const long long r0 = row0[b], r1 = row0[b + 1];
+  const int c0 = (int)(r0 + (r1 - r0) * lb / nw);
+  const int c1 = (int)(r0 + (r1 - r0) * (lb + 1) / nw);
+  double acc[kCgParts];
+#pragma unroll
+  for (int k = 0; k < kCgParts; ++k) acc[k] = 0.0;
+  for (int row = c0 + (int)threadIdx.x / G; row < c1; row += BS / G) {
+    T y0, y1, y2;
+    bsr_row_batched<G, U>(row, lane, rowptr, col, val,
+                          [&](int c, int k) {
+                            T r, z, p;
+                            cg_form(s_old[3 * c + k], a, bt, r, z, p);
+                            return p;
+                          },
+                          y0, y1, y2);
+    if (lane < 3) {
+      const int i = 3 * row + lane;
+      const CgDof<T> st = s_old[i];
+      T r, z, p;
+      cg_form(st, a, bt, r, z, p);
+      x[i] += alpha * (double)st.p;
+      const T q = lane == 0 ? y0 : (lane == 1 ? y1 : y2);
+      CgDof<T> o;
+      o.r = r;
+      o.q = q;
+      o.p = p;
+      o.d = st.d;
+      s_new[i] = o;
+      const double dq = (double)q, dz = (double)z, dr = (double)r;
+      acc[0] += (double)p * dq;
+      acc[1] += dz * dq;
+      acc[2] += dq * ((double)st.d * dq);
+      acc[3] += dr * dr;
+      acc[4] += dr * dz;
+    }
+  }
+  block_sum_n<BS, kCgParts>(acc, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kCgParts; ++k) parts_new[(size_t)k * nwg + blockIdx.x] = acc[k];
+  }
+}
+
+// Per-system control blocks: threshold max(tol_b^2 |b_b|^2, DBL_MIN) from the
+// per-system |b|^2; systems that are off (or have b = 0) start done.
+__global__ void k_cg_setup_batch(int nsys, PcgCtrl* ctrls, const double* __restrict__ b2,
+                                 const double* __restrict__ tol, const int* __restrict__ max_iters,
+                                 const int* __restrict__ on) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nsys) return;
+  PcgCtrl c;
+  c.b2 = b2[b];
+  c.rr = c.b2;
+  c.threshold = fmax(tol[b] * tol[b] * c.b2, DBL_MIN);
+  c.kern_ticks = 0.0;
+  c.iter = 0;
+  c.max_iters = max_iters[b];
+  c.stop_it = 0;
+  c.kern_launches = 0;
+  c.pad_ = 0;
+  c.done = (!on[b] || c.b2 == 0.0) ? 1 : 0;
+  ctrls[b] = c;
 }
 
 // x = 0, state = {b, 0, 0, D^-1}; partials of b.b
@@ -648,6 +754,46 @@ void launch_pcg_iteration(const PcgBuffers<T>& B, int it, hipStream_t st) {
       throw std::invalid_argument("CG kernel shape out of range");
   }
 }
+
+template <typename T>
+void launch_pcg_batch_init(const BatchCgBuffers<T>& B, double* scratch, hipStream_t st) {
+  const int gv = std::min(grid(B.n, kBS), 256);
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(gv), dim3(kBS), 0, st, B.n, B.b, B.invdiag, B.x,
+                     reinterpret_cast<CgDof<T>*>(B.state[0]), scratch);
+}
+
+void launch_pcg_batch_setup(int nsys, PcgCtrl* ctrls, const double* b2, const double* tol,
+                            const int* max_iters, const int* on, hipStream_t st) {
+  if (nsys == 0) return;
+  hipLaunchKernelGGL(k_cg_setup_batch, dim3(grid(nsys, kBS)), dim3(kBS), 0, st, nsys, ctrls, b2, tol,
+                     max_iters, on);
+}
+
+template <typename T>
+void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st) {
+  if (B.nwg <= 0) return;
+  const auto* so = reinterpret_cast<const CgDof<T>*>(B.state[it & 1]);
+  auto* sn = reinterpret_cast<CgDof<T>*>(B.state[(it + 1) & 1]);
+  const double* po = B.parts[(it + 1) & 1];
+  double* pn = B.parts[it & 1];
+  switch (B.variant) {
+#define MFEM_CGB(i)                                                                             \
+  case i:                                                                                      \
+    hipLaunchKernelGGL((k_cg_iter_batch<kCgShapes[i].G, kCgShapes[i].U, kCgShapes[i].BS, T>),    \
+                       dim3(B.nwg), dim3(kCgShapes[i].BS), 0, st, it, B.rowptr, B.col, B.K, so,  \
+                       sn, B.x, B.ctrls, po, pn, B.nwg, B.wg_sys, B.wg0, B.row0);              \
+    break;
+    MFEM_CGB(0) MFEM_CGB(1) MFEM_CGB(2) MFEM_CGB(3) MFEM_CGB(4) MFEM_CGB(5) MFEM_CGB(6) MFEM_CGB(7)
+#undef MFEM_CGB
+    default:
+      throw std::invalid_argument("CG kernel shape out of range");
+  }
+}
+
+template void launch_pcg_batch_init<double>(const BatchCgBuffers<double>&, double*, hipStream_t);
+template void launch_pcg_batch_init<float>(const BatchCgBuffers<float>&, double*, hipStream_t);
+template void launch_pcg_batch_iteration<double>(const BatchCgBuffers<double>&, int, hipStream_t);
+template void launch_pcg_batch_iteration<float>(const BatchCgBuffers<float>&, int, hipStream_t);
 
 template void launch_assemble_bsr<double>(int, const int*, const int*, const double*, const double*,
                                           double*, hipStream_t);

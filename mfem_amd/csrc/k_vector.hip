@@ -126,4 +126,71 @@ void launch_axpy(long n, double alpha, const double* x, double* y, hipStream_t s
   hipLaunchKernelGGL(k_axpy, dim3(grid(n, kBS)), dim3(kBS), 0, st, n, alpha, x, y);
 }
 
+namespace {
+
+__global__ __launch_bounds__(kBS) void k_seg_reduce(const double* __restrict__ v, int ncomp,
+                                                    long comp_stride, const int* __restrict__ seg,
+                                                    int mode, double* __restrict__ out) {
+  __shared__ double sm[kBS / kWave];
+  const int b = blockIdx.x;
+  const int lo = seg[b], hi = seg[b + 1];
+  double acc = mode == 2 ? 0.0 : 0.0;
+  for (int c = 0; c < ncomp; ++c) {
+    const double* vc = v + (size_t)c * comp_stride;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += kBS) {
+      const double a = vc[i];
+      if (mode == 0) acc += a;
+      else if (mode == 1) acc += a * a;
+      else acc = fmax(acc, fabs(a));
+    }
+  }
+  const double r = mode == 2 ? block_reduce_all<kBS, 1>(acc, sm) : block_reduce_all<kBS, 0>(acc, sm);
+  if (threadIdx.x == 0) out[b] = r;
+}
+
+__global__ __launch_bounds__(kBS) void k_accept_batch(int nV, const int* __restrict__ free_map,
+                                                      const int* __restrict__ vert_sim, int nT,
+                                                      const int* __restrict__ tet_sim,
+                                                      const double* __restrict__ alpha_sim,
+                                                      double* __restrict__ x,
+                                                      const double* __restrict__ dx,
+                                                      double* __restrict__ s,
+                                                      const double* __restrict__ ds) {
+  const long i = (long)blockIdx.x * kBS + threadIdx.x;
+  for (long v = i; v < nV; v += (long)gridDim.x * kBS) {
+    const int f = free_map[v];
+    const double a = alpha_sim[vert_sim[v]];
+    if (f >= 0 && a != 0.0) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) x[3 * v + r] += a * dx[3 * f + r];
+    }
+  }
+  if (s) {
+    const long ns = 6L * nT;
+    for (long k = i; k < ns; k += (long)gridDim.x * kBS) {
+      const double a = alpha_sim[tet_sim[k % nT]];
+      if (a != 0.0) s[k] += a * ds[k];
+    }
+  }
+}
+
+}  // namespace
+
+void launch_seg_reduce(int nseg, const double* v, int ncomp, long comp_stride, const int* seg,
+                       int mode, double* out, hipStream_t st) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_seg_reduce, dim3(nseg), dim3(kBS), 0, st, v, ncomp, comp_stride, seg, mode,
+                     out);
+}
+
+void launch_accept_batch(int nV, const int* free_map, const int* vert_sim, int nT,
+                         const int* tet_sim, const double* alpha_sim, double* x_full,
+                         const double* dx, double* s, const double* ds, hipStream_t st) {
+  const long n = std::max<long>(nV, s ? 6L * nT : 0L);
+  if (n == 0) return;
+  const int g = (int)std::min<long>((n + kBS - 1) / kBS, 65535);
+  hipLaunchKernelGGL(k_accept_batch, dim3(g), dim3(kBS), 0, st, nV, free_map, vert_sim, nT, tet_sim,
+                     alpha_sim, x_full, dx, s, ds);
+}
+
 }  // namespace mfemhip

@@ -143,6 +143,55 @@ void launch_pcg_init(const PcgBuffers<T>& B, double tol, int max_iters, hipStrea
 template <typename T>
 void launch_pcg_iteration(const PcgBuffers<T>& B, int it, hipStream_t st);
 
+// Batched CG over the diagonal blocks of a block-diagonal K (independent
+// simulations): workgroups [wg0[b], wg0[b+1]) own the block rows
+// [row0[b], row0[b+1]) of system b; ctrls[b] is that system's control block.
+template <typename T>
+struct BatchCgBuffers {
+  int n = 0;  // scalar rows of all systems
+  const int* rowptr = nullptr;
+  const int* col = nullptr;
+  const T* K = nullptr;
+  const T* invdiag = nullptr;
+  const double* b = nullptr;
+  double* x = nullptr;
+  T* state[2] = {nullptr, nullptr};
+  PcgCtrl* ctrls = nullptr;
+  double* parts[2] = {nullptr, nullptr};  // kCgParts * nwg each
+  int nwg = 0;
+  const int* wg_sys = nullptr;  // [nwg] system of each workgroup
+  const int* wg0 = nullptr;     // [nsys + 1] first workgroup of each system
+  const int* row0 = nullptr;    // [nsys + 1] first block row of each system
+  int variant = 0;
+};
+// x = 0 and the initial state for all rows (scratch: >= 256 doubles)
+template <typename T>
+void launch_pcg_batch_init(const BatchCgBuffers<T>& B, double* scratch, hipStream_t st);
+// control blocks from per-system |b|^2, tolerance, iteration budget, on flag
+void launch_pcg_batch_setup(int nsys, PcgCtrl* ctrls, const double* b2, const double* tol,
+                            const int* max_iters, const int* on, hipStream_t st);
+template <typename T>
+void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st);
+
+// Per-segment reductions, one workgroup per segment in a fixed order:
+// out[b] = sum v (mode 0), sum v^2 (mode 1) or max |v| (mode 2) over
+// indices [seg[b], seg[b+1]) of each of ncomp components (component c at
+// v + c * comp_stride).
+void launch_seg_reduce(int nseg, const double* v, int ncomp, long comp_stride, const int* seg,
+                       int mode, double* out, hipStream_t st);
+// Batched line-search acceptance: x += alpha[sim of vertex] dx on free
+// vertices, s += alpha[sim of tet] ds.
+void launch_accept_batch(int nV, const int* free_map, const int* vert_sim, int nT,
+                         const int* tet_sim, const double* alpha_sim, double* x_full,
+                         const double* dx, double* s, const double* ds, hipStream_t st);
+// Batched line-search energies: e_tet[e] = energy of tet e at the step length
+// of its simulation (0 for simulations that are off).
+void launch_mixed_energy_batch(const TetMeshDev& m, const double* x_full, const double* dx,
+                               const int* tet_sim, const double* alpha_sim, const int* sim_on,
+                               const Inertia& in, const double* R, const double* s,
+                               const double* ds, const double* lam, double h2, double* e_tet,
+                               hipStream_t st);
+
 // --- vectors ---------------------------------------------------------------
 // x_full[free] += alpha * dx ; s += alpha * ds (line-search acceptance,
 // reference linesearch.h:120-126)

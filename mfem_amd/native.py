@@ -24,6 +24,9 @@ EXPORTED = [
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
     "mfem_sim_time_spmv", "mfem_sim_spmv_tuning", "mfem_sim_set_cg_shape", "mfem_sim_cg_tuning",
     "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
+    "mfem_batch_create", "mfem_batch_destroy", "mfem_batch_step", "mfem_batch_sizes",
+    "mfem_batch_get_positions", "mfem_batch_stats", "mfem_batch_set_cg_shape",
+    "mfem_batch_cg_shape",
 ]
 
 
@@ -91,6 +94,14 @@ def lib():
         L.mfem_material_s6_batch.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mfem_snh_f9_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.mfem_polar_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
+        L.mfem_batch_create.argtypes = [vp, C.c_int, vp]
+        L.mfem_batch_destroy.argtypes = [vp]
+        L.mfem_batch_step.argtypes = [vp]
+        L.mfem_batch_sizes.argtypes = [vp, ip, ip, ip]
+        L.mfem_batch_get_positions.argtypes = [vp, C.c_int, dp]
+        L.mfem_batch_stats.argtypes = [vp, ip, ip, ip, ip, dp, dp]
+        L.mfem_batch_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
+        L.mfem_batch_cg_shape.argtypes = [vp, ip, ip]
         _lib = L
     return _lib
 
@@ -108,6 +119,37 @@ def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _params(sc, precision, p=None):
+    """mfem_sim_params for a Scene; returns (params, arrays to keep alive)."""
+    if sc.dim != 3:
+        raise MfemError("the device backend is 3D (tetrahedra)")
+    keep = dict(V=_f64(sc.V), T=_i32(sc.T), mu=_f64(sc.mu), la=_f64(sc.la), mat=_i32(sc.mat),
+                fixed=_i32(sc.fixed), solver=sc.solver.encode())
+    p = SimParams() if p is None else p
+    p.nV, p.nT = sc.nV, sc.nT
+    p.V = keep["V"].ctypes.data_as(dp)
+    p.T = keep["T"].ctypes.data_as(ip)
+    p.mu = keep["mu"].ctypes.data_as(dp)
+    p.la = keep["la"].ctypes.data_as(dp)
+    p.mat = keep["mat"].ctypes.data_as(ip)
+    p.fixed = keep["fixed"].ctypes.data_as(ip)
+    if sc.v0 is not None:
+        keep["v0"] = _f64(sc.v0)
+        p.v0 = keep["v0"].ctypes.data_as(dp)
+    else:
+        p.v0 = dp()
+    p.density, p.dt = sc.density, sc.dt
+    g = list(sc.gravity) + [0.0] * 3
+    p.gravity = (C.c_double * 3)(*g[:3])
+    p.bdf_order = sc.bdf_order
+    p.mixed = int(sc.mixed)
+    p.linear_solver = keep["solver"]
+    p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
+    p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
+    p.precision = {"fp64": 0, "mixed": 1}[precision]
+    return p, keep
+
+
 class Sim:
     """One simulation on the GPU: SimState<3> + NewtonOptimizer<3> behind the C ABI."""
 
@@ -118,36 +160,9 @@ class Sim:
             check(lib().mfem_sim_create_json(json_path.encode(), C.byref(self.h)))
             self.nV = int(lib().mfem_sim_size(self.h, b"x") // 3)
             return
-        sc = scene
-        if sc.dim != 3:
-            raise MfemError("the device backend is 3D (tetrahedra)")
-        keep = dict(V=_f64(sc.V), T=_i32(sc.T), mu=_f64(sc.mu), la=_f64(sc.la), mat=_i32(sc.mat),
-                    fixed=_i32(sc.fixed))
-        p = SimParams()
-        p.nV, p.nT = sc.nV, sc.nT
-        p.V = keep["V"].ctypes.data_as(dp)
-        p.T = keep["T"].ctypes.data_as(ip)
-        p.mu = keep["mu"].ctypes.data_as(dp)
-        p.la = keep["la"].ctypes.data_as(dp)
-        p.mat = keep["mat"].ctypes.data_as(ip)
-        p.fixed = keep["fixed"].ctypes.data_as(ip)
-        if sc.v0 is not None:
-            keep["v0"] = _f64(sc.v0)
-            p.v0 = keep["v0"].ctypes.data_as(dp)
-        else:
-            p.v0 = dp()
-        p.density, p.dt = sc.density, sc.dt
-        g = list(sc.gravity) + [0.0] * 3
-        p.gravity = (C.c_double * 3)(*g[:3])
-        p.bdf_order = sc.bdf_order
-        p.mixed = int(sc.mixed)
-        p.linear_solver = sc.solver.encode()
-        p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
-        p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
-        p.precision = {"fp64": 0, "mixed": 1}[precision]
-        self._keep = keep
+        p, self._keep = _params(scene, precision)
         check(lib().mfem_sim_create(C.byref(p), C.byref(self.h)))
-        self.nV = sc.nV
+        self.nV = scene.nV
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -245,6 +260,65 @@ class Sim:
                                        C.byref(U), C.byref(BS), C.byref(NB)))
         trials = [(int(a), int(b), float(c)) for a, b, c in zip(v, g, t)]
         return trials, ch.value, (G.value, U.value, BS.value, NB.value)
+
+
+class Batch:
+    """Independent mixed-stretch simulations advanced together on the GPU
+    (mfem_batch_*): one launch per Newton phase for all of them, each with
+    its own CG scalars, line search and Newton termination."""
+
+    def __init__(self, scenes, precision: str = "mixed"):
+        self.h = C.c_void_p()
+        params = (SimParams * len(scenes))()
+        self._keep = []
+        for k, sc in enumerate(scenes):
+            _, keep = _params(sc, precision, params[k])
+            self._keep.append(keep)
+        check(lib().mfem_batch_create(params, len(scenes), C.byref(self.h)))
+        self.count = len(scenes)
+        nV = np.zeros(self.count, dtype=np.int32)
+        nT = np.zeros(self.count, dtype=np.int32)
+        cnt = C.c_int()
+        check(lib().mfem_batch_sizes(self.h, C.byref(cnt), nV.ctypes.data_as(ip), nT.ctypes.data_as(ip)))
+        self.nV, self.nT = nV, nT
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().mfem_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self):
+        check(lib().mfem_batch_step(self.h))
+
+    def positions(self, k):
+        out = np.zeros(int(self.nV[k]) * 3)
+        check(lib().mfem_batch_get_positions(self.h, int(k), out.ctypes.data_as(dp)))
+        return out.reshape(-1, 3)
+
+    def stats(self):
+        n = self.count
+        ni, pi, rf, lf = (np.zeros(n, dtype=np.int32) for _ in range(4))
+        dec = np.zeros(n)
+        ms = C.c_double()
+        check(lib().mfem_batch_stats(self.h, ni.ctypes.data_as(ip), pi.ctypes.data_as(ip),
+                                     rf.ctypes.data_as(ip), lf.ctypes.data_as(ip),
+                                     dec.ctypes.data_as(dp), C.byref(ms)))
+        return dict(newton_iters=ni, pcg_iters=pi, refinements=rf, ls_fail=lf, decrement=dec,
+                    step_ms=ms.value)
+
+    def set_cg_shape(self, variant, rows_per_workgroup=256):
+        check(lib().mfem_batch_set_cg_shape(self.h, int(variant), int(rows_per_workgroup)))
+
+    def cg_shape(self):
+        v, r = C.c_int(), C.c_int()
+        check(lib().mfem_batch_cg_shape(self.h, C.byref(v), C.byref(r)))
+        return v.value, r.value
 
 
 # ---- batched element entry points (torch tensors as device memory) ---------
