@@ -107,6 +107,85 @@ def run_cpu_sample(scene, threads):
     return dt, st
 
 
+def run_batch(scs, precision, steps, warmup):
+    from mfem_amd import native
+    b = native.Batch(scs, precision=precision)
+    for _ in range(warmup):
+        b.step()
+    work = 0
+    pcg = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ts = time.perf_counter()
+        b.step()
+        st = b.stats()
+        work += int((st["newton_iters"].astype(np.int64) * b.nT).sum())
+        pcg += int(st["pcg_iters"].sum())
+        print(f"[bench] c4 {precision} step {k}: {(time.perf_counter() - ts) * 1e3:.1f} ms, "
+              f"newton {int(st['newton_iters'].sum())} (max {int(st['newton_iters'].max())}), "
+              f"cg {int(st['pcg_iters'].sum())} (max {int(st['pcg_iters'].max())})",
+              file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    pos = [b.positions(k) for k in range(b.count)]
+    shape = b.cg_shape()
+    b.close()
+    return dict(elapsed=elapsed, work=work, pcg=pcg, positions=pos, shape=shape)
+
+
+def run_c4(args):
+    """Batching config: independent mixed beams in one batch; value counts
+    Newton iterations x tets summed over the beams per second. Parity: the
+    smallest beams are replayed alone in the FP64 oracle for the same steps."""
+    from mfem_amd import scenes
+    from oracle import oracle as O
+    scs = scenes.config_c4(args.beams)
+    nT = np.array([sc.nT for sc in scs])
+    mixed = run_batch(scs, "mixed", args.steps, args.warmup)
+    fp64 = None if args.no_fp64 else run_batch(scs, "fp64", args.steps, args.warmup)
+    value = mixed["work"] / mixed["elapsed"]
+    out = {
+        "metric": "Newton-steps*tets/s at 1M tets, FP32-mixed vs FP64 GPU; PCG HBM GB/s",
+        "value": value, "unit": "Newton-steps*tets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * mixed["elapsed"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(scs)} independent mixed-stretch SNH jittered Kuhn beams, "
+                               f"{int(nT.min())}-{int(nT.max())} tets each ({int(nT.sum())} total), "
+                               f"E log-uniform [1e4,1e7], nu uniform [0.3,0.48], BDF1 dt 1/30, CG "
+                               f"tol 1e-4 / 500 its, 5 Newton its, one batch",
+                   "beams": len(scs), "tets": int(nT.sum()), "precision": "mixed"},
+        "pcg_iters": mixed["pcg"], "cg_shape_variant_rows_per_workgroup": list(mixed["shape"]),
+    }
+    if fp64 is not None:
+        v64 = fp64["work"] / fp64["elapsed"]
+        out["fp64_gpu"] = {"value": v64, "ms_per_step": 1000.0 * fp64["elapsed"] / args.steps}
+        out["mixed_over_fp64"] = value / v64
+        errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mixed["positions"], fp64["positions"])]
+        out["mixed_vs_fp64_rel_pos_err_max"] = float(max(errs))
+    if not args.no_cpu:
+        # FP64 oracle on the 8 smallest beams for the same number of steps
+        order = np.argsort(nT)[:8]
+        t0 = time.perf_counter()
+        work = 0
+        errs = []
+        for k in order:
+            o = O.Sim(scs[k], threads=args.cpu_threads or min(os.cpu_count() or 1, 16))
+            for _ in range(args.warmup + args.steps):
+                o.step()
+                work += o.stats()["newton_iters"] * scs[k].nT
+            xo = o.positions()
+            errs.append(float(np.linalg.norm(mixed["positions"][k] - xo) / np.linalg.norm(xo)))
+        dt_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": work / dt_cpu, "unit": "Newton-steps*tets/s",
+                               "cores": args.cpu_threads or min(os.cpu_count() or 1, 16),
+                               "kind": "port",
+                               "sample": f"FP64 oracle on the 8 smallest beams, "
+                                         f"{args.warmup + args.steps} steps each, {dt_cpu:.2f} s"}
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["mixed_vs_oracle_rel_pos_err_max_8_smallest"] = float(max(errs))
+    print(json.dumps(out), flush=True)
+
+
 def workload_name(config, nx, ny, nz, nT, scene):
     beam = f"jittered Kuhn beam {nx}x{ny}x{nz}x6 ({nT} tets)"
     if config == "c3":
@@ -125,9 +204,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=("c3", "c2", "c5"), default="c3",
+    ap.add_argument("--config", choices=("c3", "c2", "c5", "c4"), default="c3",
                     help="c3: the metric's 960k-tet mixed beam (default); c2: plain FP64 "
-                         "24k-tet beam; c5: 1.95M-tet stiff ARAP/SNH beam")
+                         "24k-tet beam; c5: 1.95M-tet stiff ARAP/SNH beam; c4: 256 independent "
+                         "beams (2e3-6e4 tets each) as one batch")
+    ap.add_argument("--beams", type=int, default=256, help="c4: number of beams")
     ap.add_argument("--nx", type=int, default=0)
     ap.add_argument("--ny", type=int, default=0)
     ap.add_argument("--nz", type=int, default=0)
@@ -142,6 +223,8 @@ def main():
         sys.exit(2)
 
     from mfem_amd import scenes
+    if args.config == "c4":
+        return run_c4(args)
     dims = {"c3": (100, 40, 40), "c2": (40, 10, 10), "c5": (130, 50, 50)}[args.config]
     dims = tuple(a or d for a, d in zip((args.nx, args.ny, args.nz), dims))
     args.nx, args.ny, args.nz = dims
