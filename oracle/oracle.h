@@ -41,7 +41,10 @@ typedef struct {
   double gravity[3];
   int bdf_order;      // 1..6
   int mixed;          // 1: "mixed-stretch", 0: "stretch"
-  int solver;         // 0: dense LLT ("eigen-llt"), 1: Jacobi CG ("eigen-pcg-diag")
+  int solver;         // 0: dense LLT ("eigen-llt"), 1: Jacobi CG ("eigen-pcg-diag"),
+                      // 2: MINRES on the KKT system ("minres-indefinite-block") with the
+                      //    GPU form of its block preconditioner, 3: the same MINRES with
+                      //    the reference's exact factorizations (dense; small meshes)
   double itr_tol;
   int max_iters;
   int max_newton;
@@ -91,6 +94,18 @@ long oracle_sim_size(void* sim, const char* name);
 int oracle_sim_get(void* sim, const char* name, double* out);
 int oracle_sim_get_int(void* sim, const char* name, int* out);
 double oracle_sim_energy(void* sim, double alpha);
+// Indefinite (KKT) system at the current state, unknowns [dx; ds; lambda]
+// (n + 12 nT, 6-vectors element-major). oracle_kkt_precond rebuilds the
+// preconditioner from the current state (exact: dense factorizations).
+long oracle_kkt_size(void* sim);
+int oracle_kkt_rhs(void* sim, double* out);
+int oracle_kkt_product(void* sim, const double* w, double* y);
+int oracle_kkt_precond(void* sim, int exact, const double* b, double* z);
+// The MINRES iteration on a dense symmetric A with a dense SPD preconditioner
+// P (z = P r), both row-major N x N. Returns the estimated relative residual
+// (negative if P is not positive definite).
+double oracle_minres_dense(int N, const double* A, const double* P, const double* b, double* x,
+                           int max_iters, double tol, int* iters);
 
 #ifdef __cplusplus
 }

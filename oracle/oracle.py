@@ -15,7 +15,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 _lib = None
 
-SOLVERS = {"eigen-llt": 0, "eigen-ldlt": 0, "cholmod": 0, "eigen-pcg-diag": 1}
+SOLVERS = {"eigen-llt": 0, "eigen-ldlt": 0, "cholmod": 0, "eigen-pcg-diag": 1,
+           "minres-indefinite-block": 2}
+# MINRES with the reference's exact (dense) preconditioner factorizations
+SOLVER_MINRES_EXACT = 3
 
 
 class Params(C.Structure):
@@ -76,6 +79,16 @@ def lib():
         L.oracle_sim_get_int.restype = C.c_int
         L.oracle_sim_energy.argtypes = [C.c_void_p, C.c_double]
         L.oracle_sim_energy.restype = C.c_double
+        L.oracle_kkt_size.argtypes = [C.c_void_p]
+        L.oracle_kkt_size.restype = C.c_long
+        L.oracle_kkt_rhs.argtypes = [C.c_void_p, dp]
+        L.oracle_kkt_rhs.restype = C.c_int
+        L.oracle_kkt_product.argtypes = [C.c_void_p, dp, dp]
+        L.oracle_kkt_product.restype = C.c_int
+        L.oracle_kkt_precond.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.oracle_kkt_precond.restype = C.c_int
+        L.oracle_minres_dense.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, C.c_double, ip]
+        L.oracle_minres_dense.restype = C.c_double
         _lib = L
     return _lib
 
@@ -178,7 +191,7 @@ def tet_geometry(V, T):
 class Sim:
     """One reference-style simulation (SimState + NewtonOptimizer) in FP64 on the CPU."""
 
-    def __init__(self, scene, threads=0):
+    def __init__(self, scene, threads=0, solver=None):
         self._keep = []
         p = Params()
         sc = scene
@@ -200,7 +213,7 @@ class Sim:
         p.gravity = (C.c_double * 3)(*g[:3])
         p.bdf_order = sc.bdf_order
         p.mixed = int(sc.mixed)
-        p.solver = SOLVERS[sc.solver]
+        p.solver = solver if solver is not None else SOLVERS[sc.solver]
         p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
         p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
         p.threads = threads
@@ -255,3 +268,38 @@ class Sim:
         if lib().oracle_sim_get_int(self.h, name.encode(), _ip(out)) != 0:
             raise KeyError(name)
         return out
+
+    # ---- indefinite (KKT) system at the current state -----------------------
+    def kkt_size(self):
+        return int(lib().oracle_kkt_size(self.h))
+
+    def kkt_rhs(self):
+        out = np.zeros(self.kkt_size())
+        if lib().oracle_kkt_rhs(self.h, _dp(out)) != 0:
+            raise RuntimeError("KKT system needs a 3D mixed scene")
+        return out
+
+    def kkt_product(self, w):
+        w = f64(w)
+        y = np.zeros(self.kkt_size())
+        if lib().oracle_kkt_product(self.h, _dp(w), _dp(y)) != 0:
+            raise RuntimeError("KKT system needs a 3D mixed scene")
+        return y
+
+    def kkt_precond(self, b, exact):
+        b = f64(b)
+        z = np.zeros(self.kkt_size())
+        if lib().oracle_kkt_precond(self.h, int(exact), _dp(b), _dp(z)) != 0:
+            raise RuntimeError("KKT system needs a 3D mixed scene")
+        return z
+
+
+def minres_dense(A, P, b, max_iters, tol):
+    """The oracle's MINRES on dense A (symmetric) with z = P r. Returns x, iters, est. error."""
+    A, P, b = f64(A), f64(P), f64(b)
+    N = b.shape[0]
+    x = np.zeros(N)
+    it = C.c_int(0)
+    err = lib().oracle_minres_dense(N, _dp(A), _dp(P), _dp(b), _dp(x), int(max_iters), float(tol),
+                                    C.byref(it))
+    return x, it.value, err

@@ -183,6 +183,14 @@ struct OracleSim {
   oracle_stats st{};
   int pcg_iters_last = 0;
   double pcg_err_last = 0;
+  // indefinite (KKT) solve: preconditioner state and the reference's
+  // refresh counter (preconditioners.h:150-158)
+  int kkt_solves = 0;
+  bool kkt_mass_ready = false;
+  std::vector<double> mlump, mdinv;   // per free dof: row sum and 1/diag of PMP
+  std::vector<double> mchol;          // dense Cholesky of PMP (exact preconditioner)
+  std::vector<double> lchol;          // dense Cholesky of L (exact preconditioner)
+  std::vector<Mat<6, 6>> linv_blk;    // element blocks of L, inverted
 };
 
 namespace {
@@ -586,6 +594,362 @@ int dense_llt(OracleSim* S_, std::vector<double>& xo) {
   return 0;
 }
 
+// ---------------- indefinite (KKT) solve: "minres-indefinite-block" -------
+// Unknowns [dx (n); ds (6 nT); lambda (6 nT)], 6-vectors element-major, as
+// SystemMatrixIndefinite lays them out (reference linear_system.h:60-99).
+
+// In-place dense Cholesky (lower, row-major n x n); false if not PD.
+bool dense_chol(std::vector<double>& A, int n) {
+  for (int j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0)) return false;
+    d = std::sqrt(d);
+    A[(size_t)j * n + j] = d;
+#pragma omp parallel for schedule(static)
+    for (int i = j + 1; i < n; ++i) {
+      double s = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = s / d;
+    }
+  }
+  return true;
+}
+
+void dense_chol_solve(const std::vector<double>& L, int n, const double* b, double* x) {
+  std::vector<double> y(n);
+  for (int i = 0; i < n; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= L[(size_t)i * n + k] * y[k];
+    y[i] = s / L[(size_t)i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < n; ++k) s -= L[(size_t)k * n + i] * x[k];
+    x[i] = s / L[(size_t)i * n + i];
+  }
+}
+
+// rhs = [-grad_x; -vol g; vol Sym (S - s)]: linear_system.h:67-86 with
+// Displacement::gradient (displacement.cpp:64-73), grad_ = vol g
+// (mixed_stretch.cpp:195-203) and grad_la_ = -vol Sym (S - s)
+// (mixed_stretch.cpp:127-131, 313-332).
+void kkt_rhs(OracleSim* S_, std::vector<double>& b) {
+  const int n = S_->n, nT = S_->nT;
+  b.assign((size_t)n + 12 * nT, 0.0);
+  disp_gradient(S_);
+  for (int i = 0; i < n; ++i) b[i] = -S_->grad_x_disp[i];
+  for (int e = 0; e < nT; ++e) {
+    const double vol = S_->vols[e];
+    for (int i = 0; i < 6; ++i) {
+      b[n + 6 * e + i] = -vol * S_->g[e][i];
+      b[n + 6 * nT + 6 * e + i] = vol * kSym[i] * (S_->S[e][i] - S_->s[6 * e + i]);
+    }
+  }
+}
+
+// y = K w with K = [[PMP, 0, Gx], [0, Hloc, C], [Gx^T, C, 0]],
+// Gx = -PJW diag(dSdF) (jacobian_x), C = vol Sym (jacobian_mixed), Hloc =
+// vol h^2 Hpsi: BlockMatrix::scaleAndAddTo (block_matrix.h:74-125) with
+// Displacement::product_hessian (displacement.h:59-63) and
+// MixedStretch::product_hessian / product_jacobian_x / product_jacobian_mixed
+// (mixed_stretch.cpp:373-434), accumulated in the same order.
+void kkt_product(const OracleSim* S_, const double* w, double* y) {
+  const int n = S_->n, nT = S_->nT;
+  const double* wx = w;
+  const double* ws = w + n;
+  const double* wl = w + n + 6 * nT;
+  double* yx = y;
+  double* ys = y + n;
+  double* yl = y + n + 6 * nT;
+  std::fill(y, y + (size_t)n + 12 * nT, 0.0);
+  // x rows: PMP dx, then -= PJW (dSdF la)
+  {
+    std::vector<double> xv(wx, wx + n), t;
+    csr_mult(S_, S_->PMP, xv, t);
+    for (int i = 0; i < n; ++i) yx[i] += t[i];
+    std::vector<double> tmp(9 * (size_t)nT);
+    for (int e = 0; e < nT; ++e) {
+      Vec<6> l; for (int i = 0; i < 6; ++i) l[i] = wl[6 * e + i];
+      Vec<9> v = S_->dSdF[e] * l;
+      for (int i = 0; i < 9; ++i) tmp[9 * (size_t)e + i] = v[i];
+    }
+    std::vector<double> acc(n, 0.0);
+    apply_PJW(S_, tmp, acc, -1.0);
+    for (int i = 0; i < n; ++i) yx[i] += acc[i];
+  }
+  // s rows: Hloc ds + vol Sym la
+  for (int e = 0; e < nT; ++e) {
+    const double vol = S_->vols[e];
+    for (int i = 0; i < 6; ++i) {
+      double a = 0;
+      for (int j = 0; j < 6; ++j) a += S_->Hloc[e](i, j) * ws[6 * e + j];
+      ys[6 * e + i] += a;
+    }
+    for (int i = 0; i < 6; ++i) ys[6 * e + i] += vol * kSym[i] * wl[6 * e + i];
+  }
+  // la rows: -= dSdF^T (PJW^T dx), then += vol Sym ds
+  {
+    std::vector<double> full(S_->nV * 3, 0.0);
+    for (int v = 0; v < S_->nV; ++v) {
+      int f = S_->free_map[v];
+      if (f < 0) continue;
+      for (int r = 0; r < 3; ++r) full[3 * v + r] = wx[3 * f + r];
+    }
+    for (int e = 0; e < nT; ++e) {
+      double Fd[9]; def_grad(S_, full, e, Fd);
+      Vec<9> t; for (int i = 0; i < 9; ++i) t[i] = S_->vols[e] * Fd[i];
+      Vec<6> u = S_->dSdF[e].T() * t;
+      for (int i = 0; i < 6; ++i) yl[6 * e + i] -= u[i];
+    }
+    for (int e = 0; e < nT; ++e)
+      for (int i = 0; i < 6; ++i) yl[6 * e + i] += S_->vols[e] * kSym[i] * ws[6 * e + i];
+  }
+}
+
+// Chebyshev steps of the mass block: kMassCheb steps of Chebyshev
+// acceleration of Jacobi on PMP (Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., Alg. 12.1) over the interval [1/2, 5/2], which holds the
+// spectrum of diag(M)^-1 M for linear tets (Wathen 1987; the element matrix
+// (1 + delta_ab)/2 has eigenvalues 1/2 and 5/2, and a principal submatrix
+// stays inside). The result is p(D^-1 M) D^-1 b for a fixed polynomial p,
+// positive on the interval, so the operator is symmetric positive definite.
+constexpr int kMassCheb = 4;
+
+void mass_cheb(const OracleSim* S_, const double* b, double* z) {
+  const int n = S_->n;
+  const double theta = 1.5, delta = 1.0, sigma = theta / delta;
+  std::vector<double> r(b, b + n), d(n), Ad;
+  double rho = 1.0 / sigma;
+  for (int i = 0; i < n; ++i) { z[i] = 0.0; d[i] = S_->mdinv[i] * r[i] / theta; }
+  for (int k = 1; k <= kMassCheb; ++k) {
+    for (int i = 0; i < n; ++i) z[i] += d[i];
+    if (k == kMassCheb) break;
+    csr_mult(S_, S_->PMP, d, Ad);
+    const double rho1 = 1.0 / (2.0 * sigma - rho);
+    for (int i = 0; i < n; ++i) {
+      r[i] -= Ad[i];
+      d[i] = rho1 * rho * d[i] + (2.0 * rho1 / delta) * S_->mdinv[i] * r[i];
+    }
+    rho = rho1;
+  }
+}
+
+// Mass-only preconditioner data (the reference's init(), preconditioners.h:81-100):
+// lumped mass Mlump = PMP 1, and for the GPU form the Jacobi scaling.
+void kkt_mass_setup(OracleSim* S_, bool exact) {
+  const int n = S_->n;
+  if (!S_->kkt_mass_ready) {
+    S_->mlump.assign(n, 0.0);
+    S_->mdinv.assign(n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int k = S_->asmb.rowptr[i]; k < S_->asmb.rowptr[i + 1]; ++k) {
+        S_->mlump[i] += S_->PMP[k];
+        if (S_->asmb.colind[k] == i) S_->mdinv[i] = 1.0 / S_->PMP[k];
+      }
+    S_->kkt_mass_ready = true;
+  }
+  if (exact && S_->mchol.empty()) {
+    S_->mchol.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int k = S_->asmb.rowptr[i]; k < S_->asmb.rowptr[i + 1]; ++k)
+        S_->mchol[(size_t)i * n + S_->asmb.colind[k]] = S_->PMP[k];
+    dense_chol(S_->mchol, n);
+  }
+}
+
+// Gx column block of element e restricted to its corners: G_e(3k + r, j) =
+// -vol dX(k, c) dSdF(r + 3c, j) for free corner k (Gx = -PJW diag(dSdF),
+// mixed_stretch.cpp:342-348).
+void gx_block(const OracleSim* S_, int e, double G[12][6]) {
+  const double vol = S_->vols[e];
+  const double* dx = &S_->dX[(size_t)e * 12];
+  for (int k = 0; k < 4; ++k)
+    for (int r = 0; r < 3; ++r)
+      for (int j = 0; j < 6; ++j) {
+        double a = 0;
+        for (int c = 0; c < 3; ++c) a += dx[3 * k + c] * S_->dSdF[e](r + 3 * c, j);
+        G[3 * k + r][j] = -vol * a;
+      }
+}
+
+// The lambda block. The reference factors L = Gx^T Mlump^-1 Gx + 1e-8 I
+// whole (preconditioners.h:102-114, 124-146; exact = true here: dense
+// Cholesky of the SPD L, the same solution as its SimplicialLDLT). The GPU
+// form is the block-diagonal preconditioner of Murphy, Golub and Wathen
+// (SIAM J. Sci. Comput. 21, 2000) with the Schur complement
+// Gx^T M^-1 Gx + C Hloc^-1 C replaced by its element blocks, lumped mass in
+// place of M: S_e = Gx_e^T Mlump^-1 Gx_e + vol Sym Hpsi^-1 Sym, inverted.
+void kkt_refresh_L(OracleSim* S_, bool exact) {
+  const int nT = S_->nT;
+  S_->linv_blk.assign(nT, Mat<6, 6>());
+  for (int e = 0; e < nT; ++e) {
+    double G[12][6];
+    gx_block(S_, e, G);
+    Mat<6, 6> Le;
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) {
+        double a = 0;
+        for (int k = 0; k < 4; ++k) {
+          int f = S_->free_map[S_->T[4 * e + k]];
+          if (f < 0) continue;
+          for (int r = 0; r < 3; ++r) a += G[3 * k + r][i] * G[3 * k + r][j] / S_->mlump[3 * f + r];
+        }
+        Le(i, j) = a + (exact ? (i == j ? 1e-8 : 0.0)
+                              : S_->vols[e] * kSym[i] * S_->Hinv[e](i, j) * kSym[j]);
+      }
+    S_->linv_blk[e] = lu_inverse<6>(Le);
+  }
+  if (!exact) return;
+  const int m = 6 * nT;
+  // dense Gx (n x 6nT), then L
+  const int n = S_->n;
+  std::vector<double> Gx((size_t)n * m, 0.0);
+  for (int e = 0; e < nT; ++e) {
+    double G[12][6];
+    gx_block(S_, e, G);
+    for (int k = 0; k < 4; ++k) {
+      int f = S_->free_map[S_->T[4 * e + k]];
+      if (f < 0) continue;
+      for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 6; ++j) Gx[(size_t)(3 * f + r) * m + 6 * e + j] += G[3 * k + r][j];
+    }
+  }
+  S_->lchol.assign((size_t)m * m, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < m; ++i)
+    for (int row = 0; row < n; ++row) {
+      double gi = Gx[(size_t)row * m + i];
+      if (gi == 0.0) continue;
+      double w = gi / S_->mlump[row];
+      for (int j = 0; j < m; ++j) S_->lchol[(size_t)i * m + j] += w * Gx[(size_t)row * m + j];
+    }
+  for (int i = 0; i < m; ++i) S_->lchol[(size_t)i * m + i] += 1e-8;
+  dense_chol(S_->lchol, m);
+}
+
+// z = P b. exact: the reference's P = diag(PMP^-1, vol Hpsi^-1, L^-1)
+// (BlockDiagonalPreconditioner::_solve_impl, preconditioners.h:160-192, with
+// product_hessian_inv, mixed_stretch.cpp:385-396; Hpsi here is h^2 times the
+// material Hessian). GPU form: diag(Chebyshev(PMP), Hloc^-1 = Hpsi^-1 / vol,
+// element blocks S_e^-1).
+void kkt_precond(const OracleSim* S_, bool exact, const double* b, double* z) {
+  const int n = S_->n, nT = S_->nT;
+  if (exact) dense_chol_solve(S_->mchol, n, b, z);
+  else mass_cheb(S_, b, z);
+  for (int e = 0; e < nT; ++e) {
+    const double vol = S_->vols[e];
+    for (int i = 0; i < 6; ++i) {
+      double a = 0;
+      for (int j = 0; j < 6; ++j) a += S_->Hinv[e](i, j) * b[n + 6 * e + j];
+      z[n + 6 * e + i] = exact ? vol * a : a / vol;
+    }
+  }
+  if (exact) {
+    dense_chol_solve(S_->lchol, 6 * nT, b + n + 6 * nT, z + n + 6 * nT);
+  } else {
+    for (int e = 0; e < nT; ++e)
+      for (int i = 0; i < 6; ++i) {
+        double a = 0;
+        for (int j = 0; j < 6; ++j) a += S_->linv_blk[e](i, j) * b[n + 6 * nT + 6 * e + j];
+        z[n + 6 * nT + 6 * e + i] = a;
+      }
+  }
+}
+
+// Preconditioned MINRES of Eigen 3.4 (unsupported/Eigen/src/IterativeSolvers/
+// MINRES.h, the Lanczos variant A(2,7) of Paige 1972 with Givens QR), with
+// x0 = 0 (IterativeSolverBase::_solve_impl), the stop test on the estimated
+// residual |b|^2 prod s_i^2 < tol^2 |b|^2 and iters counting the iterations
+// completed without stopping. Returns the estimated relative residual.
+template <class Op, class Pre>
+double minres(int N, Op op, Pre pre, const double* b, double* x, int max_iters, double tol,
+              int* iters) {
+  std::fill(x, x + N, 0.0);
+  double rhs2 = 0;
+  for (int i = 0; i < N; ++i) rhs2 += b[i] * b[i];
+  *iters = 0;
+  if (rhs2 == 0) return 0.0;
+  const double threshold2 = tol * tol * rhs2;
+  std::vector<double> v_old(N), v(N, 0.0), v_new(b, b + N), w(N), w_new(N), Aw(N);
+  std::vector<double> p_oold(N), p_old(N, 0.0), p(N, 0.0);
+  // v_new = b - K x0 with x0 = 0
+  double res2 = rhs2;
+  pre(v_new.data(), w_new.data());
+  double beta_new2 = 0;
+  for (int i = 0; i < N; ++i) beta_new2 += v_new[i] * w_new[i];
+  if (!(beta_new2 >= 0)) return -1.0;  // preconditioner not positive definite
+  double beta_new = std::sqrt(beta_new2);
+  const double beta_one = beta_new;
+  double c = 1.0, c_old = 1.0, s = 0.0, s_old = 0.0, eta = 1.0;
+  int it = 0;
+  while (it < max_iters) {
+    const double beta = beta_new;
+    v_old = v;
+    for (int i = 0; i < N; ++i) { v_new[i] /= beta_new; w_new[i] /= beta_new; }
+    v = v_new;
+    w = w_new;
+    op(w.data(), Aw.data());
+    for (int i = 0; i < N; ++i) v_new[i] = Aw[i] - beta * v_old[i];
+    double alpha = 0;
+    for (int i = 0; i < N; ++i) alpha += v_new[i] * w[i];
+    for (int i = 0; i < N; ++i) v_new[i] -= alpha * v[i];
+    pre(v_new.data(), w_new.data());
+    beta_new2 = 0;
+    for (int i = 0; i < N; ++i) beta_new2 += v_new[i] * w_new[i];
+    if (!(beta_new2 >= 0)) return -1.0;
+    beta_new = std::sqrt(beta_new2);
+    // Givens rotation (s, c, s_old, c_old still from the previous iteration)
+    const double r2 = s * alpha + c * c_old * beta;
+    const double r3 = s_old * beta;
+    const double r1_hat = c * alpha - c_old * s * beta;
+    const double r1 = std::sqrt(r1_hat * r1_hat + beta_new * beta_new);
+    c_old = c;
+    s_old = s;
+    c = r1_hat / r1;
+    s = beta_new / r1;
+    p_oold = p_old;
+    p_old = p;
+    for (int i = 0; i < N; ++i) p[i] = (w[i] - r2 * p_old[i] - r3 * p_oold[i]) / r1;
+    for (int i = 0; i < N; ++i) x[i] += beta_one * c * eta * p[i];
+    res2 *= s * s;
+    if (res2 < threshold2) break;
+    eta = -s * eta;
+    ++it;
+  }
+  *iters = it;
+  return std::sqrt(res2 / rhs2);
+}
+
+// EigenIterativeSolver<MINRES<BlockMatrix, ..., BlockDiagonalPreconditioner>>::
+// solve (eigen_iterative_solver.h:20-30) and SystemMatrixIndefinite::post_solve
+// (linear_system.h:88-99): dx, ds and lambda straight from the KKT solution.
+// The reference refactors L on every max_newton-th solve (compute(),
+// preconditioners.h:150-158; the counter here is per simulation); the GPU
+// form's element blocks are rebuilt on every solve.
+int kkt_solve(OracleSim* S_) {
+  const bool exact = S_->p.solver == 3;
+  kkt_mass_setup(S_, exact);
+  if (S_->kkt_solves == 0 || !exact) kkt_refresh_L(S_, exact);
+  S_->kkt_solves = (S_->kkt_solves + 1) % std::max(1, S_->p.max_newton);
+  std::vector<double> b;
+  kkt_rhs(S_, b);
+  const int n = S_->n, nT = S_->nT, N = n + 12 * nT;
+  std::vector<double> sol(N);
+  int iters = 0;
+  double err = minres(
+      N, [&](const double* w, double* y) { kkt_product(S_, w, y); },
+      [&](const double* r, double* z) { kkt_precond(S_, exact, r, z); }, b.data(), sol.data(),
+      S_->p.max_iters, S_->p.itr_tol, &iters);
+  if (err < 0) return -2;
+  S_->pcg_iters_last = iters;
+  S_->pcg_err_last = err;
+  S_->dx.assign(sol.begin(), sol.begin() + n);
+  S_->ds.assign(sol.begin() + n, sol.begin() + n + 6 * nT);
+  S_->lam.assign(sol.begin() + n + 6 * nT, sol.end());
+  return 0;
+}
+
 // ---------------- Newton optimizer ----------------
 void update_system(OracleSim* S_) {
   std::vector<double> full;
@@ -599,6 +963,17 @@ void update_system(OracleSim* S_) {
 }
 
 int substep(OracleSim* S_, double& decrement) {
+  if (S_->p.solver >= 2) {
+    // indefinite system (mixed scenes only)
+    if (!S_->p.mixed || S_->dim != 3) return -3;
+    int rc = kkt_solve(S_);
+    if (rc != 0) return rc;
+    S_->st.pcg_iters_total += S_->pcg_iters_last;
+    decrement = 0;
+    for (double v : S_->dx) decrement = std::max(decrement, std::fabs(v));
+    for (double v : S_->ds) decrement = std::max(decrement, std::fabs(v));
+    return 0;
+  }
   // SystemMatrixPD::pre_solve: lhs = PMP + var lhs; rhs = x rhs + var rhs
   std::vector<double> Avals;
   if (S_->p.mixed) {
@@ -997,6 +1372,50 @@ int oracle_sim_get(void* sim, const char* name, double* out) {
   if (k == "positions") { std::copy(S_->Vcur.begin(), S_->Vcur.end(), out); return 0; }
   if (k == "x_tilde") { auto v = x_tilde(S_); std::copy(v.begin(), v.end(), out); return 0; }
   return -1;
+}
+
+long oracle_kkt_size(void* sim) {
+  OracleSim* S_ = (OracleSim*)sim;
+  return (long)S_->n + 12L * S_->nT;
+}
+
+int oracle_kkt_rhs(void* sim, double* out) {
+  OracleSim* S_ = (OracleSim*)sim;
+  if (!S_->p.mixed || S_->dim != 3) return -3;
+  std::vector<double> b;
+  kkt_rhs(S_, b);
+  std::copy(b.begin(), b.end(), out);
+  return 0;
+}
+
+int oracle_kkt_product(void* sim, const double* w, double* y) {
+  OracleSim* S_ = (OracleSim*)sim;
+  if (!S_->p.mixed || S_->dim != 3) return -3;
+  kkt_product(S_, w, y);
+  return 0;
+}
+
+int oracle_kkt_precond(void* sim, int exact, const double* b, double* z) {
+  OracleSim* S_ = (OracleSim*)sim;
+  if (!S_->p.mixed || S_->dim != 3) return -3;
+  kkt_mass_setup(S_, exact != 0);
+  kkt_refresh_L(S_, exact != 0);
+  kkt_precond(S_, exact != 0, b, z);
+  return 0;
+}
+
+double oracle_minres_dense(int N, const double* A, const double* P, const double* b, double* x,
+                           int max_iters, double tol, int* iters) {
+  auto mv = [N](const double* M, const double* v, double* y) {
+    for (int i = 0; i < N; ++i) {
+      double a = 0;
+      for (int j = 0; j < N; ++j) a += M[(size_t)i * N + j] * v[j];
+      y[i] = a;
+    }
+  };
+  return minres(
+      N, [&](const double* w, double* y) { mv(A, w, y); },
+      [&](const double* r, double* z) { mv(P, r, z); }, b, x, max_iters, tol, iters);
 }
 
 int oracle_sim_get_int(void* sim, const char* name, int* out) {
