@@ -36,7 +36,8 @@ typedef struct {
   double gravity[3];      /* "body_force" (stored as float, as the reference does) */
   int bdf_order;          /* "time_integrator": BDF1..BDF6 */
   int mixed;              /* 1: "mixed_variables": ["mixed-stretch"]; 0: "variables": ["stretch"] */
-  const char* linear_solver; /* "linear_solver" name, e.g. "eigen-pcg-diag" */
+  const char* linear_solver; /* "linear_solver": "eigen-pcg-diag" (Schur complement, CG) or
+                              * "minres-indefinite-block" (KKT system, MINRES; mixed only) */
   double itr_tol;         /* "iterative_solver_tolerance" */
   int max_iters;          /* "max_iterative_solver_iters" */
   int max_newton;         /* "max_newton_iterations" */
@@ -127,6 +128,17 @@ int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks);
  * workgroup size / grid. */
 int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* variants, int* grids, int* count,
                        int* chosen, int* lanes, int* unroll, int* threads, int* blocks);
+
+/* Indefinite (KKT) system of "minres-indefinite-block" (reference
+ * SystemMatrixIndefinite, linear_system.h:60-99, and BlockMatrix::
+ * scaleAndAddTo, block_matrix.h:74-125) at the current state (after
+ * mfem_sim_update_system). Vectors are [dx (3 nfree); ds (6 nT); lambda
+ * (6 nT)] with element-major 6-vectors, size from mfem_sim_kkt_size.
+ * which 0: out = K in; 1: out = P in (the solver's block preconditioner);
+ * 2: out = the right-hand side (in unused). Errors unless the simulation's
+ * linear solver is "minres-indefinite-block". */
+int mfem_sim_kkt_size(mfem_sim* sim, long* size);
+int mfem_sim_kkt_apply(mfem_sim* sim, int which, const double* in, double* out);
 
 /* Batch of independent mixed-stretch simulations (no counterpart in the
  * reference, where each would be its own SimState + NewtonOptimizer,

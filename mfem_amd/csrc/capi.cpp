@@ -391,6 +391,25 @@ int mfem_sim_spmv_tuning(mfem_sim* sim, double* times, int* count, int* chosen, 
   });
 }
 
+int mfem_sim_kkt_size(mfem_sim* sim, long* size) {
+  return guarded([&] {
+    auto* mr = dynamic_cast<MinresSolver<3>*>(sim->opt->solver());
+    if (!mr) throw std::runtime_error("the linear solver is not \"minres-indefinite-block\"");
+    *size = mr->kkt_size();
+  });
+}
+
+int mfem_sim_kkt_apply(mfem_sim* sim, int which, const double* in, double* out) {
+  return guarded([&] {
+    auto* mr = dynamic_cast<MinresSolver<3>*>(sim->opt->solver());
+    if (!mr) throw std::runtime_error("the linear solver is not \"minres-indefinite-block\"");
+    std::vector<double> v;
+    if (which != 2) v.assign(in, in + mr->kkt_size());
+    auto y = mr->apply(which, v);
+    std::copy(y.begin(), y.end(), out);
+  });
+}
+
 int mfem_sim_set_cg_shape(mfem_sim* sim, int variant, int blocks) {
   return guarded([&] {
     auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());

@@ -65,6 +65,9 @@ class Mesh {
   DeviceArray<int> d_rowptr, d_col, d_slot_ptr, d_slots, d_diag_block, d_inc_ptr, d_inc;
   DeviceArray<double> d_mass;
   std::vector<int> rowptr_, col_;  // host copy of the BSR pattern
+  // consistent-mass coefficient per block, and the diagonal block of each row
+  const std::vector<double>& mass() const { return mass_; }
+  const std::vector<int>& diag_block() const { return diag_block_; }
 
  private:
   void build_maps();

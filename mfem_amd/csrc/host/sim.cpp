@@ -254,15 +254,17 @@ void MixedStretch<DIM>::update(const double* x_full, double dt) {
     launch_mixed_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
                                        dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
                                        g_.data(), gl_.data(), h2_, c->stream);
-    launch_mixed_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
-                                   gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
+    if (schur_blocks_)
+      launch_mixed_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
+                                     gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
   } else {
     launch_mixed_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
                                         dsdF_.f64(), Hpsi_.f64(),
                                         Hinv_.f64(), g_.data(), gl_.data(), h2_, c->stream);
-    launch_mixed_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(),
-                                    Hpsi_.f64(), gl_.data(), c->blocks.f64(),
-                                    c->fe.data(), c->stream);
+    if (schur_blocks_)
+      launch_mixed_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(),
+                                      Hpsi_.f64(), gl_.data(), c->blocks.f64(),
+                                      c->fe.data(), c->stream);
   }
 }
 
@@ -627,6 +629,212 @@ void PcgSolver<DIM>::solve() {
   system_.post_solve(this->state_, x_.data());
 }
 
+// ============================================================ MINRES
+template <int DIM>
+MinresSolver<DIM>::MinresSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
+  if (DIM != 3) throw std::runtime_error("minres-indefinite-block: the device path is 3D");
+  if (state->mixed_vars_.size() != 1 || !state->vars_.empty())
+    throw std::runtime_error(
+        "minres-indefinite-block needs exactly one mixed variable (\"mixed-stretch\") and no "
+        "displacement-only elastic variable");
+  mixed_ = dynamic_cast<MixedStretch<DIM>*>(state->mixed_vars_[0].get());
+  if (!mixed_) throw std::runtime_error("minres-indefinite-block needs \"mixed-stretch\"");
+  mixed_->schur_blocks_ = false;
+  auto& m = *state->mesh_;
+  hipStream_t st = state->ctx_->stream;
+  n_ = 3L * m.nfree();
+  N_ = n_ + 12L * m.nT();
+  // lumped mass (row sums of PMP) and the Jacobi scaling of the mass block
+  std::vector<double> ml(m.nfree(), 0.0), md(m.nfree(), 0.0);
+  const auto& mass = m.mass();
+  for (int i = 0; i < m.nfree(); ++i) {
+    for (int b = m.rowptr_[i]; b < m.rowptr_[i + 1]; ++b) ml[i] += mass[b];
+    ml[i] = 1.0 / ml[i];
+    md[i] = 1.0 / mass[m.diag_block()[i]];
+  }
+  mlump_inv_.upload(ml, st);
+  mdinv_.upload(md, st);
+  vecs_.resize(7 * (size_t)N_);
+  sinv_.resize(21 * (size_t)m.nT());
+  cheb_.resize(3 * (size_t)n_);
+  parts_.resize(2 * (size_t)(kkt_tet_blocks(m.nT()) + kkt_vertex_blocks(m.nfree())) + 2);
+  ctrl_.resize(1);
+  pinned_ = std::make_unique<PinnedBuffer>(2 * sizeof(MinresCtrl));
+  for (auto& e : poll_ev_) MFEM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+template <int DIM>
+MinresSolver<DIM>::~MinresSolver() {
+  for (auto& e : poll_ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+template <int DIM>
+template <typename Real>
+KktSystem<Real> MinresSolver<DIM>::system() {
+  auto& m = *this->state_->mesh_;
+  auto* c = this->state_->ctx_.get();
+  KktSystem<Real> S;
+  S.m = m.dev();
+  S.rowptr = m.d_rowptr.data();
+  S.col = m.d_col.data();
+  S.mass = m.d_mass.data();
+  S.inc_ptr = m.d_inc_ptr.data();
+  S.inc = m.d_inc.data();
+  S.dsdF = mixed_->dsdF_.template as<Real>();
+  S.Hpsi = mixed_->Hpsi_.template as<Real>();
+  S.Hinv = mixed_->Hinv_.template as<Real>();
+  S.Sinv = sinv_.data();
+  S.mlump_inv = mlump_inv_.data();
+  S.mdinv = mdinv_.data();
+  S.fe = c->fe.data();
+  S.cheb_r = cheb_.data();
+  S.cheb_d[0] = cheb_.data() + n_;
+  S.cheb_d[1] = cheb_.data() + 2 * n_;
+  const long np = kkt_tet_blocks(m.nT()) + kkt_vertex_blocks(m.nfree());
+  S.parts[0] = parts_.data();
+  S.parts[1] = parts_.data() + np;
+  S.ctrl = ctrl_.data();
+  return S;
+}
+
+// EigenIterativeSolver::solve (eigen_iterative_solver.h:20-30) with
+// SystemMatrixIndefinite::pre_solve / post_solve (linear_system.h:67-99):
+// MINRES from x = 0; the solution is written into dx, ds and lambda.
+template <int DIM>
+template <typename Real>
+void MinresSolver<DIM>::run() {
+  auto& m = *this->state_->mesh_;
+  auto* c = this->state_->ctx_.get();
+  auto& cfg = *this->state_->config_;
+  hipStream_t st = c->stream;
+  const KktSystem<Real> S = system<Real>();
+  MinresBuffers B;
+  auto vec = [&](int k) {
+    double* base = vecs_.data() + (size_t)k * N_;
+    return KktVec{base, base + n_, base + n_ + 6L * m.nT()};
+  };
+  B.v[0] = vec(0);
+  B.v[1] = vec(1);
+  B.w[0] = vec(2);
+  B.w[1] = vec(3);
+  for (int k = 0; k < 3; ++k) B.p[k] = vec(4 + k);
+  B.x = KktVec{this->state_->x_->delta().data(), mixed_->ds_.data(), mixed_->la_.data()};
+  // v_{-1}, p_{-1}, p_{-2} and x start at zero
+  MFEM_HIP_CHECK(hipMemsetAsync(B.v[1].x, 0, N_ * sizeof(double), st));
+  MFEM_HIP_CHECK(hipMemsetAsync(B.p[0].x, 0, 3 * N_ * sizeof(double), st));
+  this->state_->x_->delta().zero(st);
+  mixed_->ds_.zero(st);
+  mixed_->la_.zero(st);
+  launch_kkt_setup(S, st);
+  launch_minres_init(S, this->state_->x_->value().data(), c->inertia(), mixed_->g_.data(),
+                     mixed_->S_.data(), mixed_->s_.data(), B.v[0], B.w[0], cfg.itr_tol,
+                     cfg.max_iterative_solver_iters, st);
+  // iterations in chunks, two control-block snapshots in flight; kernels
+  // queued after the stop exit at once
+  const int poll = std::max(1, cfg.pcg_poll);
+  const long limit = (long)cfg.max_iterative_solver_iters + 1;
+  auto* snap = pinned_->as<MinresCtrl>();
+  long launched = 0;
+  int issued = 0, waited = 0;
+  auto issue = [&] {
+    const long chunk = std::min<long>(poll, limit - launched);
+    for (long j = 0; j < chunk; ++j) launch_minres_iteration(S, B, launched + j, st);
+    launched += chunk;
+    MFEM_HIP_CHECK(hipGetLastError());
+    const int slot = issued & 1;
+    MFEM_HIP_CHECK(hipMemcpyAsync(snap + slot, ctrl_.data(), sizeof(MinresCtrl),
+                                  hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[slot], st));
+    ++issued;
+  };
+  issue();
+  while (true) {
+    if (launched < limit && issued - waited < 2) issue();
+    const int slot = waited & 1;
+    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[slot]));
+    ++waited;
+    if (snap[slot].done) {
+      last_ = snap[slot];
+      break;
+    }
+    if (waited == issued) throw std::runtime_error("MINRES control block did not terminate");
+  }
+  // one more launch applies the update of the last iteration (it is a no-op
+  // when a queued launch already did)
+  launch_minres_iteration(S, B, launched, st);
+  if (last_.fail)
+    throw std::runtime_error("minres-indefinite-block: preconditioner is not positive definite");
+}
+
+template <int DIM>
+template <typename Real>
+void MinresSolver<DIM>::apply_device(int which) {
+  auto& m = *this->state_->mesh_;
+  auto* c = this->state_->ctx_.get();
+  hipStream_t st = c->stream;
+  const KktSystem<Real> S = system<Real>();
+  auto vec = [&](int k) {
+    double* base = vecs_.data() + (size_t)k * N_;
+    return KktVec{base, base + n_, base + n_ + 6L * m.nT()};
+  };
+  if (which == 2) {
+    launch_minres_init(S, this->state_->x_->value().data(), c->inertia(), mixed_->g_.data(),
+                       mixed_->S_.data(), mixed_->s_.data(), vec(1), vec(2), 1.0, 1, st);
+  } else {
+    launch_kkt_apply(S, which, vec(0), vec(1), vec(2), st);
+  }
+}
+
+template <int DIM>
+std::vector<double> MinresSolver<DIM>::apply(int which, const std::vector<double>& in) {
+  auto& m = *this->state_->mesh_;
+  auto* c = this->state_->ctx_.get();
+  hipStream_t st = c->stream;
+  if (which < 0 || which > 2) throw std::invalid_argument("kkt apply: which must be 0, 1 or 2");
+  const int nT = m.nT();
+  // reference layout -> device layout (element parts structure of arrays)
+  std::vector<double> dev(N_, 0.0), out(N_);
+  if (which != 2) {
+    if ((long)in.size() != N_) throw std::invalid_argument("kkt apply: wrong vector size");
+    std::copy(in.begin(), in.begin() + n_, dev.begin());
+    for (int e = 0; e < nT; ++e)
+      for (int i = 0; i < 6; ++i) {
+        dev[n_ + (long)i * nT + e] = in[n_ + 6L * e + i];
+        dev[n_ + 6L * nT + (long)i * nT + e] = in[n_ + 6L * nT + 6L * e + i];
+      }
+    MFEM_HIP_CHECK(hipMemcpyAsync(vecs_.data(), dev.data(), N_ * sizeof(double),
+                                  hipMemcpyHostToDevice, st));
+  }
+  if (c->fp32()) apply_device<float>(which);
+  else apply_device<double>(which);
+  MFEM_HIP_CHECK(hipMemcpyAsync(dev.data(), vecs_.data() + N_, N_ * sizeof(double),
+                                hipMemcpyDeviceToHost, st));
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+  std::copy(dev.begin(), dev.begin() + n_, out.begin());
+  for (int e = 0; e < nT; ++e)
+    for (int i = 0; i < 6; ++i) {
+      out[n_ + 6L * e + i] = dev[n_ + (long)i * nT + e];
+      out[n_ + 6L * nT + 6L * e + i] = dev[n_ + 6L * nT + (long)i * nT + e];
+    }
+  return out;
+}
+
+template <int DIM>
+void MinresSolver<DIM>::solve() {
+  auto* c = this->state_->ctx_.get();
+  this->stats_ = SolveStats();
+  if (c->fp32()) run<float>();
+  else run<double>();
+  this->stats_.iterations = last_.iter;
+  this->stats_.refinements = 1;
+  this->stats_.rel_residual = estimated_error();
+  // max |ds| into the slot the Newton decrement reads
+  launch_reduce_maxabs(6 * this->state_->mesh_->nT(), mixed_->ds_.data(),
+                       c->reducer.slot(RS_MAXDS), c->stream);
+}
+
 template <int DIM>
 LinearSolverFactory<DIM>::LinearSolverFactory() {
   auto unsupported = [](SimState<DIM>*) -> std::unique_ptr<LinearSolver<DIM>> {
@@ -641,6 +849,10 @@ LinearSolverFactory<DIM>::LinearSolverFactory() {
   this->register_type(SOLVER_EIGEN_CG_DIAG, "eigen-pcg-diag",
                       [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
                         return std::make_unique<PcgSolver<DIM>>(s);
+                      });
+  this->register_type(SOLVER_MINRES_ID, "minres-indefinite-block",
+                      [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
+                        return std::make_unique<MinresSolver<DIM>>(s);
                       });
 }
 
@@ -1150,6 +1362,7 @@ template class VariableFactory<3>;
 template class MixedVariableFactory<3>;
 template class SystemMatrixPD<3>;
 template class PcgSolver<3>;
+template class MinresSolver<3>;
 template class LinearSolverFactory<3>;
 template class Optimizer<3>;
 template class NewtonOptimizer<3>;

@@ -197,6 +197,9 @@ class MixedStretch : public MixedVariable<DIM> {
   // element state (structure of arrays, stride nT)
   DeviceArray<double> R_, S_, s_, la_, ds_, g_, gl_;
   RealArray dsdF_, Hpsi_, Hinv_;
+  // false when the linear solver does not use the Schur-complement system
+  // (its element stiffness blocks and residual are then not formed)
+  bool schur_blocks_ = true;
 
  private:
   double h2_ = 0;
@@ -311,6 +314,43 @@ class PcgSolver : public LinearSolver<DIM> {
   std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots
   hipEvent_t poll_ev_[2] = {nullptr, nullptr};
   double wall_khz_ = 100000.0;   // device wall-clock rate
+};
+
+// "minres-indefinite-block": MINRES on the KKT system of the mixed
+// formulation (reference linear_solver_factory.cpp:117-134 with
+// SystemMatrixIndefinite, linear_system.h:60-99, and the matrix-free
+// BlockMatrix product, block_matrix.h:74-125). dx, ds and lambda come from the
+// solution directly; there is no local solve. The block preconditioner is
+// the GPU form described in k_minres.hip. Needs one mixed-stretch variable.
+template <int DIM>
+class MinresSolver : public LinearSolver<DIM> {
+ public:
+  explicit MinresSolver(SimState<DIM>* state);
+  ~MinresSolver() override;
+  void solve() override;
+  // |r|_P / |b| as estimated by the recurrence at exit
+  double estimated_error() const { return last_.rhs2 > 0 ? std::sqrt(last_.res2 / last_.rhs2) : 0; }
+  // Diagnostics at the current state, KKT vectors in the reference's layout
+  // [dx; ds; lambda] with element-major 6-vectors: which 0: K in, 1: P in,
+  // 2: the right-hand side (in unused).
+  std::vector<double> apply(int which, const std::vector<double>& in);
+  long kkt_size() const { return N_; }
+
+ private:
+  template <typename Real>
+  mfemhip::KktSystem<Real> system();
+  template <typename Real>
+  void run();
+  template <typename Real>
+  void apply_device(int which);
+  MixedStretch<DIM>* mixed_ = nullptr;
+  long n_ = 0, N_ = 0;
+  DeviceArray<double> vecs_;   // v[2], w[2], p[3]: 7 KKT vectors
+  DeviceArray<double> sinv_, mlump_inv_, mdinv_, cheb_, parts_;
+  DeviceArray<mfemhip::MinresCtrl> ctrl_;
+  mfemhip::MinresCtrl last_{};
+  std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots
+  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
 };
 
 template <int DIM>

@@ -1,0 +1,732 @@
+// Indefinite (KKT) Newton system and MINRES ("minres-indefinite-block").
+//
+// Unknowns [dx (3 nfree); ds (6 x nT); lambda (6 x nT)], the element parts
+// stored structure-of-arrays (component i of tet e at i * nT + e) so that
+// they alias the mixed variable's ds and lambda arrays.
+//
+//   K = [[PMP, 0, Gx], [0, Hloc, C], [Gx^T, C, 0]]
+//   Gx = -PJW diag(dSdF), C = vol Sym, Hloc = vol Hpsi (Hpsi = h^2 Hessian)
+//
+// is applied matrix-free as BlockMatrix::scaleAndAddTo does (reference
+// src/linear_solvers/block_matrix.h:74-125 with Displacement::product_hessian,
+// displacement.h:59-63, and MixedStretch::product_*, mixed_stretch.cpp:373-434).
+// In this build's notation dsdF is ds/dF (6 x 9) and dSdF = dsdF^T Sym, so the
+// lambda rows read -vol Sym dsdF Jloc dx + vol Sym ds and the x rows gather
+// -vol Jloc^T dsdF^T Sym lambda per tet.
+//
+// The preconditioner is diag(Chebyshev(PMP), Hloc^-1, S_e^-1), S_e the element
+// blocks of the Schur complement Gx^T Mlump^-1 Gx + C Hloc^-1 C, and MINRES
+// follows Eigen 3.4's preconditioned Lanczos/Givens recurrence
+// (unsupported/Eigen/src/IterativeSolvers/MINRES.h). One iteration is
+//   k_mr_tet, k_mr_vertex   p/x update of the previous iteration; v_new = K w - beta v_old
+//   k_mr_fin_alpha          alpha = v_new . w
+//   k_mr_prec_tet, _vtx0    v_new -= alpha v; w_new = P v_new (element blocks,
+//                           first Chebyshev step of the mass block)
+//   k_mr_cheb x 3           Chebyshev steps of the mass block
+//   k_mr_fin_beta           beta_new, Givens rotation, stopping test
+// with every scalar in one MinresCtrl block on the device. Lanczos vectors are
+// stored unnormalised: v_k = V_k / beta_k, w_k = W_k / beta_k. Every sum is a
+// per-workgroup partial reduced in index order by a one-workgroup kernel.
+#include <hip/hip_runtime.h>
+#include "device_math.h"
+#include "kernels.h"
+#include "reduce.h"
+
+namespace mfemhip {
+
+namespace {
+
+constexpr int kTetBS = 128;
+constexpr int kVtxBS = 256;
+constexpr int kFinBS = 1024;
+
+__device__ __forceinline__ double symw(int i) { return i < 3 ? 1.0 : 2.0; }
+
+inline int blocks_for(int n, int bs) { return n > 0 ? (n + bs - 1) / bs : 0; }
+
+// Sum of parts[0 .. n) in index order (strided by lane, then the block tree).
+__device__ __forceinline__ double fin_sum(const double* parts, int n, double* sm) {
+  return reduce_array<kFinBS, 0>(parts, n, 1, 0, sm);
+}
+
+// ---------------------------------------------------------------- setup
+// S_e = Sym (vol^2 B Lambda B^T + vol Hpsi^-1) Sym with B = dsdF Jloc and
+// Lambda = 1 / Mlump on free corners (0 on pinned ones), inverted by Cholesky
+// into Sinv (21 packed, stride nT). A block that is not positive definite
+// becomes NaN, which the MINRES scalars report.
+template <typename Real>
+__global__ __launch_bounds__(kTetBS) void k_kkt_setup(TetMeshDev m, const Real* __restrict__ dsdF,
+                                                      const Real* __restrict__ Hinv,
+                                                      const double* __restrict__ mlump_inv,
+                                                      double* __restrict__ Sinv) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  const double vol = m.vol[e];
+  const int4 t = m.T[e];
+  const int v[4] = {t.x, t.y, t.z, t.w};
+  double lam[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int f = m.free_map[v[k]];
+    lam[k] = f >= 0 ? mlump_inv[f] : 0.0;
+  }
+  // G(c, c') = sum_k Lambda_k dX(k, c) dX(k, c')
+  double dx[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) dx[i] = m.dX[i * nT + e];
+  double G[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int c2 = 0; c2 < 3; ++c2) {
+      double a = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a += lam[k] * dx[3 * k + c] * dx[3 * k + c2];
+      G[c + 3 * c2] = a;
+    }
+  // B Lambda B^T = dsdF (I_3 (x) G) dsdF^T, one row r of F at a time
+  double A[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) A[i] = 0.0;
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r) {
+    double J[18];  // J[i + 6c] = dsdF(i, r + 3c)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) J[i + 6 * c] = (double)dsdF[(i + 6 * (r + 3 * c)) * nT + e];
+    double T[18];  // T = J G
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double a = 0;
+#pragma unroll
+        for (int c2 = 0; c2 < 3; ++c2) a += J[i + 6 * c2] * G[c2 + 3 * c];
+        T[i + 6 * c] = a;
+      }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double a = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a += T[i + 6 * c] * J[j + 6 * c];
+        A[i + 6 * j] += a;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double h = (double)Hinv[sym6(i < j ? i : j, i < j ? j : i) * nT + e];
+      A[i + 6 * j] = symw(i) * (vol * vol * A[i + 6 * j] + vol * h) * symw(j);
+    }
+  // Cholesky A = L L^T (lower, in place), then Sinv = L^-T L^-1
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double d = A[j + 6 * j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= A[j + 6 * k] * A[j + 6 * k];
+    ok = ok && d > 0.0;
+    d = sqrt(fmax(d, 0.0));
+    A[j + 6 * j] = d;
+    const double id = 1.0 / d;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double s = A[i + 6 * j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= A[i + 6 * k] * A[j + 6 * k];
+      A[i + 6 * j] = s * id;
+    }
+  }
+  double Li[36];  // L^-1 (lower)
+#pragma unroll
+  for (int i = 0; i < 36; ++i) Li[i] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    Li[c + 6 * c] = 1.0 / A[c + 6 * c];
+#pragma unroll
+    for (int i = c + 1; i < 6; ++i) {
+      double s = 0;
+#pragma unroll
+      for (int k = c; k < i; ++k) s -= A[i + 6 * k] * Li[k + 6 * c];
+      Li[i + 6 * c] = s / A[i + 6 * i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      double s = 0;
+#pragma unroll
+      for (int k = j; k < 6; ++k) s += Li[k + 6 * i] * Li[k + 6 * j];
+      Sinv[sym6(i, j) * nT + e] = ok ? s : NAN;
+    }
+}
+
+// ---------------------------------------------------------------- rhs
+// b = [-PM (x_full - x_tilde - h^2 g); -vol g; vol Sym (S - s)]
+// (linear_system.h:67-86): the inertia part per tet into fe (gathered per
+// free vertex by k_kkt_rhs_vtx), the element parts straight into b.
+__global__ __launch_bounds__(kTetBS) void k_kkt_rhs_tet(TetMeshDev m, const double* __restrict__ x,
+                                                        Inertia in, const double* __restrict__ g,
+                                                        const double* __restrict__ S,
+                                                        const double* __restrict__ s,
+                                                        double* __restrict__ fe, KktVec b,
+                                                        double* __restrict__ parts) {
+  __shared__ double sm[kTetBS / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (e < m.nT) {
+    const int nT = m.nT;
+    const double vol = m.vol[e];
+    const int4 t = m.T[e];
+    const int v[4] = {t.x, t.y, t.z, t.w};
+    double d[12], dsum[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double dd = x[3 * v[k] + r] - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+        d[3 * k + r] = dd;
+        dsum[r] += dd;
+      }
+    const double mc = m.density * vol / 20.0;
+    double* fo = fe + (size_t)e * 12;
+#pragma unroll
+    for (int j = 0; j < 12; ++j) fo[j] = -mc * (d[j] + dsum[j % 3]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const double bs = -vol * g[i * nT + e];
+      const double bl = vol * symw(i) * (S[i * nT + e] - s[i * nT + e]);
+      b.s[i * nT + e] = bs;
+      b.l[i * nT + e] = bl;
+      acc += bs * bs + bl * bl;
+    }
+  }
+  block_partial<kTetBS, 0>(acc, parts, sm);
+}
+
+__global__ __launch_bounds__(kVtxBS) void k_kkt_rhs_vtx(int nfree, const int* __restrict__ inc_ptr,
+                                                        const int* __restrict__ inc,
+                                                        const double* __restrict__ fe, KktVec b,
+                                                        double* __restrict__ parts) {
+  __shared__ double sm[kVtxBS / kWave];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (i < nfree) {
+    double a[3] = {0, 0, 0};
+    const int p1 = inc_ptr[i + 1];
+    for (int p = inc_ptr[i]; p < p1; ++p) {
+      const int ek = inc[p];
+      const double* f = fe + (size_t)(ek >> 2) * 12 + 3 * (ek & 3);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) a[r] += f[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      b.x[3 * i + r] = a[r];
+      acc += a[r] * a[r];
+    }
+  }
+  block_partial<kVtxBS, 0>(acc, parts, sm);
+}
+
+// ---------------------------------------------------------------- iteration
+// Update of the previous iteration, p_{k-1} = (w_{k-1} - r2 p_{k-2} -
+// r3 p_{k-3}) / r1 and x += beta_one c eta p_{k-1}, on one entry.
+__device__ __forceinline__ void pending_update(const MinresCtrl& c, long i, const double* wo,
+                                               const double* pa, const double* pb, double* pn,
+                                               double* x) {
+  const double p = (wo[i] / c.beta_prev - c.r2 * pa[i] - c.r3 * pb[i]) / c.r1;
+  pn[i] = p;
+  x[i] += c.coef * p;
+}
+
+struct MrVecs {
+  KktVec wc;  // W_k (scale beta_k)
+  KktVec vo;  // in: V_{k-1} (scale beta_{k-1}); out: V_{k+1} before alpha
+  KktVec wo;  // W_{k-1} (scale beta_{k-1})
+  KktVec pa;  // p_{k-2}
+  KktVec pb;  // p_{k-3}
+  KktVec pn;  // out: p_{k-1} (the slot of p_{k-4})
+  KktVec x;   // solution
+};
+
+template <typename Real>
+__global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtrl* __restrict__ ctrl,
+                                                   const Real* __restrict__ dsdF,
+                                                   const Real* __restrict__ Hpsi, MrVecs V,
+                                                   double* __restrict__ fe,
+                                                   double* __restrict__ parts) {
+  __shared__ double sm[kTetBS / kWave];
+  const MinresCtrl c = *ctrl;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nT = m.nT;
+  if (c.pending && e < nT) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      pending_update(c, (long)i * nT + e, V.wo.s, V.pa.s, V.pb.s, V.pn.s, V.x.s);
+      pending_update(c, (long)i * nT + e, V.wo.l, V.pa.l, V.pb.l, V.pn.l, V.x.l);
+    }
+  }
+  if (c.done) return;
+  double acc = 0.0;
+  if (e < nT) {
+    const double vol = m.vol[e];
+    const double ib = 1.0 / c.beta;
+    const double ratio = c.beta / c.beta_prev;
+    double ws[6], wl[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      ws[i] = V.wc.s[i * nT + e] * ib;
+      wl[i] = V.wc.l[i * nT + e] * ib;
+    }
+    // Jloc w_x over the free corners
+    const int4 t = m.T[e];
+    const int v[4] = {t.x, t.y, t.z, t.w};
+    double Fw[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fw[i] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int f = m.free_map[v[k]];
+      double wk[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) wk[r] = f >= 0 ? V.wc.x[3 * f + r] * ib : 0.0;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const double d = m.dX[(3 * k + cc) * nT + e];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Fw[r + 3 * cc] += wk[r] * d;
+      }
+    }
+    // u = dsdF Fw and tv = dsdF^T (Sym w_l), one pass over dsdF
+    double u[6], tv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tv[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) u[i] = 0.0;
+#pragma unroll
+    for (int mm = 0; mm < 9; ++mm)
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const double js = (double)dsdF[(i + 6 * mm) * nT + e];
+        u[i] += js * Fw[mm];
+        tv[mm] += js * symw(i) * wl[i];
+      }
+    // x rows of this tet: -vol Jloc^T tv
+    double* fo = fe + (size_t)e * 12;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double a = 0;
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) a += m.dX[(3 * k + cc) * nT + e] * tv[r + 3 * cc];
+        fo[3 * k + r] = -vol * a;
+      }
+    // s rows: Hloc w_s + vol Sym w_l ; lambda rows: vol Sym (w_s - u)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        a += (double)Hpsi[sym6(i < j ? i : j, i < j ? j : i) * nT + e] * ws[j];
+      const double ys = vol * a + vol * symw(i) * wl[i];
+      const double yl = vol * symw(i) * (ws[i] - u[i]);
+      const long is = (long)i * nT + e;
+      // v_new = K w - beta v_old, v_old = V_{k-1} / beta_{k-1}
+      const double vs = ys - ratio * V.vo.s[is];
+      const double vl = yl - ratio * V.vo.l[is];
+      V.vo.s[is] = vs;
+      V.vo.l[is] = vl;
+      acc += vs * ws[i] + vl * wl[i];
+    }
+  }
+  block_partial<kTetBS, 0>(acc, parts, sm);
+}
+
+__global__ __launch_bounds__(kVtxBS) void k_mr_vertex(int nfree, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col,
+                                                      const double* __restrict__ mass,
+                                                      const int* __restrict__ inc_ptr,
+                                                      const int* __restrict__ inc,
+                                                      const double* __restrict__ fe,
+                                                      const MinresCtrl* __restrict__ ctrl, MrVecs V,
+                                                      double* __restrict__ parts) {
+  __shared__ double sm[kVtxBS / kWave];
+  const MinresCtrl c = *ctrl;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c.pending && i < nfree) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      pending_update(c, 3L * i + r, V.wo.x, V.pa.x, V.pb.x, V.pn.x, V.x.x);
+  }
+  if (c.done) return;
+  double acc = 0.0;
+  if (i < nfree) {
+    const double ib = 1.0 / c.beta;
+    const double ratio = c.beta / c.beta_prev;
+    double y[3] = {0, 0, 0};
+    const int b1 = rowptr[i + 1];
+    for (int b = rowptr[i]; b < b1; ++b) {
+      const double mm = mass[b];
+      const int j = col[b];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) y[r] += mm * (V.wc.x[3 * j + r] * ib);
+    }
+    double z[3] = {0, 0, 0};
+    const int p1 = inc_ptr[i + 1];
+    for (int p = inc_ptr[i]; p < p1; ++p) {
+      const int ek = inc[p];
+      const double* f = fe + (size_t)(ek >> 2) * 12 + 3 * (ek & 3);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) z[r] += f[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const long ir = 3L * i + r;
+      const double vn = (y[r] + z[r]) - ratio * V.vo.x[ir];
+      V.vo.x[ir] = vn;
+      acc += vn * (V.wc.x[ir] * ib);
+    }
+  }
+  block_partial<kVtxBS, 0>(acc, parts, sm);
+}
+
+// alpha = v_new . w; the pending update has been applied by now.
+__global__ __launch_bounds__(kFinBS) void k_mr_fin_alpha(MinresCtrl* ctrl, const double* parts,
+                                                         int nparts) {
+  __shared__ double sm[kFinBS / kWave];
+  const bool done = ctrl->done;
+  __syncthreads();
+  if (threadIdx.x == 0) ctrl->pending = 0;
+  if (done) return;
+  const double a = fin_sum(parts, nparts, sm);
+  if (threadIdx.x == 0) ctrl->alpha = a;
+}
+
+// v_new -= alpha v_k (init: alpha = 0, v_k absent); w_new = P v_new on the
+// element parts: Hloc^-1 = Hpsi^-1 / vol on s, S_e^-1 on lambda.
+template <typename Real>
+__global__ __launch_bounds__(kTetBS) void k_mr_prec_tet(TetMeshDev m, const MinresCtrl* __restrict__ ctrl,
+                                                        int init, const Real* __restrict__ Hinv,
+                                                        const double* __restrict__ Sinv, KktVec vn,
+                                                        KktVec vc, KktVec wn,
+                                                        double* __restrict__ parts) {
+  __shared__ double sm[kTetBS / kWave];
+  const MinresCtrl c = *ctrl;
+  if (!init && c.done) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nT = m.nT;
+  double acc = 0.0;
+  if (e < nT) {
+    const double ab = init ? 0.0 : c.alpha / c.beta;
+    const double vol = m.vol[e];
+    double vs[6], vl[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const long is = (long)i * nT + e;
+      vs[i] = init ? vn.s[is] : vn.s[is] - ab * vc.s[is];
+      vl[i] = init ? vn.l[is] : vn.l[is] - ab * vc.l[is];
+      if (!init) {
+        vn.s[is] = vs[i];
+        vn.l[is] = vl[i];
+      }
+    }
+    const double iv = 1.0 / vol;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double a = 0, b = 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const int k = sym6(i < j ? i : j, i < j ? j : i);
+        a += (double)Hinv[k * nT + e] * vs[j];
+        b += Sinv[k * nT + e] * vl[j];
+      }
+      const long is = (long)i * nT + e;
+      const double zs = a * iv;
+      wn.s[is] = zs;
+      wn.l[is] = b;
+      acc += vs[i] * zs + vl[i] * b;
+    }
+  }
+  block_partial<kTetBS, 0>(acc, parts, sm);
+}
+
+// x part: v_new -= alpha v_k, then the first Chebyshev step of the mass block:
+// r = v_new, d = D^-1 r / theta, z = d.
+__global__ __launch_bounds__(kVtxBS) void k_mr_prec_vtx0(int nfree, const MinresCtrl* __restrict__ ctrl,
+                                                         int init, const double* __restrict__ dinv,
+                                                         double theta, KktVec vn, KktVec vc,
+                                                         KktVec wn, double* __restrict__ cr,
+                                                         double* __restrict__ cd) {
+  const MinresCtrl c = *ctrl;
+  if (!init && c.done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfree) return;
+  const double ab = init ? 0.0 : c.alpha / c.beta;
+  const double di = dinv[i];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const long ir = 3L * i + r;
+    double v = vn.x[ir];
+    if (!init) {
+      v -= ab * vc.x[ir];
+      vn.x[ir] = v;
+    }
+    const double d = di * v / theta;
+    cr[ir] = v;
+    cd[ir] = d;
+    wn.x[ir] = d;
+  }
+}
+
+// One Chebyshev step: r -= PMP d_in; d_out = c1 d_in + c2 D^-1 r; z += d_out.
+// The last step also writes the partials of v_new . z.
+__global__ __launch_bounds__(kVtxBS) void k_mr_cheb(int nfree, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ mass,
+                                                    const double* __restrict__ dinv,
+                                                    const MinresCtrl* __restrict__ ctrl, double c1,
+                                                    double c2, const double* __restrict__ d_in,
+                                                    double* __restrict__ d_out,
+                                                    double* __restrict__ cr, KktVec vn, KktVec wn,
+                                                    int last, int init,
+                                                    double* __restrict__ parts) {
+  __shared__ double sm[kVtxBS / kWave];
+  if (!init && ctrl->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (i < nfree) {
+    double y[3] = {0, 0, 0};
+    const int b1 = rowptr[i + 1];
+    for (int b = rowptr[i]; b < b1; ++b) {
+      const double mm = mass[b];
+      const int j = col[b];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) y[r] += mm * d_in[3 * j + r];
+    }
+    const double di = dinv[i];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const long ir = 3L * i + r;
+      const double rr = cr[ir] - y[r];
+      cr[ir] = rr;
+      const double d = c1 * d_in[ir] + c2 * di * rr;
+      d_out[ir] = d;
+      const double z = wn.x[ir] + d;
+      wn.x[ir] = z;
+      acc += vn.x[ir] * z;
+    }
+  }
+  if (last) block_partial<kVtxBS, 0>(acc, parts, sm);
+}
+
+// beta_new, the Givens rotation and the stopping test of one iteration.
+__global__ __launch_bounds__(kFinBS) void k_mr_fin_beta(MinresCtrl* ctrl, const double* parts,
+                                                        int nparts) {
+  __shared__ double sm[kFinBS / kWave];
+  if (ctrl->done) return;
+  const double b2 = fin_sum(parts, nparts, sm);
+  if (threadIdx.x != 0) return;
+  MinresCtrl c = *ctrl;
+  if (!(b2 >= 0.0)) {  // preconditioner not positive definite (or NaN)
+    c.fail = 1;
+    c.done = 1;
+    *ctrl = c;
+    return;
+  }
+  const double bn = sqrt(b2);
+  const double beta = c.beta, alpha = c.alpha;
+  const double r2 = c.s * alpha + c.c * c.c_old * beta;
+  const double r3 = c.s_old * beta;
+  const double r1_hat = c.c * alpha - c.c_old * c.s * beta;
+  const double r1 = sqrt(r1_hat * r1_hat + bn * bn);
+  c.c_old = c.c;
+  c.s_old = c.s;
+  c.c = r1_hat / r1;
+  c.s = bn / r1;
+  c.r1 = r1;
+  c.r2 = r2;
+  c.r3 = r3;
+  c.coef = c.beta_one * c.c * c.eta;
+  c.pending = 1;
+  c.res2 *= c.s * c.s;
+  if (c.res2 < c.threshold2) {
+    c.done = 1;
+  } else {
+    c.eta = -c.s * c.eta;
+    c.iter += 1;
+    if (c.iter >= c.max_iters) c.done = 1;
+  }
+  c.beta_prev = beta;
+  c.beta = bn;
+  *ctrl = c;
+}
+
+// Control block for a bare product (beta = 1, nothing pending, not stopped).
+__global__ void k_mr_ctrl_unit(MinresCtrl* ctrl) {
+  MinresCtrl c{};
+  c.beta = c.beta_prev = c.beta_one = c.r1 = 1.0;
+  *ctrl = c;
+}
+
+// Scalars after the right-hand side and its preconditioned image.
+__global__ __launch_bounds__(kFinBS) void k_mr_fin_init(MinresCtrl* ctrl, const double* parts_b,
+                                                        const double* parts_w, int nparts,
+                                                        double tol, int max_iters) {
+  __shared__ double sm[kFinBS / kWave];
+  const double rhs2 = fin_sum(parts_b, nparts, sm);
+  const double b2 = fin_sum(parts_w, nparts, sm);
+  if (threadIdx.x != 0) return;
+  MinresCtrl c{};
+  c.rhs2 = rhs2;
+  c.threshold2 = tol * tol * rhs2;
+  c.res2 = rhs2;
+  c.max_iters = max_iters;
+  c.c = c.c_old = 1.0;
+  c.s = c.s_old = 0.0;
+  c.eta = 1.0;
+  c.beta_prev = 1.0;
+  c.r1 = 1.0;
+  if (rhs2 == 0.0 || max_iters <= 0) {
+    c.done = 1;
+    c.beta = c.beta_one = 1.0;
+  } else if (!(b2 >= 0.0)) {
+    c.done = 1;
+    c.fail = 1;
+    c.beta = c.beta_one = 1.0;
+  } else {
+    c.beta = c.beta_one = sqrt(b2);
+  }
+  *ctrl = c;
+}
+
+}  // namespace
+
+int kkt_tet_blocks(int nT) { return blocks_for(nT, kTetBS); }
+int kkt_vertex_blocks(int nfree) { return blocks_for(nfree, kVtxBS); }
+
+template <typename Real>
+void launch_kkt_setup(const KktSystem<Real>& S, hipStream_t st) {
+  if (S.m.nT == 0) return;
+  hipLaunchKernelGGL(k_kkt_setup<Real>, dim3(kkt_tet_blocks(S.m.nT)), dim3(kTetBS), 0, st, S.m,
+                     S.dsdF, S.Hinv, S.mlump_inv, S.Sinv);
+}
+
+// Chebyshev constants for the mass block over [1/2, 5/2]: theta 1.5,
+// delta 1, sigma = theta / delta; rho_0 = 1 / sigma, rho_j = 1 / (2 sigma -
+// rho_{j-1}); step j uses c1 = rho_j rho_{j-1}, c2 = 2 rho_j / delta.
+namespace {
+constexpr double kChebTheta = 1.5, kChebDelta = 1.0;
+
+template <typename Real>
+void launch_precondition(const KktSystem<Real>& S, int init, KktVec vn, KktVec vc, KktVec wn,
+                         hipStream_t st) {
+  const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
+  if (nbT > 0)
+    hipLaunchKernelGGL(k_mr_prec_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, init,
+                       S.Hinv, S.Sinv, vn, vc, wn, S.parts[1]);
+  if (nbV == 0) return;
+  hipLaunchKernelGGL(k_mr_prec_vtx0, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.ctrl, init,
+                     S.mdinv, kChebTheta, vn, vc, wn, S.cheb_r, S.cheb_d[0]);
+  const double sigma = kChebTheta / kChebDelta;
+  double rho = 1.0 / sigma;
+  for (int j = 1; j < kMassChebSteps; ++j) {
+    const double rho1 = 1.0 / (2.0 * sigma - rho);
+    const int last = j == kMassChebSteps - 1;
+    hipLaunchKernelGGL(k_mr_cheb, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.rowptr, S.col,
+                       S.mass, S.mdinv, S.ctrl, rho1 * rho, 2.0 * rho1 / kChebDelta,
+                       S.cheb_d[(j - 1) & 1], S.cheb_d[j & 1], S.cheb_r, vn, wn, last, init,
+                       S.parts[1] + nbT);
+    rho = rho1;
+  }
+}
+}  // namespace
+
+template <typename Real>
+void launch_minres_init(const KktSystem<Real>& S, const double* x_full, const Inertia& in,
+                        const double* g, const double* Smat, const double* s, KktVec b, KktVec w,
+                        double tol, int max_iters, hipStream_t st) {
+  const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
+  if (nbT > 0)
+    hipLaunchKernelGGL(k_kkt_rhs_tet, dim3(nbT), dim3(kTetBS), 0, st, S.m, x_full, in, g, Smat, s,
+                       S.fe, b, S.parts[0]);
+  if (nbV > 0)
+    hipLaunchKernelGGL(k_kkt_rhs_vtx, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.inc_ptr,
+                       S.inc, S.fe, b, S.parts[0] + nbT);
+  launch_precondition(S, 1, b, b, w, st);
+  hipLaunchKernelGGL(k_mr_fin_init, dim3(1), dim3(kFinBS), 0, st, S.ctrl, S.parts[0],
+                     S.parts[1], nbT + nbV, tol, max_iters);
+}
+
+template <typename Real>
+void launch_minres_iteration(const KktSystem<Real>& S, const MinresBuffers& B, long k,
+                             hipStream_t st) {
+  const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
+  auto mod3 = [](long j) { return (int)(((j % 3) + 3) % 3); };
+  MrVecs V;
+  V.wc = B.w[k & 1];
+  V.wo = B.w[(k + 1) & 1];
+  V.vo = B.v[(k + 1) & 1];
+  V.pa = B.p[mod3(k - 2)];
+  V.pb = B.p[mod3(k - 3)];
+  V.pn = B.p[mod3(k - 1)];  // p_{k-1} replaces p_{k-4}
+  V.x = B.x;
+  if (nbT > 0)
+    hipLaunchKernelGGL(k_mr_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
+                       S.Hpsi, V, S.fe, S.parts[0]);
+  if (nbV > 0)
+    hipLaunchKernelGGL(k_mr_vertex, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.rowptr, S.col,
+                       S.mass, S.inc_ptr, S.inc, S.fe, S.ctrl, V, S.parts[0] + nbT);
+  hipLaunchKernelGGL(k_mr_fin_alpha, dim3(1), dim3(kFinBS), 0, st, S.ctrl, S.parts[0], nbT + nbV);
+  // v_new (in V_{k-1}'s buffer) -= alpha v_k; w_new into W_{k-1}'s buffer
+  launch_precondition(S, 0, B.v[(k + 1) & 1], B.v[k & 1], B.w[(k + 1) & 1], st);
+  hipLaunchKernelGGL(k_mr_fin_beta, dim3(1), dim3(kFinBS), 0, st, S.ctrl, S.parts[1], nbT + nbV);
+}
+
+template <typename Real>
+void launch_kkt_apply(const KktSystem<Real>& S, int which, KktVec in, KktVec out, KktVec scratch,
+                      hipStream_t st) {
+  const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
+  if (which == 1) {  // out = P in
+    launch_kkt_setup(S, st);
+    launch_precondition(S, 1, in, in, out, st);
+    return;
+  }
+  // out = K in - 1 * 0: the product kernels with beta = 1 and v_old = 0
+  hipLaunchKernelGGL(k_mr_ctrl_unit, dim3(1), dim3(1), 0, st, S.ctrl);
+  (void)hipMemsetAsync(out.x, 0, 3 * (size_t)S.m.nfree * sizeof(double), st);
+  (void)hipMemsetAsync(out.s, 0, 6 * (size_t)S.m.nT * sizeof(double), st);
+  (void)hipMemsetAsync(out.l, 0, 6 * (size_t)S.m.nT * sizeof(double), st);
+  MrVecs V;
+  V.wc = in;
+  V.wo = in;
+  V.vo = out;
+  V.pa = V.pb = V.pn = V.x = scratch;
+  if (nbT > 0)
+    hipLaunchKernelGGL(k_mr_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
+                       S.Hpsi, V, S.fe, S.parts[0]);
+  if (nbV > 0)
+    hipLaunchKernelGGL(k_mr_vertex, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.rowptr, S.col,
+                       S.mass, S.inc_ptr, S.inc, S.fe, S.ctrl, V, S.parts[0] + nbT);
+}
+
+#define MFEM_MINRES_INST(Real)                                                                  \
+  template void launch_kkt_setup<Real>(const KktSystem<Real>&, hipStream_t);                    \
+  template void launch_minres_init<Real>(const KktSystem<Real>&, const double*, const Inertia&, \
+                                         const double*, const double*, const double*, KktVec,    \
+                                         KktVec, double, int, hipStream_t);                      \
+  template void launch_minres_iteration<Real>(const KktSystem<Real>&, const MinresBuffers&, long, \
+                                              hipStream_t);                                      \
+  template void launch_kkt_apply<Real>(const KktSystem<Real>&, int, KktVec, KktVec, KktVec,      \
+                                       hipStream_t);
+MFEM_MINRES_INST(float)
+MFEM_MINRES_INST(double)
+
+}  // namespace mfemhip

@@ -192,6 +192,69 @@ void launch_mixed_energy_batch(const TetMeshDev& m, const double* x_full, const 
                                const double* ds, const double* lam, double h2, double* e_tet,
                                hipStream_t st);
 
+// --- indefinite (KKT) system and MINRES ("minres-indefinite-block";
+// reference linear_system.h:60-127, block_matrix.h:74-125) ----------------
+// Scalars of one MINRES solve (Eigen 3.4's recurrence), device resident.
+struct MinresCtrl {
+  double beta, beta_prev, alpha;     // beta_k, beta_{k-1}, alpha_k
+  double c, c_old, s, s_old, eta;    // Givens state
+  double beta_one, res2, rhs2, threshold2;
+  double r1, r2, r3, coef;           // update of the last iteration: p and x
+  int iter, done, max_iters, pending, fail, pad_;
+};
+// A KKT vector: x part (3 nfree) and the element parts (6 x nT, stride nT).
+struct KktVec {
+  double* x = nullptr;
+  double* s = nullptr;
+  double* l = nullptr;
+};
+// Chebyshev steps of the mass block of the preconditioner
+constexpr int kMassChebSteps = 4;
+template <typename Real>
+struct KktSystem {
+  TetMeshDev m;
+  const int* rowptr = nullptr;   // BSR pattern over free vertices
+  const int* col = nullptr;
+  const double* mass = nullptr;  // PMP coefficient per block
+  const int* inc_ptr = nullptr;  // (tet, corner) incidences per free vertex
+  const int* inc = nullptr;
+  const Real* dsdF = nullptr;    // [54][nT]
+  const Real* Hpsi = nullptr;    // [21][nT], h^2 Hessian (PSD-projected)
+  const Real* Hinv = nullptr;    // [21][nT]
+  double* Sinv = nullptr;        // [21][nT] inverted element Schur blocks
+  const double* mlump_inv = nullptr;  // [nfree] 1 / row sum of PMP
+  const double* mdinv = nullptr;      // [nfree] 1 / diagonal of PMP
+  double* fe = nullptr;               // [12 nT] scratch
+  double* cheb_r = nullptr;           // [3 nfree] scratch
+  double* cheb_d[2] = {nullptr, nullptr};
+  double* parts[2] = {nullptr, nullptr};  // kkt_tet_blocks + kkt_vertex_blocks each
+  MinresCtrl* ctrl = nullptr;
+};
+struct MinresBuffers {
+  KktVec v[2], w[2], p[3];  // Lanczos vectors (unnormalised) and search directions
+  KktVec x;                 // solution [dx; ds; lambda]
+};
+int kkt_tet_blocks(int nT);
+int kkt_vertex_blocks(int nfree);
+// Inverted element Schur blocks of the preconditioner from the current state.
+template <typename Real>
+void launch_kkt_setup(const KktSystem<Real>& S, hipStream_t st);
+// Right-hand side into b (= v_0 before scaling), w = P b, and the scalars.
+template <typename Real>
+void launch_minres_init(const KktSystem<Real>& S, const double* x_full, const Inertia& in,
+                        const double* g, const double* Smat, const double* s, KktVec b, KktVec w,
+                        double tol, int max_iters, hipStream_t st);
+// MINRES iteration k (k = 0, 1, ...): applies the solution update of
+// iteration k-1 first, so one more launch after the stop completes x.
+template <typename Real>
+void launch_minres_iteration(const KktSystem<Real>& S, const MinresBuffers& B, long k,
+                             hipStream_t st);
+// Diagnostics: out = K in (which 0) or out = P in (which 1) at the current
+// state; scratch is a KKT vector the product may write.
+template <typename Real>
+void launch_kkt_apply(const KktSystem<Real>& S, int which, KktVec in, KktVec out, KktVec scratch,
+                      hipStream_t st);
+
 // --- vectors ---------------------------------------------------------------
 // x_full[free] += alpha * dx ; s += alpha * ds (line-search acceptance,
 // reference linesearch.h:120-126)

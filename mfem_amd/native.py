@@ -23,6 +23,7 @@ EXPORTED = [
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
     "mfem_sim_time_spmv", "mfem_sim_spmv_tuning", "mfem_sim_set_cg_shape", "mfem_sim_cg_tuning",
+    "mfem_sim_kkt_size", "mfem_sim_kkt_apply",
     "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
     "mfem_batch_create", "mfem_batch_destroy", "mfem_batch_step", "mfem_batch_sizes",
     "mfem_batch_get_positions", "mfem_batch_stats", "mfem_batch_set_cg_shape",
@@ -91,6 +92,8 @@ def lib():
         L.mfem_sim_spmv_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
         L.mfem_sim_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
         L.mfem_sim_cg_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip, ip, ip, ip]
+        L.mfem_sim_kkt_size.argtypes = [vp, C.POINTER(C.c_long)]
+        L.mfem_sim_kkt_apply.argtypes = [vp, C.c_int, dp, dp]
         L.mfem_material_s6_batch.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.mfem_snh_f9_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
         L.mfem_polar_batch.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp]
@@ -243,6 +246,21 @@ class Sim:
     def set_cg_shape(self, variant, blocks=0):
         """Force the CG kernel shape and grid (variant -1: autotune again)."""
         check(lib().mfem_sim_set_cg_shape(self.h, int(variant), int(blocks)))
+
+    def kkt_size(self):
+        n = C.c_long(0)
+        check(lib().mfem_sim_kkt_size(self.h, C.byref(n)))
+        return n.value
+
+    def kkt_apply(self, which, v=None):
+        """KKT system of "minres-indefinite-block" at the current state: which
+        0 -> K v, 1 -> P v (block preconditioner), 2 -> right-hand side."""
+        N = self.kkt_size()
+        out = np.zeros(N)
+        vin = _f64(v) if v is not None else np.zeros(N)
+        check(lib().mfem_sim_kkt_apply(self.h, int(which), vin.ctypes.data_as(dp),
+                                       out.ctypes.data_as(dp)))
+        return out
 
     def cg_tuning(self):
         """List of (variant, grid, ms per CG iteration) the tuner timed, the
