@@ -161,6 +161,7 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
                                "' has no device kernel");
     mat[e] = (int)t;
   }
+  elem_type_ = mat;
   d_mu.upload(mu, st);
   d_la.upload(la, st);
   d_mat.upload(mat, st);

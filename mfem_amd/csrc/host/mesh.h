@@ -65,6 +65,8 @@ class Mesh {
   DeviceArray<int> d_rowptr, d_col, d_slot_ptr, d_slots, d_diag_block, d_inc_ptr, d_inc;
   DeviceArray<double> d_mass;
   std::vector<int> rowptr_, col_;  // host copy of the BSR pattern
+  // material model id of every element (after init)
+  const std::vector<int>& element_types() const { return elem_type_; }
   // consistent-mass coefficient per block, and the diagonal block of each row
   const std::vector<double>& mass() const { return mass_; }
   const std::vector<int>& diag_block() const { return diag_block_; }
@@ -74,6 +76,7 @@ class Mesh {
   int nV_ = 0, nT_ = 0, nfree_ = 0, nnzb_ = 0;
   std::vector<int> slot_ptr_, slots_, diag_block_, inc_ptr_, inc_;
   std::vector<double> mass_;
+  std::vector<int> elem_type_;
 };
 
 // Pinned set of the reference's boundary conditions

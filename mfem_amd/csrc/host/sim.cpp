@@ -183,6 +183,18 @@ int Displacement<DIM>::launch_max_abs_delta() {
 
 // ============================================================ Stretch
 template <int DIM>
+Stretch<DIM>::Stretch(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config,
+                      DeviceContext* ctx)
+    : Variable<DIM>(std::move(mesh), ctx) {
+  for (int t : this->mesh_->element_types())
+    if (t != MATERIAL_SNH && t != MATERIAL_NH && t != MATERIAL_FUNG)
+      throw std::runtime_error("material '" +
+                               MaterialModelFactory().name_by_type((MaterialModelType)t) +
+                               "' has no displacement-only (9-vector) energy; \"stretch\" needs "
+                               "Stable-Neohookean, Neohookean or Fung");
+}
+
+template <int DIM>
 void Stretch<DIM>::update(const double* x_full, double dt) {
   h2_ = dt * dt;
   auto* c = this->ctx_;

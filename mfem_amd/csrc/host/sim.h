@@ -163,8 +163,10 @@ template <int DIM>
 class Stretch : public Variable<DIM> {
  public:
   static std::string name() { return "stretch"; }
-  Stretch(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config, DeviceContext* ctx)
-      : Variable<DIM>(std::move(mesh), ctx) {}
+  // Needs the 9-vector (F) API of every element's material: SNH, NH or Fung
+  // (Corot, FCR and ARAP leave it unimplemented, reference
+  // material_model.h:43-62).
+  Stretch(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config, DeviceContext* ctx);
   void update(const double* x_full, double dt) override;
   void reset() override {}
   int size() const override { return 0; }

@@ -457,6 +457,70 @@ __device__ __forceinline__ void snh_f9_grad_hess(double mu, double la, const dou
     }
 }
 
+// Neo-Hookean on F (reference neohookean.cpp:96-231): Psi = mu/2 (|F|^2 - 3)
+// - mu log J + la/2 (log J)^2, g = mu F + a c with a = (la log J - mu) / J,
+// H = mu I + (la - la log J + mu) / J^2 c c^T + a d2J/dF2, projected with the
+// default tolerance (:230).
+__device__ __forceinline__ double nh_f9_energy(double mu, double la, const double F[9]) {
+  double J, c[9];
+  snh_f9_cof(F, J, c);
+  double n2 = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) n2 += F[i] * F[i];
+  const double lj = log(J);
+  return 0.5 * mu * (n2 - 3.0) - mu * lj + 0.5 * la * lj * lj;
+}
+
+template <typename Real>
+__device__ __forceinline__ void nh_f9_grad_hess(double mu, double la, const double F[9],
+                                                double g[9], Real H[81]) {
+  double J, c[9];
+  snh_f9_cof(F, J, c);
+  const double lj = log(J);
+  const double a = (la * lj - mu) / J;
+  const double cc = (la - la * lj + mu) / (J * J);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) g[i] = mu * F[i] + a * c[i];
+#pragma unroll
+  for (int p = 0; p < 9; ++p)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      int i = p % 3, j = p / 3, kk = q % 3, l = q / 3;
+      double v = (p == q ? mu : 0.0) + cc * c[p] * c[q];
+      if (i != kk && j != l) {
+        int mm = 3 - i - kk, n = 3 - j - l;
+        v += a * (double)(eps3(i, kk, mm) * eps3(j, l, n)) * F[mm + 3 * n];
+      }
+      H[p + 9 * q] = (Real)v;
+    }
+}
+
+// The 9-vector API of material `type` (SNH, NH, and Fung's constant 0, 0, I,
+// fung.cpp:101-107; the host admits no other type on this path), with each
+// model's PSD projection.
+__device__ __forceinline__ double f9_energy(int type, double mu, double la, const double F[9]) {
+  if (type == MAT_NH) return nh_f9_energy(mu, la, F);
+  if (type == MAT_FUNG) return 0.0;
+  return snh_f9_energy(mu, la, F);
+}
+
+template <typename Real>
+__device__ __forceinline__ void f9_grad_hess(int type, double mu, double la, const double F[9],
+                                             double g[9], Real H[81]) {
+  if (type == MAT_FUNG) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 81; ++i) H[i] = (i % 10 == 0) ? Real(1) : Real(0);
+  } else if (type == MAT_NH) {
+    nh_f9_grad_hess<Real>(mu, la, F, g, H);
+    if (!chol_pd<Real, 9>(H, Real(1e-6))) jacobi_psd_project<Real, 9>(H, Real(1e-6));
+  } else {
+    snh_f9_grad_hess<Real>(mu, la, F, g, H);
+    if (!chol_pd<Real, 9>(H, Real(0))) jacobi_psd_project<Real, 9>(H, Real(0));
+  }
+}
+
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_plain_update(
     TetMeshDev m, const double* __restrict__ x, Inertia in, double h2, Real* __restrict__ blocks,
@@ -469,8 +533,7 @@ __global__ __launch_bounds__(kElemBlock) void k_plain_update(
   gather_F(m, e, x, v, xc, F);
   double g[9];
   Real H[81];
-  snh_f9_grad_hess<Real>(m.mu[e], m.la[e], F, g, H);
-  if (!chol_pd<Real, 9>(H, Real(0))) jacobi_psd_project<Real, 9>(H, Real(0));
+  f9_grad_hess<Real>(m.mat[e], m.mu[e], m.la[e], F, g, H);
   const double vol = m.vol[e];
   const Real sc = (Real)(vol * h2);
   Real dxr[12];
@@ -557,7 +620,7 @@ __global__ __launch_bounds__(kElemBlock) void k_plain_energy(
 #pragma unroll
         for (int r = 0; r < 3; ++r) F[r + 3 * c] += q[3 * k + r] * d;
       }
-    en = e_in + h2 * vol * snh_f9_energy(m.mu[e], m.la[e], F);
+    en = e_in + h2 * vol * f9_energy(m.mat[e], m.mu[e], m.la[e], F);
   }
   block_partial<kElemBlock, 0>(en, out.partials, sm);
 }

@@ -284,6 +284,61 @@ void snh_f9(double mu, double la, const Vec<9>& f, double* E, Vec<9>* g,
   }
 }
 
+// Plain Neo-Hookean on the 9-vector F (reference neohookean.cpp:96-231, the
+// symbolic derivatives of derivs.m:50 on F): Psi = mu/2 (|F|^2 - 3) - mu log J
+// + la/2 (log J)^2; g = mu F + a c with a = (la log J - mu) / J and c = dJ/dF;
+// H = mu I + (la - la log J + mu) / J^2 c c^T + a d2J/dF2, PSD-projected with
+// the default tolerance (:230).
+void nh_f9(double mu, double la, const Vec<9>& f, double* E, Vec<9>* g, Mat<9, 9>* H) {
+  M3 F; for (int i = 0; i < 9; ++i) F[i] = f[i];
+  double J = det3(F);
+  M3 C;  // cofactor, dJ/dF
+  C(0, 0) = F(1, 1) * F(2, 2) - F(1, 2) * F(2, 1);
+  C(0, 1) = F(1, 2) * F(2, 0) - F(1, 0) * F(2, 2);
+  C(0, 2) = F(1, 0) * F(2, 1) - F(1, 1) * F(2, 0);
+  C(1, 0) = F(0, 2) * F(2, 1) - F(0, 1) * F(2, 2);
+  C(1, 1) = F(0, 0) * F(2, 2) - F(0, 2) * F(2, 0);
+  C(1, 2) = F(0, 1) * F(2, 0) - F(0, 0) * F(2, 1);
+  C(2, 0) = F(0, 1) * F(1, 2) - F(0, 2) * F(1, 1);
+  C(2, 1) = F(0, 2) * F(1, 0) - F(0, 0) * F(1, 2);
+  C(2, 2) = F(0, 0) * F(1, 1) - F(0, 1) * F(1, 0);
+  Vec<9> c = flatten(C);
+  const double lj = std::log(J);
+  if (E) *E = 0.5 * mu * (F.squaredNorm() - 3.0) - mu * lj + 0.5 * la * lj * lj;
+  const double a = (la * lj - mu) / J;
+  if (g) *g = mu * f + a * c;
+  if (H) {
+    Mat<9, 9> HH = mu * Mat<9, 9>::Identity() + ((la - la * lj + mu) / (J * J)) * (c * c.T());
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j)
+      for (int kk = 0; kk < 3; ++kk) for (int l = 0; l < 3; ++l) {
+        double v = 0;
+        for (int m = 0; m < 3; ++m) for (int n = 0; n < 3; ++n)
+          v += lc(i, kk, m) * lc(j, l, n) * F(m, n);
+        HH(i + 3 * j, kk + 3 * l) += a * v;
+      }
+    simple_psd_fix<9>(HH);
+    *H = HH;
+  }
+}
+
+// The 9-vector (displacement-only) API of each material model: SNH and NH
+// implement it; Fung returns 0, 0, I (fung.cpp:101-107); Corot, FCR and ARAP
+// print "unimplemented" and return uninitialised values
+// (material_model.h:43-62), so they are rejected here (false).
+bool material_f9(int type, double mu, double la, const Vec<9>& f, double* E, Vec<9>* g,
+                 Mat<9, 9>* H) {
+  switch (type) {
+    case MAT_SNH: snh_f9(mu, la, f, E, g, H); return true;
+    case MAT_NH: nh_f9(mu, la, f, E, g, H); return true;
+    case MAT_FUNG:
+      if (E) *E = 0.0;
+      if (g) *g = Vec<9>();
+      if (H) *H = Mat<9, 9>::Identity();
+      return true;
+    default: return false;
+  }
+}
+
 // Plain 2D SNH on the column-major 4-vector F (stable_neohookean.cpp:279-328;
 // no PSD projection).
 void snh_f4(double mu, double la, const Vec<4>& f, double* E, Vec<4>* g,

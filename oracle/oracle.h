@@ -21,6 +21,12 @@ void snh_f9(double mu, double la, const Vec<9>& f, double* E, Vec<9>* g,
             Mat<9, 9>* H);
 void snh_f4(double mu, double la, const Vec<4>& f, double* E, Vec<4>* g,
             Mat<4, 4>* H);
+void nh_f9(double mu, double la, const Vec<9>& f, double* E, Vec<9>* g, Mat<9, 9>* H);
+bool material_f9(int type, double mu, double la, const Vec<9>& f, double* E, Vec<9>* g,
+                 Mat<9, 9>* H);
+inline bool material_has_f9(int type) {
+  return type == MAT_SNH || type == MAT_NH || type == MAT_FUNG;
+}
 }  // namespace orc
 extern "C" {
 #endif
@@ -75,6 +81,9 @@ void oracle_snh_f9(double mu, double la, const double* F, double* E,
                    double* g, double* H);
 void oracle_snh_f4(double mu, double la, const double* F, double* E,
                    double* g, double* H);
+// 9-vector API of material `type` (0 when the model does not implement it)
+int oracle_material_f9(int type, double mu, double la, const double* F, double* E, double* g,
+                       double* H);
 void oracle_psd_fix(int n, double* A, double tol);
 void oracle_bc_fixed(int type, const double* V, int nV, int dim, int axis,
                      double ratio, int* fixed);

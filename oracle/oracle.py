@@ -59,6 +59,8 @@ def lib():
         L.oracle_material_s6.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, dp]
         L.oracle_snh_f9.argtypes = [C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_snh_f4.argtypes = [C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_material_f9.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_material_f9.restype = C.c_int
         L.oracle_psd_fix.argtypes = [C.c_int, dp, C.c_double]
         L.oracle_bc_fixed.argtypes = [C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_double, ip]
         L.oracle_tet_geometry.argtypes = [dp, ip, C.c_int, dp, dp]
@@ -153,6 +155,18 @@ def snh_f9(mu, la, F9):
     return E[0], g, H.reshape(9, 9).T.copy()
 
 
+def material_f9(mtype, mu, la, F9):
+    """9-vector (displacement-only) API of a material; None if it has none."""
+    F9 = f64(F9)
+    E = np.zeros(1)
+    g = np.zeros(9)
+    H = np.zeros(81)
+    if not lib().oracle_material_f9(int(mtype), float(mu), float(la), _dp(F9), _dp(E), _dp(g),
+                                    _dp(H)):
+        return None
+    return E[0], g, H.reshape(9, 9).T.copy()
+
+
 def snh_f4(mu, la, F4):
     F4 = f64(F4)
     E = np.zeros(1)
@@ -220,6 +234,9 @@ class Sim:
         self.p = p
         self.scene = sc
         self.h = lib().oracle_sim_create(C.byref(p))
+        if not self.h:
+            raise ValueError("the oracle rejected the scene (a material without the 9-vector "
+                             "API in a displacement-only simulation)")
 
     def __del__(self):
         try:

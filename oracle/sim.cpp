@@ -440,7 +440,7 @@ void plain_update(OracleSim* S_, const std::vector<double>& full) {
     if (D == 3) {
       Vec<9> fv; for (int i = 0; i < 9; ++i) fv[i] = f[i];
       Vec<9> gg; Mat<9, 9> HH;
-      snh_f9(S_->mu[e], S_->la[e], fv, nullptr, &gg, &HH);
+      material_f9(S_->mat[e], S_->mu[e], S_->la[e], fv, nullptr, &gg, &HH);
       for (int i = 0; i < 81; ++i) Hf[i] = HH[i];
       for (int i = 0; i < 9; ++i) gf[i] = gg[i];
     } else {
@@ -491,7 +491,7 @@ double plain_energy(const OracleSim* S_, const std::vector<double>& full) {
     double E;
     if (S_->dim == 3) {
       Vec<9> fv; for (int i = 0; i < 9; ++i) fv[i] = f[i];
-      snh_f9(S_->mu[e], S_->la[e], fv, &E, nullptr, nullptr);
+      material_f9(S_->mat[e], S_->mu[e], S_->la[e], fv, &E, nullptr, nullptr);
     } else {
       Vec<4> fv; for (int i = 0; i < 4; ++i) fv[i] = f[i];
       snh_f4(S_->mu[e], S_->la[e], fv, &E, nullptr, nullptr);
@@ -1139,6 +1139,16 @@ void oracle_snh_f9(double mu, double la, const double* F, double* E, double* g, 
   if (H) for (int i = 0; i < 81; ++i) H[i] = HH[i];
 }
 
+int oracle_material_f9(int type, double mu, double la, const double* F, double* E, double* g,
+                       double* H) {
+  Vec<9> f; for (int i = 0; i < 9; ++i) f[i] = F[i];
+  Vec<9> gg; Mat<9, 9> HH;
+  if (!material_f9(type, mu, la, f, E, g ? &gg : nullptr, H ? &HH : nullptr)) return 0;
+  if (g) for (int i = 0; i < 9; ++i) g[i] = gg[i];
+  if (H) for (int i = 0; i < 81; ++i) H[i] = HH[i];
+  return 1;
+}
+
 void oracle_snh_f4(double mu, double la, const double* F, double* E, double* g, double* H) {
   Vec<4> f; for (int i = 0; i < 4; ++i) f[i] = F[i];
   Vec<4> gg; Mat<4, 4> HH;
@@ -1297,6 +1307,12 @@ void* oracle_sim_create(const oracle_params* p) {
       for (int i = 0; i < 6; ++i) S_->s[6 * e + i] = S_->S[e][i];
     }
   } else {
+    if (D == 3)
+      for (int e = 0; e < p->nT; ++e)
+        if (!material_has_f9(S_->mat[e])) {
+          delete S_;
+          return nullptr;  // the material has no 9-vector API
+        }
     S_->Hp.assign((size_t)p->nT * M * M, 0.0);
     S_->gp.assign((size_t)p->nT * M, 0.0);
   }

@@ -275,3 +275,38 @@ def test_mixed_nh_fung_trajectory(native, oracle, material):
         o.step()
         g.step()
         assert relerr(g.positions(), o.positions()) < POS_TOL, (material, k)
+
+
+@pytest.mark.parametrize("material", ["Neohookean", "Fung"])
+def test_plain_nh_fung_trajectory(native, oracle, material):
+    """Displacement-only beams with the 9-vector API of Neo-Hookean
+    (neohookean.cpp:96-231, PSD-projected at tolerance 1e-6) and Fung (its
+    constant 0 / 0 / I, fung.cpp:101-107: inertia and gravity only, with an
+    identity stiffness)."""
+    sc = scenes.beam_scene(6, 3, 3, seed=12, mixed=False, E=1e5, material=material,
+                           itr_tol=1e-12, max_iters=20000, max_newton=20, newton_tol=1e-9)
+    o, g = _pair(sc)
+    for k in range(3):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < POS_TOL, (material, k)
+
+
+def test_plain_rejects_models_without_f9(native):
+    sc = scenes.beam_scene(3, 2, 2, mixed=False, material="ARAP")
+    with pytest.raises(native.MfemError):
+        native.Sim(sc)
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6])
+def test_bdf_orders_trajectory(native, order):
+    """BDF2..6 histories (reference BDF.cpp:7-28, alpha/beta tables :45-93)
+    against the oracle over more timesteps than the order, so that every
+    history slot is filled and rotated."""
+    sc = scenes.beam_scene(6, 2, 2, seed=13, mixed=True, E=1e5, bdf_order=order,
+                           itr_tol=1e-12, max_iters=20000, max_newton=10, newton_tol=1e-9)
+    o, g = _pair(sc)
+    for k in range(order + 3):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < POS_TOL, (order, k)

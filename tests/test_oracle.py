@@ -196,6 +196,32 @@ def test_snh_f9_and_f4_derivatives(oracle):
     assert np.abs(fH - H).max() <= 1e-7 * np.abs(H).max()
 
 
+def test_material_f9_api(oracle):
+    """9-vector API: NH by finite differences (its Hessian PSD-projected with
+    the default tolerance 1e-6), SNH through the same dispatch, Fung's
+    constant 0 / 0 / I, and no API for Corot, FCR and ARAP."""
+    rng = np.random.default_rng(5)
+    F = rand_pd_F(rng).T.reshape(-1)
+    E, g, H = oracle.material_f9(1, MU, LA, F)
+    J = np.linalg.det(F.reshape(3, 3).T)
+    assert abs(E - (0.5 * MU * (F @ F - 3) - MU * np.log(J) + 0.5 * LA * np.log(J) ** 2)) \
+        <= 1e-12 * abs(E)
+    fun = lambda f: oracle.material_f9(1, MU, LA, f)
+    fg, _ = fd_grad_hess(fun, F)
+    assert np.abs(fg - g).max() <= 1e-7 * np.abs(g).max()
+    _, fH = fd_grad_hess(lambda f: fun(f)[:2], F)
+    w, Q = np.linalg.eigh(0.5 * (fH + fH.T))
+    proj = (Q * np.maximum(w, 1e-6)) @ Q.T
+    assert np.abs(proj - H).max() <= 1e-6 * np.abs(H).max()
+    Es, gs, Hs = oracle.material_f9(0, MU, LA, F)
+    Eo, go, Ho = oracle.snh_f9(MU, LA, F)
+    assert Es == Eo and np.array_equal(gs, go) and np.array_equal(Hs, Ho)
+    Ef, gf, Hf = oracle.material_f9(5, MU, LA, F)
+    assert Ef == 0.0 and not gf.any() and np.array_equal(Hf, np.eye(9))
+    for t in (2, 3, 4):
+        assert oracle.material_f9(t, MU, LA, F) is None
+
+
 def test_psd_fix_properties(oracle):
     rng = np.random.default_rng(11)
     A = rng.normal(size=(6, 6))
