@@ -39,6 +39,18 @@ def cg_bytes(nfree, nnzb, vb):
     return nnzb * (9 * vb + 4) + 4 * (nfree + 1) + n * (2 * 4 * vb + 16)
 
 
+def minres_tet_bytes(nT, nV, nfree, vb):
+    """Algorithmic bytes of one k_mr_tet launch (one MINRES iteration's element
+    kernel): per tet the update of the previous iteration on its 12 element
+    entries (w, p_{k-2}, p_{k-3}, x read; p_{k-1}, x written), the product
+    (w read, v_old read and written, the 12-entry x-row contribution written,
+    ds/dF (54 values), h^2 Hessian (21), shape gradients (12 FP64), volume,
+    corner ids), plus the x part of w (3 per free vertex) and the free map
+    gathered once."""
+    per_tet = 6 * 96 + 96 + 2 * 96 + 96 + 75 * vb + 96 + 8 + 16
+    return nT * per_tet + nfree * 24 + nV * 4
+
+
 def measured_traffic(lanes, unroll, workgroup, default_workload):
     """HBM bytes per CG-kernel launch measured with rocprofv3 FETCH_SIZE and
     WRITE_SIZE (copy-calibrated) on the default workload, for the shape the
@@ -80,15 +92,89 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
     elapsed = time.perf_counter() - t0
     nfree, nnzb, vb = sim.matrix_info()
     x = sim.positions()
-    trials, chosen, shape = sim.cg_tuning()
-    spmv_alone_ms = sim.time_spmv(50)
+    tuning = None
+    if scene.solver == "eigen-pcg-diag":
+        trials, chosen, shape = sim.cg_tuning()
+        spmv_alone_ms = sim.time_spmv(50)
+        tuning = {"trials_variant_grid_ms": [[v, g, round(t, 5)] for v, g, t in trials],
+                  "chosen": chosen, "lanes_per_row": shape[0], "blocks_in_flight": shape[1],
+                  "workgroup": shape[2], "grid": shape[3], "plain_spmv_ms": spmv_alone_ms}
     sim.close()
-    tuning = {"trials_variant_grid_ms": [[v, g, round(t, 5)] for v, g, t in trials],
-              "chosen": chosen, "lanes_per_row": shape[0], "blocks_in_flight": shape[1],
-              "workgroup": shape[2], "grid": shape[3], "plain_spmv_ms": spmv_alone_ms}
     return dict(elapsed=elapsed, newton=newton, pcg=pcg, spmv_ms=spmv_ms, spmv_n=spmv_n,
                 nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st,
                 tuning=tuning)
+
+
+def run_cpu_minres(scene, threads, caps=(4, 12)):
+    """FP64 oracle, one Newton iteration of the same scene with MINRES capped
+    at each of `caps` iterations: the two timings give the fixed cost (element
+    update, line search) and the cost per MINRES iteration."""
+    from oracle import oracle as O
+    import copy
+    out = []
+    for cap in caps:
+        sc = copy.copy(scene)
+        sc.max_newton = 1
+        sc.max_iters = cap
+        sim = O.Sim(sc, threads=threads, solver=O.SOLVERS["minres-indefinite-block"])
+        print(f"[bench] CPU oracle MINRES sample (cap {cap}) on {threads} threads ...",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        sim.step()
+        out.append(time.perf_counter() - t0)
+    per_iter = (out[1] - out[0]) / (caps[1] - caps[0])
+    fixed = out[0] - caps[0] * per_iter
+    return fixed, per_iter, out
+
+
+def run_minres(args, scene):
+    """The KKT / MINRES solver ("minres-indefinite-block") on the same workload."""
+    import copy
+    scene = copy.copy(scene)
+    scene.solver = "minres-indefinite-block"
+    nT = scene.nT
+    r = run_gpu(scene, "fp64", args.steps, args.warmup)
+    value = r["newton"] * nT / r["elapsed"]
+    ms_per_step = 1000.0 * r["elapsed"] / args.steps
+    bytes_tet = minres_tet_bytes(nT, scene.nV, r["nfree"], r["vb"])
+    avg_ms = r["spmv_ms"] / max(r["spmv_n"], 1)
+    achieved = bytes_tet / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    its_per_newton = r["pcg"] / max(r["newton"], 1)
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        fixed, per_iter, raw = run_cpu_minres(scene, threads)
+        t_newton = fixed + its_per_newton * per_iter
+        cpu = {"value": nT / t_newton, "unit": "Newton-steps*tets/s", "cores": threads,
+               "kind": "port",
+               "sample": f"FP64 oracle, 1 Newton iteration of the same {nT}-tet scene with MINRES "
+                         f"capped at 4 and at 12 iterations ({raw[0]:.2f} s, {raw[1]:.2f} s): "
+                         f"{fixed:.2f} s fixed + {per_iter:.3f} s per MINRES iteration, taken at "
+                         f"the GPU run's {its_per_newton:.0f} iterations per Newton iteration"}
+    out = {
+        "metric": "Newton-steps*tets/s, KKT system with MINRES (minres-indefinite-block)",
+        "value": value, "unit": "Newton-steps*tets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload_name(args.config, args.nx, args.ny, args.nz, nT, scene)
+                   .replace("Jacobi CG", "MINRES (block-Schur preconditioner)"),
+                   "tets": nT, "kkt_size": 3 * r["nfree"] + 12 * nT, "precision": "fp64"},
+        "newton_iters": r["newton"], "minres_iters": r["pcg"],
+        "minres_iter_ms": (r["last"]["solve_ms"] / max(r["last"]["pcg_iters"], 1)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_mr_tet (MINRES element kernel: update of the previous "
+                               "iteration, KKT product rows of each tet)",
+                     "bytes_per_launch": bytes_tet, "avg_launch_ms": avg_ms,
+                     "launches": r["spmv_n"],
+                     "timing": "HIP events around the kernel on its stream, first 32 iterations "
+                               "of every solve"},
+        "cpu_baseline": cpu,
+        "phase_ms_last_step": {k: r["last"][k] for k in ("update_ms", "solve_ms", "ls_ms")},
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
 
 
 def run_cpu_sample(scene, threads):
@@ -215,6 +301,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--solver", choices=("eigen-pcg-diag", "minres-indefinite-block"),
+                    default="eigen-pcg-diag",
+                    help="linear solver of the Newton step: Schur complement + CG (default, "
+                         "the metric's) or the KKT system with MINRES (FP64)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -231,6 +321,8 @@ def main():
     make = {"c3": scenes.config_c3, "c2": scenes.config_c2, "c5": scenes.config_c5}[args.config]
     scene = make(*dims)
     nT = scene.nT
+    if args.solver == "minres-indefinite-block":
+        return run_minres(args, scene)
 
     mixed = run_gpu(scene, "mixed", args.steps, args.warmup)
     fp64 = None if args.no_fp64 else run_gpu(scene, "fp64", args.steps, args.warmup)

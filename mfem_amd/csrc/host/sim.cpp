@@ -680,6 +680,8 @@ template <int DIM>
 MinresSolver<DIM>::~MinresSolver() {
   for (auto& e : poll_ev_)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : prof_ev_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 template <int DIM>
@@ -750,9 +752,18 @@ void MinresSolver<DIM>::run() {
   auto* snap = pinned_->as<MinresCtrl>();
   long launched = 0;
   int issued = 0, waited = 0;
+  const bool prof = this->profile_spmv;
+  if (prof && prof_ev_.empty()) {
+    prof_ev_.resize(2 * kProfiledIters);
+    for (auto& e : prof_ev_) MFEM_HIP_CHECK(hipEventCreate(&e));
+  }
   auto issue = [&] {
     const long chunk = std::min<long>(poll, limit - launched);
-    for (long j = 0; j < chunk; ++j) launch_minres_iteration(S, B, launched + j, st);
+    for (long j = 0; j < chunk; ++j) {
+      const long k = launched + j;
+      hipEvent_t* ev = prof && k < kProfiledIters ? &prof_ev_[2 * k] : nullptr;
+      launch_minres_iteration(S, B, k, st, ev);
+    }
     launched += chunk;
     MFEM_HIP_CHECK(hipGetLastError());
     const int slot = issued & 1;
@@ -778,6 +789,18 @@ void MinresSolver<DIM>::run() {
   launch_minres_iteration(S, B, launched, st);
   if (last_.fail)
     throw std::runtime_error("minres-indefinite-block: preconditioner is not positive definite");
+  if (prof) {
+    // element-kernel time of the iterations that ran the product (the
+    // launches queued after the stop exit at once and are not counted)
+    const long n = std::min<long>({(long)kProfiledIters, launched, (long)last_.iter + 1});
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));
+    for (long k = 0; k < n; ++k) {
+      float ms = 0;
+      MFEM_HIP_CHECK(hipEventElapsedTime(&ms, prof_ev_[2 * k], prof_ev_[2 * k + 1]));
+      this->stats_.spmv_ms += ms;
+      this->stats_.spmv_count += 1;
+    }
+  }
 }
 
 template <int DIM>

@@ -353,6 +353,10 @@ class MinresSolver : public LinearSolver<DIM> {
   mfemhip::MinresCtrl last_{};
   std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots
   hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  // profiling: event pairs around the element kernel of the first
+  // kProfiledIters iterations of a solve
+  static constexpr int kProfiledIters = 32;
+  std::vector<hipEvent_t> prof_ev_;
 };
 
 template <int DIM>

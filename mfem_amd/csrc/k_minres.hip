@@ -42,7 +42,22 @@ constexpr int kFinBS = 1024;
 
 __device__ __forceinline__ double symw(int i) { return i < 3 ? 1.0 : 2.0; }
 
-inline int blocks_for(int n, int bs) { return n > 0 ? (n + bs - 1) / bs : 0; }
+// XCD-contiguous work split: workgroup b runs on XCD b % 8 (round-robin
+// dispatch), and XCD x owns items [n x / 8, n (x + 1) / 8), so that the
+// vertices and tets one XCD touches, and the lines their gathers read, stay
+// in that XCD's L2. Returns n for lanes past the end of their range.
+__device__ __forceinline__ int xcd_index(int n) {
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3;
+  const long long start = (long long)n * xcd / 8, end = (long long)n * (xcd + 1) / 8;
+  const long long i = start + (long long)lb * blockDim.x + threadIdx.x;
+  return i < end ? (int)i : n;
+}
+
+inline int blocks_for(int n, int bs) {
+  if (n <= 0) return 0;
+  const int per_xcd = (n + 7) / 8;
+  return 8 * ((per_xcd + bs - 1) / bs);
+}
 
 // Sum of parts[0 .. n) in index order (strided by lane, then the block tree).
 __device__ __forceinline__ double fin_sum(const double* parts, int n, double* sm) {
@@ -59,7 +74,7 @@ __global__ __launch_bounds__(kTetBS) void k_kkt_setup(TetMeshDev m, const Real* 
                                                       const Real* __restrict__ Hinv,
                                                       const double* __restrict__ mlump_inv,
                                                       double* __restrict__ Sinv) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index(m.nT);
   if (e >= m.nT) return;
   const int nT = m.nT;
   const double vol = m.vol[e];
@@ -178,7 +193,7 @@ __global__ __launch_bounds__(kTetBS) void k_kkt_rhs_tet(TetMeshDev m, const doub
                                                         double* __restrict__ fe, KktVec b,
                                                         double* __restrict__ parts) {
   __shared__ double sm[kTetBS / kWave];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index(m.nT);
   double acc = 0.0;
   if (e < m.nT) {
     const int nT = m.nT;
@@ -215,7 +230,7 @@ __global__ __launch_bounds__(kVtxBS) void k_kkt_rhs_vtx(int nfree, const int* __
                                                         const double* __restrict__ fe, KktVec b,
                                                         double* __restrict__ parts) {
   __shared__ double sm[kVtxBS / kWave];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_index(nfree);
   double acc = 0.0;
   if (i < nfree) {
     double a[3] = {0, 0, 0};
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtr
                                                    double* __restrict__ parts) {
   __shared__ double sm[kTetBS / kWave];
   const MinresCtrl c = *ctrl;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index(m.nT);
   const int nT = m.nT;
   if (c.pending && e < nT) {
 #pragma unroll
@@ -360,7 +375,7 @@ __global__ __launch_bounds__(kVtxBS) void k_mr_vertex(int nfree, const int* __re
                                                       double* __restrict__ parts) {
   __shared__ double sm[kVtxBS / kWave];
   const MinresCtrl c = *ctrl;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_index(nfree);
   if (c.pending && i < nfree) {
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -421,7 +436,7 @@ __global__ __launch_bounds__(kTetBS) void k_mr_prec_tet(TetMeshDev m, const Minr
   __shared__ double sm[kTetBS / kWave];
   const MinresCtrl c = *ctrl;
   if (!init && c.done) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_index(m.nT);
   const int nT = m.nT;
   double acc = 0.0;
   if (e < nT) {
@@ -467,7 +482,7 @@ __global__ __launch_bounds__(kVtxBS) void k_mr_prec_vtx0(int nfree, const Minres
                                                          double* __restrict__ cd) {
   const MinresCtrl c = *ctrl;
   if (!init && c.done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_index(nfree);
   if (i >= nfree) return;
   const double ab = init ? 0.0 : c.alpha / c.beta;
   const double di = dinv[i];
@@ -500,7 +515,7 @@ __global__ __launch_bounds__(kVtxBS) void k_mr_cheb(int nfree, const int* __rest
                                                     double* __restrict__ parts) {
   __shared__ double sm[kVtxBS / kWave];
   if (!init && ctrl->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_index(nfree);
   double acc = 0.0;
   if (i < nfree) {
     double y[3] = {0, 0, 0};
@@ -667,7 +682,7 @@ void launch_minres_init(const KktSystem<Real>& S, const double* x_full, const In
 
 template <typename Real>
 void launch_minres_iteration(const KktSystem<Real>& S, const MinresBuffers& B, long k,
-                             hipStream_t st) {
+                             hipStream_t st, hipEvent_t* tet_ev) {
   const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
   auto mod3 = [](long j) { return (int)(((j % 3) + 3) % 3); };
   MrVecs V;
@@ -678,9 +693,11 @@ void launch_minres_iteration(const KktSystem<Real>& S, const MinresBuffers& B, l
   V.pb = B.p[mod3(k - 3)];
   V.pn = B.p[mod3(k - 1)];  // p_{k-1} replaces p_{k-4}
   V.x = B.x;
+  if (tet_ev) (void)hipEventRecord(tet_ev[0], st);
   if (nbT > 0)
     hipLaunchKernelGGL(k_mr_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
                        S.Hpsi, V, S.fe, S.parts[0]);
+  if (tet_ev) (void)hipEventRecord(tet_ev[1], st);
   if (nbV > 0)
     hipLaunchKernelGGL(k_mr_vertex, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.rowptr, S.col,
                        S.mass, S.inc_ptr, S.inc, S.fe, S.ctrl, V, S.parts[0] + nbT);
@@ -723,7 +740,7 @@ void launch_kkt_apply(const KktSystem<Real>& S, int which, KktVec in, KktVec out
                                          const double*, const double*, const double*, KktVec,    \
                                          KktVec, double, int, hipStream_t);                      \
   template void launch_minres_iteration<Real>(const KktSystem<Real>&, const MinresBuffers&, long, \
-                                              hipStream_t);                                      \
+                                              hipStream_t, hipEvent_t*);                         \
   template void launch_kkt_apply<Real>(const KktSystem<Real>&, int, KktVec, KktVec, KktVec,      \
                                        hipStream_t);
 MFEM_MINRES_INST(float)

@@ -246,9 +246,11 @@ void launch_minres_init(const KktSystem<Real>& S, const double* x_full, const In
                         double tol, int max_iters, hipStream_t st);
 // MINRES iteration k (k = 0, 1, ...): applies the solution update of
 // iteration k-1 first, so one more launch after the stop completes x.
+// tet_ev: optional pair of events recorded around the element kernel
+// (k_mr_tet), for profiling.
 template <typename Real>
 void launch_minres_iteration(const KktSystem<Real>& S, const MinresBuffers& B, long k,
-                             hipStream_t st);
+                             hipStream_t st, hipEvent_t* tet_ev = nullptr);
 // Diagnostics: out = K in (which 0) or out = P in (which 1) at the current
 // state; scratch is a KKT vector the product may write.
 template <typename Real>

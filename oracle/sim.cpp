@@ -669,6 +669,7 @@ void kkt_product(const OracleSim* S_, const double* w, double* y) {
     csr_mult(S_, S_->PMP, xv, t);
     for (int i = 0; i < n; ++i) yx[i] += t[i];
     std::vector<double> tmp(9 * (size_t)nT);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < nT; ++e) {
       Vec<6> l; for (int i = 0; i < 6; ++i) l[i] = wl[6 * e + i];
       Vec<9> v = S_->dSdF[e] * l;
@@ -679,6 +680,7 @@ void kkt_product(const OracleSim* S_, const double* w, double* y) {
     for (int i = 0; i < n; ++i) yx[i] += acc[i];
   }
   // s rows: Hloc ds + vol Sym la
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < nT; ++e) {
     const double vol = S_->vols[e];
     for (int i = 0; i < 6; ++i) {
@@ -696,6 +698,7 @@ void kkt_product(const OracleSim* S_, const double* w, double* y) {
       if (f < 0) continue;
       for (int r = 0; r < 3; ++r) full[3 * v + r] = wx[3 * f + r];
     }
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < nT; ++e) {
       double Fd[9]; def_grad(S_, full, e, Fd);
       Vec<9> t; for (int i = 0; i < 9; ++i) t[i] = S_->vols[e] * Fd[i];
@@ -783,6 +786,7 @@ void gx_block(const OracleSim* S_, int e, double G[12][6]) {
 void kkt_refresh_L(OracleSim* S_, bool exact) {
   const int nT = S_->nT;
   S_->linv_blk.assign(nT, Mat<6, 6>());
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < nT; ++e) {
     double G[12][6];
     gx_block(S_, e, G);
@@ -837,6 +841,7 @@ void kkt_precond(const OracleSim* S_, bool exact, const double* b, double* z) {
   const int n = S_->n, nT = S_->nT;
   if (exact) dense_chol_solve(S_->mchol, n, b, z);
   else mass_cheb(S_, b, z);
+#pragma omp parallel for schedule(static)
   for (int e = 0; e < nT; ++e) {
     const double vol = S_->vols[e];
     for (int i = 0; i < 6; ++i) {
@@ -848,6 +853,7 @@ void kkt_precond(const OracleSim* S_, bool exact, const double* b, double* z) {
   if (exact) {
     dense_chol_solve(S_->lchol, 6 * nT, b + n + 6 * nT, z + n + 6 * nT);
   } else {
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < nT; ++e)
       for (int i = 0; i < 6; ++i) {
         double a = 0;
@@ -886,13 +892,16 @@ double minres(int N, Op op, Pre pre, const double* b, double* x, int max_iters, 
   while (it < max_iters) {
     const double beta = beta_new;
     v_old = v;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) { v_new[i] /= beta_new; w_new[i] /= beta_new; }
     v = v_new;
     w = w_new;
     op(w.data(), Aw.data());
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) v_new[i] = Aw[i] - beta * v_old[i];
     double alpha = 0;
     for (int i = 0; i < N; ++i) alpha += v_new[i] * w[i];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) v_new[i] -= alpha * v[i];
     pre(v_new.data(), w_new.data());
     beta_new2 = 0;
@@ -910,8 +919,11 @@ double minres(int N, Op op, Pre pre, const double* b, double* x, int max_iters, 
     s = beta_new / r1;
     p_oold = p_old;
     p_old = p;
-    for (int i = 0; i < N; ++i) p[i] = (w[i] - r2 * p_old[i] - r3 * p_oold[i]) / r1;
-    for (int i = 0; i < N; ++i) x[i] += beta_one * c * eta * p[i];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      p[i] = (w[i] - r2 * p_old[i] - r3 * p_oold[i]) / r1;
+      x[i] += beta_one * c * eta * p[i];
+    }
     res2 *= s * s;
     if (res2 < threshold2) break;
     eta = -s * eta;
