@@ -24,13 +24,13 @@ typedef struct mfem_sim mfem_sim;
  * per-element materials as with "material_ids", reference tet_mesh.h:11-15). */
 typedef struct {
   int nV, nT;
-  const double* V;        /* nV x 3, row per vertex */
-  const int* T;           /* nT x 4 */
+  const double* V;        /* nV x dim, row per vertex */
+  const int* T;           /* nT x (dim + 1) */
   const double* mu;       /* nT Lame mu (reference config.h:13-16) */
   const double* la;       /* nT Lame lambda */
   const int* mat;         /* nT material id: 0 SNH, 1 NH, 2 Corot, 3 FCR, 4 ARAP, 5 Fung */
   const int* fixed;       /* nV pin mask (1 = pinned), or NULL for bc_type/axis/ratio */
-  const double* v0;       /* nV x 3 initial velocity, or NULL */
+  const double* v0;       /* nV x dim initial velocity, or NULL */
   double density;
   double dt;              /* "dt" */
   double gravity[3];      /* "body_force" (stored as float, as the reference does) */
@@ -46,6 +46,9 @@ typedef struct {
   int precision;          /* 0: FP64, 1: mixed FP32/FP64 */
   int bc_type, bc_axis;   /* boundary condition when fixed == NULL (reference config.h:70-87) */
   double bc_ratio;
+  int dim;                /* 3 (tetrahedra; 0 also means 3) or 2 (triangles, reference
+                           * Tri2DMesh: V and v0 are nV x 2, T is nT x 3; displacement-only
+                           * Stable-Neohookean) */
 } mfem_sim_params;
 
 typedef struct {
@@ -83,7 +86,7 @@ int mfem_sim_update_system(mfem_sim* sim);
 int mfem_sim_solve(mfem_sim* sim, double* decrement);
 int mfem_sim_energy(mfem_sim* sim, double alpha, double* energy);
 int mfem_sim_stats(mfem_sim* sim, mfem_step_stats* out);
-/* Current vertex positions (nV x 3), i.e. Mesh::V_ after post_solve. */
+/* Current vertex positions (nV x dim), i.e. Mesh::V_ after post_solve. */
 int mfem_sim_get_positions(mfem_sim* sim, double* out);
 /* Internal arrays by name, element-major, for parity checks: "R" (9/tet),
  * "S", "s", "lambda", "ds", "g", "gl" (6/tet), "dsdF" (54/tet, ds/dvecF

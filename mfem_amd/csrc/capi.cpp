@@ -161,15 +161,25 @@ SceneArrays scene_of(const mfem_sim_params& p) {
   if (p.nV <= 0 || p.nT <= 0) throw std::runtime_error("empty mesh");
   if (!p.V || !p.T || !p.mu || !p.la || !p.mat) throw std::runtime_error("null mesh array");
   SceneArrays sc;
-  sc.V.assign(p.V, p.V + 3L * p.nV);
-  sc.T.assign(p.T, p.T + 4L * p.nT);
+  sc.dim = p.dim == 0 ? 3 : p.dim;
+  if (sc.dim != 2 && sc.dim != 3) throw std::runtime_error("dim must be 2 or 3");
+  const int D = sc.dim;
+  // vertices stored as nV x 3 (z = 0 in 2D)
+  sc.V.assign(3L * p.nV, 0.0);
+  for (long i = 0; i < p.nV; ++i)
+    for (int d = 0; d < D; ++d) sc.V[3 * i + d] = p.V[D * i + d];
+  sc.T.assign(p.T, p.T + (long)(D + 1) * p.nT);
   for (int v : sc.T)
-    if (v < 0 || v >= p.nV) throw std::runtime_error("tet vertex index out of range");
+    if (v < 0 || v >= p.nV) throw std::runtime_error("element vertex index out of range");
   sc.mu.assign(p.mu, p.mu + p.nT);
   sc.la.assign(p.la, p.la + p.nT);
   sc.mat.assign(p.mat, p.mat + p.nT);
   if (p.fixed) sc.fixed.assign(p.fixed, p.fixed + p.nV);
-  if (p.v0) sc.v0.assign(p.v0, p.v0 + 3L * p.nV);
+  if (p.v0) {
+    sc.v0.assign(3L * p.nV, 0.0);
+    for (long i = 0; i < p.nV; ++i)
+      for (int d = 0; d < D; ++d) sc.v0[3 * i + d] = p.v0[D * i + d];
+  }
   sc.density = p.density;
   sc.bc.type = (BCScriptType)p.bc_type;
   sc.bc.axis = p.bc_axis;
@@ -282,7 +292,10 @@ int mfem_sim_stats(mfem_sim* sim, mfem_step_stats* o) {
 int mfem_sim_get_positions(mfem_sim* sim, double* out) {
   return guarded([&] {
     auto v = sim->state().x_->value().to_host(sim->state().ctx_->stream);
-    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    const int D = sim->state().mesh_->dim();
+    const size_t nV = v.size() / 3;
+    for (size_t i = 0; i < nV; ++i)
+      for (int d = 0; d < D; ++d) out[D * i + d] = v[3 * i + d];
   });
 }
 
@@ -482,6 +495,7 @@ int mfem_batch_create(const mfem_sim_params* sims, int count, mfem_batch** out) 
         throw std::runtime_error("batch: time step, integrator, gravity, solver settings and "
                                  "precision must be the same for every simulation");
       if (!p.mixed) throw std::runtime_error("batch: only mixed-stretch simulations are batched");
+      if (p.dim != 0 && p.dim != 3) throw std::runtime_error("batch: tetrahedral meshes only");
       scenes.push_back(scene_of(p));
     }
     auto cfg = config_of(p0);

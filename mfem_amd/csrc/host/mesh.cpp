@@ -35,11 +35,14 @@ Mesh::Mesh(const std::vector<double>& V, const std::vector<int>& T,
 
 Mesh::Mesh(const std::vector<double>& V, const std::vector<int>& T,
            std::shared_ptr<MaterialModel> material, const std::vector<double>& mu,
-           const std::vector<double>& la, const std::vector<int>& mat)
+           const std::vector<double>& la, const std::vector<int>& mat, int dim)
     : V_(V), Vref_(V), Vinit_(V), T_(T), material_(material), elem_mu_(mu), elem_la_(la),
       elem_mat_(mat) {
+  if (dim != 2 && dim != 3) throw std::runtime_error("Mesh: dim must be 2 or 3");
+  dim_ = dim;
+  nodes_ = dim + 1;
   nV_ = (int)(V.size() / 3);
-  nT_ = (int)(T.size() / 4);
+  nT_ = (int)(T.size() / nodes_);
   if ((int)mu.size() != nT_ || (int)la.size() != nT_ || (int)mat.size() != nT_)
     throw std::runtime_error("Mesh: per-element parameter arrays must have nT entries");
   initial_velocity_.assign(V.size(), 0.0);
@@ -96,10 +99,28 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
   nfree_ = 0;
   for (int i = 0; i < nV_; ++i)
     if (!is_fixed_[i]) free_map_[i] = nfree_++;
-  // volumes |det Dm| / 6 and dphi/dX = rows of Dm^{-1} (phi_0 = -sum)
   vols_.assign(nT_, 0.0);
   dX_.assign((size_t)nT_ * 12, 0.0);
-  for (int e = 0; e < nT_; ++e) {
+  if (dim_ == 2) {
+    // triangles (reference tri2d_mesh.cpp:15-41): vols = igl::doublearea,
+    // signed (twice the area), dphi/dX = rows of Dm^{-1} (phi_0 = -sum), in
+    // rows 3k + c of the 12-value element layout
+    for (int e = 0; e < nT_; ++e) {
+      const double* X0 = &Vref_[3 * T_[3 * e]];
+      const double* X1 = &Vref_[3 * T_[3 * e + 1]];
+      const double* X2 = &Vref_[3 * T_[3 * e + 2]];
+      const double a = X1[0] - X0[0], b = X2[0] - X0[0], c = X1[1] - X0[1], d = X2[1] - X0[1];
+      const double det = a * d - b * c;
+      if (det == 0.0) throw std::runtime_error("Mesh: degenerate triangle " + std::to_string(e));
+      vols_[e] = det;
+      const double g1x = d / det, g1y = -b / det, g2x = -c / det, g2y = a / det;
+      double* o = &dX_[(size_t)e * 12];
+      o[3] = g1x; o[4] = g1y; o[6] = g2x; o[7] = g2y;
+      o[0] = -(g1x + g2x); o[1] = -(g1y + g2y);
+    }
+  }
+  // tets: volumes |det Dm| / 6 and dphi/dX = rows of Dm^{-1} (phi_0 = -sum)
+  for (int e = 0; e < (dim_ == 3 ? nT_ : 0); ++e) {
     const double* X0 = &Vref_[3 * T_[4 * e]];
     double D[9];
     for (int k = 1; k < 4; ++k) {
@@ -135,7 +156,9 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
   build_maps();
   // upload
   std::vector<int4> T4(nT_);
-  for (int e = 0; e < nT_; ++e) T4[e] = make_int4(T_[4 * e], T_[4 * e + 1], T_[4 * e + 2], T_[4 * e + 3]);
+  const int N = nodes_;
+  for (int e = 0; e < nT_; ++e)
+    T4[e] = make_int4(T_[N * e], T_[N * e + 1], T_[N * e + 2], N == 4 ? T_[N * e + 3] : T_[N * e]);
   d_T.upload(T4, st);
   std::vector<double> dXs((size_t)nT_ * 12);
   for (int e = 0; e < nT_; ++e)
@@ -182,9 +205,10 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
 void Mesh::build_maps() {
   // vertex -> (tet, corner) incidences of free vertices, in tet order
   inc_ptr_.assign(nfree_ + 1, 0);
+  const int N = nodes_;
   for (int e = 0; e < nT_; ++e)
-    for (int a = 0; a < 4; ++a) {
-      int f = free_map_[T_[4 * e + a]];
+    for (int a = 0; a < N; ++a) {
+      int f = free_map_[T_[N * e + a]];
       if (f >= 0) inc_ptr_[f + 1]++;
     }
   for (int i = 0; i < nfree_; ++i) inc_ptr_[i + 1] += inc_ptr_[i];
@@ -192,13 +216,15 @@ void Mesh::build_maps() {
   {
     std::vector<int> fill(inc_ptr_.begin(), inc_ptr_.end() - 1);
     for (int e = 0; e < nT_; ++e)
-      for (int a = 0; a < 4; ++a) {
-        int f = free_map_[T_[4 * e + a]];
+      for (int a = 0; a < N; ++a) {
+        int f = free_map_[T_[N * e + a]];
         if (f >= 0) inc_[fill[f]++] = (e << 2) | a;
       }
   }
   if (nT_ >= (1 << 27)) throw std::runtime_error("Mesh: too many tets for 32-bit slot ids");
-  const double denom = 20.0;
+  // consistent mass rho vol / 20 (1 + delta) for tets; rho vol / 12
+  // (1 + delta) for triangles with vol the signed doublearea
+  const double denom = dim_ == 3 ? 20.0 : 12.0;
   const double rho = material_->config()->density;
   rowptr_.assign(nfree_ + 1, 0);
   col_.clear();
@@ -212,8 +238,8 @@ void Mesh::build_maps() {
     ents.clear();
     for (int p = inc_ptr_[i]; p < inc_ptr_[i + 1]; ++p) {
       int e = inc_[p] >> 2;
-      for (int b = 0; b < 4; ++b) {
-        int fb = free_map_[T_[4 * e + b]];
+      for (int b = 0; b < N; ++b) {
+        int fb = free_map_[T_[N * e + b]];
         if (fb >= 0) ents.push_back({fb, e, b});
       }
     }
@@ -231,7 +257,7 @@ void Mesh::build_maps() {
         int e = ents[k].e;
         // local index of row vertex i in tet e
         int a = 0;
-        while (free_map_[T_[4 * e + a]] != i) ++a;
+        while (free_map_[T_[N * e + a]] != i) ++a;
         slots_.push_back((e << 4) | (a << 2) | ents[k].b);
         m += rho * vols_[e] / denom * (a == ents[k].b ? 2.0 : 1.0);
         ++k;
@@ -252,6 +278,7 @@ mfemhip::TetMeshDev Mesh::dev() const {
   m.nV = nV_;
   m.nT = nT_;
   m.nfree = nfree_;
+  m.dim = dim_;
   m.T = d_T.data();
   m.dX = d_dX.data();
   m.vol = d_vol.data();

@@ -33,8 +33,11 @@ class Mesh {
   // Per-element Lame parameters and model ids given directly (one material
   // per tet, as the reference's material_ids constructor with one model per
   // id); `material` supplies the density (reference tet_mesh.cpp:36-41).
+  // dim 2: triangles (T holds 3 corners per element) in the z = 0 plane, V
+  // still given as nV x 3 with z = 0 (reference Tri2DMesh, tri2d_mesh.cpp).
   Mesh(const std::vector<double>& V, const std::vector<int>& T, std::shared_ptr<MaterialModel> material,
-       const std::vector<double>& mu, const std::vector<double>& la, const std::vector<int>& mat);
+       const std::vector<double>& mu, const std::vector<double>& la, const std::vector<int>& mat,
+       int dim = 3);
 
   // BCs, volumes, shape gradients, assembly maps, upload (reference
   // Mesh::init, mesh.cpp:44-79). `fixed_override`, when non-empty, replaces
@@ -42,6 +45,8 @@ class Mesh {
   void init(hipStream_t st, const std::vector<int>& fixed_override = {});
 
   mfemhip::TetMeshDev dev() const;
+  int dim() const { return dim_; }
+  int nodes() const { return nodes_; }  // corners per element
   int nV() const { return nV_; }
   int nT() const { return nT_; }
   int nfree() const { return nfree_; }
@@ -73,6 +78,7 @@ class Mesh {
 
  private:
   void build_maps();
+  int dim_ = 3, nodes_ = 4;
   int nV_ = 0, nT_ = 0, nfree_ = 0, nnzb_ = 0;
   std::vector<int> slot_ptr_, slots_, diag_block_, inc_ptr_, inc_;
   std::vector<double> mass_;

@@ -186,6 +186,16 @@ template <int DIM>
 Stretch<DIM>::Stretch(std::shared_ptr<Mesh> mesh, std::shared_ptr<SimConfig> config,
                       DeviceContext* ctx)
     : Variable<DIM>(std::move(mesh), ctx) {
+  if (this->mesh_->dim() == 2) {
+    // the 4-vector (2D) F API exists only for SNH (stable_neohookean.cpp:279-328)
+    for (int t : this->mesh_->element_types())
+      if (t != MATERIAL_SNH)
+        throw std::runtime_error("material '" +
+                                 MaterialModelFactory().name_by_type((MaterialModelType)t) +
+                                 "' has no 2D displacement-only (4-vector) energy; 2D \"stretch\" "
+                                 "needs Stable-Neohookean");
+    return;
+  }
   for (int t : this->mesh_->element_types())
     if (t != MATERIAL_SNH && t != MATERIAL_NH && t != MATERIAL_FUNG)
       throw std::runtime_error("material '" +
@@ -199,6 +209,15 @@ void Stretch<DIM>::update(const double* x_full, double dt) {
   h2_ = dt * dt;
   auto* c = this->ctx_;
   auto m = this->mesh_->dev();
+  if (m.dim == 2) {
+    if (c->fp32())
+      launch_plain2d_update<float>(m, x_full, c->inertia(), h2_, c->blocks.f32(), c->fe.data(),
+                                   c->stream);
+    else
+      launch_plain2d_update<double>(m, x_full, c->inertia(), h2_, c->blocks.f64(),
+                                    c->fe.data(), c->stream);
+    return;
+  }
   if (c->fp32())
     launch_plain_update<float>(m, x_full, c->inertia(), h2_, c->blocks.f32(), c->fe.data(),
                                c->stream);
@@ -211,7 +230,11 @@ template <int DIM>
 bool Stretch<DIM>::energy_launch(const double* x_full, const double* dx, double alpha,
                                  ReduceSlot out) {
   auto* c = this->ctx_;
-  launch_plain_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), h2_, out, c->stream);
+  const auto m = this->mesh_->dev();
+  if (m.dim == 2)
+    launch_plain2d_energy(m, x_full, dx, alpha, c->inertia(), h2_, out, c->stream);
+  else
+    launch_plain_energy(m, x_full, dx, alpha, c->inertia(), h2_, out, c->stream);
   return true;
 }
 
@@ -219,6 +242,8 @@ bool Stretch<DIM>::energy_launch(const double* x_full, const double* dx, double 
 // reference mixed_stretch.cpp:262-304
 template <int DIM>
 void MixedStretch<DIM>::reset() {
+  if (this->mesh_->dim() != 3)
+    throw std::runtime_error("mixed-stretch runs on tetrahedral (3D) meshes only on the device");
   auto* c = this->ctx_;
   const int nT = this->mesh_->nT();
   hipStream_t st = c->stream;
@@ -927,12 +952,21 @@ void NewtonOptimizer<DIM>::update_system() {
   if (S.vars_.empty() && S.mixed_vars_.empty()) {
     // no elastic variable: inertia-only residual and zero stiffness blocks
     auto* c = S.ctx_.get();
-    if (c->fp32())
-      launch_plain_update<float>(S.mesh_->dev(), x, c->inertia(), 0.0, c->blocks.f32(),
-                                 c->fe.data(), c->stream);
-    else
-      launch_plain_update<double>(S.mesh_->dev(), x, c->inertia(), 0.0, c->blocks.f64(),
-                                  c->fe.data(), c->stream);
+    const auto m = S.mesh_->dev();
+    if (m.dim == 2) {
+      if (c->fp32())
+        launch_plain2d_update<float>(m, x, c->inertia(), 0.0, c->blocks.f32(), c->fe.data(),
+                                     c->stream);
+      else
+        launch_plain2d_update<double>(m, x, c->inertia(), 0.0, c->blocks.f64(), c->fe.data(),
+                                      c->stream);
+    } else if (c->fp32()) {
+      launch_plain_update<float>(m, x, c->inertia(), 0.0, c->blocks.f32(), c->fe.data(),
+                                 c->stream);
+    } else {
+      launch_plain_update<double>(m, x, c->inertia(), 0.0, c->blocks.f64(), c->fe.data(),
+                                  c->stream);
+    }
   }
 }
 
@@ -972,8 +1006,11 @@ int NewtonOptimizer<DIM>::launch_energy(double alpha, int base) {
   if (S.mixed_vars_.size() + S.vars_.size() > 2)
     throw std::runtime_error("at most two elastic variables are supported");
   if (k == 0) {  // no elastic variable: inertia only
-    launch_plain_energy(S.mesh_->dev(), x, dx, alpha, c->inertia(), 0.0, c->reducer.slot(base),
-                        c->stream);
+    const auto m = S.mesh_->dev();
+    if (m.dim == 2)
+      launch_plain2d_energy(m, x, dx, alpha, c->inertia(), 0.0, c->reducer.slot(base), c->stream);
+    else
+      launch_plain_energy(m, x, dx, alpha, c->inertia(), 0.0, c->reducer.slot(base), c->stream);
     k = 1;
   }
   return k;
@@ -1087,7 +1124,8 @@ void SimState<DIM>::load_arrays(const SceneArrays& sc, std::shared_ptr<SimConfig
                                 const std::vector<std::string>& mixed_variables,
                                 const std::vector<std::string>& variables) {
   config_ = std::move(config);
-  const int nT = (int)(sc.T.size() / 4);
+  if (sc.dim != 2 && sc.dim != 3) throw std::runtime_error("dim must be 2 or 3");
+  const int nT = (int)(sc.T.size() / (sc.dim + 1));
   // per-element Lame parameters (one model per tet, as with per-tet
   // material ids); materials[0] supplies the density
   MaterialModelFactory mf;
@@ -1098,7 +1136,7 @@ void SimState<DIM>::load_arrays(const SceneArrays& sc, std::shared_ptr<SimConfig
   if (!material_models_[0]) throw std::runtime_error("unknown material type");
   if ((int)sc.mu.size() != nT || (int)sc.la.size() != nT || (int)sc.mat.size() != nT)
     throw std::runtime_error("per-element arrays must have nT entries");
-  mesh_ = std::make_shared<Mesh>(sc.V, sc.T, material_models_[0], sc.mu, sc.la, sc.mat);
+  mesh_ = std::make_shared<Mesh>(sc.V, sc.T, material_models_[0], sc.mu, sc.la, sc.mat, sc.dim);
   mesh_->bc_config_ = sc.bc;
   if (!sc.v0.empty()) mesh_->initial_velocity_ = sc.v0;
   fixed_override_ = sc.fixed;

@@ -436,8 +436,9 @@ class OptimizerFactory
 // ---------------------------------------------------------------------------
 // Scene description accepted from the C ABI (arrays) or built by load(json).
 struct SceneArrays {
-  std::vector<double> V;      // nV x 3
-  std::vector<int> T;         // nT x 4
+  int dim = 3;                // 2: triangles in the plane
+  std::vector<double> V;      // nV x 3 (z = 0 for dim 2)
+  std::vector<int> T;         // nT x (dim + 1)
   std::vector<double> mu, la;
   std::vector<int> mat;
   std::vector<int> fixed;     // optional pin mask

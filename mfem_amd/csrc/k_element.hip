@@ -626,6 +626,160 @@ __global__ __launch_bounds__(kElemBlock) void k_plain_energy(
 }
 
 // ---------------------------------------------------------------------------
+// 2D displacement-only SNH on the 4-vector F (reference
+// stable_neohookean.cpp:279-328, no PSD projection) over triangles (corners
+// x, y, z of T; dphi/dX in rows 3k + c, k < 3, c < 2; vol = the reference's
+// signed doublearea, tri2d_mesh.cpp:32-34). Blocks and residual use the 3D
+// layout with zero z coupling, so the z rows of K hold only the mass and dz
+// stays zero.
+__device__ __forceinline__ void snh_f4_cof(const double F[4], double& J, double c[4]) {
+  // F column-major (F00, F10, F01, F11); c = dJ/dF
+  J = F[0] * F[3] - F[2] * F[1];
+  c[0] = F[3];
+  c[1] = -F[2];
+  c[2] = -F[1];
+  c[3] = F[0];
+}
+
+__device__ __forceinline__ double snh_f4_energy(double mu, double la, const double F[4]) {
+  double J, c[4];
+  snh_f4_cof(F, J, c);
+  const double n2 = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  return 0.5 * mu * (n2 - 2.0) - mu * (J - 1.0) + 0.5 * la * (J - 1.0) * (J - 1.0);
+}
+
+__device__ __forceinline__ void gather_F2(const TetMeshDev& m, int e, const double* __restrict__ x,
+                                          const double* __restrict__ dx, double alpha, int v[3],
+                                          double xc[9], double F[4]) {
+  const int4 t = m.T[e];
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int f = dx ? m.free_map[v[k]] : -1;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      xc[3 * k + r] = x[3 * v[k] + r] + (f >= 0 ? alpha * dx[3 * f + r] : 0.0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) F[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double d = m.dX[(3 * k + c) * m.nT + e];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) F[r + 2 * c] += xc[3 * k + r] * d;
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_plain2d_update(
+    TetMeshDev m, const double* __restrict__ x, Inertia in, double h2, Real* __restrict__ blocks,
+    double* __restrict__ fe) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  int v[3];
+  double xc[9], F[4];
+  gather_F2(m, e, x, nullptr, 0.0, v, xc, F);
+  double J, c[4];
+  snh_f4_cof(F, J, c);
+  const double mu = m.mu[e], la = m.la[e];
+  const double k = la * (J - 1.0) - mu;
+  double g[4], H[16];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) g[i] = mu * F[i] + k * c[i];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) H[p + 4 * q] = (p == q ? mu : 0.0) + la * c[p] * c[q];
+  H[0 + 4 * 3] += k;
+  H[3 + 4 * 0] += k;
+  H[1 + 4 * 2] -= k;
+  H[2 + 4 * 1] -= k;
+  const double vol = m.vol[e];
+  const double sc = vol * h2;
+  double d2[6];  // dphi_k/dX_c, k < 3, c < 2
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) d2[2 * kk + cc] = m.dX[(3 * kk + cc) * nT + e];
+  Real* out = blocks + (size_t)e * 90;
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int a = 0; a <= l; ++a) {
+      Real* o = out + 9 * blk_index(a, l);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          double val = 0.0;
+          if (r < 2 && q < 2) {
+#pragma unroll
+            for (int c1 = 0; c1 < 2; ++c1)
+#pragma unroll
+              for (int c2 = 0; c2 < 2; ++c2)
+                val += d2[2 * a + c1] * H[(r + 2 * c1) + 4 * (q + 2 * c2)] * d2[2 * l + c2];
+            val *= sc;
+          }
+          o[3 * r + q] = (Real)val;
+        }
+    }
+  // residual: -vol h^2 Jloc^T g - M_e (x - x_tilde - h^2 g_ext), M_e = rho vol / 12 (1 + delta)
+  double d[9], dsum[3] = {0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double dd = xc[3 * kk + r] - in.x_tilde[3 * v[kk] + r] - in.h2g[r];
+      d[3 * kk + r] = dd;
+      dsum[r] += dd;
+    }
+  const double mc = m.density * vol / 12.0;
+  double* fo = fe + (size_t)e * 12;
+#pragma unroll
+  for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double a = 0.0;
+      if (r < 2) a = d2[2 * kk] * g[r] + d2[2 * kk + 1] * g[r + 2];
+      fo[3 * kk + r] = -vol * h2 * a - mc * (d[3 * kk + r] + dsum[r]);
+    }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) fo[9 + r] = 0.0;
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_plain2d_energy(
+    TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha,
+    Inertia in, double h2, ReduceSlot out) {
+  __shared__ double sm[kElemBlock / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double en = 0.0;
+  if (e < m.nT) {
+    int v[3];
+    double xc[9], F[4];
+    gather_F2(m, e, x, dx, alpha, v, xc, F);
+    double dsum[3] = {0, 0, 0}, dsq = 0;
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double dd = xc[3 * kk + r] - in.x_tilde[3 * v[kk] + r] - in.h2g[r];
+        dsum[r] += dd;
+        dsq += dd * dd;
+      }
+    const double vol = m.vol[e];
+    const double e_in = 0.5 * (m.density * vol / 12.0) *
+                        (dsq + dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2]);
+    en = e_in + h2 * vol * snh_f4_energy(m.mu[e], m.la[e], F);
+  }
+  block_partial<kElemBlock, 0>(en, out.partials, sm);
+}
+
+// ---------------------------------------------------------------------------
 // Batched element evaluations (points in AoS, column-major per point).
 __global__ void k_batch_polar(int n, const double* __restrict__ F, double* __restrict__ R,
                               double* __restrict__ s, double* __restrict__ dsdF,
@@ -762,6 +916,23 @@ void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double
   launch_finalize(out.partials, reduce_grid(m.nT), 0, out.result, st);
 }
 
+template <typename Real>
+void launch_plain2d_update(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                           double h2, Real* blocks, double* fe, hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_plain2d_update<Real>, dim3(grid_for(m.nT, kElemBlock)), dim3(kElemBlock), 0,
+                     st, m, x_full, in, h2, blocks, fe);
+}
+
+void launch_plain2d_energy(const TetMeshDev& m, const double* x_full, const double* dx,
+                           double alpha, const Inertia& in, double h2, ReduceSlot out,
+                           hipStream_t st) {
+  const int g = reduce_grid(m.nT);
+  hipLaunchKernelGGL(k_plain2d_energy, dim3(g), dim3(kElemBlock), 0, st, m, x_full, dx, alpha, in,
+                     h2, out);
+  launch_finalize(out.partials, g, 0, out.result, st);
+}
+
 void launch_batch_polar(int n, const double* F, double* R, double* s, double* dsdF, int* iters,
                         hipStream_t st) {
   if (n <= 0) return;
@@ -794,7 +965,9 @@ void launch_batch_snh_f9(int n, const double* mu, const double* la, const double
                                                const double*, double*, double*, ReduceSlot,     \
                                                hipStream_t);                                    \
   template void launch_plain_update<Real>(const TetMeshDev&, const double*, const Inertia&,     \
-                                          double, Real*, double*, hipStream_t);
+                                          double, Real*, double*, hipStream_t);                 \
+  template void launch_plain2d_update<Real>(const TetMeshDev&, const double*, const Inertia&,   \
+                                            double, Real*, double*, hipStream_t);
 MFEM_INST(double)
 MFEM_INST(float)
 #undef MFEM_INST

@@ -11,6 +11,7 @@ namespace mfemhip {
 // Mesh data on the device, structure-of-arrays with stride nT.
 struct TetMeshDev {
   int nV = 0, nT = 0, nfree = 0;
+  int dim = 3;                     // 2: triangles in the z = 0 plane (corners x, y, z of T)
   const int4* T = nullptr;         // [nT] vertex ids
   const double* dX = nullptr;      // [12][nT]: dphi_k/dX_c at (3k+c)*nT + e
   const double* vol = nullptr;     // [nT]
@@ -68,6 +69,15 @@ void launch_plain_update(const TetMeshDev& m, const double* x_full, const Inerti
 void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double* dx,
                          double alpha, const Inertia& in, double h2, ReduceSlot out,
                          hipStream_t st);
+// 2D (triangles, reference tri2d_mesh.cpp and the 4-vector SNH API,
+// stable_neohookean.cpp:279-328, no PSD projection): the same outputs with
+// the z components zero (z rows carry only the mass).
+template <typename Real>
+void launch_plain2d_update(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                           double h2, Real* blocks, double* fe, hipStream_t st);
+void launch_plain2d_energy(const TetMeshDev& m, const double* x_full, const double* dx,
+                           double alpha, const Inertia& in, double h2, ReduceSlot out,
+                           hipStream_t st);
 
 // --- assembly (reference src/utils/sparse_utils.cpp:141-203) ---------------
 // K block b = mass[b] * I + sum over its slot list of element blocks.

@@ -40,6 +40,7 @@ class SimParams(C.Structure):
         ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
         ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("precision", C.c_int),
         ("bc_type", C.c_int), ("bc_axis", C.c_int), ("bc_ratio", C.c_double),
+        ("dim", C.c_int),
     ]
 
 
@@ -124,8 +125,6 @@ def _i32(a):
 
 def _params(sc, precision, p=None):
     """mfem_sim_params for a Scene; returns (params, arrays to keep alive)."""
-    if sc.dim != 3:
-        raise MfemError("the device backend is 3D (tetrahedra)")
     keep = dict(V=_f64(sc.V), T=_i32(sc.T), mu=_f64(sc.mu), la=_f64(sc.la), mat=_i32(sc.mat),
                 fixed=_i32(sc.fixed), solver=sc.solver.encode())
     p = SimParams() if p is None else p
@@ -150,6 +149,7 @@ def _params(sc, precision, p=None):
     p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
     p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
     p.precision = {"fp64": 0, "mixed": 1}[precision]
+    p.dim = sc.dim
     return p, keep
 
 
@@ -166,6 +166,7 @@ class Sim:
         p, self._keep = _params(scene, precision)
         check(lib().mfem_sim_create(C.byref(p), C.byref(self.h)))
         self.nV = scene.nV
+        self.dim = scene.dim
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -200,9 +201,10 @@ class Sim:
         return {k: getattr(st, k) for k, _ in StepStats._fields_}
 
     def positions(self):
-        out = np.zeros(self.nV * 3)
+        d = getattr(self, "dim", 3)
+        out = np.zeros(self.nV * d)
         check(lib().mfem_sim_get_positions(self.h, out.ctypes.data_as(dp)))
-        return out.reshape(-1, 3)
+        return out.reshape(-1, d)
 
     def get(self, name):
         n = lib().mfem_sim_size(self.h, name.encode())

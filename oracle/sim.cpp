@@ -1319,12 +1319,11 @@ void* oracle_sim_create(const oracle_params* p) {
       for (int i = 0; i < 6; ++i) S_->s[6 * e + i] = S_->S[e][i];
     }
   } else {
-    if (D == 3)
-      for (int e = 0; e < p->nT; ++e)
-        if (!material_has_f9(S_->mat[e])) {
-          delete S_;
-          return nullptr;  // the material has no 9-vector API
-        }
+    for (int e = 0; e < p->nT; ++e)
+      if (D == 3 ? !material_has_f9(S_->mat[e]) : S_->mat[e] != MAT_SNH) {
+        delete S_;
+        return nullptr;  // no 9-vector (3D) or 4-vector (2D, SNH only) API
+      }
     S_->Hp.assign((size_t)p->nT * M * M, 0.0);
     S_->gp.assign((size_t)p->nT * M, 0.0);
   }

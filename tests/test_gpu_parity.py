@@ -310,3 +310,49 @@ def test_bdf_orders_trajectory(native, order):
         o.step()
         g.step()
         assert relerr(g.positions(), o.positions()) < POS_TOL, (order, k)
+
+
+# ------------------------------------------------------------------ 2D (triangles)
+
+def test_2d_plain_trajectory(native, oracle):
+    """The 2D displacement-only path (reference Tri2DMesh and the 4-vector
+    SNH API, stable_neohookean.cpp:279-328): the benchmark set's 40x10
+    rectangle of 800 triangles with its pinned end, solved with Jacobi CG to
+    rel-res 1e-12 on both sides (the device has no direct solver)."""
+    sc = scenes.config_c1()
+    sc.solver, sc.itr_tol, sc.max_iters = "eigen-pcg-diag", 1e-12, 20000
+    o = oracle.Sim(sc)
+    g = native.Sim(sc)
+    for k in range(5):
+        o.step()
+        g.step()
+        xo, xg = o.positions(), g.positions()
+        assert xg.shape == xo.shape == (sc.nV, 2)
+        assert relerr(xg, xo) < POS_TOL, k
+        assert g.stats()["newton_iters"] == o.stats()["newton_iters"], k
+    assert np.abs(o.positions() - sc.V).max() > 1e-3
+
+
+def test_2d_matches_direct_solve(native, oracle):
+    """The same rectangle against the oracle's dense Cholesky (the config's
+    own "eigen-llt" solver)."""
+    sc = scenes.config_c1()
+    o = oracle.Sim(sc)
+    sc.solver, sc.itr_tol, sc.max_iters = "eigen-pcg-diag", 1e-12, 20000
+    g = native.Sim(sc)
+    for k in range(3):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < POS_TOL, k
+
+
+def test_2d_rejects_mixed_and_other_models(native):
+    sc = scenes.config_c1()
+    sc.solver = "eigen-pcg-diag"
+    sc.mixed = True
+    with pytest.raises(native.MfemError):
+        native.Sim(sc)
+    sc.mixed = False
+    sc.mat = np.full(sc.nT, scenes.MATERIALS["Neohookean"], dtype=np.int32)
+    with pytest.raises(native.MfemError):
+        native.Sim(sc)
