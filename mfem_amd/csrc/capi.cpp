@@ -68,14 +68,32 @@ std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
     if (!ms) throw std::runtime_error("'" + k + "' needs a mixed-stretch variable");
     return ms;
   };
-  if (k == "R") return soa_to_elem(need_ms()->R_.to_host(st), nT, 9);
-  if (k == "S") return soa_to_elem(need_ms()->S_.to_host(st), nT, 6);
-  if (k == "s") return soa_to_elem(need_ms()->s_.to_host(st), nT, 6);
-  if (k == "lambda") return soa_to_elem(need_ms()->la_.to_host(st), nT, 6);
-  if (k == "ds") return soa_to_elem(need_ms()->ds_.to_host(st), nT, 6);
-  if (k == "g") return soa_to_elem(need_ms()->g_.to_host(st), nT, 6);
-  if (k == "gl") return soa_to_elem(need_ms()->gl_.to_host(st), nT, 6);
-  if (k == "dsdF") return soa_to_elem(need_ms()->dsdF_.to_host_double(st), nT, 54);
+  // element widths: 3D s is a 6-vector, 2D a 3-vector (leading rows)
+  const bool two = m.dim() == 2;
+  const int ws = two ? 3 : 6;
+  if (k == "R") return soa_to_elem(need_ms()->R_.to_host(st), nT, two ? 4 : 9);
+  if (k == "S") return soa_to_elem(need_ms()->S_.to_host(st), nT, ws);
+  if (k == "s") return soa_to_elem(need_ms()->s_.to_host(st), nT, ws);
+  if (k == "lambda") return soa_to_elem(need_ms()->la_.to_host(st), nT, ws);
+  if (k == "ds") return soa_to_elem(need_ms()->ds_.to_host(st), nT, ws);
+  if (k == "g") return soa_to_elem(need_ms()->g_.to_host(st), nT, ws);
+  if (k == "gl") return soa_to_elem(need_ms()->gl_.to_host(st), nT, ws);
+  if (k == "dsdF") return soa_to_elem(need_ms()->dsdF_.to_host_double(st), nT, two ? 12 : 54);
+  if (two && (k == "H" || k == "Hinv")) {
+    auto P = soa_to_elem(k == "H" ? need_ms()->Hpsi_.to_host_double(st)
+                                  : need_ms()->Hinv_.to_host_double(st), nT, 6);
+    std::vector<double> o((size_t)nT * 9);
+    const double syminv[3] = {1, 1, 0.5};
+    for (int e = 0; e < nT; ++e)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const int a = std::min(i, j), b = std::max(i, j);
+          double v = P[(size_t)e * 6 + (a * 3 - (a * (a - 1)) / 2 + (b - a))];
+          if (k == "H") v = syminv[i] * v * syminv[j] / m.vols_[e];
+          o[(size_t)e * 9 + i + 3 * j] = v;
+        }
+    return o;
+  }
   if (k == "H" || k == "Hinv") {
     auto P = soa_to_elem(k == "H" ? need_ms()->Hpsi_.to_host_double(st)
                                   : need_ms()->Hinv_.to_host_double(st), nT, 21);

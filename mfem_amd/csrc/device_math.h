@@ -339,6 +339,129 @@ MFEM_DEV void material_grad_hess(int type, double mu, double la, const double s[
   H[sym6(4, 5)] += k * (2.0 * d);
 }
 
+// ---------------------------------------------------------------------------
+// 2D (reference DIM = 2 paths). Symmetric 3x3 matrices are packed
+// upper-triangular row by row: sym3(i, j) in 0..5.
+MFEM_DEV constexpr int sym3(int i, int j) {
+  return i <= j ? (i * 3 - (i * (i - 1)) / 2 + (j - i)) : (j * 3 - (j * (j - 1)) / 2 + (i - j));
+}
+
+// 2D polar fit (mixed_stretch.cpp:42-77 with svd_eigen.h:7-31 and
+// dsvd.cpp:138-191): R = the rotation by atan2(F10 - F01, F00 + F11), which
+// is the flip-corrected U V^T; S = R^T F, s = (S00, S11, (S10 + S01) / 2);
+// dsdF (3 x 4 column-major, entry (i, c) at i + 3c) = rows {0, 3, (1 + 2)/2}
+// of (I (x) R^T)(I - (S^T (x) I) dR/dF), with dR/dF dropped when
+// |sigma_1^2 - sigma_0^2| < 1e-4 as the reference's dsvd does. F, R and S
+// are 2x2 column-major.
+MFEM_DEV void polar2d(const double F[4], double R[4], double S[4], double s[3], double* dsdF) {
+  const double a = F[0] + F[3], b = F[1] - F[2];
+  const double th = atan2(b, a), c = cos(th), sn = sin(th);
+  R[0] = c; R[1] = sn; R[2] = -sn; R[3] = c;
+  // S = R^T F
+  S[0] = c * F[0] + sn * F[1];
+  S[1] = -sn * F[0] + c * F[1];
+  S[2] = c * F[2] + sn * F[3];
+  S[3] = -sn * F[2] + c * F[3];
+  s[0] = S[0];
+  s[1] = S[3];
+  s[2] = 0.5 * (S[1] + S[2]);
+  if (!dsdF) return;
+  const double n2 = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const double det = F[0] * F[3] - F[2] * F[1];
+  const double gap = sqrt(fmax(n2 * n2 - 4.0 * det * det, 0.0));
+  double dRdF[16];  // (i, j) at i + 4j: d vec(R)_i / d vec(F)_j
+#pragma unroll
+  for (int i = 0; i < 16; ++i) dRdF[i] = 0.0;
+  if (gap >= 1e-4) {
+    const double r2 = a * a + b * b;
+    const double dth[4] = {-b / r2, a / r2, -a / r2, -b / r2};
+    const double dR[4] = {-sn, c, -c, -sn};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dRdF[i + 4 * j] = dR[i] * dth[j];
+  }
+  // K = I - (S^T (x) I) dRdF; (S^T (x) I)(bi*2 + i, bj*2 + i) = S(bj, bi)
+  double K[16];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int bi = i / 2, ii = i % 2;
+      double v = 0.0;
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) v += S[bj + 2 * bi] * dRdF[(bj * 2 + ii) + 4 * j];
+      K[i + 4 * j] = (i == j ? 1.0 : 0.0) - v;
+    }
+  // J = (I (x) R^T) K; (I (x) R^T)(blk*2 + i, blk*2 + j) = R(j, i)
+  double J[16];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int blk = i / 2, ii = i % 2;
+      J[i + 4 * j] = R[0 + 2 * ii] * K[(blk * 2 + 0) + 4 * j] + R[1 + 2 * ii] * K[(blk * 2 + 1) + 4 * j];
+    }
+#pragma unroll
+  for (int col = 0; col < 4; ++col) {
+    dsdF[0 + 3 * col] = J[0 + 4 * col];
+    dsdF[1 + 3 * col] = J[3 + 4 * col];
+    dsdF[2 + 3 * col] = 0.5 * (J[1 + 4 * col] + J[2 + 4 * col]);
+  }
+}
+
+// Materials on the 2D 3-vector s = (S00, S11, S01) (stable_neohookean.cpp:
+// 236-277, arap.cpp:67-98, corotational.cpp:62-96, fixed_corotational.cpp:
+// 93-132), with q = 1 - det S. SNH's Hessian is PSD-projected by the caller;
+// the host admits no other type on this path.
+MFEM_DEV double material2d_energy(int type, double mu, double la, const double s[3]) {
+  const double S1 = s[0], S2 = s[1], S3 = s[2];
+  const double q = 1.0 - S1 * S2 + S3 * S3;
+  const double dev = (S1 - 1) * (S1 - 1) + (S2 - 1) * (S2 - 1) + 2.0 * S3 * S3;
+  if (type == MAT_ARAP) return 0.5 * mu * dev;
+  if (type == MAT_COROT) return 0.5 * la * (S1 + S2 - 2.0) * (S1 + S2 - 2.0) + mu * dev;
+  if (type == MAT_FCR) return 0.5 * la * q * q + mu * dev;
+  return 0.5 * la * q * q + 0.5 * mu * (S1 * S1 + S2 * S2 + 2.0 * S3 * S3 - 2.0) + mu * q;
+}
+
+MFEM_DEV void material2d_grad_hess(int type, double mu, double la, const double s[3], double g[3],
+                                   double H[6]) {
+  const double S1 = s[0], S2 = s[1], S3 = s[2];
+  const double q = 1.0 - S1 * S2 + S3 * S3;
+  const double dq[3] = {-S2, -S1, 2.0 * S3};
+#pragma unroll
+  for (int i = 0; i < 6; ++i) H[i] = 0.0;
+  if (type == MAT_ARAP) {
+    g[0] = mu * (S1 - 1); g[1] = mu * (S2 - 1); g[2] = 2.0 * mu * S3;
+    H[sym3(0, 0)] = mu; H[sym3(1, 1)] = mu; H[sym3(2, 2)] = 2.0 * mu;
+    return;
+  }
+  if (type == MAT_COROT) {
+    const double t = S1 + S2 - 2.0;
+    g[0] = la * t + 2.0 * mu * (S1 - 1); g[1] = la * t + 2.0 * mu * (S2 - 1); g[2] = 4.0 * mu * S3;
+    H[sym3(0, 0)] = H[sym3(1, 1)] = la + 2.0 * mu;
+    H[sym3(0, 1)] = la;
+    H[sym3(2, 2)] = 4.0 * mu;
+    return;
+  }
+  // SNH: (la q + mu) dq + mu diag(1,1,2) s; FCR: la q dq + mu diag(2,2,4)(s - (1,1,0))
+  const bool fcr = type == MAT_FCR;
+  const double k = fcr ? la * q : la * q + mu;
+  const double v[3] = {1, 1, 2};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    g[i] = k * dq[i] + (fcr ? 2.0 * mu * v[i] * (s[i] - (i < 2 ? 1.0 : 0.0)) : mu * v[i] * s[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) H[sym3(i, j)] = la * dq[i] * dq[j];
+  // d2q: (0,1) = -1, (2,2) = 2
+  H[sym3(0, 1)] += -k;
+  H[sym3(2, 2)] += 2.0 * k;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) H[sym3(i, i)] += (fcr ? 2.0 : 1.0) * mu * v[i];
+}
+
 MFEM_DEV bool material_needs_psd(int type) {
   // SNH (reference stable_neohookean.cpp:91-93) and NH/Fung project; ARAP,
   // Corot and FCR do not.

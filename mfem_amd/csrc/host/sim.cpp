@@ -242,8 +242,14 @@ bool Stretch<DIM>::energy_launch(const double* x_full, const double* dx, double 
 // reference mixed_stretch.cpp:262-304
 template <int DIM>
 void MixedStretch<DIM>::reset() {
-  if (this->mesh_->dim() != 3)
-    throw std::runtime_error("mixed-stretch runs on tetrahedral (3D) meshes only on the device");
+  const bool two = this->mesh_->dim() == 2;
+  if (two)
+    for (int t : this->mesh_->element_types())
+      if (t != MATERIAL_SNH && t != MATERIAL_ARAP && t != MATERIAL_COROT && t != MATERIAL_FCR)
+        throw std::runtime_error("material '" +
+                                 MaterialModelFactory().name_by_type((MaterialModelType)t) +
+                                 "' has no 2D mixed (3-vector) energy; 2D \"mixed-stretch\" needs "
+                                 "Stable-Neohookean, ARAP, Corot or FixedCorot");
   auto* c = this->ctx_;
   const int nT = this->mesh_->nT();
   hipStream_t st = c->stream;
@@ -262,12 +268,27 @@ void MixedStretch<DIM>::reset() {
   s_.zero(st);
   // R = I, then fit at the rest configuration and set s = S
   std::vector<double> Rh(9 * (size_t)nT, 0.0);
-  for (int e = 0; e < nT; ++e) { Rh[0 * nT + e] = 1; Rh[4 * nT + e] = 1; Rh[8 * nT + e] = 1; }
+  for (int e = 0; e < nT; ++e) {
+    if (two) {
+      Rh[0 * nT + e] = 1; Rh[3 * nT + e] = 1;
+    } else {
+      Rh[0 * nT + e] = 1; Rh[4 * nT + e] = 1; Rh[8 * nT + e] = 1;
+    }
+  }
   R_.upload(Rh, st);
   DeviceArray<double> x0;
   x0.upload(this->mesh_->V_, st);
   auto m = this->mesh_->dev();
-  if (c->fp32())
+  S_.zero(st);
+  if (two && c->fp32())
+    launch_mixed2d_polar_material<float>(m, x0.data(), R_.data(), S_.data(), s_.data(),
+                                         dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
+                                         gl_.data(), 0.0, st);
+  else if (two)
+    launch_mixed2d_polar_material<double>(m, x0.data(), R_.data(), S_.data(), s_.data(),
+                                          dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
+                                          gl_.data(), 0.0, st);
+  else if (c->fp32())
     launch_mixed_polar_material<float>(m, x0.data(), R_.data(), S_.data(), s_.data(),
                                        dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
                                        g_.data(), gl_.data(), 0.0, st);
@@ -287,6 +308,22 @@ void MixedStretch<DIM>::update(const double* x_full, double dt) {
   h2_ = dt * dt;
   auto* c = this->ctx_;
   auto m = this->mesh_->dev();
+  if (m.dim == 2) {
+    if (c->fp32()) {
+      launch_mixed2d_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
+                                           dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
+                                           gl_.data(), h2_, c->stream);
+      launch_mixed2d_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
+                                       gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
+    } else {
+      launch_mixed2d_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
+                                            dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
+                                            gl_.data(), h2_, c->stream);
+      launch_mixed2d_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(), Hpsi_.f64(),
+                                        gl_.data(), c->blocks.f64(), c->fe.data(), c->stream);
+    }
+    return;
+  }
   if (c->fp32()) {
     launch_mixed_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
                                        dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
@@ -309,6 +346,17 @@ template <int DIM>
 void MixedStretch<DIM>::solve(const double* dx) {
   auto* c = this->ctx_;
   auto m = this->mesh_->dev();
+  if (m.dim == 2) {
+    if (c->fp32())
+      launch_mixed2d_local_solve<float>(m, dx, dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
+                                        gl_.data(), la_.data(), ds_.data(),
+                                        c->reducer.slot(RS_MAXDS), c->stream);
+    else
+      launch_mixed2d_local_solve<double>(m, dx, dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
+                                         gl_.data(), la_.data(), ds_.data(),
+                                         c->reducer.slot(RS_MAXDS), c->stream);
+    return;
+  }
   if (c->fp32())
     launch_mixed_local_solve<float>(m, dx, dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
                                     g_.data(), gl_.data(), la_.data(), ds_.data(),
@@ -333,8 +381,13 @@ template <int DIM>
 bool MixedStretch<DIM>::energy_launch(const double* x_full, const double* dx, double alpha,
                                       ReduceSlot out) {
   auto* c = this->ctx_;
-  launch_mixed_energy(this->mesh_->dev(), x_full, dx, alpha, c->inertia(), R_.data(), s_.data(),
-                      ds_.data(), la_.data(), h2_, out, c->stream);
+  const auto m = this->mesh_->dev();
+  if (m.dim == 2)
+    launch_mixed2d_energy(m, x_full, dx, alpha, c->inertia(), s_.data(), ds_.data(), la_.data(),
+                          h2_, out, c->stream);
+  else
+    launch_mixed_energy(m, x_full, dx, alpha, c->inertia(), R_.data(), s_.data(), ds_.data(),
+                        la_.data(), h2_, out, c->stream);
   return true;
 }
 
@@ -669,7 +722,8 @@ void PcgSolver<DIM>::solve() {
 // ============================================================ MINRES
 template <int DIM>
 MinresSolver<DIM>::MinresSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
-  if (DIM != 3) throw std::runtime_error("minres-indefinite-block: the device path is 3D");
+  if (DIM != 3 || state->mesh_->dim() != 3)
+    throw std::runtime_error("minres-indefinite-block: the device path is 3D (tetrahedra)");
   if (state->mixed_vars_.size() != 1 || !state->vars_.empty())
     throw std::runtime_error(
         "minres-indefinite-block needs exactly one mixed variable (\"mixed-stretch\") and no "

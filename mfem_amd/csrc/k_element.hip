@@ -752,6 +752,257 @@ __global__ __launch_bounds__(kElemBlock) void k_plain2d_update(
   for (int r = 0; r < 3; ++r) fo[9 + r] = 0.0;
 }
 
+// 2D mixed stretch (the DIM = 2 instantiation of mixed_stretch.cpp:134-332):
+// s = (S00, S11, S01), Sym = diag(1, 1, 2); per triangle R (4), S, s, g, gl
+// (3 each), dsdF (3 x 4 at (i + 3c)), Hpsi and its inverse (6 packed), all
+// structure of arrays with stride nT in the 3D arrays' leading rows.
+__device__ __forceinline__ double sym3w(int i) { return i < 2 ? 1.0 : 2.0; }
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed2d_polar_material(
+    TetMeshDev m, const double* __restrict__ x, double* __restrict__ R, double* __restrict__ S,
+    const double* __restrict__ s, Real* __restrict__ dsdF, Real* __restrict__ Hpsi,
+    Real* __restrict__ Hinv, double* __restrict__ g, double* __restrict__ gl, double h2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  int v[3];
+  double xc[9], F[4];
+  gather_F2(m, e, x, nullptr, 0.0, v, xc, F);
+  double Rl[4], Sm[4], sv[3], d[12];
+  polar2d(F, Rl, Sm, sv, d);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) R[i * nT + e] = Rl[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) S[i * nT + e] = sv[i];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) dsdF[i * nT + e] = (Real)d[i];
+  double se[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) se[i] = s[i * nT + e];
+  const int type = m.mat[e];
+  double gg[3], HH[6];
+  material2d_grad_hess(type, m.mu[e], m.la[e], se, gg, HH);
+  Real A[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i + 3 * j] = (Real)HH[sym3(i, j)];
+  if (type == MAT_SNH && !chol_pd<Real, 3>(A, Real(1e-6))) jacobi_psd_project<Real, 3>(A, Real(1e-6));
+  double Hs[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Hs[i] = h2 * (double)A[i];
+  // inverse of the symmetric 3x3 by cofactors
+  const double c00 = Hs[4] * Hs[8] - Hs[7] * Hs[5], c01 = Hs[7] * Hs[2] - Hs[1] * Hs[8],
+               c02 = Hs[1] * Hs[5] - Hs[4] * Hs[2];
+  const double det = Hs[0] * c00 + Hs[3] * c01 + Hs[6] * c02;
+  double Hi[9];
+  Hi[0] = c00 / det; Hi[1] = c01 / det; Hi[2] = c02 / det;
+  Hi[3] = (Hs[6] * Hs[5] - Hs[3] * Hs[8]) / det;
+  Hi[4] = (Hs[0] * Hs[8] - Hs[6] * Hs[2]) / det;
+  Hi[5] = (Hs[3] * Hs[2] - Hs[0] * Hs[5]) / det;
+  Hi[6] = (Hs[3] * Hs[7] - Hs[6] * Hs[4]) / det;
+  Hi[7] = (Hs[6] * Hs[1] - Hs[0] * Hs[7]) / det;
+  Hi[8] = (Hs[0] * Hs[4] - Hs[3] * Hs[1]) / det;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      Hpsi[sym3(i, j) * nT + e] = (Real)Hs[i + 3 * j];
+      Hinv[sym3(i, j) * nT + e] = (Real)(0.5 * (Hi[i + 3 * j] + Hi[j + 3 * i]));
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double a = h2 * gg[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a += (double)(Real)Hs[i + 3 * j] * (sv[j] - se[j]);
+    g[i * nT + e] = h2 * gg[i];
+    gl[i * nT + e] = (i < 2 ? 1.0 : 0.5) * a;
+  }
+}
+
+// B = dsdF Jloc (3 x 9 over the 3 corners' xyz; z columns zero):
+// B(i, 3k + r) = sum_c dsdF(i, r + 2c) dX(k, c) for r < 2.
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed2d_blocks_rhs(
+    TetMeshDev m, const double* __restrict__ x, Inertia in, const Real* __restrict__ dsdF,
+    const Real* __restrict__ Hpsi, const double* __restrict__ gl, Real* __restrict__ blocks,
+    double* __restrict__ fe) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.nT) return;
+  const int nT = m.nT;
+  const double vol = m.vol[e];
+  double d2[6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) d2[2 * k + c] = m.dX[(3 * k + c) * nT + e];
+  double B[27];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double a = 0.0;
+        if (r < 2)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) a += (double)dsdF[(i + 3 * (r + 2 * c)) * nT + e] * d2[2 * k + c];
+        B[i + 3 * (3 * k + r)] = a;
+      }
+  double H[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) H[i + 3 * j] = (double)Hpsi[sym3(i, j) * nT + e];
+  Real* out = blocks + (size_t)e * 90;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    double C[9];  // H B_l (3 x 3)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double a = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) a += H[i + 3 * j] * B[j + 3 * (3 * l + q)];
+        C[i + 3 * q] = a;
+      }
+#pragma unroll
+    for (int k = 0; k <= l; ++k) {
+      Real* o = out + 9 * blk_index(k, l);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          double a = 0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) a += B[i + 3 * (3 * k + r)] * C[i + 3 * q];
+          o[3 * r + q] = (Real)(vol * a);
+        }
+    }
+  }
+  // residual: -vol B^T (Sym gl) - M_e (x - x_tilde - h^2 g), M_e = rho vol / 12 (1 + delta)
+  double w[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) w[i] = sym3w(i) * gl[i * nT + e];
+  const int4 t = m.T[e];
+  const int v[3] = {t.x, t.y, t.z};
+  double dd[9], dsum[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double a = x[3 * v[k] + r] - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+      dd[3 * k + r] = a;
+      dsum[r] += a;
+    }
+  const double mc = m.density * vol / 12.0;
+  double* fo = fe + (size_t)e * 12;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    double a = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) a += B[i + 3 * j] * w[i];
+    fo[j] = -vol * a - mc * (dd[j] + dsum[j % 3]);
+  }
+#pragma unroll
+  for (int j = 9; j < 12; ++j) fo[j] = 0.0;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed2d_local_solve(
+    TetMeshDev m, const double* __restrict__ dx, const Real* __restrict__ dsdF,
+    const Real* __restrict__ Hpsi, const Real* __restrict__ Hinv, const double* __restrict__ g,
+    const double* __restrict__ gl, double* __restrict__ lam, double* __restrict__ ds,
+    ReduceSlot max_ds) {
+  __shared__ double sm[kElemBlock / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double local_max = 0.0;
+  if (e < m.nT) {
+    const int nT = m.nT;
+    const int4 t = m.T[e];
+    const int v[3] = {t.x, t.y, t.z};
+    double Fd[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int f = m.free_map[v[k]];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double d = m.dX[(3 * k + c) * nT + e];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) Fd[r + 2 * c] += (f >= 0 ? dx[3 * f + r] : 0.0) * d;
+      }
+    }
+    double y[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a += (double)dsdF[(i + 3 * c) * nT + e] * Fd[c];
+      y[i] = a;
+    }
+    double l[3], rr[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) a += (double)Hpsi[sym3(i, j) * nT + e] * y[j];
+      l[i] = -gl[i * nT + e] - (i < 2 ? 1.0 : 0.5) * a;
+      rr[i] = sym3w(i) * l[i] + g[i * nT + e];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) a += (double)Hinv[sym3(i, j) * nT + e] * rr[j];
+      lam[i * nT + e] = l[i];
+      ds[i * nT + e] = -a;
+      local_max = fmax(local_max, fabs(a));
+    }
+  }
+  block_partial<kElemBlock, 1>(local_max, max_ds.partials, sm);
+}
+
+// Line-search energy of triangle e at (x + alpha dx, s + alpha ds): inertia,
+// h^2 vol Psi(s) and -vol lambda^T Sym (S(x) - s), S(x) by the 2D polar fit.
+__global__ __launch_bounds__(kElemBlock) void k_mixed2d_energy(
+    TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha,
+    Inertia in, const double* __restrict__ s, const double* __restrict__ ds,
+    const double* __restrict__ lam, double h2, ReduceSlot out) {
+  __shared__ double sm[kElemBlock / kWave];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double en = 0.0;
+  if (e < m.nT) {
+    const int nT = m.nT;
+    int v[3];
+    double xc[9], F[4];
+    gather_F2(m, e, x, dx, alpha, v, xc, F);
+    double dsum[3] = {0, 0, 0}, dsq = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double dd = xc[3 * k + r] - in.x_tilde[3 * v[k] + r] - in.h2g[r];
+        dsum[r] += dd;
+        dsq += dd * dd;
+      }
+    const double vol = m.vol[e];
+    const double e_in = 0.5 * (m.density * vol / 12.0) *
+                        (dsq + dsum[0] * dsum[0] + dsum[1] * dsum[1] + dsum[2] * dsum[2]);
+    double Rl[4], Sm[4], st[3];
+    polar2d(F, Rl, Sm, st, nullptr);
+    double si[3], c = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      si[i] = s[i * nT + e] + alpha * ds[i * nT + e];
+      c += lam[i * nT + e] * sym3w(i) * (st[i] - si[i]);
+    }
+    en = e_in + h2 * vol * material2d_energy(m.mat[e], m.mu[e], m.la[e], si) - vol * c;
+  }
+  block_partial<kElemBlock, 0>(en, out.partials, sm);
+}
+
 __global__ __launch_bounds__(kElemBlock) void k_plain2d_energy(
     TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha,
     Inertia in, double h2, ReduceSlot out) {
@@ -933,6 +1184,44 @@ void launch_plain2d_energy(const TetMeshDev& m, const double* x_full, const doub
   launch_finalize(out.partials, g, 0, out.result, st);
 }
 
+template <typename Real>
+void launch_mixed2d_polar_material(const TetMeshDev& m, const double* x_full, double* R, double* S,
+                                   const double* s, Real* dsdF, Real* Hpsi, Real* Hinv, double* g,
+                                   double* gl, double h2, hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_mixed2d_polar_material<Real>, dim3(grid_for(m.nT, kElemBlock)),
+                     dim3(kElemBlock), 0, st, m, x_full, R, S, s, dsdF, Hpsi, Hinv, g, gl, h2);
+}
+
+template <typename Real>
+void launch_mixed2d_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                               const Real* dsdF, const Real* Hpsi, const double* gl, Real* blocks,
+                               double* fe, hipStream_t st) {
+  if (m.nT == 0) return;
+  hipLaunchKernelGGL(k_mixed2d_blocks_rhs<Real>, dim3(grid_for(m.nT, kElemBlock)), dim3(kElemBlock),
+                     0, st, m, x_full, in, dsdF, Hpsi, gl, blocks, fe);
+}
+
+template <typename Real>
+void launch_mixed2d_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
+                                const Real* Hpsi, const Real* Hinv, const double* g,
+                                const double* gl, double* lam, double* ds, ReduceSlot max_ds,
+                                hipStream_t st) {
+  const int gsz = reduce_grid(m.nT);
+  hipLaunchKernelGGL(k_mixed2d_local_solve<Real>, dim3(gsz), dim3(kElemBlock), 0, st, m, dx, dsdF,
+                     Hpsi, Hinv, g, gl, lam, ds, max_ds);
+  launch_finalize(max_ds.partials, gsz, 1, max_ds.result, st);
+}
+
+void launch_mixed2d_energy(const TetMeshDev& m, const double* x_full, const double* dx,
+                           double alpha, const Inertia& in, const double* s, const double* ds,
+                           const double* lam, double h2, ReduceSlot out, hipStream_t st) {
+  const int gsz = reduce_grid(m.nT);
+  hipLaunchKernelGGL(k_mixed2d_energy, dim3(gsz), dim3(kElemBlock), 0, st, m, x_full, dx, alpha,
+                     in, s, ds, lam, h2, out);
+  launch_finalize(out.partials, gsz, 0, out.result, st);
+}
+
 void launch_batch_polar(int n, const double* F, double* R, double* s, double* dsdF, int* iters,
                         hipStream_t st) {
   if (n <= 0) return;
@@ -967,7 +1256,17 @@ void launch_batch_snh_f9(int n, const double* mu, const double* la, const double
   template void launch_plain_update<Real>(const TetMeshDev&, const double*, const Inertia&,     \
                                           double, Real*, double*, hipStream_t);                 \
   template void launch_plain2d_update<Real>(const TetMeshDev&, const double*, const Inertia&,   \
-                                            double, Real*, double*, hipStream_t);
+                                            double, Real*, double*, hipStream_t);               \
+  template void launch_mixed2d_polar_material<Real>(const TetMeshDev&, const double*, double*,  \
+                                                    double*, const double*, Real*, Real*, Real*, \
+                                                    double*, double*, double, hipStream_t);     \
+  template void launch_mixed2d_blocks_rhs<Real>(const TetMeshDev&, const double*, const Inertia&, \
+                                                const Real*, const Real*, const double*, Real*,   \
+                                                double*, hipStream_t);                            \
+  template void launch_mixed2d_local_solve<Real>(const TetMeshDev&, const double*, const Real*,   \
+                                                 const Real*, const Real*, const double*,         \
+                                                 const double*, double*, double*, ReduceSlot,     \
+                                                 hipStream_t);
 MFEM_INST(double)
 MFEM_INST(float)
 #undef MFEM_INST

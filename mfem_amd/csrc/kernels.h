@@ -69,6 +69,25 @@ void launch_plain_update(const TetMeshDev& m, const double* x_full, const Inerti
 void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double* dx,
                          double alpha, const Inertia& in, double h2, ReduceSlot out,
                          hipStream_t st);
+// 2D mixed stretch on triangles (same phases as the 3D launchers above;
+// per triangle R 4, S/s/g/gl 3, dsdF 12, Hpsi/Hinv 6 packed, in the leading
+// rows of the 3D arrays).
+template <typename Real>
+void launch_mixed2d_polar_material(const TetMeshDev& m, const double* x_full, double* R, double* S,
+                                   const double* s, Real* dsdF, Real* Hpsi, Real* Hinv, double* g,
+                                   double* gl, double h2, hipStream_t st);
+template <typename Real>
+void launch_mixed2d_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
+                               const Real* dsdF, const Real* Hpsi, const double* gl, Real* blocks,
+                               double* fe, hipStream_t st);
+template <typename Real>
+void launch_mixed2d_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
+                                const Real* Hpsi, const Real* Hinv, const double* g,
+                                const double* gl, double* lam, double* ds, ReduceSlot max_ds,
+                                hipStream_t st);
+void launch_mixed2d_energy(const TetMeshDev& m, const double* x_full, const double* dx,
+                           double alpha, const Inertia& in, const double* s, const double* ds,
+                           const double* lam, double h2, ReduceSlot out, hipStream_t st);
 // 2D (triangles, reference tri2d_mesh.cpp and the 4-vector SNH API,
 // stable_neohookean.cpp:279-328, no PSD projection): the same outputs with
 // the z components zero (z rows carry only the mass).

@@ -139,6 +139,103 @@ int polar_svd3(M3& R, Vec<6>& s, const M3& F, bool compute_gradients,
 }
 
 // ---------------------------------------------------------------------------
+// 2D polar fit of mixed_stretch.cpp:42-77 (DIM = 2). The reference takes
+// R = U V^T from JacobiSVD with the flip fix of svd_eigen.h:7-31; for any F
+// this is the rotation by theta = atan2(F10 - F01, F00 + F11), the rotation
+// that makes R^T F symmetric with a non-negative trace. S = R^T F,
+// s = (S00, S11, (S10 + S01) / 2); dsdF holds rows {0, 3, (1 + 2) / 2} of
+// J = (I (x) R^T)(I - (S^T (x) I) dRdF). dRdF follows dsvd.cpp:138-191: zero
+// when |sigma_1^2 - sigma_0^2| < 1e-4 (its "no angular velocity" guard),
+// else the exact derivative of the rotation, vec(dR/dtheta) dtheta/dF.
+int polar2d(Mat<2, 2>& R, Vec<3>& s, const Mat<2, 2>& F, bool compute_gradients,
+            Mat<3, 4>* dsdF) {
+  const double a = F(0, 0) + F(1, 1), b = F(1, 0) - F(0, 1);
+  const double th = std::atan2(b, a), c = std::cos(th), sn = std::sin(th);
+  R(0, 0) = c; R(1, 0) = sn; R(0, 1) = -sn; R(1, 1) = c;
+  Mat<2, 2> S = R.T() * F;
+  s[0] = S(0, 0); s[1] = S(1, 1); s[2] = 0.5 * (S(1, 0) + S(0, 1));
+  if (compute_gradients && dsdF) {
+    Mat<4, 4> dRdF;
+    double n2 = 0;
+    for (int i = 0; i < 4; ++i) n2 += F[i] * F[i];
+    const double det = F(0, 0) * F(1, 1) - F(0, 1) * F(1, 0);
+    const double gap = std::sqrt(std::max(n2 * n2 - 4.0 * det * det, 0.0));  // s1^2 - s0^2
+    if (gap >= 1e-4) {
+      const double r2 = a * a + b * b;
+      const double dth[4] = {-b / r2, a / r2, -a / r2, -b / r2};  // d theta / vec F
+      const double dR[4] = {-sn, c, -c, -sn};                     // vec dR/dtheta
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) dRdF(i, j) = dR[i] * dth[j];
+    }
+    Mat<4, 4> J = flatten_multiply<2>(R.T()) *
+                  (Mat<4, 4>::Identity() - flatten_multiply_right<2>(S) * dRdF);
+    for (int col = 0; col < 4; ++col) {
+      (*dsdF)(0, col) = J(0, col);
+      (*dsdF)(1, col) = J(3, col);
+      (*dsdF)(2, col) = 0.5 * (J(1, col) + J(2, col));
+    }
+  }
+  return 0;
+}
+
+// Materials on the 2D symmetric 3-vector s = (S00, S11, S01), the 3-vector
+// API of stable_neohookean.cpp:236-277 (PSD fix), arap.cpp:67-98,
+// corotational.cpp:62-96 and fixed_corotational.cpp:93-132 (closed forms of
+// their symbolic expressions, q = 1 - det S = 1 - S00 S11 + S01^2). Neo-Hookean
+// and Fung have no 3-vector API (material_model.h, base "unimplemented"):
+// false.
+bool material_s3(int type, double mu, double la, const Vec<3>& s, bool psd_fix, double* E,
+                 Vec<3>* g, Mat<3, 3>* H) {
+  const double S1 = s[0], S2 = s[1], S3 = s[2];
+  const double q = 1.0 - S1 * S2 + S3 * S3;
+  Vec<3> dq; dq[0] = -S2; dq[1] = -S1; dq[2] = 2.0 * S3;
+  Mat<3, 3> d2q; d2q(0, 1) = d2q(1, 0) = -1.0; d2q(2, 2) = 2.0;
+  const double v[3] = {1, 1, 2};
+  double e = 0;
+  Vec<3> gg;
+  Mat<3, 3> HH;
+  switch (type) {
+    case MAT_SNH: {
+      e = 0.5 * la * q * q + 0.5 * mu * (S1 * S1 + S2 * S2 + 2.0 * S3 * S3 - 2.0) + mu * q;
+      for (int i = 0; i < 3; ++i) gg[i] = (la * q + mu) * dq[i] + mu * v[i] * s[i];
+      HH = la * (dq * dq.T()) + (la * q + mu) * d2q;
+      for (int i = 0; i < 3; ++i) HH(i, i) += mu * v[i];
+      if (psd_fix) simple_psd_fix<3>(HH);
+      break;
+    }
+    case MAT_ARAP: {
+      e = 0.5 * mu * ((S1 - 1) * (S1 - 1) + (S2 - 1) * (S2 - 1) + 2.0 * S3 * S3);
+      gg[0] = mu * (S1 - 1); gg[1] = mu * (S2 - 1); gg[2] = 2.0 * mu * S3;
+      HH(0, 0) = mu; HH(1, 1) = mu; HH(2, 2) = 2.0 * mu;
+      break;
+    }
+    case MAT_COROT: {
+      const double t = S1 + S2 - 2.0;
+      e = 0.5 * la * t * t + mu * ((S1 - 1) * (S1 - 1) + (S2 - 1) * (S2 - 1) + 2.0 * S3 * S3);
+      gg[0] = la * t + 2.0 * mu * (S1 - 1); gg[1] = la * t + 2.0 * mu * (S2 - 1);
+      gg[2] = 4.0 * mu * S3;
+      HH(0, 0) = HH(1, 1) = la + 2.0 * mu; HH(0, 1) = HH(1, 0) = la; HH(2, 2) = 4.0 * mu;
+      break;
+    }
+    case MAT_FCR: {
+      e = 0.5 * la * q * q + mu * ((S1 - 1) * (S1 - 1) + (S2 - 1) * (S2 - 1) + 2.0 * S3 * S3);
+      gg[0] = la * q * dq[0] + 2.0 * mu * (S1 - 1);
+      gg[1] = la * q * dq[1] + 2.0 * mu * (S2 - 1);
+      gg[2] = la * q * dq[2] + 4.0 * mu * S3;
+      HH = la * (dq * dq.T()) + (la * q) * d2q;
+      HH(0, 0) += 2.0 * mu; HH(1, 1) += 2.0 * mu; HH(2, 2) += 4.0 * mu;
+      break;
+    }
+    default:
+      return false;
+  }
+  if (E) *E = e;
+  if (g) *g = gg;
+  if (H) *H = HH;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // Materials on the symmetric 6-vector s = (S00,S11,S22,S01,S02,S12).
 // J = det S, with gradient dJ and Hessian d2J with respect to s.
 static void det_s6(const Vec<6>& s, double& J, Vec<6>& dJ, Mat<6, 6>& d2J) {

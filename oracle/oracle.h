@@ -17,6 +17,13 @@ int polar_svd3(M3& R, Vec<6>& s, const M3& F, bool compute_gradients,
                Mat<6, 9>* dsdF, int* iters);
 void material_s6(int type, double mu, double la, const Vec<6>& s, bool psd_fix,
                  double* E, Vec<6>* g, Mat<6, 6>* H);
+int polar2d(Mat<2, 2>& R, Vec<3>& s, const Mat<2, 2>& F, bool compute_gradients,
+            Mat<3, 4>* dsdF);
+bool material_s3(int type, double mu, double la, const Vec<3>& s, bool psd_fix, double* E,
+                 Vec<3>* g, Mat<3, 3>* H);
+inline bool material_has_s3(int type) {
+  return type == MAT_SNH || type == MAT_ARAP || type == MAT_COROT || type == MAT_FCR;
+}
 void snh_f9(double mu, double la, const Vec<9>& f, double* E, Vec<9>* g,
             Mat<9, 9>* H);
 void snh_f4(double mu, double la, const Vec<4>& f, double* E, Vec<4>* g,
@@ -81,6 +88,11 @@ void oracle_snh_f9(double mu, double la, const double* F, double* E,
                    double* g, double* H);
 void oracle_snh_f4(double mu, double la, const double* F, double* E,
                    double* g, double* H);
+// 2D polar fit (F, R column-major 2x2; dsdF 3x4 column-major)
+int oracle_polar2d(double* R, double* s, const double* F, int grad, double* dsdF);
+// 3-vector (2D mixed) API of material `type` (0 when the model has none)
+int oracle_material_s3(int type, double mu, double la, const double* s, int psd_fix, double* E,
+                       double* g, double* H);
 // 9-vector API of material `type` (0 when the model does not implement it)
 int oracle_material_f9(int type, double mu, double la, const double* F, double* E, double* g,
                        double* H);

@@ -61,6 +61,10 @@ def lib():
         L.oracle_snh_f4.argtypes = [C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_material_f9.argtypes = [C.c_int, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_material_f9.restype = C.c_int
+        L.oracle_polar2d.argtypes = [dp, dp, dp, C.c_int, dp]
+        L.oracle_polar2d.restype = C.c_int
+        L.oracle_material_s3.argtypes = [C.c_int, C.c_double, C.c_double, dp, C.c_int, dp, dp, dp]
+        L.oracle_material_s3.restype = C.c_int
         L.oracle_psd_fix.argtypes = [C.c_int, dp, C.c_double]
         L.oracle_bc_fixed.argtypes = [C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_double, ip]
         L.oracle_tet_geometry.argtypes = [dp, ip, C.c_int, dp, dp]
@@ -153,6 +157,28 @@ def snh_f9(mu, la, F9):
     H = np.zeros(81)
     lib().oracle_snh_f9(float(mu), float(la), _dp(F9), _dp(E), _dp(g), _dp(H))
     return E[0], g, H.reshape(9, 9).T.copy()
+
+
+def polar2d(F, grad=True):
+    """2D polar fit: F (2x2) -> R (2x2), s (3), dsdF (3x4, column-major vec F)."""
+    Fc = f64(np.asarray(F, dtype=np.float64).T.reshape(-1))
+    R = np.zeros(4)
+    s = np.zeros(3)
+    d = np.zeros(12)
+    lib().oracle_polar2d(_dp(R), _dp(s), _dp(Fc), int(grad), _dp(d))
+    return R.reshape(2, 2).T.copy(), s, d.reshape(4, 3).T.copy()
+
+
+def material_s3(mtype, mu, la, s, psd_fix=True):
+    """3-vector (2D mixed) API of a material; None if it has none."""
+    s = f64(s)
+    E = np.zeros(1)
+    g = np.zeros(3)
+    H = np.zeros(9)
+    if not lib().oracle_material_s3(int(mtype), float(mu), float(la), _dp(s), int(psd_fix),
+                                    _dp(E), _dp(g), _dp(H)):
+        return None
+    return E[0], g, H.reshape(3, 3).T.copy()
 
 
 def material_f9(mtype, mu, la, F9):

@@ -176,6 +176,11 @@ struct OracleSim {
   std::vector<Mat<6, 6>> Hm, Hinv, Hloc;
   std::vector<double> Aloc;  // per element (nodes*dim)^2
   std::vector<double> grad_x_mixed, grad_s;
+  // mixed stretch (2D): 3-vector s, 2x2 R, 4x3 dSdF, 3x3 Hessians
+  std::vector<Mat<2, 2>> R2;
+  std::vector<Vec<3>> S3, g3, gl3;
+  std::vector<Mat<4, 3>> dSdF2;
+  std::vector<Mat<3, 3>> Hm3, Hinv3, Hloc3;
   // plain stretch
   std::vector<double> Hp, gp, grad_plain;
   // system
@@ -421,6 +426,117 @@ double mixed_constraint_value(const OracleSim* S_, const std::vector<double>& fu
     double el = 0;
     for (int k = 0; k < 6; ++k)
       el += S_->lam[6 * i + k] * kSym[k] * (stmp[k] - sv[6 * i + k]);
+    per[i] = el * S_->vols[i];
+  }
+  double e = 0;
+  for (int i = 0; i < S_->nT; ++i) e += per[i];
+  return e;
+}
+
+// ---------------- MixedStretch (2D) ----------------
+// The DIM = 2 instantiation of mixed_stretch.cpp:134-332: s = (S00, S11, S01),
+// Sym = diag(1, 1, 2), dSdF = dsdF^T Sym (4 x 3), the same element formulas as
+// the 3D path with 3x3 material Hessians and 6x6 triangle blocks.
+const double kSym3[3] = {1, 1, 2};
+const double kSyminv3[3] = {1, 1, 0.5};
+
+Mat<2, 2> def_grad2(const OracleSim* S_, const std::vector<double>& full, int e) {
+  double f[4]; def_grad(S_, full, e, f);
+  Mat<2, 2> F; for (int i = 0; i < 4; ++i) F[i] = f[i];
+  return F;
+}
+
+void mixed2_update_rotations(OracleSim* S_, const std::vector<double>& full) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    Mat<3, 4> dsdF;
+    polar2d(S_->R2[e], S_->S3[e], def_grad2(S_, full, e), true, &dsdF);
+    S_->dSdF2[e] = dsdF.T() * diag<3>(kSym3);
+  }
+}
+
+void mixed2_update_derivatives(OracleSim* S_) {
+  const double h2 = S_->h * S_->h;
+  const int M = 6;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    const double vol = S_->vols[e];
+    Vec<3> si; for (int i = 0; i < 3; ++i) si[i] = S_->s[3 * e + i];
+    Vec<3> gg; Mat<3, 3> HH;
+    material_s3(S_->mat[e], S_->mu[e], S_->la[e], si, true, nullptr, &gg, &HH);
+    Mat<3, 3> H = h2 * HH;
+    S_->Hinv3[e] = lu_inverse<3>(H);
+    S_->g3[e] = h2 * gg;
+    S_->Hm3[e] = (1.0 / vol) * (diag<3>(kSyminv3) * H * diag<3>(kSyminv3));
+    S_->Hloc3[e] = vol * H;
+    Mat<4, 4> inner = S_->dSdF2[e] * S_->Hm3[e] * S_->dSdF2[e].T();
+    double* A = &S_->Aloc[(size_t)e * M * M];
+    const double* dx = &S_->dX[(size_t)e * 6];
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 2; ++r)
+        for (int l = 0; l < 3; ++l)
+          for (int q = 0; q < 2; ++q) {
+            double a = 0;
+            for (int c = 0; c < 2; ++c)
+              for (int c2 = 0; c2 < 2; ++c2)
+                a += dx[2 * k + c] * inner(r + 2 * c, q + 2 * c2) * dx[2 * l + c2];
+            A[(2 * k + r) + M * (2 * l + q)] = a * vol * vol;
+          }
+  }
+  S_->asmb.update_matrix(S_->Aloc, S_->lhs);
+}
+
+std::vector<double> mixed2_rhs(OracleSim* S_) {
+  std::vector<double> out(S_->n, 0.0), tmp(4 * (size_t)S_->nT);
+  for (int e = 0; e < S_->nT; ++e) {
+    const double vol = S_->vols[e];
+    Vec<3> d; for (int i = 0; i < 3; ++i) d[i] = S_->S3[e][i] - S_->s[3 * e + i];
+    S_->gl3[e] = vol * (S_->Hm3[e] * (diag<3>(kSym3) * d)) + diag<3>(kSyminv3) * S_->g3[e];
+    Vec<4> t = S_->dSdF2[e] * S_->gl3[e];
+    for (int i = 0; i < 4; ++i) tmp[4 * e + i] = t[i];
+  }
+  apply_PJW(S_, tmp, out, -1.0);
+  return out;
+}
+
+void mixed2_solve(OracleSim* S_, const std::vector<double>& dxv) {
+  std::vector<double> full(S_->nV * 2, 0.0);
+  for (int v = 0; v < S_->nV; ++v) {
+    const int f = S_->free_map[v];
+    if (f < 0) continue;
+    for (int r = 0; r < 2; ++r) full[2 * v + r] = dxv[2 * f + r];
+  }
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < S_->nT; ++e) {
+    Mat<2, 2> Fd = def_grad2(S_, full, e);
+    Vec<4> jdx; for (int i = 0; i < 4; ++i) jdx[i] = -S_->vols[e] * Fd[i];
+    Vec<3> l = -S_->gl3[e] + S_->Hm3[e] * (S_->dSdF2[e].T() * jdx);
+    Vec<3> ls = diag<3>(kSym3) * l + S_->g3[e];
+    Vec<3> d = -(S_->Hinv3[e] * ls);
+    for (int i = 0; i < 3; ++i) { S_->lam[3 * e + i] = l[i]; S_->ds[3 * e + i] = d[i]; }
+  }
+}
+
+double mixed2_energy(const OracleSim* S_, const std::vector<double>& sv) {
+  double e = 0;
+  for (int i = 0; i < S_->nT; ++i) {
+    Vec<3> si; for (int k = 0; k < 3; ++k) si[k] = sv[3 * i + k];
+    double E;
+    material_s3(S_->mat[i], S_->mu[i], S_->la[i], si, false, &E, nullptr, nullptr);
+    e += E * S_->vols[i];
+  }
+  return e;
+}
+
+double mixed2_constraint_value(const OracleSim* S_, const std::vector<double>& full,
+                               const std::vector<double>& sv) {
+  std::vector<double> per(S_->nT);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < S_->nT; ++i) {
+    Mat<2, 2> R; Vec<3> st;
+    polar2d(R, st, def_grad2(S_, full, i), false, nullptr);
+    double el = 0;
+    for (int k = 0; k < 3; ++k) el += S_->lam[3 * i + k] * kSym3[k] * (st[k] - sv[3 * i + k]);
     per[i] = el * S_->vols[i];
   }
   double e = 0;
@@ -966,7 +1082,10 @@ int kkt_solve(OracleSim* S_) {
 void update_system(OracleSim* S_) {
   std::vector<double> full;
   unproject(S_, S_->x, full);
-  if (S_->p.mixed) {
+  if (S_->p.mixed && S_->dim == 2) {
+    mixed2_update_rotations(S_, full);
+    mixed2_update_derivatives(S_);
+  } else if (S_->p.mixed) {
     mixed_update_rotations(S_, full);
     mixed_update_derivatives(S_);
   } else {
@@ -999,7 +1118,7 @@ int substep(OracleSim* S_, double& decrement) {
   S_->rhs.assign(S_->n, 0.0);
   for (int i = 0; i < S_->n; ++i) S_->rhs[i] = -S_->grad_x_disp[i];
   if (S_->p.mixed) {
-    std::vector<double> mr = mixed_rhs(S_);
+    std::vector<double> mr = S_->dim == 2 ? mixed2_rhs(S_) : mixed_rhs(S_);
     for (int i = 0; i < S_->n; ++i) S_->rhs[i] += mr[i];
   } else {
     for (int i = 0; i < S_->n; ++i) S_->rhs[i] += -S_->grad_plain[i];
@@ -1010,7 +1129,8 @@ int substep(OracleSim* S_, double& decrement) {
   S_->st.pcg_iters_total += S_->pcg_iters_last;
   // post_solve
   S_->dx = sol;
-  if (S_->p.mixed) mixed_solve(S_, sol);
+  if (S_->p.mixed && S_->dim == 2) mixed2_solve(S_, sol);
+  else if (S_->p.mixed) mixed_solve(S_, sol);
   decrement = 0;
   for (double v : S_->dx) decrement = std::max(decrement, std::fabs(v));
   if (S_->p.mixed)
@@ -1028,7 +1148,10 @@ double energy_func(const OracleSim* S_, double a) {
   if (S_->p.mixed) {
     std::vector<double> si(S_->s.size());
     for (size_t i = 0; i < si.size(); ++i) si[i] = S_->s[i] + a * S_->ds[i];
-    val += h2 * mixed_energy(S_, si) - mixed_constraint_value(S_, full, si);
+    if (S_->dim == 2)
+      val += h2 * mixed2_energy(S_, si) - mixed2_constraint_value(S_, full, si);
+    else
+      val += h2 * mixed_energy(S_, si) - mixed_constraint_value(S_, full, si);
   } else {
     val += h2 * plain_energy(S_, full);
   }
@@ -1149,6 +1272,27 @@ void oracle_snh_f9(double mu, double la, const double* F, double* E, double* g, 
   snh_f9(mu, la, f, E, &gg, H ? &HH : nullptr);
   if (g) for (int i = 0; i < 9; ++i) g[i] = gg[i];
   if (H) for (int i = 0; i < 81; ++i) H[i] = HH[i];
+}
+
+int oracle_polar2d(double* R, double* s, const double* F, int grad, double* dsdF) {
+  Mat<2, 2> Rm, Fm; for (int i = 0; i < 4; ++i) Fm[i] = F[i];
+  Vec<3> sv; Mat<3, 4> d;
+  int rc = polar2d(Rm, sv, Fm, grad != 0, &d);
+  for (int i = 0; i < 4; ++i) R[i] = Rm[i];
+  for (int i = 0; i < 3; ++i) s[i] = sv[i];
+  if (grad && dsdF) for (int i = 0; i < 12; ++i) dsdF[i] = d[i];
+  return rc;
+}
+
+int oracle_material_s3(int type, double mu, double la, const double* s, int psd_fix, double* E,
+                       double* g, double* H) {
+  Vec<3> sv; for (int i = 0; i < 3; ++i) sv[i] = s[i];
+  Vec<3> gg; Mat<3, 3> HH;
+  if (!material_s3(type, mu, la, sv, psd_fix != 0, E, g ? &gg : nullptr, H ? &HH : nullptr))
+    return 0;
+  if (g) for (int i = 0; i < 3; ++i) g[i] = gg[i];
+  if (H) for (int i = 0; i < 9; ++i) H[i] = HH[i];
+  return 1;
 }
 
 int oracle_material_f9(int type, double mu, double la, const double* F, double* E, double* g,
@@ -1300,7 +1444,27 @@ void* oracle_sim_create(const oracle_params* p) {
     for (int d = 0; d < D; ++d) S_->f_ext[D * v + d] = (double)(float)p->gravity[d];
   // variables
   int M = S_->nodes * D;
-  if (p->mixed) {
+  if (p->mixed && D == 2) {
+    const int nT = p->nT;
+    for (int e = 0; e < nT; ++e)
+      if (!material_has_s3(S_->mat[e])) {
+        delete S_;
+        return nullptr;  // no 3-vector (2D mixed) API
+      }
+    S_->R2.assign(nT, Mat<2, 2>::Identity());
+    S_->S3.assign(nT, Vec<3>()); S_->g3.assign(nT, Vec<3>()); S_->gl3.assign(nT, Vec<3>());
+    S_->dSdF2.assign(nT, Mat<4, 3>());
+    S_->Hm3.assign(nT, Mat<3, 3>::Identity()); S_->Hinv3.assign(nT, Mat<3, 3>::Identity());
+    S_->Hloc3.assign(nT, Mat<3, 3>());
+    S_->s.assign(3 * nT, 0.0); S_->lam.assign(3 * nT, 0.0); S_->ds.assign(3 * nT, 0.0);
+    S_->Aloc.assign((size_t)nT * M * M, 0.0);
+    for (int e = 0; e < nT; ++e) {
+      Mat<3, 4> dsdF;
+      polar2d(S_->R2[e], S_->S3[e], def_grad2(S_, S_->Vcur, e), true, &dsdF);
+      S_->dSdF2[e] = dsdF.T() * diag<3>(kSym3);
+      for (int i = 0; i < 3; ++i) S_->s[3 * e + i] = S_->S3[e][i];
+    }
+  } else if (p->mixed) {
     int nT = p->nT;
     S_->R.assign(nT, M3::Identity());
     S_->S.assign(nT, Vec<6>()); S_->g.assign(nT, Vec<6>()); S_->gl.assign(nT, Vec<6>());
@@ -1353,10 +1517,12 @@ long oracle_sim_size(void* sim, const char* name) {
   OracleSim* S_ = (OracleSim*)sim;
   std::string k(name);
   long nT = S_->nT;
-  if (k == "R") return 9 * nT;
-  if (k == "S" || k == "s" || k == "lambda" || k == "ds" || k == "g" || k == "gl") return 6 * nT;
-  if (k == "dSdF") return 54 * nT;
-  if (k == "H" || k == "Hinv") return 36 * nT;
+  const bool two = S_->dim == 2;
+  if (k == "R") return (two ? 4 : 9) * nT;
+  if (k == "S" || k == "s" || k == "lambda" || k == "ds" || k == "g" || k == "gl")
+    return (two ? 3 : 6) * nT;
+  if (k == "dSdF") return (two ? 12 : 54) * nT;
+  if (k == "H" || k == "Hinv") return (two ? 9 : 36) * nT;
   if (k == "Aloc") return (long)S_->Aloc.size();
   if (k == "Hplain") return (long)S_->Hp.size();
   if (k == "gplain") return (long)S_->gp.size();
@@ -1376,6 +1542,18 @@ int oracle_sim_get(void* sim, const char* name, double* out) {
   auto put6 = [&](const std::vector<Vec<6>>& v) {
     for (int e = 0; e < nT; ++e) for (int i = 0; i < 6; ++i) out[6 * e + i] = v[e][i];
   };
+  if (S_->dim == 2 && S_->p.mixed) {
+    auto put = [&](const auto& v, int w) {
+      for (int e = 0; e < nT; ++e) for (int i = 0; i < w; ++i) out[(size_t)w * e + i] = v[e][i];
+    };
+    if (k == "R") { put(S_->R2, 4); return 0; }
+    if (k == "S") { put(S_->S3, 3); return 0; }
+    if (k == "g") { put(S_->g3, 3); return 0; }
+    if (k == "gl") { put(S_->gl3, 3); return 0; }
+    if (k == "dSdF") { put(S_->dSdF2, 12); return 0; }
+    if (k == "H") { put(S_->Hm3, 9); return 0; }
+    if (k == "Hinv") { put(S_->Hinv3, 9); return 0; }
+  }
   if (k == "R") { for (int e = 0; e < nT; ++e) for (int i = 0; i < 9; ++i) out[9 * e + i] = S_->R[e][i]; return 0; }
   if (k == "S") { put6(S_->S); return 0; }
   if (k == "g") { put6(S_->g); return 0; }

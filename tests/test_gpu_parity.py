@@ -346,13 +346,70 @@ def test_2d_matches_direct_solve(native, oracle):
         assert relerr(g.positions(), o.positions()) < POS_TOL, k
 
 
-def test_2d_rejects_mixed_and_other_models(native):
+def test_2d_rejects_models_without_2d_api(native):
+    """2D plain needs the 4-vector API (SNH only); 2D mixed the 3-vector API
+    (no Neo-Hookean or Fung); MINRES and batches are 3D only."""
     sc = scenes.config_c1()
     sc.solver = "eigen-pcg-diag"
-    sc.mixed = True
-    with pytest.raises(native.MfemError):
-        native.Sim(sc)
-    sc.mixed = False
     sc.mat = np.full(sc.nT, scenes.MATERIALS["Neohookean"], dtype=np.int32)
     with pytest.raises(native.MfemError):
         native.Sim(sc)
+    sc.mixed = True
+    with pytest.raises(native.MfemError):
+        native.Sim(sc)
+    sc.mat[:] = scenes.MATERIALS["Stable-Neohookean"]
+    sc.solver = "minres-indefinite-block"
+    with pytest.raises(native.MfemError):
+        native.Sim(sc)
+
+
+def _rect_mixed(material="Stable-Neohookean", **kw):
+    args = dict(solver="eigen-pcg-diag", itr_tol=1e-12, max_iters=20000, max_newton=20,
+                newton_tol=1e-9)
+    args.update(kw)
+    sc = scenes.rect2d_scene(16, 5, 0.1, E=1e5, **args)
+    sc.mixed = True
+    sc.mat = np.full(sc.nT, scenes.MATERIALS[material], dtype=np.int32)
+    return sc
+
+
+def test_2d_mixed_phases(native, oracle):
+    """One update + solve of the 2D mixed path (reference mixed_stretch.cpp,
+    DIM = 2: SVD polar fit with the dsvd guard, 3-vector SNH) against the
+    oracle after a timestep of motion."""
+    sc = _rect_mixed()
+    o, g = oracle.Sim(sc), native.Sim(sc)
+    o.step()
+    g.step()
+    o.update_system()
+    g.update_system()
+    assert relerr(g.get("R"), o.get("R")) < 1e-9
+    assert relerr(g.get("S"), o.get("S")) < 1e-9
+    o.solve()
+    g.solve()
+    assert relerr(g.get("ds"), o.get("ds")) < 1e-7
+    assert relerr(g.get("lambda"), o.get("lambda")) < 1e-7
+    for a in (0.0, 0.5, 1.0):
+        eo, eg = o.energy(a), g.energy(a)
+        assert abs(eg - eo) <= 1e-10 * abs(eo), a
+
+
+@pytest.mark.parametrize("material", ["Stable-Neohookean", "ARAP", "Corot", "FixedCorot"])
+def test_2d_mixed_trajectory(native, oracle, material):
+    sc = _rect_mixed(material)
+    o, g = oracle.Sim(sc), native.Sim(sc)
+    for k in range(4):
+        o.step()
+        g.step()
+        assert relerr(g.positions(), o.positions()) < POS_TOL, (material, k)
+    assert np.abs(o.positions() - sc.V).max() > 1e-3
+
+
+def test_2d_mixed_precision(native, oracle):
+    sc = _rect_mixed(itr_tol=1e-10)
+    o, g = oracle.Sim(sc), native.Sim(sc, precision="mixed")
+    for k in range(4):
+        o.step()
+        g.step()
+        x_o = o.positions()
+        assert np.linalg.norm(g.positions() - x_o) / np.linalg.norm(x_o) < 1e-6, k

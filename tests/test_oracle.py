@@ -382,3 +382,91 @@ def test_golden_trajectories(oracle):
             sim.step()
             ref = data[f"{name}_x{k}"]
             assert np.abs(sim.positions() - ref).max() <= 1e-11 * np.abs(ref).max(), (name, k)
+
+
+# ------------------------------------------------------------------ 2D mixed
+
+def _svd_polar2(F):
+    """Polar rotation by the flip-corrected SVD (the reference's svd_eigen.h)."""
+    U, sig, Vt = np.linalg.svd(F)
+    V = Vt.T
+    if np.linalg.det(U) < 0:
+        U[:, 1] *= -1
+    if np.linalg.det(V) < 0:
+        V[:, 1] *= -1
+    return U @ V.T
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_polar2d_matches_svd_and_fd(oracle, seed):
+    rng = np.random.default_rng(seed)
+    F = np.eye(2) + 0.5 * rng.uniform(-1, 1, (2, 2))
+    if seed == 3:
+        F[:, 1] *= -1  # det F < 0: the flip fix applies
+    R, s, dsdF = oracle.polar2d(F)
+    assert np.abs(R - _svd_polar2(F)).max() < 1e-12
+    assert abs(np.linalg.det(R) - 1) < 1e-12
+    S = R.T @ F
+    assert abs(S[0, 1] - S[1, 0]) < 1e-12
+    assert np.allclose(s, [S[0, 0], S[1, 1], S[1, 0]], atol=1e-14)
+    f = F.T.reshape(-1)
+    h = 1e-6
+    fd = np.zeros((3, 4))
+    for j in range(4):
+        fp, fm = f.copy(), f.copy()
+        fp[j] += h
+        fm[j] -= h
+        fd[:, j] = (oracle.polar2d(fp.reshape(2, 2).T, grad=False)[1]
+                    - oracle.polar2d(fm.reshape(2, 2).T, grad=False)[1]) / (2 * h)
+    assert np.abs(fd - dsdF).max() < 1e-8
+
+
+def test_polar2d_isotropic_guard(oracle):
+    """|sigma_1^2 - sigma_0^2| < 1e-4: the reference's dsvd drops the rotation
+    derivative, leaving ds/dF = rows of (I (x) R^T) (dsvd.cpp:150-159)."""
+    th = 0.3
+    Rr = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    F = Rr @ np.diag([1.2, 1.2 + 1e-6])
+    R, s, dsdF = oracle.polar2d(F)
+    J = np.kron(np.eye(2), R.T)
+    ref = np.stack([J[0], J[3], 0.5 * (J[1] + J[2])])
+    assert np.abs(dsdF - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("mtype", [0, 2, 3, 4])
+def test_material_s3_derivatives(oracle, mtype):
+    rng = np.random.default_rng(20 + mtype)
+    s = np.array([1.0, 1.0, 0.0]) + 0.2 * rng.uniform(-1, 1, 3)
+    E, g, H = oracle.material_s3(mtype, MU, LA, s, psd_fix=False)
+    fun = lambda x: oracle.material_s3(mtype, MU, LA, x, psd_fix=False)
+    fg, fH = fd_grad_hess(fun, s)
+    assert np.abs(fg - g).max() <= 1e-7 * np.abs(g).max()
+    assert np.abs(fH - H).max() <= 1e-6 * np.abs(H).max()
+    E0, g0, _ = oracle.material_s3(mtype, MU, LA, np.array([1.0, 1.0, 0.0]))
+    assert abs(E0) < 1e-9 * MU and np.abs(g0).max() < 1e-9 * MU
+    if mtype == 0:  # SNH: the F-form energy at S = R^T F, and its PSD fix
+        S = np.array([[s[0], s[2]], [s[2], s[1]]])
+        J = np.linalg.det(S)
+        ref = 0.5 * MU * (np.sum(S * S) - 2) - MU * (J - 1) + 0.5 * LA * (J - 1) ** 2
+        assert abs(E - ref) <= 1e-12 * abs(ref)
+        _, _, Hp = oracle.material_s3(0, MU, LA, s, psd_fix=True)
+        assert np.linalg.eigvalsh(Hp).min() >= 1e-6 * (1 - 1e-9)
+    for t in (1, 5):
+        assert oracle.material_s3(t, MU, LA, s) is None
+
+
+def test_2d_mixed_equals_plain_at_convergence(oracle):
+    """2D SNH depends on F only through invariants of S (|F| = |S|, det F =
+    det S), so converged mixed and displacement-only steps agree."""
+    kw = dict(itr_tol=1e-12, max_iters=20000, max_newton=40, newton_tol=1e-10,
+              solver="eigen-llt")
+    a = scenes.rect2d_scene(12, 4, 0.1, E=1e5, **kw)
+    a.mixed = True
+    b = scenes.rect2d_scene(12, 4, 0.1, E=1e5, **kw)
+    sa, sb = oracle.Sim(a), oracle.Sim(b)
+    for _ in range(4):
+        sa.step()
+        sb.step()
+        disp = np.abs(sb.positions() - b.V).max()
+        assert disp > 1e-3
+        assert np.abs(sa.positions() - sb.positions()).max() < 1e-6 * disp
