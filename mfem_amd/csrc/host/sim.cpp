@@ -1,5 +1,7 @@
 #include "sim.h"
 
+#include <rocsolver/rocsolver.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -719,6 +721,72 @@ void PcgSolver<DIM>::solve() {
   system_.post_solve(this->state_, x_.data());
 }
 
+// ============================================================ direct
+template <int DIM>
+DirectSolver<DIM>::DirectSolver(SimState<DIM>* state, bool lu) : LinearSolver<DIM>(state), lu_(lu) {
+  const long n = 3L * state->mesh_->nfree();
+  if (n > kMaxRows)
+    throw std::runtime_error("direct solvers factor the system densely on the device and take at "
+                             "most " + std::to_string(kMaxRows) + " unknowns (this system has " +
+                             std::to_string(n) + "); use \"eigen-pcg-diag\"");
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success)
+    throw std::runtime_error("rocblas_create_handle failed");
+  handle_ = h;
+  if (rocblas_set_stream(h, state->ctx_->stream) != rocblas_status_success)
+    throw std::runtime_error("rocblas_set_stream failed");
+  info_.resize(1);
+}
+
+template <int DIM>
+DirectSolver<DIM>::~DirectSolver() {
+  if (handle_) (void)rocblas_destroy_handle((rocblas_handle)handle_);
+}
+
+template <int DIM>
+void DirectSolver<DIM>::solve() {
+  auto* c = this->state_->ctx_.get();
+  auto& m = *this->state_->mesh_;
+  hipStream_t st = c->stream;
+  system_.pre_solve(this->state_);
+  const int n = 3 * m.nfree();
+  this->stats_ = SolveStats();
+  dense_.resize((size_t)n * n);
+  x_.resize(n);
+  MFEM_HIP_CHECK(hipMemsetAsync(dense_.data(), 0, dense_.bytes(), st));
+  if (c->fp32())
+    launch_bsr_to_dense<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
+                               dense_.data(), n, st);
+  else
+    launch_bsr_to_dense<double>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f64(),
+                                dense_.data(), n, st);
+  MFEM_HIP_CHECK(hipMemcpyAsync(x_.data(), system_.rhs.data(), (size_t)n * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+  auto h = (rocblas_handle)handle_;
+  rocblas_status rs;
+  if (lu_) {
+    ipiv_.resize(n);
+    rs = rocsolver_dgetrf(h, n, n, dense_.data(), n, ipiv_.data(), info_.data());
+    if (rs == rocblas_status_success)
+      rs = rocsolver_dgetrs(h, rocblas_operation_none, n, 1, dense_.data(), n, ipiv_.data(),
+                            x_.data(), n);
+  } else {
+    rs = rocsolver_dpotrf(h, rocblas_fill_lower, n, dense_.data(), n, info_.data());
+    if (rs == rocblas_status_success)
+      rs = rocsolver_dpotrs(h, rocblas_fill_lower, n, 1, dense_.data(), n, x_.data(), n);
+  }
+  if (rs != rocblas_status_success)
+    throw std::runtime_error("rocSOLVER factorization call failed (status " +
+                             std::to_string((int)rs) + ")");
+  const int info = info_.to_host(st)[0];
+  if (info != 0)
+    throw std::runtime_error(std::string(lu_ ? "LU" : "Cholesky") +
+                             " factorization of the system matrix failed at column " +
+                             std::to_string(info));
+  this->stats_.refinements = 1;
+  system_.post_solve(this->state_, x_.data());
+}
+
 // ============================================================ MINRES
 template <int DIM>
 MinresSolver<DIM>::MinresSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
@@ -951,15 +1019,16 @@ void MinresSolver<DIM>::solve() {
 
 template <int DIM>
 LinearSolverFactory<DIM>::LinearSolverFactory() {
-  auto unsupported = [](SimState<DIM>*) -> std::unique_ptr<LinearSolver<DIM>> {
-    throw std::runtime_error(
-        "direct sparse factorisations (eigen-llt/ldlt/lu, cholmod) have no device "
-        "implementation; use \"eigen-pcg-diag\"");
+  auto cholesky = [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
+    return std::make_unique<DirectSolver<DIM>>(s, false);
   };
-  this->register_type(SOLVER_EIGEN_LLT, "eigen-llt", unsupported);
-  this->register_type(SOLVER_EIGEN_LDLT, "eigen-ldlt", unsupported);
-  this->register_type(SOLVER_EIGEN_LU, "eigen-lu", unsupported);
-  this->register_type(SOLVER_CHOLMOD, "cholmod", unsupported);
+  this->register_type(SOLVER_EIGEN_LLT, "eigen-llt", cholesky);
+  this->register_type(SOLVER_EIGEN_LDLT, "eigen-ldlt", cholesky);
+  this->register_type(SOLVER_CHOLMOD, "cholmod", cholesky);
+  this->register_type(SOLVER_EIGEN_LU, "eigen-lu",
+                      [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
+                        return std::make_unique<DirectSolver<DIM>>(s, true);
+                      });
   this->register_type(SOLVER_EIGEN_CG_DIAG, "eigen-pcg-diag",
                       [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
                         return std::make_unique<PcgSolver<DIM>>(s);
@@ -1489,6 +1558,7 @@ template class VariableFactory<3>;
 template class MixedVariableFactory<3>;
 template class SystemMatrixPD<3>;
 template class PcgSolver<3>;
+template class DirectSolver<3>;
 template class MinresSolver<3>;
 template class LinearSolverFactory<3>;
 template class Optimizer<3>;

@@ -318,6 +318,29 @@ class PcgSolver : public LinearSolver<DIM> {
   double wall_khz_ = 100000.0;   // device wall-clock rate
 };
 
+// "eigen-llt", "eigen-ldlt", "cholmod" and "eigen-lu": a direct solve of the
+// Schur-complement system K dx = rhs, refactored on every call as the
+// reference's EigenSolver does (eigen_solver.h:17-31), here as a dense FP64
+// factorization on the device (rocSOLVER potrf/potrs; getrf/getrs for LU).
+// K is SPD, so the LLT, LDLT and supernodal Cholesky variants give the same
+// solution. Systems above kMaxRows scalar rows are rejected.
+template <int DIM>
+class DirectSolver : public LinearSolver<DIM> {
+ public:
+  static constexpr long kMaxRows = 50000;
+  DirectSolver(SimState<DIM>* state, bool lu);
+  ~DirectSolver() override;
+  void solve() override;
+  SystemMatrixPD<DIM>& system() { return system_; }
+
+ private:
+  bool lu_ = false;
+  void* handle_ = nullptr;  // rocblas_handle
+  SystemMatrixPD<DIM> system_;
+  DeviceArray<double> dense_, x_;
+  DeviceArray<int> ipiv_, info_;
+};
+
 // "minres-indefinite-block": MINRES on the KKT system of the mixed
 // formulation (reference linear_solver_factory.cpp:117-134 with
 // SystemMatrixIndefinite, linear_system.h:60-99, and the matrix-free

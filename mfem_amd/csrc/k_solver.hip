@@ -699,6 +699,36 @@ void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* inv
                      diag_block, K, invdiag);
 }
 
+// Dense column-major copy of the BSR matrix (the direct-solver path):
+// A[(3 col + c) lda + 3 row + r] = K_b(r, c), one block row per lane.
+template <typename Real>
+__global__ __launch_bounds__(kBS) void k_bsr_to_dense(int nrows, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col,
+                                                      const Real* __restrict__ K,
+                                                      double* __restrict__ A, long lda) {
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= nrows) return;
+  for (int b = rowptr[i]; b < rowptr[i + 1]; ++b) {
+    const long j = col[b];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) A[(3 * j + c) * lda + 3L * i + r] = (double)K[9L * b + 3 * r + c];
+  }
+}
+
+template <typename Real>
+void launch_bsr_to_dense(int nrows, const int* rowptr, const int* col, const Real* K, double* A,
+                         long lda, hipStream_t st) {
+  if (nrows == 0) return;
+  hipLaunchKernelGGL(k_bsr_to_dense<Real>, dim3(grid(nrows, kBS)), dim3(kBS), 0, st, nrows, rowptr,
+                     col, K, A, lda);
+}
+template void launch_bsr_to_dense<float>(int, const int*, const int*, const float*, double*, long,
+                                         hipStream_t);
+template void launch_bsr_to_dense<double>(int, const int*, const int*, const double*, double*,
+                                          long, hipStream_t);
+
 void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const double* fe,
                          double* rhs, hipStream_t st) {
   if (nfree == 0) return;

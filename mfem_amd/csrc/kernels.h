@@ -108,6 +108,10 @@ void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const 
 template <typename Real, typename VT>
 void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* invdiag,
                         hipStream_t st);
+// Dense column-major FP64 copy of a BSR 3x3 matrix (zero A first).
+template <typename Real>
+void launch_bsr_to_dense(int nrows, const int* rowptr, const int* col, const Real* K, double* A,
+                         long lda, hipStream_t st);
 // rhs at free vertex i = sum over its (tet, corner) incidences of f_e[corner]
 void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const double* fe,
                          double* rhs, hipStream_t st);

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 _lib = None
 
-SOLVERS = {"eigen-llt": 0, "eigen-ldlt": 0, "cholmod": 0, "eigen-pcg-diag": 1,
+SOLVERS = {"eigen-llt": 0, "eigen-ldlt": 0, "cholmod": 0, "eigen-lu": 0, "eigen-pcg-diag": 1,
            "minres-indefinite-block": 2}
 # MINRES with the reference's exact (dense) preconditioner factorizations
 SOLVER_MINRES_EXACT = 3

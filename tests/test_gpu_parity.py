@@ -218,8 +218,8 @@ def test_json_scene_matches_array_scene(native, oracle):
 
 
 def test_errors_are_reported(native):
-    sc = scenes.beam_scene(2, 1, 1, mixed=True, solver="eigen-llt")
-    with pytest.raises(native.MfemError, match="no device implementation"):
+    sc = scenes.beam_scene(2, 1, 1, mixed=True, solver="no-such-solver")
+    with pytest.raises(native.MfemError, match="unknown linear solver"):
         native.Sim(sc)
     bad = scenes.beam_scene(2, 1, 1, mixed=True)
     bad.T = bad.T.copy()
@@ -318,7 +318,7 @@ def test_2d_plain_trajectory(native, oracle):
     """The 2D displacement-only path (reference Tri2DMesh and the 4-vector
     SNH API, stable_neohookean.cpp:279-328): the benchmark set's 40x10
     rectangle of 800 triangles with its pinned end, solved with Jacobi CG to
-    rel-res 1e-12 on both sides (the device has no direct solver)."""
+    rel-res 1e-12 on both sides."""
     sc = scenes.config_c1()
     sc.solver, sc.itr_tol, sc.max_iters = "eigen-pcg-diag", 1e-12, 20000
     o = oracle.Sim(sc)
@@ -335,7 +335,7 @@ def test_2d_plain_trajectory(native, oracle):
 
 def test_2d_matches_direct_solve(native, oracle):
     """The same rectangle against the oracle's dense Cholesky (the config's
-    own "eigen-llt" solver)."""
+    own "eigen-llt" solver), with the device's CG to 1e-12."""
     sc = scenes.config_c1()
     o = oracle.Sim(sc)
     sc.solver, sc.itr_tol, sc.max_iters = "eigen-pcg-diag", 1e-12, 20000
@@ -344,6 +344,32 @@ def test_2d_matches_direct_solve(native, oracle):
         o.step()
         g.step()
         assert relerr(g.positions(), o.positions()) < POS_TOL, k
+
+
+@pytest.mark.parametrize("solver", ["eigen-llt", "eigen-ldlt", "cholmod", "eigen-lu"])
+def test_direct_solvers(native, oracle, solver):
+    """The direct solvers (reference EigenSolver, eigen_solver.h:17-31,
+    refactored every call) as dense FP64 factorizations on the device: C1 as
+    configured (2D plain, "eigen-llt", Newton to 1e-8) and a small mixed
+    beam, against the oracle's dense Cholesky; Newton counts equal."""
+    c1 = scenes.config_c1()
+    c1.solver = solver
+    beam = scenes.beam_scene(4, 2, 2, seed=3, mixed=True, solver=solver, max_newton=10,
+                             newton_tol=1e-9)
+    for sc in (c1, beam):
+        o = oracle.Sim(sc)
+        g = native.Sim(sc)
+        for k in range(3):
+            o.step()
+            g.step()
+            assert relerr(g.positions(), o.positions()) < POS_TOL, (solver, k)
+            assert g.stats()["newton_iters"] == o.stats()["newton_iters"], (solver, k)
+
+
+def test_direct_solver_size_limit(native):
+    sc = scenes.beam_scene(40, 20, 20, mixed=True, solver="eigen-llt")
+    with pytest.raises(native.MfemError, match="at most"):
+        native.Sim(sc)
 
 
 def test_2d_rejects_models_without_2d_api(native):
