@@ -47,8 +47,10 @@ typedef struct {
   int bc_type, bc_axis;   /* boundary condition when fixed == NULL (reference config.h:70-87) */
   double bc_ratio;
   int dim;                /* 3 (tetrahedra; 0 also means 3) or 2 (triangles, reference
-                           * Tri2DMesh: V and v0 are nV x 2, T is nT x 3; displacement-only
-                           * Stable-Neohookean) */
+                           * Tri2DMesh: V and v0 are nV x 2, T is nT x 3) */
+  double bc_velocity;     /* "velocity" of the stretch / twist / bend boundary scripts, which
+                           * move their pinned end groups after every timestep
+                           * (reference boundary_conditions/*.h) */
 } mfem_sim_params;
 
 typedef struct {

@@ -202,6 +202,7 @@ SceneArrays scene_of(const mfem_sim_params& p) {
   sc.bc.type = (BCScriptType)p.bc_type;
   sc.bc.axis = p.bc_axis;
   sc.bc.ratio = p.bc_ratio;
+  sc.bc.velocity = p.bc_velocity;
   return sc;
 }
 

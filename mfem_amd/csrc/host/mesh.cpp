@@ -95,6 +95,52 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
     if (v < 0 || v >= nV_) throw std::runtime_error("Mesh: tet vertex index out of range");
   is_fixed_ = fixed_override.empty() ? boundary_fixed(bc_config_, V_, 3) : fixed_override;
   if ((int)is_fixed_.size() != nV_) throw std::runtime_error("Mesh: pin mask size");
+  // boundary-condition scripts (BoundaryCondition constructor + init,
+  // boundary_condition.cpp:5-21 and the *_boundary_condition.h headers)
+  bc_groups_.assign(2, {});
+  bc_group_center_.clear();
+  if (nV_ > 0) {
+    // groups from the rest positions (the constructor's V), centers from the
+    // start positions (init's V)
+    const int ax = bc_config_.axis;
+    auto bounds = [&](const std::vector<double>& X, double mn[3], double mx[3]) {
+      for (int d = 0; d < 3; ++d) mn[d] = mx[d] = X[d];
+      for (int i = 0; i < nV_; ++i)
+        for (int d = 0; d < 3; ++d) {
+          mn[d] = std::min(mn[d], X[3 * i + d]);
+          mx[d] = std::max(mx[d], X[3 * i + d]);
+        }
+    };
+    double mn[3], mx[3];
+    bounds(Vref_, mn, mx);
+    const double range = mx[ax] - mn[ax];
+    for (int i = 0; i < nV_; ++i) {
+      const double v = Vref_[3 * i + ax];
+      if (v < mn[ax] + range * bc_config_.ratio) bc_groups_[0].push_back(i);
+      else if (v > mx[ax] - range * bc_config_.ratio) bc_groups_[1].push_back(i);
+    }
+    bounds(V_, mn, mx);
+    for (int d = 0; d < 3; ++d) bc_center_[d] = 0.5 * (mn[d] + mx[d]);
+    for (const auto& g : bc_groups_) {
+      std::array<double, 3> c{0, 0, 0};
+      if (!g.empty())
+        for (int d = 0; d < 3; ++d) c[d] = V_[3 * g.back() + d];
+      bc_group_center_.push_back(c);
+    }
+    if (bc_config_.type == BC_SCALEF) {
+      // ScaleBC::init: the start configuration is the mesh scaled by 1.5
+      // about its bounding-box center (no pins)
+      for (int i = 0; i < nV_; ++i)
+        for (int d = 0; d < dim_; ++d)
+          V_[3 * i + d] = (V_[3 * i + d] - bc_center_[d]) * 1.5 + bc_center_[d];
+    } else if (bc_config_.type == BC_RANDOM) {
+      throw std::runtime_error("the \"randomize\" boundary condition (C rand() positions) has no "
+                               "reproducible device counterpart");
+    }
+  }
+  pinned_.clear();
+  for (int i = 0; i < nV_; ++i)
+    if (is_fixed_[i]) pinned_.push_back(i);
   free_map_.assign(nV_, -1);
   nfree_ = 0;
   for (int i = 0; i < nV_; ++i)
@@ -271,6 +317,47 @@ void Mesh::build_maps() {
       throw std::runtime_error("Mesh: free vertex not referenced by any tetrahedron (singular system)");
   }
   nnzb_ = (int)col_.size();
+}
+
+bool Mesh::bc_moves() const {
+  const auto t = bc_config_.type;
+  return t == BC_STRETCH || t == BC_TWIST || t == BC_BEND;
+}
+
+// StretchBC / TwistBC / BendBC::step: group i moves with (-1)^i times the
+// configured velocity: along the axis (stretch), or rotating by
+// (-1)^i velocity pi dt about the x axis through the bounding-box center
+// (twist; about z in 2D) or about the z axis through the group's last
+// vertex (bend). Rotations act on the current positions.
+void Mesh::update_bcs(double dt) {
+  if (!bc_moves()) return;
+  const auto t = bc_config_.type;
+  for (size_t gi = 0; gi < bc_groups_.size(); ++gi) {
+    const double sgn = gi % 2 == 0 ? 1.0 : -1.0;
+    if (t == BC_STRETCH) {
+      const double dv = sgn * bc_config_.velocity * dt;
+      for (int j : bc_groups_[gi]) V_[3 * j + bc_config_.axis] += dv;
+      continue;
+    }
+    const double a = sgn * bc_config_.velocity * M_PI * dt;
+    const double c = std::cos(a), sn = std::sin(a);
+    const double* o = t == BC_TWIST ? bc_center_ : bc_group_center_[gi].data();
+    // rotation plane: (y, z) for the 3D twist, (x, y) otherwise
+    const int p0 = (t == BC_TWIST && dim_ == 3) ? 1 : 0;
+    const int p1 = p0 + 1;
+    for (int j : bc_groups_[gi]) {
+      const double u = V_[3 * j + p0] - o[p0], w = V_[3 * j + p1] - o[p1];
+      V_[3 * j + p0] = c * u - sn * w + o[p0];
+      V_[3 * j + p1] = sn * u + c * w + o[p1];
+    }
+  }
+}
+
+std::vector<double> Mesh::pinned_positions() const {
+  std::vector<double> out(3 * pinned_.size());
+  for (size_t k = 0; k < pinned_.size(); ++k)
+    for (int d = 0; d < 3; ++d) out[3 * k + d] = V_[3 * pinned_[k] + d];
+  return out;
 }
 
 mfemhip::TetMeshDev Mesh::dev() const {

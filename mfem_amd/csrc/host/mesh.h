@@ -8,6 +8,7 @@
 // its sorted (tet, local pair) slot list and its consistent-mass coefficient;
 // for every free vertex its sorted (tet, corner) incidence list.
 #pragma once
+#include <array>
 #include <memory>
 #include <vector>
 
@@ -45,6 +46,15 @@ class Mesh {
   void init(hipStream_t st, const std::vector<int>& fixed_override = {});
 
   mfemhip::TetMeshDev dev() const;
+
+  // Scripted boundary motion (reference boundary_conditions/{stretch,twist,
+  // bend}_boundary_condition.h, applied after every timestep by
+  // Mesh::update_bcs, mesh.h:130-132): moves the pinned rows of V_.
+  bool bc_moves() const;
+  void update_bcs(double dt);
+  // pinned vertex ids (ascending) and their current positions (3 each)
+  const std::vector<int>& pinned() const { return pinned_; }
+  std::vector<double> pinned_positions() const;
   int dim() const { return dim_; }
   int nodes() const { return nodes_; }  // corners per element
   int nV() const { return nV_; }
@@ -83,6 +93,10 @@ class Mesh {
   std::vector<int> slot_ptr_, slots_, diag_block_, inc_ptr_, inc_;
   std::vector<double> mass_;
   std::vector<int> elem_type_;
+  std::vector<int> pinned_;
+  std::vector<std::vector<int>> bc_groups_;   // min and max end groups along the axis
+  double bc_center_[3] = {0, 0, 0};           // twist: bounding-box center
+  std::vector<std::array<double, 3>> bc_group_center_;  // bend: rotation center per group
 };
 
 // Pinned set of the reference's boundary conditions

@@ -170,10 +170,21 @@ void Displacement<DIM>::pre_solve() {
   integrator_->x_tilde(this->ctx_->x_tilde.data(), this->ctx_->stream);
 }
 
-// reference displacement.cpp:33-50 (pinned values are static here)
+// reference displacement.cpp:33-50: the pinned entries of x_full take the
+// mesh's (possibly scripted) pinned positions, then the integrator update.
 template <int DIM>
 void Displacement<DIM>::post_solve() {
-  integrator_->update(x_full_.data(), this->ctx_->stream);
+  auto& m = *this->mesh_;
+  hipStream_t st = this->ctx_->stream;
+  if (m.bc_moves() && !m.pinned().empty()) {
+    const auto pos = m.pinned_positions();
+    if (pinned_ids_.size() != m.pinned().size()) pinned_ids_.upload(m.pinned(), st);
+    pinned_pos_.upload(pos, st);
+    launch_scatter3((int)m.pinned().size(), pinned_ids_.data(), pinned_pos_.data(),
+                    x_full_.data(), st);
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));  // the host copies above are pageable
+  }
+  integrator_->update(x_full_.data(), st);
 }
 
 template <int DIM>
@@ -1226,7 +1237,9 @@ void NewtonOptimizer<DIM>::step() {
   stats_.energy = E;
   stats_.res = res;
   stats_.step_ms = stats_.update_ms + stats_.solve_ms + stats_.ls_ms;
-  // post-solve updates (boundary conditions are static here)
+  // reference newton_optimizer.cpp:78-87: scripted boundary motion, then the
+  // variables' post_solve
+  S.mesh_->update_bcs(S.config_->h);
   S.x_->post_solve();
   for (auto& v : S.mixed_vars_) v->post_solve();
   for (auto& v : S.vars_) v->post_solve();

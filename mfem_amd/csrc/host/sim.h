@@ -157,6 +157,8 @@ class Displacement : public Variable<DIM> {
   std::shared_ptr<SimConfig> config_;
   std::shared_ptr<ImplicitIntegrator> integrator_;
   DeviceArray<double> x_full_, dx_;
+  DeviceArray<int> pinned_ids_;      // scripted boundary motion
+  DeviceArray<double> pinned_pos_;
 };
 
 template <int DIM>

@@ -116,6 +116,22 @@ void launch_reduce_maxabs(int n, const double* v, ReduceSlot out, hipStream_t st
   launch_finalize(out.partials, g, 1, out.result, st);
 }
 
+// x[3 ids[k] + r] = pos[3k + r]: prescribed positions of pinned vertices
+__global__ __launch_bounds__(kBS) void k_scatter3(int n, const int* __restrict__ ids,
+                                                  const double* __restrict__ pos,
+                                                  double* __restrict__ x) {
+  const int k = blockIdx.x * kBS + threadIdx.x;
+  if (k >= n) return;
+  const long v = ids[k];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) x[3 * v + r] = pos[3 * k + r];
+}
+
+void launch_scatter3(int n, const int* ids, const double* pos, double* x, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter3, dim3(grid(n, kBS)), dim3(kBS), 0, st, n, ids, pos, x);
+}
+
 void launch_fill(double* p, double v, long n, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill, dim3(grid(n, kBS)), dim3(kBS), 0, st, p, v, n);

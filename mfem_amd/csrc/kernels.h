@@ -301,6 +301,8 @@ void launch_bdf_update(int n, int order, const double* const* xh, const double* 
                        const double* x_full, double* x_new, double* v_new, hipStream_t st);
 void launch_reduce_maxabs(int n, const double* v, ReduceSlot out, hipStream_t st);
 void launch_fill(double* p, double v, long n, hipStream_t st);
+// x[3 ids[k] + r] = pos[3k + r] for k < n (pinned vertices' prescribed positions)
+void launch_scatter3(int n, const int* ids, const double* pos, double* x, hipStream_t st);
 void launch_axpy(long n, double alpha, const double* x, double* y, hipStream_t st);  // y += a x
 
 // --- batched element entry points (MaterialModel / polar on many points) ---

@@ -40,7 +40,7 @@ class SimParams(C.Structure):
         ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
         ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("precision", C.c_int),
         ("bc_type", C.c_int), ("bc_axis", C.c_int), ("bc_ratio", C.c_double),
-        ("dim", C.c_int),
+        ("dim", C.c_int), ("bc_velocity", C.c_double),
     ]
 
 
@@ -127,6 +127,7 @@ def _params(sc, precision, p=None):
     """mfem_sim_params for a Scene; returns (params, arrays to keep alive)."""
     keep = dict(V=_f64(sc.V), T=_i32(sc.T), mu=_f64(sc.mu), la=_f64(sc.la), mat=_i32(sc.mat),
                 fixed=_i32(sc.fixed), solver=sc.solver.encode())
+    bc = getattr(sc, "bc", None)
     p = SimParams() if p is None else p
     p.nV, p.nT = sc.nV, sc.nT
     p.V = keep["V"].ctypes.data_as(dp)
@@ -134,7 +135,13 @@ def _params(sc, precision, p=None):
     p.mu = keep["mu"].ctypes.data_as(dp)
     p.la = keep["la"].ctypes.data_as(dp)
     p.mat = keep["mat"].ctypes.data_as(ip)
-    p.fixed = keep["fixed"].ctypes.data_as(ip)
+    if bc:  # the library derives the pins from the boundary condition (and moves them)
+        from .scenes import BCS
+        p.fixed = ip()
+        p.bc_type, p.bc_axis = BCS[bc["type"]], int(bc.get("axis", 0))
+        p.bc_ratio, p.bc_velocity = float(bc.get("ratio", 0.1)), float(bc.get("velocity", 0.1))
+    else:
+        p.fixed = keep["fixed"].ctypes.data_as(ip)
     if sc.v0 is not None:
         keep["v0"] = _f64(sc.v0)
         p.v0 = keep["v0"].ctypes.data_as(dp)

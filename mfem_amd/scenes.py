@@ -163,6 +163,9 @@ class Scene:
     newton_tol: float = 1e-10
     ls_iters: int = 20
     v0: np.ndarray | None = None
+    # boundary condition script {"type", "axis", "ratio", "velocity"}; when set,
+    # `fixed` holds the pins it implies and the end groups move after every step
+    bc: dict | None = None
     name: str = ""
     extra: dict = field(default_factory=dict)
 
@@ -234,6 +237,10 @@ def scene_from_json(path: str) -> Scene:
     mat = np.full(T.shape[0], MATERIALS[m.get("energy", "Stable-Neohookean")], dtype=np.int32)
     bc = obj.get("boundary_condition", {})
     fixed = bc_fixed(bc.get("type", "null"), V, bc.get("axis", 0), bc.get("ratio", 0.1))
+    script = None
+    if bc.get("type", "null") in ("stretch", "twist", "bend", "scale"):
+        script = dict(type=bc["type"], axis=bc.get("axis", 0), ratio=bc.get("ratio", 0.1),
+                      velocity=bc.get("velocity", 0.1))
     g = d.get("body_force", [0, -9.8, 0])
     return Scene(V=V, T=T, mu=mu, la=la, mat=mat, fixed=fixed,
                  density=mats[0].get("density", 1000.0), dt=d.get("dt", 0.034),
@@ -244,7 +251,8 @@ def scene_from_json(path: str) -> Scene:
                  itr_tol=d.get("iterative_solver_tolerance", 1e-4),
                  max_iters=d.get("max_iterative_solver_iters", 500),
                  max_newton=d.get("max_newton_iterations", 5),
-                 ls_iters=d.get("max_linesearch_iterations", 20), name=os.path.basename(path))
+                 ls_iters=d.get("max_linesearch_iterations", 20), bc=script,
+                 name=os.path.basename(path))
 
 
 # --- the benchmark configurations --------------------------------------------

@@ -64,6 +64,11 @@ typedef struct {
   double newton_tol;
   int ls_iters;
   int threads;        // OpenMP threads (<= 0: runtime default)
+  // boundary-condition script (reference config.h:70-87 numbering): 1 scale
+  // (start positions scaled 1.5x about the box center), 4 stretch, 7 bend,
+  // 8 twist (end groups moved after every timestep); others: static pins
+  int bc_type, bc_axis;
+  double bc_ratio, bc_velocity;
 } oracle_params;
 
 typedef struct {

@@ -32,6 +32,8 @@ class Params(C.Structure):
         ("bdf_order", C.c_int), ("mixed", C.c_int), ("solver", C.c_int),
         ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
         ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("threads", C.c_int),
+        ("bc_type", C.c_int), ("bc_axis", C.c_int), ("bc_ratio", C.c_double),
+        ("bc_velocity", C.c_double),
     ]
 
 
@@ -257,6 +259,11 @@ class Sim:
         p.itr_tol, p.max_iters = sc.itr_tol, sc.max_iters
         p.max_newton, p.newton_tol, p.ls_iters = sc.max_newton, sc.newton_tol, sc.ls_iters
         p.threads = threads
+        bc = getattr(sc, "bc", None)
+        if bc:
+            from mfem_amd.scenes import BCS
+            p.bc_type, p.bc_axis = BCS[bc["type"]], int(bc.get("axis", 0))
+            p.bc_ratio, p.bc_velocity = float(bc.get("ratio", 0.1)), float(bc.get("velocity", 0.1))
         self.p = p
         self.scene = sc
         self.h = lib().oracle_sim_create(C.byref(p))

@@ -190,6 +190,10 @@ struct OracleSim {
   double pcg_err_last = 0;
   // indefinite (KKT) solve: preconditioner state and the reference's
   // refresh counter (preconditioners.h:150-158)
+  // boundary-condition script: end groups, twist center, bend centers
+  std::vector<int> bc_g[2];
+  double bc_center[3] = {0, 0, 0};
+  double bc_gc[2][3] = {{0, 0, 0}, {0, 0, 0}};
   int kkt_solves = 0;
   bool kkt_mass_ready = false;
   std::vector<double> mlump, mdinv;   // per free dof: row sum and 1/diag of PMP
@@ -1158,6 +1162,33 @@ double energy_func(const OracleSim* S_, double a) {
   return val;
 }
 
+// Moves the pinned values b of the end groups: stretch translates group i by
+// (-1)^i v dt along the axis; twist rotates it by (-1)^i v pi dt about the x
+// axis through the start bounding-box center (about z in 2D); bend about the
+// z axis through the group's last vertex. Rotations act on the current b.
+void bc_step(OracleSim* S_, double dt) {
+  const int t = S_->p.bc_type, D = S_->dim;
+  if (t != 4 && t != 7 && t != 8) return;
+  for (int gi = 0; gi < 2; ++gi) {
+    const double sgn = gi == 0 ? 1.0 : -1.0;
+    if (t == 4) {
+      for (int j : S_->bc_g[gi])
+        if (S_->fixed[j]) S_->b[D * j + S_->p.bc_axis] += sgn * S_->p.bc_velocity * dt;
+      continue;
+    }
+    const double a = sgn * S_->p.bc_velocity * M_PI * dt;
+    const double c = std::cos(a), s = std::sin(a);
+    const double* o = t == 8 ? S_->bc_center : S_->bc_gc[gi];
+    const int p0 = (t == 8 && D == 3) ? 1 : 0, p1 = p0 + 1;
+    for (int j : S_->bc_g[gi]) {
+      if (S_->fixed[j] == 0) continue;
+      const double u = S_->b[D * j + p0] - o[p0], w = S_->b[D * j + p1] - o[p1];
+      S_->b[D * j + p0] = c * u - s * w + o[p0];
+      S_->b[D * j + p1] = s * u + c * w + o[p1];
+    }
+  }
+}
+
 int step(OracleSim* S_) {
   int i = 0;
   double grad_norm = 0, E = 0, E_prev = 0, res = 0;
@@ -1195,7 +1226,10 @@ int step(OracleSim* S_) {
   S_->st.last_decrement = grad_norm;
   S_->st.last_energy = E;
   S_->st.last_res = res;
-  // post_solve: Displacement (b unchanged for static pins), BDF update
+  // update_bcs (newton_optimizer.cpp:78-79): the scripted end groups move
+  // (stretch_boundary_condition.h, twist_..., bend_...; group i with
+  // (-1)^i velocity); then Displacement::post_solve takes the pinned values
+  bc_step(S_, S_->p.dt);
   std::vector<double> full;
   unproject(S_, S_->x, full);
   {
@@ -1422,10 +1456,35 @@ void* oracle_sim_create(const oracle_params* p) {
     }
     S_->asmb.update_matrix(mb, S_->PMP);
   }
+  // boundary-condition script (BoundaryCondition ctor on the rest positions,
+  // boundary_condition.cpp:5-21; init on the start positions)
+  if (p->nV > 0) {
+    const int ax = p->bc_axis;
+    std::vector<double> mn(D, 1e300), mx(D, -1e300);
+    for (int i = 0; i < p->nV; ++i)
+      for (int d = 0; d < D; ++d) {
+        mn[d] = std::min(mn[d], p->V[i * D + d]);
+        mx[d] = std::max(mx[d], p->V[i * D + d]);
+      }
+    const double range = mx[ax] - mn[ax];
+    for (int i = 0; i < p->nV; ++i) {
+      const double v = p->V[i * D + ax];
+      if (v < mn[ax] + range * p->bc_ratio) S_->bc_g[0].push_back(i);
+      else if (v > mx[ax] - range * p->bc_ratio) S_->bc_g[1].push_back(i);
+    }
+    for (int d = 0; d < D; ++d) S_->bc_center[d] = 0.5 * (mn[d] + mx[d]);
+    for (int gi = 0; gi < 2; ++gi)
+      if (!S_->bc_g[gi].empty())
+        for (int d = 0; d < D; ++d) S_->bc_gc[gi][d] = p->V[S_->bc_g[gi].back() * D + d];
+    if (p->bc_type == 1)  // ScaleBC::init
+      for (int i = 0; i < p->nV; ++i)
+        for (int d = 0; d < D; ++d)
+          S_->Vcur[i * D + d] = (S_->Vcur[i * D + d] - S_->bc_center[d]) * 1.5 + S_->bc_center[d];
+  }
   // Displacement::reset
   bdf_tables(p->bdf_order, S_->alphas, S_->beta);
   S_->h = S_->beta * p->dt;
-  std::vector<double> x0 = S_->Vref;
+  std::vector<double> x0 = S_->Vcur;
   std::vector<double> v0(x0.size(), 0.0);
   if (p->v0) v0.assign(p->v0, p->v0 + x0.size());
   for (int k = 0; k < p->bdf_order; ++k) { S_->x_prevs.push_front(x0); S_->v_prevs.push_front(v0); }

@@ -470,3 +470,55 @@ def test_2d_mixed_equals_plain_at_convergence(oracle):
         disp = np.abs(sb.positions() - b.V).max()
         assert disp > 1e-3
         assert np.abs(sa.positions() - sb.positions()).max() < 1e-6 * disp
+
+
+# ------------------------------------------------------------------ boundary scripts
+
+@pytest.mark.parametrize("kind", ["stretch", "twist", "bend"])
+def test_bc_scripts_move_pins(oracle, kind):
+    """The pinned end groups follow the reference's scripts in closed form
+    (stretch_/twist_/bend_boundary_condition.h): after n steps group i has
+    moved by (-1)^i v n dt along the axis, or rotated by (-1)^i v pi n dt about
+    the x axis through the box center (twist) or the z axis through the
+    group's last vertex (bend)."""
+    sc = scenes.beam_scene(6, 2, 2, seed=1, mixed=True, solver="eigen-llt", gravity=(0, 0, 0))
+    v, ax, ratio = 0.3, 0, 0.1
+    sc.bc = dict(type=kind, axis=ax, ratio=ratio, velocity=v)
+    sc.fixed = scenes.bc_fixed(kind, sc.V, ax, ratio)
+    sim = oracle.Sim(sc)
+    V = sc.V
+    lo, hi = V[:, ax].min(), V[:, ax].max()
+    rng = hi - lo
+    g0 = np.nonzero(V[:, ax] < lo + rng * ratio)[0]
+    g1 = np.nonzero(V[:, ax] > hi - rng * ratio)[0]
+    center = 0.5 * (V.min(0) + V.max(0))
+    n = 3
+    for _ in range(n):
+        sim.step()
+    X = sim.positions()
+    for i, g in enumerate((g0, g1)):
+        sgn = 1.0 if i == 0 else -1.0
+        if kind == "stretch":
+            exp = V[g].copy()
+            exp[:, ax] += sgn * v * n * sc.dt
+        else:
+            a = sgn * v * np.pi * n * sc.dt
+            c, s = np.cos(a), np.sin(a)
+            if kind == "twist":
+                Rm, o = np.array([[1, 0, 0], [0, c, -s], [0, s, c]]), center
+            else:
+                Rm, o = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]]), V[g[-1]]
+            exp = (V[g] - o) @ Rm.T + o
+        assert np.abs(X[g] - exp).max() < 1e-12
+    # the free vertices follow
+    assert np.abs(X - V).max() > 1e-3
+
+
+def test_scale_bc_start_configuration(oracle):
+    sc = scenes.beam_scene(4, 2, 2, seed=1, mixed=True, solver="eigen-llt", gravity=(0, 0, 0),
+                           dt=1e-4)
+    sc.bc = dict(type="scale", axis=0, ratio=0.1, velocity=0.0)
+    sc.fixed = np.zeros(sc.nV, dtype=np.int32)
+    sim = oracle.Sim(sc)
+    c = 0.5 * (sc.V.min(0) + sc.V.max(0))
+    assert np.abs(sim.positions() - ((sc.V - c) * 1.5 + c)).max() < 1e-14
