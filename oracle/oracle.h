@@ -57,7 +57,9 @@ typedef struct {
   int solver;         // 0: dense LLT ("eigen-llt"), 1: Jacobi CG ("eigen-pcg-diag"),
                       // 2: MINRES on the KKT system ("minres-indefinite-block") with the
                       //    GPU form of its block preconditioner, 3: the same MINRES with
-                      //    the reference's exact factorizations (dense; small meshes)
+                      //    the reference's exact factorizations (dense; small meshes),
+                      // 4: CG with the incomplete-Cholesky preconditioner of the
+                      //    device ("eigen-pcg-IC": multicolour block IC(0))
   double itr_tol;
   int max_iters;
   int max_newton;
@@ -130,6 +132,10 @@ int oracle_kkt_precond(void* sim, int exact, const double* b, double* z);
 // The MINRES iteration on a dense symmetric A with a dense SPD preconditioner
 // P (z = P r), both row-major N x N. Returns the estimated relative residual
 // (negative if P is not positive definite).
+int oracle_ic_factor_dense(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                           double* L, double* scale, double* shift, int* color, int* attempts);
+int oracle_ic_pcg(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                  const double* b, double* x, double tol, int max_iters, double* err);
 double oracle_minres_dense(int N, const double* A, const double* P, const double* b, double* x,
                            int max_iters, double tol, int* iters);
 

@@ -16,7 +16,7 @@ _LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 _lib = None
 
 SOLVERS = {"eigen-llt": 0, "eigen-ldlt": 0, "cholmod": 0, "eigen-lu": 0, "eigen-pcg-diag": 1,
-           "minres-indefinite-block": 2}
+           "minres-indefinite-block": 2, "eigen-pcg-IC": 4}
 # MINRES with the reference's exact (dense) preconditioner factorizations
 SOLVER_MINRES_EXACT = 3
 
@@ -97,6 +97,10 @@ def lib():
         L.oracle_kkt_precond.restype = C.c_int
         L.oracle_minres_dense.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, C.c_double, ip]
         L.oracle_minres_dense.restype = C.c_double
+        L.oracle_ic_factor_dense.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, ip, ip]
+        L.oracle_ic_factor_dense.restype = C.c_int
+        L.oracle_ic_pcg.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.oracle_ic_pcg.restype = C.c_int
         _lib = L
     return _lib
 
@@ -353,3 +357,41 @@ def minres_dense(A, P, b, max_iters, tol):
     err = lib().oracle_minres_dense(N, _dp(A), _dp(P), _dp(b), _dp(x), int(max_iters), float(tol),
                                     C.byref(it))
     return x, it.value, err
+
+
+def _csr(rowptr, colind, vals):
+    return (np.ascontiguousarray(rowptr, dtype=np.int32), np.ascontiguousarray(colind, dtype=np.int32),
+            f64(vals))
+
+
+def ic_factor_dense(rowptr, colind, vals, bs):
+    """The "eigen-pcg-IC" preconditioner of a scalar CSR matrix with bs x bs
+    vertex blocks: (L in the original numbering as a dense array, scale,
+    shift, vertex colours, attempts). Raises on a factor that breaks down ten
+    times."""
+    rp, ci, va = _csr(rowptr, colind, vals)
+    n = rp.shape[0] - 1
+    L = np.zeros((n, n))
+    scale = np.zeros(n)
+    shift = np.zeros(1)
+    color = np.zeros(n // bs, dtype=np.int32)
+    att = np.zeros(1, dtype=np.int32)
+    rc = lib().oracle_ic_factor_dense(n, int(bs), _ip(rp), _ip(ci), _dp(va), _dp(L), _dp(scale),
+                                      _dp(shift), _ip(color), _ip(att))
+    if rc != 0:
+        raise RuntimeError(f"incomplete Cholesky broke down after {att[0]} attempts")
+    return L, scale, float(shift[0]), color, int(att[0])
+
+
+def ic_pcg(rowptr, colind, vals, bs, b, tol, max_iters):
+    """Eigen's CG (x0 = 0) with that preconditioner: (x, iterations, error)."""
+    rp, ci, va = _csr(rowptr, colind, vals)
+    n = rp.shape[0] - 1
+    b = f64(b)
+    x = np.zeros(n)
+    err = np.zeros(1)
+    it = lib().oracle_ic_pcg(n, int(bs), _ip(rp), _ip(ci), _dp(va), _dp(b), _dp(x), float(tol),
+                             int(max_iters), _dp(err))
+    if it < 0:
+        raise RuntimeError("incomplete Cholesky broke down")
+    return x, it, float(err[0])

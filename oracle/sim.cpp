@@ -188,6 +188,8 @@ struct OracleSim {
   oracle_stats st{};
   int pcg_iters_last = 0;
   double pcg_err_last = 0;
+  double ic_shift_last = 0;  // "eigen-pcg-IC": diagonal shift and colours of the last factor
+  int ic_colors_last = 0;
   // indefinite (KKT) solve: preconditioner state and the reference's
   // refresh counter (preconditioners.h:150-158)
   // boundary-condition script: end groups, twist center, bend centers
@@ -677,6 +679,220 @@ int pcg(OracleSim* S_, std::vector<double>& xo) {
   return 0;
 }
 
+// ---------------- "eigen-pcg-IC" ----------------
+// The reference runs Eigen 3.4's ConjugateGradient with IncompleteCholesky
+// (linear_solver_factory.cpp:87-97; Eigen is not vendored in the reference).
+// That preconditioner (Eigen/src/IterativeLinearSolvers/IncompleteCholesky.h,
+// after Lin & More, SIAM J. Sci. Comput. 21(1), 1999) orders the matrix by
+// AMD, scales it symmetrically by 1/sqrt(|column|_2), shifts the diagonal by
+// 1e-3 - min diag when that is not positive, factors with as many kept
+// entries per column as the column had, and on a non-positive pivot restarts
+// with the shift max(1e-3, 2 shift), at most 10 attempts. The device keeps the
+// scaling and the shift loop and replaces the ordering and fill rule with
+// IC(0) on the 3x3 vertex blocks in greedy multicolour order (vertices of one
+// colour share no matrix entry, so a colour is one parallel step of the
+// factorization and of both triangular solves). This is that algorithm, in
+// scalar form, as the parity reference for the device.
+struct IcFactor {
+  int n = 0, bs = 1, nb = 0, ncolors = 0, attempts = 0;
+  std::vector<int> color, order, pos;  // vertex colour; vertex order; vertex -> position
+  std::vector<double> scale;           // n
+  double shift = 0;
+  // L by rows in the permuted scalar numbering p = pos[v] bs + d: strictly
+  // lower entries (sorted) in lptr/lcol/lval, the diagonal in ldiag
+  std::vector<int> lptr, lcol;
+  std::vector<double> lval, ldiag;
+};
+
+// Greedy first-fit colouring of the vertex graph of a block pattern, in
+// vertex order. Returns the number of colours.
+int ic_color(int nb, const std::vector<int>& browptr, const std::vector<int>& bcol,
+             std::vector<int>& color) {
+  color.assign(nb, -1);
+  std::vector<int> mark;  // mark[c] == v: colour c taken by a neighbour of v
+  int ncol = 0;
+  for (int v = 0; v < nb; ++v) {
+    for (int k = browptr[v]; k < browptr[v + 1]; ++k) {
+      const int u = bcol[k];
+      if (u != v && color[u] >= 0) {
+        if ((int)mark.size() <= color[u]) mark.resize(color[u] + 1, -1);
+        mark[color[u]] = v;
+      }
+    }
+    int c = 0;
+    while (c < (int)mark.size() && mark[c] == v) ++c;
+    color[v] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return ncol;
+}
+
+// Block pattern of a scalar CSR whose rows come in groups of bs (every
+// block dense in the pattern, as the assembled K is).
+void ic_block_pattern(int n, int bs, const int* rowptr, const int* colind, std::vector<int>& bp,
+                      std::vector<int>& bc) {
+  const int nb = n / bs;
+  bp.assign(nb + 1, 0);
+  bc.clear();
+  for (int v = 0; v < nb; ++v) {
+    const int r = v * bs;
+    for (int k = rowptr[r]; k < rowptr[r + 1]; k += bs) bc.push_back(colind[k] / bs);
+    bp[v + 1] = (int)bc.size();
+  }
+}
+
+// One IC(0) attempt of the scaled, shifted matrix; false on a non-positive
+// pivot. Up-looking: row p of L from the rows before it, within the pattern.
+bool ic_attempt(IcFactor& f, const int* rowptr, const int* colind, const double* vals) {
+  const int n = f.n, bs = f.bs;
+  std::vector<double> w(n, 0.0);
+  std::vector<char> in(n, 0);
+  std::vector<int> cols;
+  f.lptr.assign(n + 1, 0);
+  f.lcol.clear();
+  f.lval.clear();
+  f.ldiag.assign(n, 0.0);
+  for (int p = 0; p < n; ++p) {
+    const int i = f.order[p / bs] * bs + p % bs;  // original scalar row
+    double diag = 0;
+    cols.clear();
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int j = colind[k];
+      const int q = f.pos[j / bs] * bs + j % bs;
+      const double a = vals[k] * f.scale[i] * f.scale[j];
+      if (q == p) diag = a + f.shift;
+      else if (q < p) { w[q] = a; in[q] = 1; cols.push_back(q); }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (int q : cols) {
+      double s = w[q];
+      for (int k = f.lptr[q]; k < f.lptr[q + 1]; ++k)
+        if (in[f.lcol[k]]) s -= f.lval[k] * w[f.lcol[k]];
+      w[q] = s / f.ldiag[q];
+    }
+    for (int q : cols) diag -= w[q] * w[q];
+    if (!(diag > 0)) {
+      for (int q : cols) { w[q] = 0; in[q] = 0; }
+      return false;
+    }
+    f.ldiag[p] = std::sqrt(diag);
+    for (int q : cols) {
+      f.lcol.push_back(q);
+      f.lval.push_back(w[q]);
+      w[q] = 0;
+      in[q] = 0;
+    }
+    f.lptr[p + 1] = (int)f.lcol.size();
+  }
+  return true;
+}
+
+// Colouring, ordering, scaling and the shift loop. Returns 0, or -5 when ten
+// attempts break down (Eigen then reports NumericalIssue).
+int ic_factor(IcFactor& f, int n, int bs, const int* rowptr, const int* colind,
+              const double* vals) {
+  f.n = n;
+  f.bs = bs;
+  f.nb = n / bs;
+  std::vector<int> bp, bc;
+  ic_block_pattern(n, bs, rowptr, colind, bp, bc);
+  f.ncolors = ic_color(f.nb, bp, bc, f.color);
+  f.order.clear();
+  for (int c = 0; c < f.ncolors; ++c)
+    for (int v = 0; v < f.nb; ++v)
+      if (f.color[v] == c) f.order.push_back(v);
+  f.pos.assign(f.nb, 0);
+  for (int p = 0; p < f.nb; ++p) f.pos[f.order[p]] = p;
+  // scale_j = |column j|_2^(-1/2) (1 when the column is zero)
+  f.scale.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double s = 0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s += vals[k] * vals[k];
+    s = std::sqrt(std::sqrt(s));
+    f.scale[i] = s > 2.2250738585072014e-308 ? 1.0 / s : 1.0;
+  }
+  double mindiag = 1.79769313486231570e308;
+  for (int i = 0; i < n; ++i)
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (colind[k] == i) mindiag = std::min(mindiag, vals[k] * f.scale[i] * f.scale[i]);
+  f.shift = mindiag <= 0 ? 1e-3 - mindiag : 0.0;
+  for (f.attempts = 1;; ++f.attempts) {
+    if (ic_attempt(f, rowptr, colind, vals)) return 0;
+    if (f.attempts >= 10) return -5;
+    f.shift = std::max(1e-3, 2 * f.shift);
+  }
+}
+
+// z = S P^T L^-T L^-1 P S r
+void ic_apply(const IcFactor& f, const double* r, double* z) {
+  const int n = f.n, bs = f.bs;
+  std::vector<double> y(n);
+  for (int p = 0; p < n; ++p) {
+    const int i = f.order[p / bs] * bs + p % bs;
+    double s = f.scale[i] * r[i];
+    for (int k = f.lptr[p]; k < f.lptr[p + 1]; ++k) s -= f.lval[k] * y[f.lcol[k]];
+    y[p] = s / f.ldiag[p];
+  }
+  for (int p = n - 1; p >= 0; --p) {
+    y[p] /= f.ldiag[p];
+    for (int k = f.lptr[p]; k < f.lptr[p + 1]; ++k) y[f.lcol[k]] -= f.lval[k] * y[p];
+  }
+  for (int p = 0; p < n; ++p) {
+    const int i = f.order[p / bs] * bs + p % bs;
+    z[i] = f.scale[i] * y[p];
+  }
+}
+
+// Eigen's ConjugateGradient loop (x0 = 0) with a preconditioner callback.
+template <class Mult, class Pre>
+int cg_loop(int n, Mult mult, Pre pre, const double* bb, double* xo, double tol, int max_iters,
+            double* err) {
+  std::vector<double> b(bb, bb + n);
+  std::fill(xo, xo + n, 0.0);
+  const double rhsNorm2 = dotv(b, b);
+  if (rhsNorm2 == 0) { *err = 0; return 0; }
+  const double threshold = std::max(tol * tol * rhsNorm2, 2.2250738585072014e-308);
+  std::vector<double> r = b, z(n), p(n), tmp(n);
+  double rn2 = dotv(r, r);
+  if (rn2 < threshold) { *err = std::sqrt(rn2 / rhsNorm2); return 0; }
+  pre(r.data(), p.data());
+  double absNew = dotv(r, p);
+  int it = 0;
+  while (it < max_iters) {
+    mult(p, tmp);
+    const double alpha = absNew / dotv(p, tmp);
+    for (int i = 0; i < n; ++i) { xo[i] += alpha * p[i]; r[i] -= alpha * tmp[i]; }
+    rn2 = dotv(r, r);
+    if (rn2 < threshold) break;
+    pre(r.data(), z.data());
+    const double absOld = absNew;
+    absNew = dotv(r, z);
+    const double beta = absNew / absOld;
+    for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  *err = std::sqrt(rn2 / rhsNorm2);
+  return it;
+}
+
+int ic_pcg(OracleSim* S_, std::vector<double>& xo) {
+  const int n = S_->n;
+  IcFactor f;
+  int rc = ic_factor(f, n, S_->dim, S_->asmb.rowptr.data(), S_->asmb.colind.data(),
+                     S_->lhs.data());
+  if (rc != 0) return rc;
+  S_->ic_shift_last = f.shift;
+  S_->ic_colors_last = f.ncolors;
+  xo.assign(n, 0.0);
+  auto mult = [&](const std::vector<double>& v, std::vector<double>& y) {
+    csr_mult(S_, S_->lhs, v, y);
+  };
+  auto pre = [&](const double* r, double* z) { ic_apply(f, r, z); };
+  S_->pcg_iters_last = cg_loop(n, mult, pre, S_->rhs.data(), xo.data(), S_->p.itr_tol,
+                               S_->p.max_iters, &S_->pcg_err_last);
+  return 0;
+}
+
 // dense Cholesky (stands in for the reference's SimplicialLLT / CHOLMOD;
 // exact-arithmetic solution is identical). Returns -1 if not PD.
 int dense_llt(OracleSim* S_, std::vector<double>& xo) {
@@ -1098,7 +1314,7 @@ void update_system(OracleSim* S_) {
 }
 
 int substep(OracleSim* S_, double& decrement) {
-  if (S_->p.solver >= 2) {
+  if (S_->p.solver == 2 || S_->p.solver == 3) {
     // indefinite system (mixed scenes only)
     if (!S_->p.mixed || S_->dim != 3) return -3;
     int rc = kkt_solve(S_);
@@ -1128,7 +1344,7 @@ int substep(OracleSim* S_, double& decrement) {
     for (int i = 0; i < S_->n; ++i) S_->rhs[i] += -S_->grad_plain[i];
   }
   std::vector<double> sol;
-  int rc = (S_->p.solver == 0) ? dense_llt(S_, sol) : pcg(S_, sol);
+  int rc = S_->p.solver == 0 ? dense_llt(S_, sol) : S_->p.solver == 4 ? ic_pcg(S_, sol) : pcg(S_, sol);
   if (rc != 0) return rc;
   S_->st.pcg_iters_total += S_->pcg_iters_last;
   // post_solve
@@ -1591,6 +1807,7 @@ long oracle_sim_size(void* sim, const char* name) {
   if (k == "dX") return (long)S_->dX.size();
   if (k == "positions") return (long)S_->Vcur.size();
   if (k == "x_tilde") return (long)S_->Vcur.size();
+  if (k == "ic_shift" || k == "pcg_error") return 1;
   return -1;
 }
 
@@ -1635,6 +1852,8 @@ int oracle_sim_get(void* sim, const char* name, double* out) {
   if (k == "dX") { std::copy(S_->dX.begin(), S_->dX.end(), out); return 0; }
   if (k == "positions") { std::copy(S_->Vcur.begin(), S_->Vcur.end(), out); return 0; }
   if (k == "x_tilde") { auto v = x_tilde(S_); std::copy(v.begin(), v.end(), out); return 0; }
+  if (k == "ic_shift") { *out = S_->ic_shift_last; return 0; }
+  if (k == "pcg_error") { *out = S_->pcg_err_last; return 0; }
   return -1;
 }
 
@@ -1689,7 +1908,50 @@ int oracle_sim_get_int(void* sim, const char* name, int* out) {
   if (k == "colind") { std::copy(S_->asmb.colind.begin(), S_->asmb.colind.end(), out); return 0; }
   if (k == "free_map") { std::copy(S_->free_map.begin(), S_->free_map.end(), out); return 0; }
   if (k == "pcg_iters") { *out = S_->pcg_iters_last; return 0; }
+  if (k == "ic_colors") { *out = S_->ic_colors_last; return 0; }
   return -1;
+}
+
+// "eigen-pcg-IC" preconditioner on a scalar CSR matrix with bs x bs vertex
+// blocks: the factor as a dense n x n matrix in the original numbering
+// (L_orig = P^T L P, lower triangular in the colour order), the scaling,
+// the final shift, the vertex colours and the number of attempts.
+int oracle_ic_factor_dense(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                           double* L, double* scale, double* shift, int* color, int* attempts) {
+  IcFactor f;
+  int rc = ic_factor(f, n, bs, rowptr, colind, vals);
+  *attempts = f.attempts;
+  *shift = f.shift;
+  for (int v = 0; v < f.nb; ++v) color[v] = f.color[v];
+  if (rc != 0) return rc;
+  std::copy(f.scale.begin(), f.scale.end(), scale);
+  std::fill(L, L + (size_t)n * n, 0.0);
+  auto orig = [&](int p) { return f.order[p / bs] * bs + p % bs; };
+  for (int p = 0; p < n; ++p) {
+    L[(size_t)orig(p) * n + orig(p)] = f.ldiag[p];
+    for (int k = f.lptr[p]; k < f.lptr[p + 1]; ++k)
+      L[(size_t)orig(p) * n + orig(f.lcol[k])] = f.lval[k];
+  }
+  return 0;
+}
+
+// Eigen's CG with that preconditioner on a scalar CSR matrix (x0 = 0).
+// Returns the iteration count, or a negative code when the factor fails.
+int oracle_ic_pcg(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                  const double* b, double* x, double tol, int max_iters, double* err) {
+  IcFactor f;
+  int rc = ic_factor(f, n, bs, rowptr, colind, vals);
+  if (rc != 0) return rc;
+  auto mult = [&](const std::vector<double>& v, std::vector<double>& y) {
+    y.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double a = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) a += vals[k] * v[colind[k]];
+      y[i] = a;
+    }
+  };
+  auto pre = [&](const double* r, double* z) { ic_apply(f, r, z); };
+  return cg_loop(n, mult, pre, b, x, tol, max_iters, err);
 }
 
 }  // extern "C"
