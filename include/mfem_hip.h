@@ -36,7 +36,10 @@ typedef struct {
   double gravity[3];      /* "body_force" (stored as float, as the reference does) */
   int bdf_order;          /* "time_integrator": BDF1..BDF6 */
   int mixed;              /* 1: "mixed_variables": ["mixed-stretch"]; 0: "variables": ["stretch"] */
-  const char* linear_solver; /* "linear_solver": "eigen-pcg-diag" (Schur complement, CG) or
+  const char* linear_solver; /* "linear_solver": "eigen-pcg-diag" (Schur complement, Jacobi
+                              * CG), "eigen-pcg-IC" (Schur complement, CG with a multicolour
+                              * block incomplete Cholesky), "eigen-llt", "eigen-ldlt",
+                              * "cholmod", "eigen-lu" (dense factorization on the device) or
                               * "minres-indefinite-block" (KKT system, MINRES; mixed only) */
   double itr_tol;         /* "iterative_solver_tolerance" */
   int max_iters;          /* "max_iterative_solver_iters" */
@@ -97,12 +100,14 @@ int mfem_sim_get_positions(mfem_sim* sim, double* out);
  * values in the reference's row order), "rhs", "dx" (3/free vertex),
  * "x" (3/vertex), "x_tilde", "mass" (per nonzero block), "cg_stamps"
  * (profiling: per-workgroup start/end of the last two CG launches, in us
- * from the earliest stamp, [2][grid][2]). */
+ * from the earliest stamp, [2][grid][2]), "ic_shift" ("eigen-pcg-IC":
+ * diagonal shift of the last factorization). */
 long mfem_sim_size(mfem_sim* sim, const char* name);
 int mfem_sim_get(mfem_sim* sim, const char* name, double* out);
 /* "rowptr", "colind" (scalar CSR pattern), "free_map", "fixed",
  * "bsr_rowptr", "bsr_col", "slot_ptr", "slots", "pcg_iters" (CG iterations
- * of the last solve) */
+ * of the last solve); "eigen-pcg-IC": "ic_color" (colour of every free
+ * vertex), "ic_colors", "ic_attempts" (factorizations of the last solve) */
 int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out);
 /* bit 0: time every CG kernel launch with per-workgroup device clock
  * stamps (span of the launch, summed in the stats as spmv_ms/spmv_count) */

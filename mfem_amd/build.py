@@ -17,6 +17,7 @@ LIB = os.path.join(OUT_DIR, "libmfem_hip.so")
 
 SOURCES = [
     "k_element.hip",
+    "k_ic.hip",
     "k_minres.hip",
     "k_solver.hip",
     "k_vector.hip",

@@ -114,10 +114,17 @@ std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
   if (k == "x_tilde") return S.ctx_->x_tilde.to_host(st);
   if (k == "dx") return S.x_->delta().to_host(st);
   auto* pcg = dynamic_cast<PcgSolver<3>*>(sim->opt->solver());
+  auto* ic = dynamic_cast<IcPcgSolver<3>*>(sim->opt->solver());
+  if (k == "ic_shift") {
+    if (!ic) throw std::runtime_error("the linear solver is not \"eigen-pcg-IC\"");
+    return {ic->shift()};
+  }
   if (k == "rhs" || k == "lhs") {
-    if (!pcg) throw std::runtime_error("no assembled system");
-    if (k == "rhs") return pcg->system().rhs.to_host(st);
-    auto Kb = pcg->system().K.to_host_double(st);
+    SystemMatrixPD<3>* sys = pcg ? &pcg->system() : ic ? &ic->system() : nullptr;
+    if (auto* d = dynamic_cast<DirectSolver<3>*>(sim->opt->solver())) sys = &d->system();
+    if (!sys) throw std::runtime_error("no assembled system");
+    if (k == "rhs") return sys->rhs.to_host(st);
+    auto Kb = sys->K.to_host_double(st);
     std::vector<double> o;
     o.reserve(Kb.size());
     for (int i = 0; i < m.nfree(); ++i)
@@ -149,6 +156,12 @@ std::vector<int> fetch_int(mfem_sim* sim, const std::string& k) {
   if (k == "slot_ptr") return m.d_slot_ptr.to_host(st);
   if (k == "slots") return m.d_slots.to_host(st);
   if (k == "pcg_iters") return {sim->opt->solver()->stats().iterations};
+  if (k == "ic_color" || k == "ic_colors" || k == "ic_attempts") {
+    auto* ic = dynamic_cast<IcPcgSolver<3>*>(sim->opt->solver());
+    if (!ic) throw std::runtime_error("the linear solver is not \"eigen-pcg-IC\"");
+    if (k == "ic_color") return ic->vertex_colors();
+    return {k == "ic_colors" ? ic->colors() : ic->attempts()};
+  }
   if (k == "rowptr" || k == "colind") {
     std::vector<int> rp(1, 0), ci;
     for (int i = 0; i < m.nfree(); ++i)
@@ -323,7 +336,8 @@ long mfem_sim_size(mfem_sim* sim, const char* name) {
   int rc = guarded([&] {
     const std::string k(name);
     if (k == "rowptr" || k == "colind" || k == "free_map" || k == "fixed" || k == "bsr_rowptr" ||
-        k == "bsr_col" || k == "slot_ptr" || k == "slots" || k == "pcg_iters")
+        k == "bsr_col" || k == "slot_ptr" || k == "slots" || k == "pcg_iters" ||
+        k == "ic_color" || k == "ic_colors" || k == "ic_attempts")
       n = (long)fetch_int(sim, k).size();
     else
       n = (long)fetch(sim, k).size();

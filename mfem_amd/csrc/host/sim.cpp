@@ -798,6 +798,213 @@ void DirectSolver<DIM>::solve() {
   system_.post_solve(this->state_, x_.data());
 }
 
+// ============================================================ IC-CG
+template <int DIM>
+IcPcgSolver<DIM>::IcPcgSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
+  pinned_ = std::make_unique<PinnedBuffer>(2 * sizeof(IcCtrl) + 64);
+  ctrl_.resize(1);
+  fail_.resize(1);
+  for (auto& e : poll_ev_) MFEM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+template <int DIM>
+IcPcgSolver<DIM>::~IcPcgSolver() {
+  for (auto e : poll_ev_)
+    if (e) (void)hipEventDestroy(e);
+}
+
+// Pattern work, once per simulation (the BSR pattern is fixed): greedy
+// first-fit colouring in vertex order, the colour order, and the lower and
+// upper block lists of every row.
+template <int DIM>
+void IcPcgSolver<DIM>::setup() {
+  auto& m = *this->state_->mesh_;
+  hipStream_t st = this->state_->ctx_->stream;
+  const int nb = m.nfree();
+  const auto& rp = m.rowptr_;
+  const auto& cl = m.col_;
+  color_.assign(nb, -1);
+  std::vector<int> mark;
+  int ncol = 0;
+  for (int v = 0; v < nb; ++v) {
+    for (int k = rp[v]; k < rp[v + 1]; ++k) {
+      const int u = cl[k];
+      if (u != v && color_[u] >= 0) {
+        if ((int)mark.size() <= color_[u]) mark.resize(color_[u] + 1, -1);
+        mark[color_[u]] = v;
+      }
+    }
+    int c = 0;
+    while (c < (int)mark.size() && mark[c] == v) ++c;
+    color_[v] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  color_ptr_.assign(ncol + 1, 0);
+  for (int v = 0; v < nb; ++v) color_ptr_[color_[v] + 1]++;
+  for (int c = 0; c < ncol; ++c) color_ptr_[c + 1] += color_ptr_[c];
+  std::vector<int> rows(nb), pos(nb), fill(color_ptr_.begin(), color_ptr_.end() - 1);
+  for (int v = 0; v < nb; ++v) rows[fill[color_[v]]++] = v;  // by (colour, index)
+  for (int p = 0; p < nb; ++p) pos[rows[p]] = p;
+  std::vector<int> lo_ptr(nb + 1, 0), lo_col, lo_pos, lo_k;
+  std::vector<std::array<int, 3>> ent;  // (position, column, K block)
+  for (int i = 0; i < nb; ++i) {
+    ent.clear();
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      if (pos[cl[k]] < pos[i]) ent.push_back({pos[cl[k]], cl[k], k});
+    std::sort(ent.begin(), ent.end());
+    for (const auto& e : ent) {
+      lo_pos.push_back(e[0]);
+      lo_col.push_back(e[1]);
+      lo_k.push_back(e[2]);
+    }
+    lo_ptr[i + 1] = (int)lo_col.size();
+  }
+  // upper lists: row i gathers L_ji from the rows j after it
+  std::vector<int> up_ptr(nb + 1, 0), up_col(lo_col.size()), up_l(lo_col.size());
+  for (int i : lo_col) up_ptr[i + 1]++;
+  for (int i = 0; i < nb; ++i) up_ptr[i + 1] += up_ptr[i];
+  std::vector<int> up_fill(up_ptr.begin(), up_ptr.end() - 1);
+  for (int p = 0; p < nb; ++p) {  // rows j in colour order: each upper list sorted by position
+    const int j = rows[p];
+    for (int a = lo_ptr[j]; a < lo_ptr[j + 1]; ++a) {
+      const int i = lo_col[a];
+      up_col[up_fill[i]] = j;
+      up_l[up_fill[i]++] = a;
+    }
+  }
+  color_rows_.upload(rows, st);
+  lo_ptr_.upload(lo_ptr, st);
+  lo_col_.upload(lo_col, st);
+  lo_pos_.upload(lo_pos, st);
+  lo_k_.upload(lo_k, st);
+  up_ptr_.upload(up_ptr, st);
+  up_col_.upload(up_col, st);
+  up_l_.upload(up_l, st);
+  L_.resize(9 * std::max<size_t>(lo_col.size(), 1));
+  Dinv_.resize(9 * (size_t)nb);
+  scale_.resize(3 * (size_t)nb);
+  vecs_.resize(6 * 3 * (size_t)nb);
+  parts_.resize(std::max(ic_partials(nb), 1));
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+  ready_ = true;
+}
+
+template <int DIM>
+IcFactorDev IcPcgSolver<DIM>::factor_dev() {
+  auto& m = *this->state_->mesh_;
+  IcFactorDev f;
+  f.nfree = m.nfree();
+  f.dim = m.dim();
+  f.rowptr = m.d_rowptr.data();
+  f.col = m.d_col.data();
+  f.diag = m.d_diag_block.data();
+  f.color_rows = color_rows_.data();
+  f.lo_ptr = lo_ptr_.data();
+  f.lo_col = lo_col_.data();
+  f.lo_pos = lo_pos_.data();
+  f.lo_k = lo_k_.data();
+  f.up_ptr = up_ptr_.data();
+  f.up_col = up_col_.data();
+  f.up_l = up_l_.data();
+  f.L = L_.data();
+  f.Dinv = Dinv_.data();
+  f.scale = scale_.data();
+  f.fail = fail_.data();
+  return f;
+}
+
+template <int DIM>
+template <typename MT>
+void IcPcgSolver<DIM>::run(const MT* K) {
+  auto& cfg = *this->state_->config_;
+  hipStream_t st = this->state_->ctx_->stream;
+  const IcFactorDev f = factor_dev();
+  const size_t n3 = 3 * (size_t)f.nfree;
+  if (n3 == 0) {  // every vertex pinned: dx is empty
+    last_ = IcCtrl{};
+    attempts_ = 0;
+    return;
+  }
+  auto* snap = pinned_->as<IcCtrl>();
+  int* flag = reinterpret_cast<int*>(snap + 2);
+  // scaling, then factorizations with the shift loop of Eigen's
+  // IncompleteCholesky::factorize (initial shift 1e-3 - min diag when that
+  // is not positive; on a failed pivot max(1e-3, 2 shift); 10 attempts)
+  launch_ic_scale<MT>(f, K, parts_.data(), ctrl_.data(), st);
+  MFEM_HIP_CHECK(hipMemcpyAsync(snap, ctrl_.data(), sizeof(IcCtrl), hipMemcpyDeviceToHost, st));
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
+  const double mindiag = snap[0].mindiag;
+  shift_ = mindiag <= 0 ? 1e-3 - mindiag : 0.0;
+  for (attempts_ = 1;; ++attempts_) {
+    fail_.zero(st);
+    launch_ic_factor<MT>(f, K, shift_, color_ptr_.data(), colors(), st);
+    MFEM_HIP_CHECK(hipGetLastError());
+    MFEM_HIP_CHECK(hipMemcpyAsync(flag, fail_.data(), sizeof(int), hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));
+    if (*flag == 0) break;
+    if (attempts_ >= 10)
+      throw std::runtime_error("eigen-pcg-IC: the incomplete Cholesky factorization broke down "
+                               "with every shift up to " + std::to_string(shift_));
+    shift_ = std::max(1e-3, 2 * shift_);
+  }
+  // CG: one chunk of iterations queued behind the one being polled
+  IcVectors v;
+  double* w = vecs_.data();
+  v.b = system_.rhs.data();
+  v.x = w;
+  v.r = w + n3;
+  v.p = w + 2 * n3;
+  v.q = w + 3 * n3;
+  v.y = w + 4 * n3;
+  v.zt = w + 5 * n3;
+  v.parts = parts_.data();
+  v.ctrl = ctrl_.data();
+  launch_ic_cg_init(f, v, cfg.itr_tol, cfg.max_iterative_solver_iters, color_ptr_.data(), colors(),
+                    st);
+  const int poll = std::max(1, cfg.pcg_poll);
+  const long limit = std::max(cfg.max_iterative_solver_iters, 0);
+  long launched = 0;
+  int issued = 0, waited = 0;
+  auto issue = [&] {
+    const long chunk = std::min<long>(poll, limit - launched);
+    for (long j = 0; j < chunk; ++j)
+      launch_ic_cg_iteration<MT>(f, K, v, color_ptr_.data(), colors(), st);
+    launched += chunk;
+    MFEM_HIP_CHECK(hipGetLastError());
+    const int slot = issued & 1;
+    MFEM_HIP_CHECK(hipMemcpyAsync(snap + slot, ctrl_.data(), sizeof(IcCtrl),
+                                  hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[slot], st));
+    ++issued;
+  };
+  issue();
+  while (true) {
+    if (launched < limit && issued - waited < 2) issue();
+    const int slot = waited & 1;
+    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[slot]));
+    ++waited;
+    if (snap[slot].done) {  // set at the latest by iteration max_iters
+      last_ = snap[slot];
+      break;
+    }
+    if (waited == issued) throw std::runtime_error("IC-CG control block did not terminate");
+  }
+}
+
+// reference eigen_iterative_solver.h:20-29 (compute + solve, x0 = 0)
+template <int DIM>
+void IcPcgSolver<DIM>::solve() {
+  system_.pre_solve(this->state_);
+  if (!ready_) setup();
+  this->stats_ = SolveStats();
+  if (system_.K.fp32()) run<float>(system_.K.f32());
+  else run<double>(system_.K.f64());
+  this->stats_.iterations = last_.iter;
+  this->stats_.refinements = 1;
+  this->stats_.rel_residual = last_.b2 > 0 ? std::sqrt(last_.rr / last_.b2) : 0.0;
+  system_.post_solve(this->state_, vecs_.data());
+}
+
 // ============================================================ MINRES
 template <int DIM>
 MinresSolver<DIM>::MinresSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
@@ -1043,6 +1250,10 @@ LinearSolverFactory<DIM>::LinearSolverFactory() {
   this->register_type(SOLVER_EIGEN_CG_DIAG, "eigen-pcg-diag",
                       [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
                         return std::make_unique<PcgSolver<DIM>>(s);
+                      });
+  this->register_type(SOLVER_EIGEN_CG_IC, "eigen-pcg-IC",
+                      [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
+                        return std::make_unique<IcPcgSolver<DIM>>(s);
                       });
   this->register_type(SOLVER_MINRES_ID, "minres-indefinite-block",
                       [](SimState<DIM>* s) -> std::unique_ptr<LinearSolver<DIM>> {
@@ -1572,6 +1783,7 @@ template class MixedVariableFactory<3>;
 template class SystemMatrixPD<3>;
 template class PcgSolver<3>;
 template class DirectSolver<3>;
+template class IcPcgSolver<3>;
 template class MinresSolver<3>;
 template class LinearSolverFactory<3>;
 template class Optimizer<3>;

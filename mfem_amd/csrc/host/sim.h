@@ -343,6 +343,42 @@ class DirectSolver : public LinearSolver<DIM> {
   DeviceArray<int> ipiv_, info_;
 };
 
+// "eigen-pcg-IC": CG with an incomplete-Cholesky preconditioner (reference
+// linear_solver_factory.cpp:87-97, Eigen's ConjugateGradient with
+// IncompleteCholesky). The factor is IC(0) on the 3x3 vertex blocks of K in
+// greedy multicolour order, with Eigen's row scaling and diagonal-shift
+// restarts (k_ic.hip), refactored on every call as Eigen's compute() does.
+// Vectors and the factor are FP64; K is read in the simulation's precision.
+template <int DIM>
+class IcPcgSolver : public LinearSolver<DIM> {
+ public:
+  explicit IcPcgSolver(SimState<DIM>* state);
+  ~IcPcgSolver() override;
+  void solve() override;
+  SystemMatrixPD<DIM>& system() { return system_; }
+  int colors() const { return color_ptr_.empty() ? 0 : (int)color_ptr_.size() - 1; }
+  const std::vector<int>& vertex_colors() const { return color_; }  // per free vertex
+  double shift() const { return shift_; }     // diagonal shift of the last factor
+  int attempts() const { return attempts_; }  // factorizations of the last solve
+
+ private:
+  void setup();
+  mfemhip::IcFactorDev factor_dev();
+  template <typename MT>
+  void run(const MT* K);
+  SystemMatrixPD<DIM> system_;
+  bool ready_ = false;
+  std::vector<int> color_, color_ptr_;
+  DeviceArray<int> color_rows_, lo_ptr_, lo_col_, lo_pos_, lo_k_, up_ptr_, up_col_, up_l_, fail_;
+  DeviceArray<double> L_, Dinv_, scale_, vecs_, parts_;
+  DeviceArray<mfemhip::IcCtrl> ctrl_;
+  mfemhip::IcCtrl last_{};
+  std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots + a flag
+  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  double shift_ = 0;
+  int attempts_ = 0;
+};
+
 // "minres-indefinite-block": MINRES on the KKT system of the mixed
 // formulation (reference linear_solver_factory.cpp:117-134 with
 // SystemMatrixIndefinite, linear_system.h:60-99, and the matrix-free

@@ -316,4 +316,54 @@ void launch_batch_snh_f9(int n, const double* mu, const double* la, const double
 
 int reduce_grid(long n);  // grid size used by the reductions for n items
 
+// --- "eigen-pcg-IC" (k_ic.hip; reference linear_solver_factory.cpp:87-97) --
+// CG scalars of one solve, device resident.
+struct IcCtrl {
+  double b2, threshold, rr, abs_new, alpha, beta, tol2, mindiag;
+  int iter, done, max_iters, first;
+};
+// Multicolour block IC(0) factor of K over the free vertices.
+struct IcFactorDev {
+  int nfree = 0, dim = 3;
+  const int* rowptr = nullptr;  // BSR pattern of K
+  const int* col = nullptr;
+  const int* diag = nullptr;    // diagonal block of each row
+  const int* color_rows = nullptr;  // rows by (colour, index)
+  // strictly lower blocks (earlier colours), by position in the colour
+  // order: column, position, K block
+  const int* lo_ptr = nullptr;
+  const int* lo_col = nullptr;
+  const int* lo_pos = nullptr;
+  const int* lo_k = nullptr;
+  // strictly upper blocks: row j and the index of L_ji in the lower lists
+  const int* up_ptr = nullptr;
+  const int* up_col = nullptr;
+  const int* up_l = nullptr;
+  double* L = nullptr;      // 9 per lower block, row-major
+  double* Dinv = nullptr;   // 9 per row: inverse of the diagonal factor
+  double* scale = nullptr;  // 3 per row
+  int* fail = nullptr;      // set when a pivot of the last factorization failed
+};
+struct IcVectors {
+  const double* b = nullptr;
+  double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *y = nullptr, *zt = nullptr;
+  double* parts = nullptr;  // ic_partials(nfree) values
+  IcCtrl* ctrl = nullptr;
+};
+int ic_partials(int nfree);
+// Row scaling and min scaled diagonal (into ctrl->mindiag)
+template <typename MT>
+void launch_ic_scale(const IcFactorDev& f, const MT* K, double* parts, IcCtrl* ctrl,
+                     hipStream_t st);
+// The factorization, colour by colour (color_ptr on the host)
+template <typename MT>
+void launch_ic_factor(const IcFactorDev& f, const MT* K, double shift, const int* color_ptr,
+                      int ncolors, hipStream_t st);
+// CG set-up (x = 0, r = b, first preconditioned direction) and one iteration
+void launch_ic_cg_init(const IcFactorDev& f, const IcVectors& v, double tol, int max_iters,
+                       const int* color_ptr, int ncolors, hipStream_t st);
+template <typename MT>
+void launch_ic_cg_iteration(const IcFactorDev& f, const MT* K, const IcVectors& v,
+                            const int* color_ptr, int ncolors, hipStream_t st);
+
 }  // namespace mfemhip
