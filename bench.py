@@ -177,6 +177,43 @@ def run_minres(args, scene):
     print(json.dumps(out), flush=True)
 
 
+def run_ic(args, scene):
+    """CG with the multicolour block IC(0) preconditioner ("eigen-pcg-IC") on
+    the same workload and precision."""
+    import copy
+    scene = copy.copy(scene)
+    scene.solver = "eigen-pcg-IC"
+    nT = scene.nT
+    r = run_gpu(scene, "mixed", args.steps, args.warmup, profile=False)
+    value = r["newton"] * nT / r["elapsed"]
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        dt_cpu, st_cpu = run_cpu_sample(scene, threads)
+        cpu = {"value": st_cpu["newton_iters"] * nT / dt_cpu, "unit": "Newton-steps*tets/s",
+               "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle, 1 Newton iteration of the same {nT}-tet scene with its "
+                         f"IC-CG (serial triangular solves): {dt_cpu:.2f} s, "
+                         f"{st_cpu['pcg_iters_total']} CG iterations"}
+    out = {
+        "metric": "Newton-steps*tets/s, Schur complement with IC-preconditioned CG (eigen-pcg-IC)",
+        "value": value, "unit": "Newton-steps*tets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * r["elapsed"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": workload_name(args.config, args.nx, args.ny, args.nz, nT, scene)
+                   .replace("Jacobi CG", "IC(0)-preconditioned CG"),
+                   "tets": nT, "free_dofs": 3 * r["nfree"], "precision": "mixed"},
+        "newton_iters": r["newton"], "pcg_iters": r["pcg"],
+        "cg_iter_ms": r["last"]["solve_ms"] / max(r["last"]["pcg_iters"], 1),
+        "cpu_baseline": cpu,
+        "phase_ms_last_step": {k: r["last"][k] for k in ("update_ms", "solve_ms", "ls_ms")},
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
 def run_cpu_sample(scene, threads):
     """FP64 oracle on one Newton iteration of the same scene (bounded sample)."""
     from oracle import oracle as O
@@ -301,10 +338,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--solver", choices=("eigen-pcg-diag", "minres-indefinite-block"),
+    ap.add_argument("--solver", choices=("eigen-pcg-diag", "eigen-pcg-IC",
+                                         "minres-indefinite-block"),
                     default="eigen-pcg-diag",
-                    help="linear solver of the Newton step: Schur complement + CG (default, "
-                         "the metric's) or the KKT system with MINRES (FP64)")
+                    help="linear solver of the Newton step: Schur complement + Jacobi CG "
+                         "(default, the metric's), Schur complement + IC(0) CG, or the KKT "
+                         "system with MINRES (FP64)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -323,6 +362,8 @@ def main():
     nT = scene.nT
     if args.solver == "minres-indefinite-block":
         return run_minres(args, scene)
+    if args.solver == "eigen-pcg-IC":
+        return run_ic(args, scene)
 
     mixed = run_gpu(scene, "mixed", args.steps, args.warmup)
     fp64 = None if args.no_fp64 else run_gpu(scene, "fp64", args.steps, args.warmup)

@@ -9,6 +9,7 @@
 #include <fstream>
 #include <iostream>
 #include <sstream>
+#include <type_traits>
 
 namespace mfem {
 
@@ -811,6 +812,8 @@ template <int DIM>
 IcPcgSolver<DIM>::~IcPcgSolver() {
   for (auto e : poll_ev_)
     if (e) (void)hipEventDestroy(e);
+  for (auto g : pair_graph_)
+    if (g) (void)hipGraphExecDestroy(g);
 }
 
 // Pattern work, once per simulation (the BSR pattern is fixed): greedy
@@ -845,33 +848,34 @@ void IcPcgSolver<DIM>::setup() {
   std::vector<int> rows(nb), pos(nb), fill(color_ptr_.begin(), color_ptr_.end() - 1);
   for (int v = 0; v < nb; ++v) rows[fill[color_[v]]++] = v;  // by (colour, index)
   for (int p = 0; p < nb; ++p) pos[rows[p]] = p;
+  // lower lists by position, each sorted by the position of its columns
   std::vector<int> lo_ptr(nb + 1, 0), lo_col, lo_pos, lo_k;
   std::vector<std::array<int, 3>> ent;  // (position, column, K block)
-  for (int i = 0; i < nb; ++i) {
+  for (int p = 0; p < nb; ++p) {
+    const int i = rows[p];
     ent.clear();
     for (int k = rp[i]; k < rp[i + 1]; ++k)
-      if (pos[cl[k]] < pos[i]) ent.push_back({pos[cl[k]], cl[k], k});
+      if (pos[cl[k]] < p) ent.push_back({pos[cl[k]], cl[k], k});
     std::sort(ent.begin(), ent.end());
     for (const auto& e : ent) {
       lo_pos.push_back(e[0]);
       lo_col.push_back(e[1]);
       lo_k.push_back(e[2]);
     }
-    lo_ptr[i + 1] = (int)lo_col.size();
+    lo_ptr[p + 1] = (int)lo_col.size();
   }
-  // upper lists: row i gathers L_ji from the rows j after it
+  // upper lists by position: position q gathers L_jq from the later rows j,
+  // in increasing position of j
   std::vector<int> up_ptr(nb + 1, 0), up_col(lo_col.size()), up_l(lo_col.size());
-  for (int i : lo_col) up_ptr[i + 1]++;
-  for (int i = 0; i < nb; ++i) up_ptr[i + 1] += up_ptr[i];
+  for (int q : lo_pos) up_ptr[q + 1]++;
+  for (int p = 0; p < nb; ++p) up_ptr[p + 1] += up_ptr[p];
   std::vector<int> up_fill(up_ptr.begin(), up_ptr.end() - 1);
-  for (int p = 0; p < nb; ++p) {  // rows j in colour order: each upper list sorted by position
-    const int j = rows[p];
-    for (int a = lo_ptr[j]; a < lo_ptr[j + 1]; ++a) {
-      const int i = lo_col[a];
-      up_col[up_fill[i]] = j;
-      up_l[up_fill[i]++] = a;
+  for (int p = 0; p < nb; ++p)
+    for (int a = lo_ptr[p]; a < lo_ptr[p + 1]; ++a) {
+      const int q = lo_pos[a];
+      up_col[up_fill[q]] = rows[p];
+      up_l[up_fill[q]++] = a;
     }
-  }
   color_rows_.upload(rows, st);
   lo_ptr_.upload(lo_ptr, st);
   lo_col_.upload(lo_col, st);
@@ -883,8 +887,8 @@ void IcPcgSolver<DIM>::setup() {
   L_.resize(9 * std::max<size_t>(lo_col.size(), 1));
   Dinv_.resize(9 * (size_t)nb);
   scale_.resize(3 * (size_t)nb);
-  vecs_.resize(6 * 3 * (size_t)nb);
-  parts_.resize(std::max(ic_partials(nb), 1));
+  vecs_.resize(8 * 3 * (size_t)nb);
+  parts_.resize(ic_partials(nb, color_ptr_.data(), ncol));
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
   ready_ = true;
 }
@@ -953,22 +957,38 @@ void IcPcgSolver<DIM>::run(const MT* K) {
   v.b = system_.rhs.data();
   v.x = w;
   v.r = w + n3;
-  v.p = w + 2 * n3;
-  v.q = w + 3 * n3;
-  v.y = w + 4 * n3;
-  v.zt = w + 5 * n3;
+  v.p[0] = w + 2 * n3;
+  v.p[1] = w + 3 * n3;
+  v.q = w + 4 * n3;
+  v.y = w + 5 * n3;
+  v.zt = w + 6 * n3;
+  v.z = w + 7 * n3;
   v.parts = parts_.data();
   v.ctrl = ctrl_.data();
   launch_ic_cg_init(f, v, cfg.itr_tol, cfg.max_iterative_solver_iters, color_ptr_.data(), colors(),
                     st);
-  const int poll = std::max(1, cfg.pcg_poll);
-  const long limit = std::max(cfg.max_iterative_solver_iters, 0);
+  // the iteration pair as a graph (its kernels take only pointers that stay
+  // fixed for the solver's lifetime, and the matrix, checked here)
+  const int gi = std::is_same<MT, float>::value ? 1 : 0;
+  if (!pair_graph_[gi] || pair_graph_K_[gi] != (const void*)K) {
+    if (pair_graph_[gi]) MFEM_HIP_CHECK(hipGraphExecDestroy(pair_graph_[gi]));
+    pair_graph_[gi] = nullptr;
+    hipGraph_t g = nullptr;
+    MFEM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch_ic_cg_iteration<MT>(f, K, v, 0, color_ptr_.data(), colors(), st);
+    launch_ic_cg_iteration<MT>(f, K, v, 1, color_ptr_.data(), colors(), st);
+    MFEM_HIP_CHECK(hipStreamEndCapture(st, &g));
+    MFEM_HIP_CHECK(hipGraphInstantiate(&pair_graph_[gi], g, nullptr, nullptr, 0));
+    MFEM_HIP_CHECK(hipGraphDestroy(g));
+    pair_graph_K_[gi] = K;
+  }
+  const int poll = std::max(1, (cfg.pcg_poll + 1) / 2);              // pairs per poll
+  const long limit = (std::max(cfg.max_iterative_solver_iters, 0) + 1) / 2;  // pairs
   long launched = 0;
   int issued = 0, waited = 0;
   auto issue = [&] {
     const long chunk = std::min<long>(poll, limit - launched);
-    for (long j = 0; j < chunk; ++j)
-      launch_ic_cg_iteration<MT>(f, K, v, color_ptr_.data(), colors(), st);
+    for (long j = 0; j < chunk; ++j) MFEM_HIP_CHECK(hipGraphLaunch(pair_graph_[gi], st));
     launched += chunk;
     MFEM_HIP_CHECK(hipGetLastError());
     const int slot = issued & 1;

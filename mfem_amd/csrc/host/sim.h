@@ -375,6 +375,10 @@ class IcPcgSolver : public LinearSolver<DIM> {
   mfemhip::IcCtrl last_{};
   std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots + a flag
   hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  // two CG iterations (both parities of the direction buffers) captured as
+  // one graph per matrix precision, replayed for every pair
+  hipGraphExec_t pair_graph_[2] = {nullptr, nullptr};
+  const void* pair_graph_K_[2] = {nullptr, nullptr};
   double shift_ = 0;
   int attempts_ = 0;
 };

@@ -322,15 +322,17 @@ struct IcCtrl {
   double b2, threshold, rr, abs_new, alpha, beta, tol2, mindiag;
   int iter, done, max_iters, first;
 };
-// Multicolour block IC(0) factor of K over the free vertices.
+// Multicolour block IC(0) factor of K over the free vertices. Rows are
+// numbered by position in the colour order; the lists, L and Dinv are
+// stored by position, vectors and `scale` by row.
 struct IcFactorDev {
   int nfree = 0, dim = 3;
-  const int* rowptr = nullptr;  // BSR pattern of K
+  const int* rowptr = nullptr;  // BSR pattern of K (by row)
   const int* col = nullptr;
   const int* diag = nullptr;    // diagonal block of each row
-  const int* color_rows = nullptr;  // rows by (colour, index)
-  // strictly lower blocks (earlier colours), by position in the colour
-  // order: column, position, K block
+  const int* color_rows = nullptr;  // row at each position
+  // strictly lower blocks (earlier colours) of each position, sorted by
+  // position: column row, column position, K block
   const int* lo_ptr = nullptr;
   const int* lo_col = nullptr;
   const int* lo_pos = nullptr;
@@ -340,17 +342,19 @@ struct IcFactorDev {
   const int* up_col = nullptr;
   const int* up_l = nullptr;
   double* L = nullptr;      // 9 per lower block, row-major
-  double* Dinv = nullptr;   // 9 per row: inverse of the diagonal factor
+  double* Dinv = nullptr;   // 9 per position: inverse of the diagonal factor
   double* scale = nullptr;  // 3 per row
   int* fail = nullptr;      // set when a pivot of the last factorization failed
 };
 struct IcVectors {
   const double* b = nullptr;
-  double *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *y = nullptr, *zt = nullptr;
-  double* parts = nullptr;  // ic_partials(nfree) values
+  double *x = nullptr, *r = nullptr, *q = nullptr, *y = nullptr, *zt = nullptr, *z = nullptr;
+  double* p[2] = {nullptr, nullptr};  // search direction, ping-pong between iterations
+  double* parts = nullptr;            // ic_partials(...) values
   IcCtrl* ctrl = nullptr;
 };
-int ic_partials(int nfree);
+// partial-sum slots the solver kernels need (color_ptr: positions per colour)
+int ic_partials(int nfree, const int* color_ptr, int ncolors);
 // Row scaling and min scaled diagonal (into ctrl->mindiag)
 template <typename MT>
 void launch_ic_scale(const IcFactorDev& f, const MT* K, double* parts, IcCtrl* ctrl,
@@ -359,11 +363,12 @@ void launch_ic_scale(const IcFactorDev& f, const MT* K, double* parts, IcCtrl* c
 template <typename MT>
 void launch_ic_factor(const IcFactorDev& f, const MT* K, double shift, const int* color_ptr,
                       int ncolors, hipStream_t st);
-// CG set-up (x = 0, r = b, first preconditioned direction) and one iteration
+// CG set-up (x = 0, r = b, first preconditioned residual) and iteration k
+// (counted from 0; selects the ping-pong direction buffers)
 void launch_ic_cg_init(const IcFactorDev& f, const IcVectors& v, double tol, int max_iters,
                        const int* color_ptr, int ncolors, hipStream_t st);
 template <typename MT>
-void launch_ic_cg_iteration(const IcFactorDev& f, const MT* K, const IcVectors& v,
+void launch_ic_cg_iteration(const IcFactorDev& f, const MT* K, const IcVectors& v, int k,
                             const int* color_ptr, int ncolors, hipStream_t st);
 
 }  // namespace mfemhip
