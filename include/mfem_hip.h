@@ -54,6 +54,9 @@ typedef struct {
   double bc_velocity;     /* "velocity" of the stretch / twist / bend boundary scripts, which
                            * move their pinned end groups after every timestep
                            * (reference boundary_conditions/*.h) */
+  const double* x0;       /* nV x dim start positions, or NULL to start at V: the scene's
+                           * "initial_state" x_path (reference simulation_state.cpp:263-285),
+                           * with V the rest shape */
 } mfem_sim_params;
 
 typedef struct {
@@ -79,6 +82,11 @@ int mfem_version(void);
  * (reference simulation_state.cpp:72-89, optimizer_factory.cpp:10-16,
  * newton_optimizer.cpp:127-140). */
 int mfem_sim_create_json(const char* json_path, mfem_sim** out);
+/* SimState<2>::load(json) for the 2D application (reference
+ * simulation_state.cpp:46-60 and apps/sim2d.cpp): OBJ triangle meshes,
+ * unreferenced vertices dropped, z truncated, no rescaling; positions come
+ * back nV x 2. */
+int mfem_sim_create_json_2d(const char* json_path, mfem_sim** out);
 /* Same from arrays. */
 int mfem_sim_create(const mfem_sim_params* p, mfem_sim** out);
 int mfem_sim_destroy(mfem_sim* sim);

@@ -211,6 +211,11 @@ SceneArrays scene_of(const mfem_sim_params& p) {
     for (long i = 0; i < p.nV; ++i)
       for (int d = 0; d < D; ++d) sc.v0[3 * i + d] = p.v0[D * i + d];
   }
+  if (p.x0) {
+    sc.x0.assign(3L * p.nV, 0.0);
+    for (long i = 0; i < p.nV; ++i)
+      for (int d = 0; d < D; ++d) sc.x0[3 * i + d] = p.x0[D * i + d];
+  }
   sc.density = p.density;
   sc.bc.type = (BCScriptType)p.bc_type;
   sc.bc.axis = p.bc_axis;
@@ -253,6 +258,15 @@ int mfem_sim_create_json(const char* json_path, mfem_sim** out) {
     if (!out) throw std::runtime_error("null output handle");
     auto state = std::make_unique<SimState<3>>();
     if (!state->load(std::string(json_path))) throw std::runtime_error("scene load failed");
+    finish_create(std::move(state), out);
+  });
+}
+
+int mfem_sim_create_json_2d(const char* json_path, mfem_sim** out) {
+  return guarded([&] {
+    if (!out) throw std::runtime_error("null output handle");
+    auto state = std::make_unique<SimState<3>>();
+    if (!state->load(std::string(json_path), 2)) throw std::runtime_error("scene load failed");
     finish_create(std::move(state), out);
   });
 }

@@ -93,7 +93,9 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
   V_ = Vinit_;
   for (int v : T_)
     if (v < 0 || v >= nV_) throw std::runtime_error("Mesh: tet vertex index out of range");
-  is_fixed_ = fixed_override.empty() ? boundary_fixed(bc_config_, V_, 3) : fixed_override;
+  // pins from the rest positions: the reference builds its boundary
+  // condition on Vref_ (mesh.cpp:84) and only init() sees the start positions
+  is_fixed_ = fixed_override.empty() ? boundary_fixed(bc_config_, Vref_, 3) : fixed_override;
   if ((int)is_fixed_.size() != nV_) throw std::runtime_error("Mesh: pin mask size");
   // boundary-condition scripts (BoundaryCondition constructor + init,
   // boundary_condition.cpp:5-21 and the *_boundary_condition.h headers)

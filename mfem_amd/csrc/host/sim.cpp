@@ -1506,6 +1506,10 @@ void SimState<DIM>::load_arrays(const SceneArrays& sc, std::shared_ptr<SimConfig
   mesh_ = std::make_shared<Mesh>(sc.V, sc.T, material_models_[0], sc.mu, sc.la, sc.mat, sc.dim);
   mesh_->bc_config_ = sc.bc;
   if (!sc.v0.empty()) mesh_->initial_velocity_ = sc.v0;
+  if (!sc.x0.empty()) {
+    if (sc.x0.size() != mesh_->Vinit_.size()) throw std::runtime_error("x0 has the wrong size");
+    mesh_->Vinit_ = mesh_->V_ = sc.x0;
+  }
   fixed_override_ = sc.fixed;
   MixedVariableFactory<DIM> mvf;
   VariableFactory<DIM> vf;
@@ -1557,7 +1561,7 @@ void SimState<DIM>::finish_setup() {
 }
 
 template <int DIM>
-bool SimState<DIM>::load(const std::string& json_file) {
+bool SimState<DIM>::load(const std::string& json_file, int dim) {
   if (std::filesystem::path(json_file).extension() != ".json") {
     std::cerr << "File: " << json_file << " needs to be json" << std::endl;
     return false;
@@ -1569,7 +1573,7 @@ bool SimState<DIM>::load(const std::string& json_file) {
     std::cerr << e.what() << std::endl;
     return false;
   }
-  return load(args, std::filesystem::path(json_file).parent_path().string());
+  return load(args, std::filesystem::path(json_file).parent_path().string(), dim);
 }
 
 // reference simulation_state.cpp:344-366
@@ -1597,9 +1601,10 @@ void SimState<DIM>::load_params(const Json& a) {
   if (a.has("pcg_poll")) config_->pcg_poll = a["pcg_poll"].as_int();
 }
 
-// reference simulation_state.cpp:92-341 (3D branch)
+// reference simulation_state.cpp:92-341
 template <int DIM>
-bool SimState<DIM>::load(const Json& args, const std::string& base_dir) {
+bool SimState<DIM>::load(const Json& args, const std::string& base_dir, int dim) {
+  if (dim != 2 && dim != 3) throw std::runtime_error("scene dimension must be 2 or 3");
   config_ = std::make_shared<SimConfig>();
   load_params(args);
   if (args.has("optimizer")) {
@@ -1646,45 +1651,67 @@ bool SimState<DIM>::load(const Json& args, const std::string& base_dir) {
     std::cerr << "Object missing path!" << std::endl;
     return false;
   }
-  read_mesh_file(resolve(obj["path"].as_string()), V, T);
-  // 3D meshes are scaled by 1/max coefficient (reference :62-68)
-  double fac = *std::max_element(V.begin(), V.end());
-  for (auto& v : V) v /= fac;
+  if (dim == 3) {
+    read_mesh_file(resolve(obj["path"].as_string()), V, T);
+    // 3D meshes are scaled by 1/max coefficient (reference :62-68)
+    double fac = *std::max_element(V.begin(), V.end());
+    for (auto& v : V) v /= fac;
+  } else {
+    // igl::read_triangle_mesh + remove_unreferenced, z truncated (:46-60)
+    read_obj_file(resolve(obj["path"].as_string()), V, T);
+    remove_unreferenced(V, T, 3);
+    for (size_t i = 2; i < V.size(); i += 3) V[i] = 0.0;
+  }
+  const int nV = (int)(V.size() / 3);
+  // offset, then the DIM x DIM transformation (column-major) about the
+  // centroid (reference :187-198)
   std::vector<double> offset = {0, 0, 0};
   if (obj.has("offset")) offset = obj["offset"].as_numbers();
-  const int nV = (int)(V.size() / 3);
   for (int i = 0; i < nV; ++i)
-    for (int d = 0; d < 3; ++d) V[3 * i + d] += offset[d];
+    for (int d = 0; d < dim && d < (int)offset.size(); ++d) V[3 * i + d] += offset[d];
   if (obj.has("transformation")) {
-    auto tr = obj["transformation"].as_numbers();  // column-major DIM x DIM
+    auto tr = obj["transformation"].as_numbers();
+    if ((int)tr.size() != dim * dim) throw std::runtime_error("transformation must have DIM^2 entries");
     double cen[3] = {0, 0, 0};
     for (int i = 0; i < nV; ++i)
-      for (int d = 0; d < 3; ++d) cen[d] += V[3 * i + d] / nV;
+      for (int d = 0; d < dim; ++d) cen[d] += V[3 * i + d] / nV;
     for (int i = 0; i < nV; ++i) {
       double p[3];
-      for (int d = 0; d < 3; ++d) p[d] = V[3 * i + d] - cen[d];
-      for (int r = 0; r < 3; ++r) {
+      for (int d = 0; d < dim; ++d) p[d] = V[3 * i + d] - cen[d];
+      for (int r = 0; r < dim; ++r) {
         double a = 0;
-        for (int c = 0; c < 3; ++c) a += tr[r + 3 * c] * p[c];
+        for (int c = 0; c < dim; ++c) a += tr[r + dim * c] * p[c];
         V[3 * i + r] = a + cen[r];
       }
     }
   }
-  const int nT = (int)(T.size() / 4);
+  const int nT = (int)(T.size() / (dim + 1));
+  int idx = 0;
   if (obj.has("material_ids")) {
-    int rows, cols;
-    auto ids = read_dmat(resolve(obj["material_ids"].as_string()), rows, cols);
-    mat_ids.resize(nT);
-    for (int e = 0; e < nT; ++e) mat_ids[e] = (int)ids.at(e);
-  } else {
-    int idx = obj.has("material_index") ? obj["material_index"].as_int() : 0;
-    mat_ids.assign(nT, idx);
+    if (dim == 3) {
+      int rows, cols;
+      auto ids = read_dmat(resolve(obj["material_ids"].as_string()), rows, cols);
+      if ((long)rows * cols < nT) throw std::runtime_error("material_ids has too few entries");
+      mat_ids.resize(nT);
+      for (int e = 0; e < nT; ++e) {
+        mat_ids[e] = (int)ids[e];
+        if (mat_ids[e] < 0 || mat_ids[e] >= (int)materials.size())
+          throw std::runtime_error("material id out of range");
+      }
+    } else {
+      // the reference's 2D branch keeps materials[0] (:200-205)
+      std::cerr << "Missing heterogeneous 2Dtrimesh support!" << std::endl;
+    }
+  } else if (obj.has("material_index")) {
+    idx = obj["material_index"].as_int();
+    if (idx < 0 || idx >= (int)materials.size()) throw std::runtime_error("material_index out of range");
   }
+  if (mat_ids.empty()) mat_ids.assign(nT, idx);
   if (obj.has("initial_velocity")) {
     auto v = obj["initial_velocity"].as_numbers();
-    v0.resize(V.size());
+    v0.assign(V.size(), 0.0);
     for (int i = 0; i < nV; ++i)
-      for (int d = 0; d < 3; ++d) v0[3 * i + d] = v[d];
+      for (int d = 0; d < dim && d < (int)v.size(); ++d) v0[3 * i + d] = v[d];
   }
   if (obj.has("boundary_condition")) {
     const Json& b = obj["boundary_condition"];
@@ -1704,23 +1731,33 @@ bool SimState<DIM>::load(const Json& args, const std::string& base_dir) {
     if (b.has("ratio")) bc.ratio = b["ratio"].as_number();
     if (b.has("velocity")) bc.velocity = b["velocity"].as_number();
   }
-  mesh_ = std::make_shared<Mesh>(V, T, mat_ids, materials);
+  if (dim == 3) {
+    mesh_ = std::make_shared<Mesh>(V, T, mat_ids, materials);
+  } else {
+    // per-element parameters of the object's material; materials[0] gives
+    // the density (tri2d_mesh.cpp:36-41)
+    const auto& mc = *materials[mat_ids.empty() ? 0 : mat_ids[0]]->config();
+    std::vector<double> mu(nT, mc.mu), la(nT, mc.la);
+    std::vector<int> mt(nT, (int)mc.material_model);
+    mesh_ = std::make_shared<Mesh>(V, T, materials[0], mu, la, mt, 2);
+  }
   mesh_->bc_config_ = bc;
   if (!v0.empty()) mesh_->initial_velocity_ = v0;
   if (args.has("initial_state")) {
+    // DMAT start positions and velocities, nV x DIM (:263-308)
     const Json& is = args["initial_state"];
     int rows, cols;
     if (is.has("x_path")) {
       auto x = read_dmat(resolve(is["x_path"].as_string()), rows, cols);
-      if (rows != nV || cols != 3) throw std::runtime_error("initial_state x_path has wrong shape");
+      if (rows != nV || cols != dim) throw std::runtime_error("initial_state x_path has wrong shape");
       for (int i = 0; i < nV; ++i)
-        for (int d = 0; d < 3; ++d) mesh_->Vinit_[3 * i + d] = mesh_->V_[3 * i + d] = x[i + nV * d];
+        for (int d = 0; d < dim; ++d) mesh_->Vinit_[3 * i + d] = mesh_->V_[3 * i + d] = x[i + nV * d];
     }
     if (is.has("v_path")) {
       auto v = read_dmat(resolve(is["v_path"].as_string()), rows, cols);
-      if (rows != nV || cols != 3) throw std::runtime_error("initial_state v_path has wrong shape");
+      if (rows != nV || cols != dim) throw std::runtime_error("initial_state v_path has wrong shape");
       for (int i = 0; i < nV; ++i)
-        for (int d = 0; d < 3; ++d) mesh_->initial_velocity_[3 * i + d] = v[i + nV * d];
+        for (int d = 0; d < dim; ++d) mesh_->initial_velocity_[3 * i + d] = v[i + nV * d];
     }
   }
   MixedVariableFactory<DIM> mvf;
@@ -1787,6 +1824,62 @@ std::vector<double> read_dmat(const std::string& path, int& rows, int& cols) {
   for (auto& x : v) f >> x;
   if (!f) throw std::runtime_error(path + ": short dmat");
   return v;  // column-major rows x cols
+}
+
+void read_obj_file(const std::string& path, std::vector<double>& V, std::vector<int>& T) {
+  std::ifstream f(path);
+  if (!f.good()) throw std::runtime_error("Unable to open mesh: " + path);
+  V.clear();
+  T.clear();
+  std::string line;
+  std::vector<int> poly;
+  while (std::getline(f, line)) {
+    std::istringstream ss(line);
+    std::string key;
+    if (!(ss >> key)) continue;
+    if (key == "v") {
+      double x = 0, y = 0, z = 0;
+      ss >> x >> y;
+      if (!(ss >> z)) z = 0;
+      V.push_back(x);
+      V.push_back(y);
+      V.push_back(z);
+    } else if (key == "f") {
+      poly.clear();
+      std::string tok;
+      while (ss >> tok) {
+        const int k = std::stoi(tok.substr(0, tok.find('/')));  // v, v/vt, v/vt/vn, v//vn
+        const int nv = (int)(V.size() / 3);
+        const int id = k > 0 ? k - 1 : nv + k;  // negative: relative to the last vertex
+        if (k == 0 || id < 0) throw std::runtime_error(path + ": bad face index");
+        poly.push_back(id);
+      }
+      for (size_t c = 1; c + 1 < poly.size(); ++c) {  // fan from the first corner
+        T.push_back(poly[0]);
+        T.push_back(poly[c]);
+        T.push_back(poly[c + 1]);
+      }
+    }
+  }
+  const int nv = (int)(V.size() / 3);
+  for (int v : T)
+    if (v >= nv) throw std::runtime_error(path + ": face index past the vertex list");
+  if (V.empty() || T.empty()) throw std::runtime_error(path + ": no vertices or faces");
+}
+
+void remove_unreferenced(std::vector<double>& V, std::vector<int>& T, int nodes) {
+  (void)nodes;
+  const int nv = (int)(V.size() / 3);
+  std::vector<int> used(nv, 0), map(nv, -1);
+  for (int v : T) used[v] = 1;
+  std::vector<double> NV;
+  for (int i = 0, k = 0; i < nv; ++i)
+    if (used[i]) {
+      map[i] = k++;
+      NV.insert(NV.end(), V.begin() + 3 * i, V.begin() + 3 * i + 3);
+    }
+  for (int& v : T) v = map[v];
+  V.swap(NV);
 }
 
 template class BDF<1>;

@@ -508,6 +508,7 @@ struct SceneArrays {
   std::vector<int> mat;
   std::vector<int> fixed;     // optional pin mask
   std::vector<double> v0;     // optional initial velocity
+  std::vector<double> x0;     // optional start positions (V is then the rest shape)
   double density = 1000.0;
   BoundaryConditionConfig bc;  // used when `fixed` is empty
 };
@@ -527,9 +528,12 @@ struct SimState {
     for (const auto& v : mixed_vars_) n += v->size() + v->size_dual();
     return n;
   }
-  // Scene JSON (reference simulation_state.cpp:72-366 key set).
-  bool load(const std::string& json_file);
-  bool load(const Json& args, const std::string& base_dir = ".");
+  // Scene JSON (reference simulation_state.cpp:72-366 key set). dim 3:
+  // Medit .mesh objects scaled by 1/max coefficient (SimState<3>); dim 2:
+  // OBJ triangle meshes, unreferenced vertices dropped and z truncated
+  // (SimState<2>, simulation_state.cpp:46-60).
+  bool load(const std::string& json_file, int dim = 3);
+  bool load(const Json& args, const std::string& base_dir = ".", int dim = 3);
   // Arrays (per-element materials as in the material_ids constructor,
   // reference tet_mesh.h:11-15); names of variables / solver as in JSON.
   void load_arrays(const SceneArrays& sc, std::shared_ptr<SimConfig> config,
@@ -545,5 +549,12 @@ struct SimState {
 // Medit .mesh and libigl .dmat readers used by the scene loader.
 void read_mesh_file(const std::string& path, std::vector<double>& V, std::vector<int>& T);
 std::vector<double> read_dmat(const std::string& path, int& rows, int& cols);
+// OBJ as a triangle mesh (igl::read_triangle_mesh: "v" rows, "f" polygons
+// fan-triangulated from their first corner; v/vt/vn and negative indices
+// accepted). V is nV x 3.
+void read_obj_file(const std::string& path, std::vector<double>& V, std::vector<int>& T);
+// Drops the vertices no element references and renumbers T, keeping the
+// order of the others (igl::remove_unreferenced). `nodes` corners per element.
+void remove_unreferenced(std::vector<double>& V, std::vector<int>& T, int nodes);
 
 }  // namespace mfem

@@ -45,6 +45,55 @@ def read_mesh(path: str):
     return V, T
 
 
+def read_obj(path: str):
+    """Read an OBJ file as a triangle mesh, as the reference's 2D loader does
+    with igl::read_triangle_mesh (src/simulation_state.cpp:46-60): "v" rows
+    (z optional), "f" polygons fan-triangulated from their first corner,
+    v/vt/vn and negative (relative) indices. Returns V (nV x 3), T (nT x 3)."""
+    V, T = [], []
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                xyz = [float(t) for t in tok[1:4]]
+                V.append(xyz + [0.0] * (3 - len(xyz)))
+            elif tok[0] == "f":
+                poly = []
+                for t in tok[1:]:
+                    k = int(t.split("/")[0])
+                    if k == 0:
+                        raise ValueError(f"{path}: bad face index")
+                    poly.append(k - 1 if k > 0 else len(V) + k)
+                for c in range(1, len(poly) - 1):
+                    T.append([poly[0], poly[c], poly[c + 1]])
+    V = np.array(V, dtype=np.float64).reshape(-1, 3)
+    T = np.array(T, dtype=np.int32).reshape(-1, 3)
+    if V.size == 0 or T.size == 0 or T.min() < 0 or T.max() >= V.shape[0]:
+        raise ValueError(f"{path}: no vertices or faces, or an index out of range")
+    return V, T
+
+
+def remove_unreferenced(V: np.ndarray, T: np.ndarray):
+    """Drop the vertices no element references, keeping the order of the
+    others (igl::remove_unreferenced); returns (V, T renumbered)."""
+    used = np.zeros(V.shape[0], dtype=bool)
+    used[T.reshape(-1)] = True
+    new_id = np.cumsum(used) - 1
+    return V[used].copy(), new_id[T].astype(np.int32)
+
+
+def write_obj(path: str, V: np.ndarray, T: np.ndarray) -> None:
+    """Write vertices (nV x 2 or 3) and triangles as an OBJ file (1-based)."""
+    with open(path, "w") as f:
+        for v in np.asarray(V, dtype=np.float64):
+            xyz = list(v) + [0.0] * (3 - len(v))
+            f.write("v %.17g %.17g %.17g\n" % tuple(xyz[:3]))
+        for t in np.asarray(T):
+            f.write("f " + " ".join(str(int(i) + 1) for i in t) + "\n")
+
+
 def write_mesh(path: str, V: np.ndarray, T: np.ndarray) -> None:
     """Write a Medit .mesh file (1-based indices, reference zero labels)."""
     with open(path, "w") as f:

@@ -18,7 +18,8 @@ dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
 
 EXPORTED = [
-    "mfem_last_error", "mfem_version", "mfem_sim_create_json", "mfem_sim_create",
+    "mfem_last_error", "mfem_version", "mfem_sim_create_json", "mfem_sim_create_json_2d",
+    "mfem_sim_create",
     "mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system", "mfem_sim_solve",
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
@@ -40,7 +41,7 @@ class SimParams(C.Structure):
         ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
         ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("precision", C.c_int),
         ("bc_type", C.c_int), ("bc_axis", C.c_int), ("bc_ratio", C.c_double),
-        ("dim", C.c_int), ("bc_velocity", C.c_double),
+        ("dim", C.c_int), ("bc_velocity", C.c_double), ("x0", dp),
     ]
 
 
@@ -76,6 +77,7 @@ def lib():
         vp = C.c_void_p
         L.mfem_last_error.restype = C.c_char_p
         L.mfem_sim_create_json.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.mfem_sim_create_json_2d.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.mfem_sim_create.argtypes = [C.POINTER(SimParams), C.POINTER(vp)]
         for f in ("mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system"):
             getattr(L, f).argtypes = [vp]
@@ -147,6 +149,11 @@ def _params(sc, precision, p=None):
         p.v0 = keep["v0"].ctypes.data_as(dp)
     else:
         p.v0 = dp()
+    if getattr(sc, "x0", None) is not None:
+        keep["x0"] = _f64(sc.x0)
+        p.x0 = keep["x0"].ctypes.data_as(dp)
+    else:
+        p.x0 = dp()
     p.density, p.dt = sc.density, sc.dt
     g = list(sc.gravity) + [0.0] * 3
     p.gravity = (C.c_double * 3)(*g[:3])
@@ -163,12 +170,17 @@ def _params(sc, precision, p=None):
 class Sim:
     """One simulation on the GPU: SimState<3> + NewtonOptimizer<3> behind the C ABI."""
 
-    def __init__(self, scene=None, precision: str = "fp64", json_path: str | None = None):
+    def __init__(self, scene=None, precision: str = "fp64", json_path: str | None = None,
+                 dim: int = 3):
+        """From a Scene, or from a scene JSON file (dim 2: the 2D
+        application's loader)."""
         self.h = C.c_void_p()
         self.scene = scene
         if json_path is not None:
-            check(lib().mfem_sim_create_json(json_path.encode(), C.byref(self.h)))
+            create = lib().mfem_sim_create_json_2d if dim == 2 else lib().mfem_sim_create_json
+            check(create(json_path.encode(), C.byref(self.h)))
             self.nV = int(lib().mfem_sim_size(self.h, b"x") // 3)
+            self.dim = dim
             return
         p, self._keep = _params(scene, precision)
         check(lib().mfem_sim_create(C.byref(p), C.byref(self.h)))

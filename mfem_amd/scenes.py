@@ -163,6 +163,9 @@ class Scene:
     newton_tol: float = 1e-10
     ls_iters: int = 20
     v0: np.ndarray | None = None
+    # start positions (nV x dim) when they differ from the rest shape V: the
+    # reference's initial_state x_path
+    x0: np.ndarray | None = None
     # boundary condition script {"type", "axis", "ratio", "velocity"}; when set,
     # `fixed` holds the pins it implies and the end groups move after every step
     bc: dict | None = None
@@ -215,32 +218,70 @@ def rect2d_scene(nx=40, ny=10, h=0.1, E=1e5, nu=0.45, **kw) -> Scene:
                  name=f"rect{nx}x{ny}", **kw)
 
 
-def scene_from_json(path: str) -> Scene:
-    """Arrays of a one-object 3D scene JSON, loaded with the reference's rules
-    (reference src/simulation_state.cpp:92-366): mesh scaled by 1/max
-    coefficient, offset, material_index, boundary_condition, solver
-    parameters."""
+def scene_from_json(path: str, dim: int = 3) -> Scene:
+    """Arrays of a one-object scene JSON, loaded with the reference's rules
+    (reference src/simulation_state.cpp:46-366): 3D Medit meshes scaled by
+    1/max coefficient, 2D OBJ triangle meshes with unreferenced vertices
+    dropped and z truncated; offset, then the DIM x DIM "transformation"
+    (column-major) about the centroid; "material_ids" (3D; the 2D branch
+    keeps materials[0]) or "material_index"; "initial_velocity";
+    "boundary_condition"; "initial_state" x_path / v_path (DMAT start
+    positions and velocities, the rest shape staying the file's mesh);
+    solver parameters. Multi-object scenes are rejected."""
     import json
     import os
 
     from . import meshio
     d = json.load(open(path))
     base = os.path.dirname(os.path.abspath(path))
-    obj = d["objects"][0]
-    V, T = meshio.read_mesh(os.path.join(base, obj["path"]))
-    V = V / V.max()
-    V = V + np.asarray(obj.get("offset", [0, 0, 0]), dtype=np.float64)[None, :]
+    res = lambda q: q if os.path.isabs(q) else os.path.join(base, q)
+    objs = d.get("objects", [])
+    if len(objs) != 1:
+        raise ValueError("scene_from_json loads scenes with exactly one object")
+    obj = objs[0]
+    if dim == 3:
+        V, T = meshio.read_mesh(res(obj["path"]))
+        V = V / V.max()
+    else:
+        V, T = meshio.read_obj(res(obj["path"]))
+        V, T = meshio.remove_unreferenced(V, T)
+        V = V[:, :2].copy()
+    off = np.asarray(obj.get("offset", [0, 0, 0]), dtype=np.float64)
+    V = V + off[None, :dim]
+    if "transformation" in obj:
+        A = np.asarray(obj["transformation"], dtype=np.float64).reshape(dim, dim, order="F")
+        c = V.sum(axis=0) / V.shape[0]
+        V = (V - c) @ A.T + c
+    nT = T.shape[0]
     mats = d.get("material_models", [{}])
-    m = mats[obj.get("material_index", 0)]
-    E, nu = m.get("youngs_modulus", 1e6), m.get("poissons_ratio", 0.45)
-    mu, la = enu_to_lame(np.full(T.shape[0], E), nu)
-    mat = np.full(T.shape[0], MATERIALS[m.get("energy", "Stable-Neohookean")], dtype=np.int32)
+    if "material_ids" in obj and dim == 3:
+        ids = meshio.read_dmat(res(obj["material_ids"])).reshape(-1)[:nT].astype(np.int64)
+    else:
+        idx = 0 if "material_ids" in obj else int(obj.get("material_index", 0))
+        ids = np.full(nT, idx, dtype=np.int64)
+    E = np.array([m.get("youngs_modulus", 1e6) for m in mats])[ids]
+    nu = np.array([m.get("poissons_ratio", 0.45) for m in mats])[ids]
+    mu, la = enu_to_lame(E, nu)
+    mat = np.array([MATERIALS[m.get("energy", "Stable-Neohookean")] for m in mats],
+                   dtype=np.int32)[ids]
     bc = obj.get("boundary_condition", {})
     fixed = bc_fixed(bc.get("type", "null"), V, bc.get("axis", 0), bc.get("ratio", 0.1))
     script = None
     if bc.get("type", "null") in ("stretch", "twist", "bend", "scale"):
         script = dict(type=bc["type"], axis=bc.get("axis", 0), ratio=bc.get("ratio", 0.1),
                       velocity=bc.get("velocity", 0.1))
+    v0 = x0 = None
+    if "initial_velocity" in obj:
+        v0 = np.tile(np.asarray(obj["initial_velocity"], dtype=np.float64)[:dim], (V.shape[0], 1))
+    st = d.get("initial_state", {})
+    if "x_path" in st:
+        x0 = meshio.read_dmat(res(st["x_path"]))
+        if x0.shape != V.shape:
+            raise ValueError("initial_state x_path has the wrong shape")
+    if "v_path" in st:
+        v0 = meshio.read_dmat(res(st["v_path"]))
+        if v0.shape != V.shape:
+            raise ValueError("initial_state v_path has the wrong shape")
     g = d.get("body_force", [0, -9.8, 0])
     return Scene(V=V, T=T, mu=mu, la=la, mat=mat, fixed=fixed,
                  density=mats[0].get("density", 1000.0), dt=d.get("dt", 0.034),
@@ -251,7 +292,8 @@ def scene_from_json(path: str) -> Scene:
                  itr_tol=d.get("iterative_solver_tolerance", 1e-4),
                  max_iters=d.get("max_iterative_solver_iters", 500),
                  max_newton=d.get("max_newton_iterations", 5),
-                 ls_iters=d.get("max_linesearch_iterations", 20), bc=script,
+                 newton_tol=d.get("newton_tol", 1e-10),
+                 ls_iters=d.get("max_linesearch_iterations", 20), bc=script, v0=v0, x0=x0,
                  name=os.path.basename(path))
 
 

@@ -71,6 +71,10 @@ typedef struct {
   // 8 twist (end groups moved after every timestep); others: static pins
   int bc_type, bc_axis;
   double bc_ratio, bc_velocity;
+  // start positions (nV*dim), or NULL for V: the reference's initial_state
+  // x_path (simulation_state.cpp:263-285), which replaces Mesh::Vinit_ while
+  // the rest shape stays Vref_
+  const double* x0;
 } oracle_params;
 
 typedef struct {

@@ -33,7 +33,7 @@ class Params(C.Structure):
         ("itr_tol", C.c_double), ("max_iters", C.c_int), ("max_newton", C.c_int),
         ("newton_tol", C.c_double), ("ls_iters", C.c_int), ("threads", C.c_int),
         ("bc_type", C.c_int), ("bc_axis", C.c_int), ("bc_ratio", C.c_double),
-        ("bc_velocity", C.c_double),
+        ("bc_velocity", C.c_double), ("x0", C.POINTER(C.c_double)),
     ]
 
 
@@ -254,6 +254,12 @@ class Sim:
             p.v0 = _dp(v0)
         else:
             p.v0 = C.POINTER(C.c_double)()
+        if getattr(sc, "x0", None) is not None:
+            x0 = f64(sc.x0)
+            self._keep.append(x0)
+            p.x0 = _dp(x0)
+        else:
+            p.x0 = C.POINTER(C.c_double)()
         p.density, p.dt = sc.density, sc.dt
         g = list(sc.gravity) + [0.0] * (3 - len(sc.gravity))
         p.gravity = (C.c_double * 3)(*g[:3])

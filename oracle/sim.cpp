@@ -1628,6 +1628,7 @@ void* oracle_sim_create(const oracle_params* p) {
   S_->dim = D; S_->nodes = D + 1; S_->nV = p->nV; S_->nT = p->nT;
   S_->Vref.assign(p->V, p->V + (size_t)p->nV * D);
   S_->Vcur = S_->Vref;
+  if (p->x0) S_->Vcur.assign(p->x0, p->x0 + (size_t)p->nV * D);
   S_->T.assign(p->T, p->T + (size_t)p->nT * S_->nodes);
   S_->mu.assign(p->mu, p->mu + p->nT);
   S_->la.assign(p->la, p->la + p->nT);
@@ -1672,8 +1673,9 @@ void* oracle_sim_create(const oracle_params* p) {
     }
     S_->asmb.update_matrix(mb, S_->PMP);
   }
-  // boundary-condition script (BoundaryCondition ctor on the rest positions,
-  // boundary_condition.cpp:5-21; init on the start positions)
+  // boundary-condition script: groups from the rest positions (the
+  // BoundaryCondition constructor, boundary_condition.cpp:5-21), centers from
+  // the start positions (init(V) of twist_/bend_/fixed_boundary_conditions.h)
   if (p->nV > 0) {
     const int ax = p->bc_axis;
     std::vector<double> mn(D, 1e300), mx(D, -1e300);
@@ -1688,10 +1690,17 @@ void* oracle_sim_create(const oracle_params* p) {
       if (v < mn[ax] + range * p->bc_ratio) S_->bc_g[0].push_back(i);
       else if (v > mx[ax] - range * p->bc_ratio) S_->bc_g[1].push_back(i);
     }
+    std::fill(mn.begin(), mn.end(), 1e300);
+    std::fill(mx.begin(), mx.end(), -1e300);
+    for (int i = 0; i < p->nV; ++i)
+      for (int d = 0; d < D; ++d) {
+        mn[d] = std::min(mn[d], S_->Vcur[i * D + d]);
+        mx[d] = std::max(mx[d], S_->Vcur[i * D + d]);
+      }
     for (int d = 0; d < D; ++d) S_->bc_center[d] = 0.5 * (mn[d] + mx[d]);
     for (int gi = 0; gi < 2; ++gi)
       if (!S_->bc_g[gi].empty())
-        for (int d = 0; d < D; ++d) S_->bc_gc[gi][d] = p->V[S_->bc_g[gi].back() * D + d];
+        for (int d = 0; d < D; ++d) S_->bc_gc[gi][d] = S_->Vcur[S_->bc_g[gi].back() * D + d];
     if (p->bc_type == 1)  // ScaleBC::init
       for (int i = 0; i < p->nV; ++i)
         for (int d = 0; d < D; ++d)

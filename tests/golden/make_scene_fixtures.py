@@ -1,0 +1,105 @@
+"""Write the scene-file fixtures of the loader tests (inputs only; the tests
+compare the device against the oracle on them).
+
+* square2d.obj / square2d_scene.json: a 2D scene for the 2D application's
+  loader. The OBJ has a nonzero z (truncated by the loader), an unreferenced
+  vertex in front (dropped, the rest renumbered), v/vt/vn-style indices, one
+  quad (fan-triangulated) and one face with negative (relative) indices.
+  The scene offsets, rotates (column-major 2x2 "transformation" about the
+  centroid) and gives an initial velocity.
+* beam_restart_scene.json (+ .dmat files): the 3D beam of beam_scene.json
+  with two materials by "material_ids", a 3x3 "transformation", and an
+  "initial_state" whose x_path / v_path were taken from two oracle timesteps
+  of the same scene without it.
+
+Run:  python -m tests.golden.make_scene_fixtures
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def write_square_obj(path, nx=8, ny=3, h=0.125, z=0.25):
+    lines = ["# 2D square fixture", "v 9 9 9"]  # vertex 1 is never referenced
+
+    def vid(i, j):  # 1-based OBJ index
+        return 2 + i * (ny + 1) + j
+
+    for i in range(nx + 1):
+        for j in range(ny + 1):
+            lines.append(f"v {i * h:.17g} {j * h:.17g} {z:.17g}")
+    nv = 1 + (nx + 1) * (ny + 1)
+    for i in range(nx):
+        for j in range(ny):
+            a, b, c, d = vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)
+            if (i, j) == (2, 1):
+                lines.append(f"f {a} {b} {c} {d}")  # quad, fanned from a
+            elif (i, j) == (5, 0):
+                rel = lambda k: k - nv - 1  # -1 is the last vertex read
+                lines.append(f"f {rel(a)} {rel(b)} {rel(c)}")
+                lines.append(f"f {rel(a)} {rel(c)} {rel(d)}")
+            else:
+                lines.append(f"f {a}/{a}/1 {b}/{b}/1 {c}/{c}/1")
+                lines.append(f"f {a}//1 {c}//1 {d}//1")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def main():
+    write_square_obj(os.path.join(HERE, "square2d.obj"))
+    ang = math.radians(30.0)
+    rot2 = [math.cos(ang), math.sin(ang), -math.sin(ang), math.cos(ang)]  # column-major
+    scene2d = {
+        "dt": 1.0 / 30.0, "body_force": [0.0, -9.8, 0.0], "optimizer": "newton",
+        "time_integrator": "BDF1", "linear_solver": "eigen-pcg-diag",
+        "max_newton_iterations": 20, "max_linesearch_iterations": 20,
+        "max_iterative_solver_iters": 20000, "iterative_solver_tolerance": 1e-12,
+        "newton_tol": 1e-9,
+        "objects": [{"path": "square2d.obj", "offset": [1.0, 2.0, 0.0], "transformation": rot2,
+                     "initial_velocity": [0.1, 0.2, 0.0],
+                     "boundary_condition": {"type": "hangends", "ratio": 0.01, "axis": 0}}],
+        "material_models": [{"youngs_modulus": 1e5, "poissons_ratio": 0.45, "density": 1e3,
+                             "energy": "Stable-Neohookean"}],
+        "mixed_variables": ["mixed-stretch"], "variables": []}
+    with open(os.path.join(HERE, "square2d_scene.json"), "w") as f:
+        json.dump(scene2d, f, indent=2)
+
+    # 3D restart scene: materials by id, a rotation, a deformed start
+    from mfem_amd import meshio, scenes
+    from oracle import oracle
+    base = json.load(open(os.path.join(HERE, "beam_scene.json")))
+    V, T = meshio.read_mesh(os.path.join(HERE, "beam6x2x2.mesh"))
+    ids = (np.arange(T.shape[0]) % 3 == 0).astype(np.float64)  # every third tet stiffer
+    meshio.write_dmat(os.path.join(HERE, "beam_restart_material_ids.dmat"), ids[:, None])
+    c, s = math.cos(0.3), math.sin(0.3)
+    rot3 = [c, s, 0.0, -s, c, 0.0, 0.0, 0.0, 1.0]  # column-major, about z
+    sc = dict(base)
+    sc["newton_tol"] = 1e-9
+    sc["objects"] = [dict(base["objects"][0], transformation=rot3,
+                          material_ids="beam_restart_material_ids.dmat")]
+    sc["objects"][0].pop("material_index", None)
+    sc["material_models"] = base["material_models"] + [
+        {"youngs_modulus": 1e6, "poissons_ratio": 0.4, "density": 1e3, "energy": "ARAP"}]
+    tmp = os.path.join(HERE, "_restart_tmp.json")
+    with open(tmp, "w") as f:
+        json.dump(sc, f)
+    scene = scenes.scene_from_json(tmp)
+    os.remove(tmp)
+    o = oracle.Sim(scene)
+    o.step()
+    x1 = o.positions()
+    o.step()
+    x2 = o.positions()
+    meshio.write_dmat(os.path.join(HERE, "beam_restart_x.dmat"), x2)
+    meshio.write_dmat(os.path.join(HERE, "beam_restart_v.dmat"), (x2 - x1) / scene.dt)
+    sc["initial_state"] = {"x_path": "beam_restart_x.dmat", "v_path": "beam_restart_v.dmat"}
+    with open(os.path.join(HERE, "beam_restart_scene.json"), "w") as f:
+        json.dump(sc, f, indent=2)
+
+
+if __name__ == "__main__":
+    main()

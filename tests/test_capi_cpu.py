@@ -65,3 +65,49 @@ def test_scene_json_files_parse():
         if name.endswith(".json"):
             d = json.load(open(os.path.join(ROOT, "tests", "golden", name)))
             assert "objects" in d and "material_models" in d
+
+
+def test_obj_reader_and_remove_unreferenced():
+    """square2d.obj: one unreferenced leading vertex, v/vt/vn indices, a quad
+    and a face with negative indices. After loading it must be the same
+    triangle set as the tri_rectangle generator's 8x3 grid."""
+    V, T = meshio.read_obj(os.path.join(ROOT, "tests", "golden", "square2d.obj"))
+    assert V.shape == (37, 3) and T.shape == (48, 3)
+    assert np.array_equal(V[0], [9, 9, 9])
+    assert 0 not in T
+    V2, T2 = meshio.remove_unreferenced(V, T)
+    assert V2.shape == (36, 3)
+    assert np.array_equal(V2, V[1:])  # order kept
+    assert np.array_equal(T2, T - 1)
+    Vr, Tr = scenes.tri_rectangle(8, 3, 0.125)
+    assert np.allclose(V2[:, :2], Vr)
+    assert np.allclose(V2[:, 2], 0.25)
+    assert sorted(map(tuple, T2.tolist())) == sorted(map(tuple, Tr.tolist()))
+
+
+def test_scene_loader_transforms():
+    """Offset, then the column-major transformation about the centroid; 2D
+    drops z and does not rescale; 3D rescales by 1/max first."""
+    s = scenes.scene_from_json(os.path.join(ROOT, "tests", "golden", "square2d_scene.json"), dim=2)
+    Vr, _ = scenes.tri_rectangle(8, 3, 0.125)
+    P = Vr + np.array([1.0, 2.0])
+    c = P.mean(axis=0)
+    a = np.radians(30.0)
+    R = np.array([[np.cos(a), -np.sin(a)], [np.sin(a), np.cos(a)]])
+    assert np.allclose(s.V, (P - c) @ R.T + c, atol=1e-14)
+    assert s.dim == 2 and np.allclose(s.v0, [0.1, 0.2])
+    assert abs(np.linalg.norm(s.V[1] - s.V[0]) - 0.125) < 1e-14  # rigid
+    r = scenes.scene_from_json(os.path.join(ROOT, "tests", "golden", "beam_restart_scene.json"))
+    V, T = meshio.read_mesh(os.path.join(ROOT, "tests", "golden", "beam6x2x2.mesh"))
+    V = V / V.max()
+    c3 = V.mean(axis=0)
+    t = 0.3
+    Rz = np.array([[np.cos(t), -np.sin(t), 0], [np.sin(t), np.cos(t), 0], [0, 0, 1]])
+    assert np.allclose(r.V, (V - c3) @ Rz.T + c3, atol=1e-14)
+    ids = np.arange(T.shape[0]) % 3 == 0
+    assert np.all(r.mat[ids] == scenes.MATERIALS["ARAP"])
+    assert np.all(r.mat[~ids] == scenes.MATERIALS["Stable-Neohookean"])
+    mu1, _ = scenes.enu_to_lame(np.array([1e6]), 0.4)
+    assert np.allclose(r.mu[ids], mu1[0])
+    assert r.x0.shape == r.V.shape and r.v0.shape == r.V.shape
+    assert np.abs(r.x0 - r.V).max() > 1e-4  # a deformed start

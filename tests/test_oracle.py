@@ -522,3 +522,23 @@ def test_scale_bc_start_configuration(oracle):
     sim = oracle.Sim(sc)
     c = 0.5 * (sc.V.min(0) + sc.V.max(0))
     assert np.abs(sim.positions() - ((sc.V - c) * 1.5 + c)).max() < 1e-14
+
+
+def test_restart_continues_trajectory(oracle):
+    """initial_state: beam_restart_scene.json starts from the positions and
+    BDF1 velocity of step 2 of the same scene without it; two steps from
+    there reproduce steps 3 and 4 of the uninterrupted run (to the Newton
+    tolerance: the restart re-fits s from the positions)."""
+    import copy
+    r = scenes.scene_from_json(os.path.join(GOLDEN, "beam_restart_scene.json"))
+    full = copy.copy(r)
+    full.x0 = full.v0 = None
+    a, b = oracle.Sim(full), oracle.Sim(r)
+    a.step()
+    a.step()
+    assert np.array_equal(a.positions(), r.x0)
+    for _ in range(2):
+        a.step()
+        b.step()
+        disp = np.abs(a.positions() - r.V).max()
+        assert np.abs(a.positions() - b.positions()).max() < 1e-7 * disp
