@@ -117,6 +117,21 @@ int mfem_sim_get(mfem_sim* sim, const char* name, double* out);
  * of the last solve); "eigen-pcg-IC": "ic_color" (colour of every free
  * vertex), "ic_colors", "ic_attempts" (factorizations of the last solve) */
 int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out);
+/* Per-step output of the reference's application (apps/polyscope_app.h:
+ * 131-149 write_obj, 335-370 the .mesh export and the sim_x0 / sim_v DMAT
+ * pair): the current positions as an OBJ surface (3D: the boundary
+ * triangles, normals out of the mesh; 2D: the triangles, z = 0) or as a
+ * Medit .mesh (3D), and the integrator's last positions and velocities
+ * (x_prev, v_prev, nV x dim) as DMAT files that a scene's "initial_state"
+ * x_path / v_path restarts from. Either path of write_state may be NULL. */
+int mfem_sim_write_obj(mfem_sim* sim, const char* path);
+int mfem_sim_write_mesh(mfem_sim* sim, const char* path);
+int mfem_sim_write_state(mfem_sim* sim, const char* x_path, const char* v_path);
+/* Boundary triangles of a tetrahedral mesh (V nV x 3, T nT x 4): the faces
+ * of exactly one tet, oriented out of it, in (tet, face) order. Host only.
+ * F may be NULL to query the count; *nF is its capacity in triangles. */
+int mfem_boundary_faces(const double* V, int nV, const int* T, int nT, int* F, int* nF);
+
 /* bit 0: time every CG kernel launch with per-workgroup device clock
  * stamps (span of the launch, summed in the stats as spmv_ms/spmv_count) */
 int mfem_sim_set_profiling(mfem_sim* sim, int flags);

@@ -25,6 +25,7 @@ SOURCES = [
     "host/json.cpp",
     "host/material_model.cpp",
     "host/mesh.cpp",
+    "host/output.cpp",
     "host/sim.cpp",
     "capi.cpp",
 ]

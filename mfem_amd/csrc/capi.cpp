@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "host/batch.h"
+#include "host/output.h"
 #include "host/sim.h"
 #include "kernels.h"
 
@@ -18,6 +19,7 @@ using namespace mfem;
 struct mfem_sim {
   std::unique_ptr<NewtonOptimizer<3>> opt;
   SimState<3>& state() { return opt->state(); }
+  std::vector<int> surface;  // output triangles (boundary faces in 3D), built on first use
 };
 
 struct mfem_batch {
@@ -370,6 +372,72 @@ int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out) {
   return guarded([&] {
     auto v = fetch_int(sim, name);
     std::memcpy(out, v.data(), v.size() * sizeof(int));
+  });
+}
+
+int mfem_boundary_faces(const double* V, int nV, const int* T, int nT, int* F, int* nF) {
+  return guarded([&] {
+    if (!V || !T || !nF || nV < 0 || nT < 0) throw std::runtime_error("null argument");
+    std::vector<int> t(T, T + 4L * nT);
+    for (int v : t)
+      if (v < 0 || v >= nV) throw std::runtime_error("tet vertex index out of range");
+    auto f = boundary_faces(std::vector<double>(V, V + 3L * nV), t);
+    if (F) {
+      if (*nF < (int)(f.size() / 3)) throw std::runtime_error("face buffer too small");
+      std::copy(f.begin(), f.end(), F);
+    }
+    *nF = (int)(f.size() / 3);
+  });
+}
+
+namespace {
+const std::vector<int>& output_surface(mfem_sim* sim) {
+  auto& m = *sim->state().mesh_;
+  if (sim->surface.empty()) sim->surface = m.dim() == 2 ? m.T_ : boundary_faces(m.Vref_, m.T_);
+  return sim->surface;
+}
+
+std::vector<double> current_positions(mfem_sim* sim) {
+  return sim->state().x_->value().to_host(sim->state().ctx_->stream);
+}
+}  // namespace
+
+int mfem_sim_write_obj(mfem_sim* sim, const char* path) {
+  return guarded([&] {
+    if (!path) throw std::runtime_error("null path");
+    write_obj(path, current_positions(sim), output_surface(sim));
+  });
+}
+
+int mfem_sim_write_mesh(mfem_sim* sim, const char* path) {
+  return guarded([&] {
+    if (!path) throw std::runtime_error("null path");
+    auto& m = *sim->state().mesh_;
+    if (m.dim() != 3) throw std::runtime_error(".mesh output is for tetrahedral meshes");
+    write_medit(path, current_positions(sim), m.T_, output_surface(sim));
+  });
+}
+
+int mfem_sim_write_state(mfem_sim* sim, const char* x_path, const char* v_path) {
+  return guarded([&] {
+    auto& S = sim->state();
+    const int D = S.mesh_->dim(), nV = S.mesh_->nV();
+    hipStream_t st = S.ctx_->stream;
+    const auto& I = *S.x_->integrator();
+    std::vector<double> x(3L * nV), v(3L * nV);
+    MFEM_HIP_CHECK(hipMemcpyAsync(x.data(), I.x_prev(), x.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipMemcpyAsync(v.data(), I.v_prev(), v.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));
+    auto take = [&](const std::vector<double>& a) {
+      std::vector<double> o((size_t)nV * D);
+      for (int i = 0; i < nV; ++i)
+        for (int d = 0; d < D; ++d) o[(size_t)i * D + d] = a[3L * i + d];
+      return o;
+    };
+    if (x_path) write_dmat(x_path, take(x), nV, D);
+    if (v_path) write_dmat(v_path, take(v), nV, D);
   });
 }
 

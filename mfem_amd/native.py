@@ -19,7 +19,8 @@ ip = C.POINTER(C.c_int)
 
 EXPORTED = [
     "mfem_last_error", "mfem_version", "mfem_sim_create_json", "mfem_sim_create_json_2d",
-    "mfem_sim_create",
+    "mfem_sim_create", "mfem_sim_write_obj", "mfem_sim_write_mesh", "mfem_sim_write_state",
+    "mfem_boundary_faces",
     "mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system", "mfem_sim_solve",
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
@@ -78,6 +79,10 @@ def lib():
         L.mfem_last_error.restype = C.c_char_p
         L.mfem_sim_create_json.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.mfem_sim_create_json_2d.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.mfem_sim_write_obj.argtypes = [vp, C.c_char_p]
+        L.mfem_sim_write_mesh.argtypes = [vp, C.c_char_p]
+        L.mfem_sim_write_state.argtypes = [vp, C.c_char_p, C.c_char_p]
+        L.mfem_boundary_faces.argtypes = [dp, C.c_int, ip, C.c_int, ip, C.POINTER(C.c_int)]
         L.mfem_sim_create.argtypes = [C.POINTER(SimParams), C.POINTER(vp)]
         for f in ("mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system"):
             getattr(L, f).argtypes = [vp]
@@ -200,6 +205,17 @@ class Sim:
 
     def step(self):
         check(lib().mfem_sim_step(self.h))
+
+    # per-step output (the reference application's OBJ / .mesh / DMAT export)
+    def write_obj(self, path):
+        check(lib().mfem_sim_write_obj(self.h, str(path).encode()))
+
+    def write_mesh(self, path):
+        check(lib().mfem_sim_write_mesh(self.h, str(path).encode()))
+
+    def write_state(self, x_path=None, v_path=None):
+        enc = lambda p: None if p is None else str(p).encode()
+        check(lib().mfem_sim_write_state(self.h, enc(x_path), enc(v_path)))
 
     def update_system(self):
         check(lib().mfem_sim_update_system(self.h))
@@ -429,3 +445,15 @@ def snh_f9_batch(mu, la, F9):
                                   C.c_void_p(g.data_ptr()), C.c_void_p(H.data_ptr()), _stream()))
     torch.cuda.synchronize()
     return E.cpu().numpy(), g.cpu().numpy(), H.cpu().numpy().reshape(n, 9, 9).transpose(0, 2, 1)
+
+
+def boundary_faces(V, T):
+    """Boundary triangles of a tetrahedral mesh (host code of the library)."""
+    V, T = _f64(V), _i32(T)
+    n = C.c_int(0)
+    check(lib().mfem_boundary_faces(V.ctypes.data_as(dp), V.shape[0], T.ctypes.data_as(ip),
+                                    T.shape[0], ip(), C.byref(n)))
+    F = np.zeros((n.value, 3), dtype=np.int32)
+    check(lib().mfem_boundary_faces(V.ctypes.data_as(dp), V.shape[0], T.ctypes.data_as(ip),
+                                    T.shape[0], F.ctypes.data_as(ip), C.byref(n)))
+    return F

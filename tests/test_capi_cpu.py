@@ -111,3 +111,31 @@ def test_scene_loader_transforms():
     assert np.allclose(r.mu[ids], mu1[0])
     assert r.x0.shape == r.V.shape and r.v0.shape == r.V.shape
     assert np.abs(r.x0 - r.V).max() > 1e-4  # a deformed start
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_boundary_faces_close_the_surface(flip):
+    """The library's boundary triangles of a Kuhn beam (host code, no GPU):
+    2 per boundary cell face, every edge used once in each direction (a
+    closed, consistently oriented surface), and the divergence theorem's
+    volume of the surface equals the summed tet volumes (outward normals).
+    With every tet's corner order flipped the surface must not change
+    orientation."""
+    from mfem_amd import native
+    nx, ny, nz = 4, 3, 2
+    V, T = scenes.kuhn_beam(nx, ny, nz, 0.1, 0.1, 7)
+    if flip:
+        T = T[:, [0, 2, 1, 3]]
+    F = native.boundary_faces(V, T)
+    assert F.shape[0] == 4 * (nx * ny + ny * nz + nx * nz)
+    edges = {}
+    for a, b, c in F:
+        for e in ((a, b), (b, c), (c, a)):
+            edges[e] = edges.get(e, 0) + 1
+    assert all(n == 1 for n in edges.values())
+    assert all((b, a) in edges for a, b in edges)
+    P = V[F]
+    vol_surface = np.einsum("ij,ij->i", P[:, 0], np.cross(P[:, 1], P[:, 2])).sum() / 6
+    D = V[T[:, 1:]] - V[T[:, [0]]]
+    vol_tets = np.abs(np.linalg.det(D)).sum() / 6
+    assert abs(vol_surface - vol_tets) < 1e-12 * vol_tets

@@ -546,3 +546,51 @@ def test_json_restart_scene(native, oracle):
         assert relerr(g.positions(), o.positions()) < POS_TOL, k
         disp = np.abs(o.positions() - sc.V).max()
         assert np.abs(c.positions() - g.positions()).max() < 1e-7 * disp, k
+
+
+def test_output_files_and_restart(native, tmp_path):
+    """Per-step output: the DMAT pair equals the positions and the BDF1
+    velocity, the OBJ holds the positions and the boundary triangles, the
+    .mesh the positions and tets; a scene restarted from the DMAT pair
+    continues the run (to the Newton tolerance: the restart re-fits s)."""
+    import json
+    from mfem_amd import meshio
+    path = os.path.join(GOLDEN, "beam_scene.json")
+    sc = scenes.scene_from_json(path)
+    g = native.Sim(json_path=path)
+    g.step()
+    x1 = g.positions()
+    g.step()
+    x2 = g.positions()
+    xp, vp = tmp_path / "sim_x0_0002.dmat", tmp_path / "sim_v_0002.dmat"
+    g.write_state(xp, vp)
+    assert np.array_equal(meshio.read_dmat(str(xp)), x2)
+    assert relerr(meshio.read_dmat(str(vp)), (x2 - x1) / sc.dt) < 1e-12
+    g.write_obj(tmp_path / "tet_0002.obj")
+    Vo, Fo = meshio.read_obj(str(tmp_path / "tet_0002.obj"))
+    assert np.array_equal(Vo, x2)
+    assert np.array_equal(Fo, native.boundary_faces(sc.V, sc.T))
+    g.write_mesh(tmp_path / "tet_0002.mesh")
+    Vm, Tm = meshio.read_mesh(str(tmp_path / "tet_0002.mesh"))
+    assert np.array_equal(Vm, x2) and np.array_equal(Tm, sc.T)
+    d = json.load(open(path))
+    d["objects"][0]["path"] = os.path.join(GOLDEN, d["objects"][0]["path"])
+    d["initial_state"] = {"x_path": str(xp), "v_path": str(vp)}
+    rpath = tmp_path / "restart.json"
+    json.dump(d, open(rpath, "w"))
+    r = native.Sim(json_path=str(rpath))
+    assert np.array_equal(r.positions(), x2)
+    for k in range(2):
+        print(f"restart from output, step {k}", flush=True)
+        g.step()
+        r.step()
+        disp = np.abs(g.positions() - sc.V).max()
+        assert np.abs(g.positions() - r.positions()).max() < 1e-7 * disp, k
+    # 2D: the triangles, z = 0
+    p2 = os.path.join(GOLDEN, "square2d_scene.json")
+    s2 = native.Sim(json_path=p2, dim=2)
+    s2.step()
+    s2.write_obj(tmp_path / "tri.obj")
+    V2, F2 = meshio.read_obj(str(tmp_path / "tri.obj"))
+    assert np.array_equal(V2[:, :2], s2.positions()) and not V2[:, 2].any()
+    assert np.array_equal(F2, scenes.scene_from_json(p2, dim=2).T)
