@@ -117,6 +117,13 @@ int mfem_sim_get(mfem_sim* sim, const char* name, double* out);
  * of the last solve); "eigen-pcg-IC": "ic_color" (colour of every free
  * vertex), "ic_colors", "ic_attempts" (factorizations of the last solve) */
 int mfem_sim_get_int(mfem_sim* sim, const char* name, int* out);
+/* The last step's Newton iterations, one row of 10 doubles each: iteration,
+ * energy, energy res, Newton dec, alpha (the columns the reference records,
+ * newton_optimizer.cpp:60-66, and prints with "print_stats"), then the
+ * device time in ms of the update, solve and line search ("print_timing"'s
+ * update / global / LS), the linear solver's iterations and the line-search
+ * halvings. out may be NULL to query the count; *rows is its capacity. */
+int mfem_sim_newton_history(mfem_sim* sim, double* out, int* rows);
 /* Per-step output of the reference's application (apps/polyscope_app.h:
  * 131-149 write_obj, 335-370 the .mesh export and the sim_x0 / sim_v DMAT
  * pair): the current positions as an OBJ surface (3D: the boundary

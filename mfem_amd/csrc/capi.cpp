@@ -228,6 +228,8 @@ SceneArrays scene_of(const mfem_sim_params& p) {
 
 std::shared_ptr<SimConfig> config_of(const mfem_sim_params& p) {
   auto cfg = std::make_shared<SimConfig>();
+  // no print_stats / print_timing keys on this path: quiet
+  cfg->show_data = cfg->show_timing = false;
   cfg->h = p.dt;
   for (int i = 0; i < 3; ++i) cfg->ext[i] = (float)p.gravity[i];
   if (p.bdf_order < 1 || p.bdf_order > 6) throw std::runtime_error("bdf_order must be 1..6");
@@ -438,6 +440,23 @@ int mfem_sim_write_state(mfem_sim* sim, const char* x_path, const char* v_path) 
     };
     if (x_path) write_dmat(x_path, take(x), nV, D);
     if (v_path) write_dmat(v_path, take(v), nV, D);
+  });
+}
+
+int mfem_sim_newton_history(mfem_sim* sim, double* out, int* rows) {
+  return guarded([&] {
+    if (!rows) throw std::runtime_error("null argument");
+    const auto& h = sim->opt->history();
+    if (out) {
+      if (*rows < (int)h.size()) throw std::runtime_error("history buffer too small");
+      for (size_t k = 0; k < h.size(); ++k) {
+        const double row[10] = {(double)h[k].iteration, h[k].energy, h[k].energy_res,
+                                h[k].newton_dec, h[k].alpha, h[k].update_ms, h[k].solve_ms,
+                                h[k].ls_ms, (double)h[k].solver_iters, (double)h[k].ls_halvings};
+        std::copy(row, row + 10, out + 10 * k);
+      }
+    }
+    *rows = (int)h.size();
   });
 }
 

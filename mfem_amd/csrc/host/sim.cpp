@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -1401,6 +1402,7 @@ void NewtonOptimizer<DIM>::step() {
   auto* c = S.ctx_.get();
   hipStream_t st = c->stream;
   stats_ = StepStats();
+  history_.clear();
   S.x_->pre_solve();
   for (auto& v : S.mixed_vars_) v->pre_solve();
   for (auto& v : S.vars_) v->pre_solve();
@@ -1460,6 +1462,7 @@ void NewtonOptimizer<DIM>::step() {
     stats_.solve_ms += t12;
     stats_.ls_ms += t23;
     stats_.alpha = alpha;
+    history_.push_back({i + 1, E, res, grad_norm, alpha, t01, t12, t23, ss.iterations, iter});
     ++i;
     this->callback(S);
   } while (i < S.config_->outer_steps && grad_norm > S.config_->newton_tol && res > 1e-12);
@@ -1468,6 +1471,7 @@ void NewtonOptimizer<DIM>::step() {
   stats_.energy = E;
   stats_.res = res;
   stats_.step_ms = stats_.update_ms + stats_.solve_ms + stats_.ls_ms;
+  if (S.config_->show_data) print_history();
   // reference newton_optimizer.cpp:78-87: scripted boundary motion, then the
   // variables' post_solve
   S.mesh_->update_bcs(S.config_->h);
@@ -1475,6 +1479,30 @@ void NewtonOptimizer<DIM>::step() {
   for (auto& v : S.mixed_vars_) v->post_solve();
   for (auto& v : S.vars_) v->post_solve();
   MFEM_HIP_CHECK(hipGetLastError());
+}
+
+template <int DIM>
+void NewtonOptimizer<DIM>::print_history() const {
+  std::printf("%10s | %14s | %11s | %11s | %9s | %8s\n", "Iteration", "Energy", "Energy res",
+              "Newton dec", "alpha", "solver");
+  for (const auto& r : history_)
+    std::printf("%10d | %14.8e | %11.4e | %11.4e | %9.3e | %8d\n", r.iteration, r.energy,
+                r.energy_res, r.newton_dec, r.alpha, r.solver_iters);
+  if (this->state_->config_->show_timing && !history_.empty()) {
+    // the reference's Timer keys: "update", "global" (the solve), "LS"
+    double tu = 0, tg = 0, tl = 0;
+    for (const auto& r : history_) {
+      tu += r.update_ms;
+      tg += r.solve_ms;
+      tl += r.ls_ms;
+    }
+    const double n = (double)history_.size();
+    std::printf("Timing (in ms, device events):\n");
+    std::printf("  [%10s]  Avg: %10.4f   Total: %10.4f\n", "LS", tl / n, tl);
+    std::printf("  [%10s]  Avg: %10.4f   Total: %10.4f\n", "global", tg / n, tg);
+    std::printf("  [%10s]  Avg: %10.4f   Total: %10.4f\n", "update", tu / n, tu);
+  }
+  std::fflush(stdout);
 }
 
 template <int DIM>

@@ -478,8 +478,22 @@ class NewtonOptimizer : public Optimizer<DIM> {
   double energy(double alpha);
   const StepStats& last_stats() const { return stats_; }
   LinearSolver<DIM>* solver() { return solver_.get(); }
+  // One row per Newton iteration of the last step: the reference's
+  // OptimizerData columns (newton_optimizer.cpp:60-66) and the device time
+  // of its phases.
+  struct Record {
+    int iteration;
+    double energy, energy_res, newton_dec, alpha;
+    double update_ms, solve_ms, ls_ms;
+    int solver_iters, ls_halvings;
+  };
+  const std::vector<Record>& history() const { return history_; }
 
  private:
+  // the per-step table and timing of the reference's print_stats /
+  // print_timing (optimizer_data.cpp:21-183)
+  void print_history() const;
+  std::vector<Record> history_;
   // E(alpha) as one partial per elastic variable in reducer slots base, ...;
   // returns the number of slots written (at most 2)
   int launch_energy(double alpha, int base);

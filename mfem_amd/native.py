@@ -20,7 +20,7 @@ ip = C.POINTER(C.c_int)
 EXPORTED = [
     "mfem_last_error", "mfem_version", "mfem_sim_create_json", "mfem_sim_create_json_2d",
     "mfem_sim_create", "mfem_sim_write_obj", "mfem_sim_write_mesh", "mfem_sim_write_state",
-    "mfem_boundary_faces",
+    "mfem_boundary_faces", "mfem_sim_newton_history",
     "mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system", "mfem_sim_solve",
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
@@ -83,6 +83,7 @@ def lib():
         L.mfem_sim_write_mesh.argtypes = [vp, C.c_char_p]
         L.mfem_sim_write_state.argtypes = [vp, C.c_char_p, C.c_char_p]
         L.mfem_boundary_faces.argtypes = [dp, C.c_int, ip, C.c_int, ip, C.POINTER(C.c_int)]
+        L.mfem_sim_newton_history.argtypes = [vp, dp, C.POINTER(C.c_int)]
         L.mfem_sim_create.argtypes = [C.POINTER(SimParams), C.POINTER(vp)]
         for f in ("mfem_sim_destroy", "mfem_sim_step", "mfem_sim_update_system"):
             getattr(L, f).argtypes = [vp]
@@ -205,6 +206,18 @@ class Sim:
 
     def step(self):
         check(lib().mfem_sim_step(self.h))
+
+    HISTORY_COLUMNS = ("iteration", "energy", "energy_res", "newton_dec", "alpha", "update_ms",
+                       "solve_ms", "ls_ms", "solver_iters", "ls_halvings")
+
+    def newton_history(self):
+        """The last step's Newton iterations: an (iterations x 10) array with
+        the columns of HISTORY_COLUMNS."""
+        n = C.c_int(0)
+        check(lib().mfem_sim_newton_history(self.h, dp(), C.byref(n)))
+        out = np.zeros((n.value, 10))
+        check(lib().mfem_sim_newton_history(self.h, out.ctypes.data_as(dp), C.byref(n)))
+        return out
 
     # per-step output (the reference application's OBJ / .mesh / DMAT export)
     def write_obj(self, path):

@@ -136,6 +136,7 @@ int oracle_kkt_precond(void* sim, int exact, const double* b, double* z);
 // The MINRES iteration on a dense symmetric A with a dense SPD preconditioner
 // P (z = P r), both row-major N x N. Returns the estimated relative residual
 // (negative if P is not positive definite).
+int oracle_sim_history(void* sim, double* out);
 int oracle_ic_factor_dense(int n, int bs, const int* rowptr, const int* colind, const double* vals,
                            double* L, double* scale, double* shift, int* color, int* attempts);
 int oracle_ic_pcg(int n, int bs, const int* rowptr, const int* colind, const double* vals,

@@ -97,6 +97,8 @@ def lib():
         L.oracle_kkt_precond.restype = C.c_int
         L.oracle_minres_dense.argtypes = [C.c_int, dp, dp, dp, dp, C.c_int, C.c_double, ip]
         L.oracle_minres_dense.restype = C.c_double
+        L.oracle_sim_history.argtypes = [C.c_void_p, dp]
+        L.oracle_sim_history.restype = C.c_int
         L.oracle_ic_factor_dense.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, dp, ip, ip]
         L.oracle_ic_factor_dense.restype = C.c_int
         L.oracle_ic_pcg.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
@@ -314,6 +316,14 @@ class Sim:
         out = np.zeros(self.scene.nV * self.scene.dim)
         lib().oracle_sim_get_positions(self.h, _dp(out))
         return out.reshape(self.scene.nV, self.scene.dim)
+
+    def newton_history(self):
+        """The last step's Newton iterations: (iterations x 5) array of
+        iteration, energy, energy res, Newton dec, alpha."""
+        n = lib().oracle_sim_history(self.h, C.POINTER(C.c_double)())
+        out = np.zeros((n, 5))
+        lib().oracle_sim_history(self.h, _dp(out))
+        return out
 
     def get(self, name):
         n = lib().oracle_sim_size(self.h, name.encode())

@@ -18,6 +18,7 @@
 // Element-local loops run under OpenMP (as the reference's #pragma omp loops);
 // every reduction here is serial and therefore deterministic.
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -188,6 +189,9 @@ struct OracleSim {
   oracle_stats st{};
   int pcg_iters_last = 0;
   double pcg_err_last = 0;
+  // per Newton iteration of the last step: iteration, energy, energy res,
+  // Newton dec, alpha (the reference's OptimizerData rows)
+  std::vector<std::array<double, 5>> hist;
   double ic_shift_last = 0;  // "eigen-pcg-IC": diagonal shift and colours of the last factor
   int ic_colors_last = 0;
   // indefinite (KKT) solve: preconditioner state and the reference's
@@ -1411,6 +1415,7 @@ int step(OracleSim* S_) {
   S_->st.newton_iters = 0;
   S_->st.pcg_iters_total = 0;
   S_->st.ls_fail = 0;
+  S_->hist.clear();
   do {
     update_system(S_);
     int rc = substep(S_, grad_norm);
@@ -1436,6 +1441,8 @@ int step(OracleSim* S_) {
       S_->st.ls_fail++;
     }
     S_->st.last_alpha = alpha;
+    // OptimizerData rows (newton_optimizer.cpp:60-66)
+    S_->hist.push_back({(double)(i + 1), E, res, grad_norm, alpha});
     ++i;
   } while (i < S_->p.max_newton && grad_norm > S_->p.newton_tol && res > 1e-12);
   S_->st.newton_iters = i;
@@ -1919,6 +1926,15 @@ int oracle_sim_get_int(void* sim, const char* name, int* out) {
   if (k == "pcg_iters") { *out = S_->pcg_iters_last; return 0; }
   if (k == "ic_colors") { *out = S_->ic_colors_last; return 0; }
   return -1;
+}
+
+// The last step's Newton iterations, 5 values each (iteration, energy,
+// energy res, Newton dec, alpha); returns the row count (out may be NULL).
+int oracle_sim_history(void* sim, double* out) {
+  const auto& h = ((OracleSim*)sim)->hist;
+  if (out)
+    for (size_t k = 0; k < h.size(); ++k) std::copy(h[k].begin(), h[k].end(), out + 5 * k);
+  return (int)h.size();
 }
 
 // "eigen-pcg-IC" preconditioner on a scalar CSR matrix with bs x bs vertex

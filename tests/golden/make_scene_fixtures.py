@@ -54,7 +54,8 @@ def main():
     ang = math.radians(30.0)
     rot2 = [math.cos(ang), math.sin(ang), -math.sin(ang), math.cos(ang)]  # column-major
     scene2d = {
-        "dt": 1.0 / 30.0, "body_force": [0.0, -9.8, 0.0], "optimizer": "newton",
+        "dt": 1.0 / 30.0, "body_force": [0.0, -9.8, 0.0], "print_stats": False,
+        "print_timing": False, "optimizer": "newton",
         "time_integrator": "BDF1", "linear_solver": "eigen-pcg-diag",
         "max_newton_iterations": 20, "max_linesearch_iterations": 20,
         "max_iterative_solver_iters": 20000, "iterative_solver_tolerance": 1e-12,

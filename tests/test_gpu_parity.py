@@ -594,3 +594,31 @@ def test_output_files_and_restart(native, tmp_path):
     V2, F2 = meshio.read_obj(str(tmp_path / "tri.obj"))
     assert np.array_equal(V2[:, :2], s2.positions()) and not V2[:, 2].any()
     assert np.array_equal(F2, scenes.scene_from_json(p2, dim=2).T)
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_newton_history_matches_oracle(native, oracle, mixed):
+    """Newton iteration by iteration (the print_stats rows): same count,
+    energies to 1e-10, decrements to 1e-6 relative above a 1e-8 noise floor,
+    and the same
+    line-search steps while the update is above 1e-7. Below that (within 10x
+    of the rotation fit's 1e-8 resolution: rodrigues drops smaller updates,
+    and FP64 rounding of E below ~1e-9), E(alpha) - E(0) is noise and the
+    strict decrease test (c = 0) accepts or rejects by it, on either side.
+    Such a step, taken on one side only, starts the next timestep from
+    states ~1e-9 apart, so energies after the first step agree to 1e-8."""
+    sc = scenes.beam_scene(3, 2, 2, seed=2, mixed=mixed, itr_tol=1e-12, max_iters=5000,
+                           max_newton=10, newton_tol=1e-9)
+    g, o = native.Sim(sc), oracle.Sim(sc)
+    for k in range(3):
+        g.step()
+        o.step()
+        hg, ho = g.newton_history(), o.newton_history()
+        print(f"history step {k}: {hg.shape[0]} iterations", flush=True)
+        assert hg.shape[0] == ho.shape[0]
+        assert np.array_equal(hg[:, 0], ho[:, 0])
+        big = ho[:, 3] > 1e-7
+        assert big[0] and np.array_equal(hg[big, 4], ho[big, 4])
+        assert relerr(hg[:, 1], ho[:, 1]) < (1e-10 if k == 0 else 1e-8)
+        assert np.all(np.abs(hg[:, 3] - ho[:, 3]) <= 1e-6 * ho[:, 3] + 1e-8)
+        assert np.all(hg[:, 8] > 0) and np.all(hg[:, 6] > 0)  # solver iterations, solve time

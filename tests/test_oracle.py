@@ -542,3 +542,26 @@ def test_restart_continues_trajectory(oracle):
         b.step()
         disp = np.abs(a.positions() - r.V).max()
         assert np.abs(a.positions() - b.positions()).max() < 1e-7 * disp
+
+
+def test_newton_history_rows(oracle):
+    """The per-iteration rows (the reference's print_stats table): iterations
+    numbered from 1, the energy residual |E_k - E_(k-1)| / (E_k + 1e-6) from
+    the energy column (E_(-1) = 0), alpha a power of 1/2, the last decrement
+    the step's, and on the plain path (a true energy) non-increasing energies
+    across accepted steps."""
+    sc = scenes.beam_scene(3, 2, 2, seed=1, mixed=False, max_newton=8, newton_tol=1e-12,
+                           solver="eigen-llt")
+    o = oracle.Sim(sc)
+    for _ in range(2):
+        o.step()
+        h = o.newton_history()
+        st = o.stats()
+        assert h.shape == (st["newton_iters"], 5)
+        assert np.array_equal(h[:, 0], np.arange(1, h.shape[0] + 1))
+        E = h[:, 1]
+        prev = np.concatenate([[0.0], E[:-1]])
+        assert np.allclose(h[:, 2], np.abs((E - prev) / (E + 1e-6)), rtol=1e-12, atol=0)
+        assert np.all(np.log2(h[:, 4]) == np.round(np.log2(h[:, 4])))
+        assert h[-1, 3] == st["last_decrement"]
+        assert np.all(np.diff(E) <= 1e-12 * np.abs(E[:-1]))
