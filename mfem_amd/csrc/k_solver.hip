@@ -47,40 +47,56 @@ inline int xcd_grid(int n, int bs) {
   return 8 * std::max(1, (per_xcd + bs - 1) / bs);
 }
 
+// Four lanes per K block: lane q sums entries q, q + 4, q + 8 (row-major
+// 3x3) over the block's slots, so the four lanes of a block read the same
+// 36 bytes of each element block with at most 3 loads each, and the diagonal
+// entries (0, 4, 8) with the mass term all fall to lane 0. Every entry is
+// summed in slot order, as with one lane per block.
 template <typename Real>
 __global__ __launch_bounds__(kBS) void k_assemble_bsr(int nnzb, const int* __restrict__ slot_ptr,
                                                       const int* __restrict__ slots,
                                                       const double* __restrict__ mass,
                                                       const Real* __restrict__ blocks,
                                                       Real* __restrict__ K) {
-  const int b = xcd_item(nnzb);
-  if (b < 0) return;
-  double acc[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) acc[i] = 0.0;
+  const int t = xcd_item(4 * nnzb);
+  if (t < 0) return;
+  const int q = t & 3;
+  const int b = t >> 2;
+  const int ne = q == 0 ? 3 : 2;
+  double acc[3] = {0.0, 0.0, 0.0};
   const int p1 = slot_ptr[b + 1];
-  for (int p = slot_ptr[b]; p < p1; ++p) {
-    const int sl = slots[p];
-    const int e = sl >> 4, a = (sl >> 2) & 3, c = sl & 3;
-    if (a <= c) {
-      const Real* src = blocks + (size_t)e * 90 + 9 * blk_index(a, c);
+  // slots in batches of kAsmBatch: all slot ids, then all element values,
+  // are in flight together (the sums still run in slot order)
+  constexpr int kAsmBatch = 8;
+  for (int p = slot_ptr[b]; p < p1; p += kAsmBatch) {
+    int sl[kAsmBatch];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) acc[i] += (double)src[i];
-    } else {
-      const Real* src = blocks + (size_t)e * 90 + 9 * blk_index(c, a);
+    for (int u = 0; u < kAsmBatch; ++u) sl[u] = p + u < p1 ? slots[p + u] : -1;
+    Real val[kAsmBatch][3];
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+    for (int u = 0; u < kAsmBatch; ++u) {
+      if (sl[u] < 0) continue;
+      const int e = sl[u] >> 4, a = (sl[u] >> 2) & 3, c = sl[u] & 3;
+      const Real* src = blocks + (size_t)e * 90 + 9 * (a <= c ? blk_index(a, c) : blk_index(c, a));
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[3 * r + q] += (double)src[3 * q + r];
+      for (int j = 0; j < 3; ++j) {
+        const int o = q + 4 * j;                             // entry 3r + s of the K block
+        const int src_o = a <= c ? o : 3 * (o % 3) + o / 3;  // transposed for a > c
+        val[u][j] = j < ne ? src[src_o] : (Real)0;
+      }
     }
-  }
-  const double m = mass[b];
-  acc[0] += m;
-  acc[4] += m;
-  acc[8] += m;
-  Real* o = K + (size_t)9 * b;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) o[i] = (Real)acc[i];
+    for (int u = 0; u < kAsmBatch; ++u)
+      if (sl[u] >= 0)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          if (j < ne) acc[j] += (double)val[u][j];
+  }
+  Real* out = K + (size_t)9 * b;
+  const double m = q == 0 ? mass[b] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    if (j < ne) out[q + 4 * j] = (Real)(acc[j] + m);
 }
 
 template <typename Real, typename VT>
@@ -102,12 +118,29 @@ __global__ void k_assemble_rhs(int nfree, const int* __restrict__ inc_ptr, const
   if (i < 0) return;
   double a0 = 0, a1 = 0, a2 = 0;
   const int p1 = inc_ptr[i + 1];
-  for (int p = inc_ptr[i]; p < p1; ++p) {
-    const int ek = inc[p];
-    const double* f = fe + (size_t)(ek >> 2) * 12 + 3 * (ek & 3);
-    a0 += f[0];
-    a1 += f[1];
-    a2 += f[2];
+  // incidences in batches: ids, then element values, in flight together
+  // (the sums still run in incidence order)
+  constexpr int kRhsBatch = 8;
+  for (int p = inc_ptr[i]; p < p1; p += kRhsBatch) {
+    int ek[kRhsBatch];
+#pragma unroll
+    for (int u = 0; u < kRhsBatch; ++u) ek[u] = p + u < p1 ? inc[p + u] : -1;
+    double f[kRhsBatch][3];
+#pragma unroll
+    for (int u = 0; u < kRhsBatch; ++u)
+      if (ek[u] >= 0) {
+        const double* src = fe + (size_t)(ek[u] >> 2) * 12 + 3 * (ek[u] & 3);
+        f[u][0] = src[0];
+        f[u][1] = src[1];
+        f[u][2] = src[2];
+      }
+#pragma unroll
+    for (int u = 0; u < kRhsBatch; ++u)
+      if (ek[u] >= 0) {
+        a0 += f[u][0];
+        a1 += f[u][1];
+        a2 += f[u][2];
+      }
   }
   rhs[3 * i] = a0;
   rhs[3 * i + 1] = a1;
@@ -687,8 +720,8 @@ template <typename Real>
 void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const double* mass,
                          const Real* blocks, Real* K, hipStream_t st) {
   if (nnzb == 0) return;
-  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(xcd_grid(nnzb, kBS)), dim3(kBS), 0, st, nnzb, slot_ptr,
-                     slots, mass, blocks, K);
+  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(xcd_grid(4 * nnzb, kBS)), dim3(kBS), 0, st, nnzb,
+                     slot_ptr, slots, mass, blocks, K);
 }
 
 template <typename Real, typename VT>
