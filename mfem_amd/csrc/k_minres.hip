@@ -39,6 +39,7 @@ namespace {
 constexpr int kTetBS = 128;
 constexpr int kVtxBS = 256;
 constexpr int kFinBS = 1024;
+constexpr int kGather = 8;  // indices / values in flight per gather batch
 
 __device__ __forceinline__ double symw(int i) { return i < 3 ? 1.0 : 2.0; }
 
@@ -235,11 +236,23 @@ __global__ __launch_bounds__(kVtxBS) void k_kkt_rhs_vtx(int nfree, const int* __
   if (i < nfree) {
     double a[3] = {0, 0, 0};
     const int p1 = inc_ptr[i + 1];
-    for (int p = inc_ptr[i]; p < p1; ++p) {
-      const int ek = inc[p];
-      const double* f = fe + (size_t)(ek >> 2) * 12 + 3 * (ek & 3);
+    for (int p = inc_ptr[i]; p < p1; p += kGather) {  // batched gather, sums in order
+      int ek[kGather];
+      double f[kGather][3];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) a[r] += f[r];
+      for (int u = 0; u < kGather; ++u) ek[u] = p + u < p1 ? inc[p + u] : -1;
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (ek[u] >= 0) {
+          const double* src = fe + (size_t)(ek[u] >> 2) * 12 + 3 * (ek[u] & 3);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) f[u][r] = src[r];
+        }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (ek[u] >= 0)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) a[r] += f[u][r];
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -386,21 +399,48 @@ __global__ __launch_bounds__(kVtxBS) void k_mr_vertex(int nfree, const int* __re
   if (i < nfree) {
     const double ib = 1.0 / c.beta;
     const double ratio = c.beta / c.beta_prev;
+    // both gathers in batches of kGather: indices, then values, in flight
+    // together (the sums keep their order)
     double y[3] = {0, 0, 0};
     const int b1 = rowptr[i + 1];
-    for (int b = rowptr[i]; b < b1; ++b) {
-      const double mm = mass[b];
-      const int j = col[b];
+    for (int b = rowptr[i]; b < b1; b += kGather) {
+      int j[kGather];
+      double mm[kGather], w[kGather][3];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) y[r] += mm * (V.wc.x[3 * j + r] * ib);
+      for (int u = 0; u < kGather; ++u) {
+        j[u] = b + u < b1 ? col[b + u] : -1;
+        mm[u] = b + u < b1 ? mass[b + u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (j[u] >= 0)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) w[u][r] = V.wc.x[3 * j[u] + r];
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (j[u] >= 0)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) y[r] += mm[u] * (w[u][r] * ib);
     }
     double z[3] = {0, 0, 0};
     const int p1 = inc_ptr[i + 1];
-    for (int p = inc_ptr[i]; p < p1; ++p) {
-      const int ek = inc[p];
-      const double* f = fe + (size_t)(ek >> 2) * 12 + 3 * (ek & 3);
+    for (int p = inc_ptr[i]; p < p1; p += kGather) {
+      int ek[kGather];
+      double f[kGather][3];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) z[r] += f[r];
+      for (int u = 0; u < kGather; ++u) ek[u] = p + u < p1 ? inc[p + u] : -1;
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (ek[u] >= 0) {
+          const double* src = fe + (size_t)(ek[u] >> 2) * 12 + 3 * (ek[u] & 3);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) f[u][r] = src[r];
+        }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (ek[u] >= 0)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) z[r] += f[u][r];
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -520,11 +560,24 @@ __global__ __launch_bounds__(kVtxBS) void k_mr_cheb(int nfree, const int* __rest
   if (i < nfree) {
     double y[3] = {0, 0, 0};
     const int b1 = rowptr[i + 1];
-    for (int b = rowptr[i]; b < b1; ++b) {
-      const double mm = mass[b];
-      const int j = col[b];
+    for (int b = rowptr[i]; b < b1; b += kGather) {  // batched gather, sums in order
+      int j[kGather];
+      double mm[kGather], w[kGather][3];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) y[r] += mm * d_in[3 * j + r];
+      for (int u = 0; u < kGather; ++u) {
+        j[u] = b + u < b1 ? col[b + u] : -1;
+        mm[u] = b + u < b1 ? mass[b + u] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (j[u] >= 0)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) w[u][r] = d_in[3 * j[u] + r];
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (j[u] >= 0)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) y[r] += mm[u] * w[u][r];
     }
     const double di = dinv[i];
 #pragma unroll
