@@ -51,16 +51,16 @@ MFEM_DEV void matmul3(const double A[9], const double B[9], double C[9]) {
 // Rodrigues' formula (reference src/svd/newton_procrustes.h:87-106).
 MFEM_DEV void rodrigues(const double w[3], double dR[9]) {
   double angle = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-  if (fabs(angle) < 1e-8) {
-#pragma unroll
-    for (int i = 0; i < 9; ++i) dR[i] = (i % 4 == 0) ? 1.0 : 0.0;
-    return;
-  }
-  double ia = 1.0 / angle;
+  // below 1e-8 the rotation is the identity: sin and 1 - cos are taken as
+  // 0 (and the axis as 0), for which the formula below gives exactly I;
+  // selecting instead of branching keeps dR in registers
+  const bool small = fabs(angle) < 1e-8;
+  double ia = small ? 0.0 : 1.0 / angle;
   double kx = w[0] * ia, ky = w[1] * ia, kz = w[2] * ia;
   double sn, cs;
   sincos(angle, &sn, &cs);
-  double oc = 1.0 - cs;
+  sn = small ? 0.0 : sn;
+  double oc = small ? 0.0 : 1.0 - cs;
   // K = [0 -kz ky; kz 0 -kx; -ky kx 0]; K^2 = k k^T - I
   dR[0] = 1.0 + oc * (kx * kx - 1.0);
   dR[4] = 1.0 + oc * (ky * ky - 1.0);

@@ -57,11 +57,16 @@ __device__ __forceinline__ void lu_inverse6(const Real* Ain, Real* X) {
 #pragma unroll
     for (int i = k + 1; i < 6; ++i)
       if (tabs(A[i + 6 * k]) > best) { best = tabs(A[i + 6 * k]); p = i; }
-    if (p != k) {
+    // the swap with row p, written as one predicated swap per candidate
+    // row so that every index is a compile-time constant (a run-time row
+    // index would put A in scratch memory)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) { Real t = A[k + 6 * j]; A[k + 6 * j] = A[p + 6 * j]; A[p + 6 * j] = t; }
-      int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
-    }
+    for (int i = k + 1; i < 6; ++i)
+      if (p == i) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) { Real t = A[k + 6 * j]; A[k + 6 * j] = A[i + 6 * j]; A[i + 6 * j] = t; }
+        int t = piv[k]; piv[k] = piv[i]; piv[i] = t;
+      }
     Real id = Real(1) / A[k + 6 * k];
 #pragma unroll
     for (int i = k + 1; i < 6; ++i) {
