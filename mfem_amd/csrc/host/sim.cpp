@@ -9,6 +9,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <queue>
 #include <sstream>
 #include <type_traits>
 
@@ -817,9 +818,12 @@ IcPcgSolver<DIM>::~IcPcgSolver() {
     if (g) (void)hipGraphExecDestroy(g);
 }
 
-// Pattern work, once per simulation (the BSR pattern is fixed): greedy
-// first-fit colouring in vertex order, the colour order, and the lower and
-// upper block lists of every row.
+// Pattern work, once per simulation (the BSR pattern is fixed): the
+// colouring -- the smallest-last order (repeatedly remove a vertex of least
+// remaining degree, lowest index on ties), then greedy first fit in its
+// reverse (Matula and Beck 1983; 4 colours on the Kuhn beams, where first
+// fit in vertex order needs 8) -- the colour order, and the lower and upper
+// block lists of every row.
 template <int DIM>
 void IcPcgSolver<DIM>::setup() {
   auto& m = *this->state_->mesh_;
@@ -827,10 +831,29 @@ void IcPcgSolver<DIM>::setup() {
   const int nb = m.nfree();
   const auto& rp = m.rowptr_;
   const auto& cl = m.col_;
+  std::vector<int> deg(nb, 0), removal;
+  std::vector<char> removed(nb, 0);
+  for (int v = 0; v < nb; ++v)
+    for (int k = rp[v]; k < rp[v + 1]; ++k) deg[v] += cl[k] != v;
+  std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>,
+                      std::greater<std::pair<int, int>>> heap;
+  for (int v = 0; v < nb; ++v) heap.push({deg[v], v});
+  while (!heap.empty()) {
+    const auto [d, v] = heap.top();
+    heap.pop();
+    if (removed[v] || d != deg[v]) continue;  // stale entry
+    removed[v] = 1;
+    removal.push_back(v);
+    for (int k = rp[v]; k < rp[v + 1]; ++k) {
+      const int u = cl[k];
+      if (u != v && !removed[u]) heap.push({--deg[u], u});
+    }
+  }
   color_.assign(nb, -1);
   std::vector<int> mark;
   int ncol = 0;
-  for (int v = 0; v < nb; ++v) {
+  for (auto it = removal.rbegin(); it != removal.rend(); ++it) {
+    const int v = *it;
     for (int k = rp[v]; k < rp[v + 1]; ++k) {
       const int u = cl[k];
       if (u != v && color_[u] >= 0) {

