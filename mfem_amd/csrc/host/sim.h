@@ -346,7 +346,7 @@ class DirectSolver : public LinearSolver<DIM> {
 // "eigen-pcg-IC": CG with an incomplete-Cholesky preconditioner (reference
 // linear_solver_factory.cpp:87-97, Eigen's ConjugateGradient with
 // IncompleteCholesky). The factor is IC(0) on the 3x3 vertex blocks of K in
-// greedy multicolour order, with Eigen's row scaling and diagonal-shift
+// smallest-last multicolour order, with Eigen's row scaling and diagonal-shift
 // restarts (k_ic.hip), refactored on every call as Eigen's compute() does.
 // Vectors and the factor are FP64; K is read in the simulation's precision.
 template <int DIM>

@@ -5,9 +5,10 @@
 // kept here: K~ = S K S with S = diag(|row|_2^-1/2), a diagonal shift of
 // 1e-3 - min diag(K~) when that is not positive, and on a non-positive pivot
 // a restart with max(1e-3, 2 shift) (the host drives the restarts). The
-// ordering and fill rule become IC(0) on the 3x3 vertex blocks of K in greedy
-// multicolour order: vertices of one colour share no block, so one colour is
-// one parallel step of the factorization and of each triangular solve.
+// ordering and fill rule become IC(0) on the 3x3 vertex blocks of K in
+// multicolour order (smallest-last colouring, built on the host): vertices of
+// one colour share no block, so one colour is one parallel step of the
+// factorization and of each triangular solve.
 //
 //   L_ij = (K~_ij - sum_k L_ik L_jk^T) L_jj^-T   for j before i (k before j,
 //                                               k in both rows' lower lists)

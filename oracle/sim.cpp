@@ -694,7 +694,7 @@ int pcg(OracleSim* S_, std::vector<double>& xo) {
 // entries per column as the column had, and on a non-positive pivot restarts
 // with the shift max(1e-3, 2 shift), at most 10 attempts. The device keeps the
 // scaling and the shift loop and replaces the ordering and fill rule with
-// IC(0) on the 3x3 vertex blocks in greedy multicolour order (vertices of one
+// IC(0) on the 3x3 vertex blocks in smallest-last multicolour order (vertices of one
 // colour share no matrix entry, so a colour is one parallel step of the
 // factorization and of both triangular solves). This is that algorithm, in
 // scalar form, as the parity reference for the device.
