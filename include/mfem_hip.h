@@ -172,8 +172,9 @@ int mfem_sim_cg_tuning(mfem_sim* sim, double* times, int* variants, int* grids, 
 /* Indefinite (KKT) system of "minres-indefinite-block" (reference
  * SystemMatrixIndefinite, linear_system.h:60-99, and BlockMatrix::
  * scaleAndAddTo, block_matrix.h:74-125) at the current state (after
- * mfem_sim_update_system). Vectors are [dx (3 nfree); ds (6 nT); lambda
- * (6 nT)] with element-major 6-vectors, size from mfem_sim_kkt_size.
+ * mfem_sim_update_system). Vectors are [dx (D nfree); ds (ns nT); lambda
+ * (ns nT)] with element-major s-vectors (D = 3, ns = 6 on tetrahedra; D = 2,
+ * ns = 3 on triangles), size from mfem_sim_kkt_size.
  * which 0: out = K in; 1: out = P in (the solver's block preconditioner);
  * 2: out = the right-hand side (in unused). Errors unless the simulation's
  * linear solver is "minres-indefinite-block". */

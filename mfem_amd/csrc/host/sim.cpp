@@ -329,13 +329,15 @@ void MixedStretch<DIM>::update(const double* x_full, double dt) {
       launch_mixed2d_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
                                            dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
                                            gl_.data(), h2_, c->stream);
-      launch_mixed2d_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
+      if (schur_blocks_)
+        launch_mixed2d_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
                                        gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
     } else {
       launch_mixed2d_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
                                             dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
                                             gl_.data(), h2_, c->stream);
-      launch_mixed2d_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(), Hpsi_.f64(),
+      if (schur_blocks_)
+        launch_mixed2d_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(), Hpsi_.f64(),
                                         gl_.data(), c->blocks.f64(), c->fe.data(), c->stream);
     }
     return;
@@ -1052,8 +1054,6 @@ void IcPcgSolver<DIM>::solve() {
 // ============================================================ MINRES
 template <int DIM>
 MinresSolver<DIM>::MinresSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
-  if (DIM != 3 || state->mesh_->dim() != 3)
-    throw std::runtime_error("minres-indefinite-block: the device path is 3D (tetrahedra)");
   if (state->mixed_vars_.size() != 1 || !state->vars_.empty())
     throw std::runtime_error(
         "minres-indefinite-block needs exactly one mixed variable (\"mixed-stretch\") and no "
@@ -1065,6 +1065,8 @@ MinresSolver<DIM>::MinresSolver(SimState<DIM>* state) : LinearSolver<DIM>(state)
   hipStream_t st = state->ctx_->stream;
   n_ = 3L * m.nfree();
   N_ = n_ + 12L * m.nT();
+  ns_ = m.dim() == 2 ? 3 : 6;
+  Nref_ = (long)m.dim() * m.nfree() + 2L * ns_ * m.nT();
   // lumped mass (row sums of PMP) and the Jacobi scaling of the mass block
   std::vector<double> ml(m.nfree(), 0.0), md(m.nfree(), 0.0);
   const auto& mass = m.mass();
@@ -1237,16 +1239,20 @@ std::vector<double> MinresSolver<DIM>::apply(int which, const std::vector<double
   auto* c = this->state_->ctx_.get();
   hipStream_t st = c->stream;
   if (which < 0 || which > 2) throw std::invalid_argument("kkt apply: which must be 0, 1 or 2");
-  const int nT = m.nT();
-  // reference layout -> device layout (element parts structure of arrays)
-  std::vector<double> dev(N_, 0.0), out(N_);
+  const int nT = m.nT(), D = m.dim(), nf = m.nfree();
+  const long nr = (long)D * nf, ns = ns_;
+  // reference layout [dx (D nfree); ds (ns nT); lambda (ns nT)], element
+  // parts element-major -> device layout (3 per free vertex, element parts
+  // structure of arrays)
+  std::vector<double> dev(N_, 0.0), out(Nref_);
   if (which != 2) {
-    if ((long)in.size() != N_) throw std::invalid_argument("kkt apply: wrong vector size");
-    std::copy(in.begin(), in.begin() + n_, dev.begin());
+    if ((long)in.size() != Nref_) throw std::invalid_argument("kkt apply: wrong vector size");
+    for (int f = 0; f < nf; ++f)
+      for (int r = 0; r < D; ++r) dev[3L * f + r] = in[(long)D * f + r];
     for (int e = 0; e < nT; ++e)
-      for (int i = 0; i < 6; ++i) {
-        dev[n_ + (long)i * nT + e] = in[n_ + 6L * e + i];
-        dev[n_ + 6L * nT + (long)i * nT + e] = in[n_ + 6L * nT + 6L * e + i];
+      for (int i = 0; i < ns; ++i) {
+        dev[n_ + (long)i * nT + e] = in[nr + ns * e + i];
+        dev[n_ + 6L * nT + (long)i * nT + e] = in[nr + ns * nT + ns * e + i];
       }
     MFEM_HIP_CHECK(hipMemcpyAsync(vecs_.data(), dev.data(), N_ * sizeof(double),
                                   hipMemcpyHostToDevice, st));
@@ -1256,11 +1262,12 @@ std::vector<double> MinresSolver<DIM>::apply(int which, const std::vector<double
   MFEM_HIP_CHECK(hipMemcpyAsync(dev.data(), vecs_.data() + N_, N_ * sizeof(double),
                                 hipMemcpyDeviceToHost, st));
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
-  std::copy(dev.begin(), dev.begin() + n_, out.begin());
+  for (int f = 0; f < nf; ++f)
+    for (int r = 0; r < D; ++r) out[(long)D * f + r] = dev[3L * f + r];
   for (int e = 0; e < nT; ++e)
-    for (int i = 0; i < 6; ++i) {
-      out[n_ + 6L * e + i] = dev[n_ + (long)i * nT + e];
-      out[n_ + 6L * nT + 6L * e + i] = dev[n_ + 6L * nT + (long)i * nT + e];
+    for (int i = 0; i < ns; ++i) {
+      out[nr + ns * e + i] = dev[n_ + (long)i * nT + e];
+      out[nr + ns * nT + ns * e + i] = dev[n_ + 6L * nT + (long)i * nT + e];
     }
   return out;
 }
@@ -1275,7 +1282,7 @@ void MinresSolver<DIM>::solve() {
   this->stats_.refinements = 1;
   this->stats_.rel_residual = estimated_error();
   // max |ds| into the slot the Newton decrement reads
-  launch_reduce_maxabs(6 * this->state_->mesh_->nT(), mixed_->ds_.data(),
+  launch_reduce_maxabs(ns_ * this->state_->mesh_->nT(), mixed_->ds_.data(),
                        c->reducer.slot(RS_MAXDS), c->stream);
 }
 

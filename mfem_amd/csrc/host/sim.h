@@ -398,10 +398,11 @@ class MinresSolver : public LinearSolver<DIM> {
   // |r|_P / |b| as estimated by the recurrence at exit
   double estimated_error() const { return last_.rhs2 > 0 ? std::sqrt(last_.res2 / last_.rhs2) : 0; }
   // Diagnostics at the current state, KKT vectors in the reference's layout
-  // [dx; ds; lambda] with element-major 6-vectors: which 0: K in, 1: P in,
+  // [dx (D per free vertex); ds; lambda] with element-major s-vectors (6 in
+  // 3D, 3 in 2D): which 0: K in, 1: P in,
   // 2: the right-hand side (in unused).
   std::vector<double> apply(int which, const std::vector<double>& in);
-  long kkt_size() const { return N_; }
+  long kkt_size() const { return Nref_; }  // reference layout: D nfree + 2 ns nT
 
  private:
   template <typename Real>
@@ -411,7 +412,8 @@ class MinresSolver : public LinearSolver<DIM> {
   template <typename Real>
   void apply_device(int which);
   MixedStretch<DIM>* mixed_ = nullptr;
-  long n_ = 0, N_ = 0;
+  long n_ = 0, N_ = 0, Nref_ = 0;  // device x part; device and reference vector sizes
+  int ns_ = 6;                     // entries of s per element (3 in 2D)
   DeviceArray<double> vecs_;   // v[2], w[2], p[3]: 7 KKT vectors
   DeviceArray<double> sinv_, mlump_inv_, mdinv_, cheb_, parts_;
   DeviceArray<mfemhip::MinresCtrl> ctrl_;

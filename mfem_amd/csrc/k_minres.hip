@@ -1,8 +1,10 @@
 // Indefinite (KKT) Newton system and MINRES ("minres-indefinite-block").
 //
-// Unknowns [dx (3 nfree); ds (6 x nT); lambda (6 x nT)], the element parts
-// stored structure-of-arrays (component i of tet e at i * nT + e) so that
-// they alias the mixed variable's ds and lambda arrays.
+// Unknowns [dx (3 nfree); ds (ns x nT); lambda (ns x nT)], ns = 6 on
+// tetrahedra and 3 on triangles, the element parts stored structure-of-arrays
+// (component i of element e at i * nT + e) so that they alias the mixed
+// variable's ds and lambda arrays. On triangles the z rows of dx hold only
+// the mass, which never couples to the rest, and stay exactly zero.
 //
 //   K = [[PMP, 0, Gx], [0, Hloc, C], [Gx^T, C, 0]]
 //   Gx = -PJW diag(dSdF), C = vol Sym, Hloc = vol Hpsi (Hpsi = h^2 Hessian)
@@ -41,7 +43,19 @@ constexpr int kVtxBS = 256;
 constexpr int kFinBS = 1024;
 constexpr int kGather = 8;  // indices / values in flight per gather batch
 
-__device__ __forceinline__ double symw(int i) { return i < 3 ? 1.0 : 2.0; }
+// Per-dimension element shape: corners, entries of s (Voigt, off-diagonals
+// weighted 2 by Sym), consistent-mass divisor (rho vol / 20 on a tet, / 12 on
+// a triangle), packed index of a symmetric ns x ns matrix (i <= j).
+template <int DIM>
+struct Elem {
+  static constexpr int NC = DIM + 1, NS = DIM == 3 ? 6 : 3, NF = DIM * DIM;
+  static constexpr double kMassDiv = DIM == 3 ? 20.0 : 12.0;
+  __device__ static double symw(int i) { return i < DIM ? 1.0 : 2.0; }
+  __device__ static int sym(int i, int j) {
+    if (i > j) { const int t = i; i = j; j = t; }
+    return DIM == 3 ? sym6(i, j) : sym3(i, j);
+  }
+};
 
 // XCD-contiguous work split: workgroup b runs on XCD b % 8 (round-robin
 // dispatch), and XCD x owns items [n x / 8, n (x + 1) / 8), so that the
@@ -68,118 +82,122 @@ __device__ __forceinline__ double fin_sum(const double* parts, int n, double* sm
 // ---------------------------------------------------------------- setup
 // S_e = Sym (vol^2 B Lambda B^T + vol Hpsi^-1) Sym with B = dsdF Jloc and
 // Lambda = 1 / Mlump on free corners (0 on pinned ones), inverted by Cholesky
-// into Sinv (21 packed, stride nT). A block that is not positive definite
-// becomes NaN, which the MINRES scalars report.
-template <typename Real>
+// into Sinv (packed symmetric, stride nT). A block that is not positive
+// definite becomes NaN, which the MINRES scalars report.
+template <typename Real, int DIM>
 __global__ __launch_bounds__(kTetBS) void k_kkt_setup(TetMeshDev m, const Real* __restrict__ dsdF,
                                                       const Real* __restrict__ Hinv,
                                                       const double* __restrict__ mlump_inv,
                                                       double* __restrict__ Sinv) {
+  using E = Elem<DIM>;
+  constexpr int NC = E::NC, NS = E::NS;
   const int e = xcd_index(m.nT);
   if (e >= m.nT) return;
   const int nT = m.nT;
   const double vol = m.vol[e];
   const int4 t = m.T[e];
   const int v[4] = {t.x, t.y, t.z, t.w};
-  double lam[4];
+  double lam[NC];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < NC; ++k) {
     const int f = m.free_map[v[k]];
     lam[k] = f >= 0 ? mlump_inv[f] : 0.0;
   }
   // G(c, c') = sum_k Lambda_k dX(k, c) dX(k, c')
-  double dx[12];
+  double dx[3 * NC];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) dx[i] = m.dX[i * nT + e];
-  double G[9];
+  for (int k = 0; k < NC; ++k)
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < DIM; ++c) dx[3 * k + c] = m.dX[(3 * k + c) * nT + e];
+  double G[DIM * DIM];
 #pragma unroll
-    for (int c2 = 0; c2 < 3; ++c2) {
+  for (int c = 0; c < DIM; ++c)
+#pragma unroll
+    for (int c2 = 0; c2 < DIM; ++c2) {
       double a = 0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) a += lam[k] * dx[3 * k + c] * dx[3 * k + c2];
-      G[c + 3 * c2] = a;
+      for (int k = 0; k < NC; ++k) a += lam[k] * dx[3 * k + c] * dx[3 * k + c2];
+      G[c + DIM * c2] = a;
     }
-  // B Lambda B^T = dsdF (I_3 (x) G) dsdF^T, one row r of F at a time
-  double A[36];
+  // B Lambda B^T = dsdF (I (x) G) dsdF^T, one row r of F at a time
+  double A[NS * NS];
 #pragma unroll
-  for (int i = 0; i < 36; ++i) A[i] = 0.0;
+  for (int i = 0; i < NS * NS; ++i) A[i] = 0.0;
 #pragma unroll 1
-  for (int r = 0; r < 3; ++r) {
-    double J[18];  // J[i + 6c] = dsdF(i, r + 3c)
+  for (int r = 0; r < DIM; ++r) {
+    double J[NS * DIM];  // J[i + NS c] = dsdF(i, r + DIM c)
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < DIM; ++c)
 #pragma unroll
-      for (int i = 0; i < 6; ++i) J[i + 6 * c] = (double)dsdF[(i + 6 * (r + 3 * c)) * nT + e];
-    double T[18];  // T = J G
+      for (int i = 0; i < NS; ++i) J[i + NS * c] = (double)dsdF[(i + NS * (r + DIM * c)) * nT + e];
+    double T[NS * DIM];  // T = J G
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < DIM; ++c)
 #pragma unroll
-      for (int i = 0; i < 6; ++i) {
+      for (int i = 0; i < NS; ++i) {
         double a = 0;
 #pragma unroll
-        for (int c2 = 0; c2 < 3; ++c2) a += J[i + 6 * c2] * G[c2 + 3 * c];
-        T[i + 6 * c] = a;
+        for (int c2 = 0; c2 < DIM; ++c2) a += J[i + NS * c2] * G[c2 + DIM * c];
+        T[i + NS * c] = a;
       }
 #pragma unroll
-    for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < NS; ++i)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
+      for (int j = 0; j < NS; ++j) {
         double a = 0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a += T[i + 6 * c] * J[j + 6 * c];
-        A[i + 6 * j] += a;
+        for (int c = 0; c < DIM; ++c) a += T[i + NS * c] * J[j + NS * c];
+        A[i + NS * j] += a;
       }
   }
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < NS; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const double h = (double)Hinv[sym6(i < j ? i : j, i < j ? j : i) * nT + e];
-      A[i + 6 * j] = symw(i) * (vol * vol * A[i + 6 * j] + vol * h) * symw(j);
+    for (int j = 0; j < NS; ++j) {
+      const double h = (double)Hinv[E::sym(i, j) * nT + e];
+      A[i + NS * j] = E::symw(i) * (vol * vol * A[i + NS * j] + vol * h) * E::symw(j);
     }
   // Cholesky A = L L^T (lower, in place), then Sinv = L^-T L^-1
   bool ok = true;
 #pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    double d = A[j + 6 * j];
+  for (int j = 0; j < NS; ++j) {
+    double d = A[j + NS * j];
 #pragma unroll
-    for (int k = 0; k < j; ++k) d -= A[j + 6 * k] * A[j + 6 * k];
+    for (int k = 0; k < j; ++k) d -= A[j + NS * k] * A[j + NS * k];
     ok = ok && d > 0.0;
     d = sqrt(fmax(d, 0.0));
-    A[j + 6 * j] = d;
+    A[j + NS * j] = d;
     const double id = 1.0 / d;
 #pragma unroll
-    for (int i = j + 1; i < 6; ++i) {
-      double s = A[i + 6 * j];
+    for (int i = j + 1; i < NS; ++i) {
+      double s = A[i + NS * j];
 #pragma unroll
-      for (int k = 0; k < j; ++k) s -= A[i + 6 * k] * A[j + 6 * k];
-      A[i + 6 * j] = s * id;
+      for (int k = 0; k < j; ++k) s -= A[i + NS * k] * A[j + NS * k];
+      A[i + NS * j] = s * id;
     }
   }
-  double Li[36];  // L^-1 (lower)
+  double Li[NS * NS];  // L^-1 (lower)
 #pragma unroll
-  for (int i = 0; i < 36; ++i) Li[i] = 0.0;
+  for (int i = 0; i < NS * NS; ++i) Li[i] = 0.0;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    Li[c + 6 * c] = 1.0 / A[c + 6 * c];
+  for (int c = 0; c < NS; ++c) {
+    Li[c + NS * c] = 1.0 / A[c + NS * c];
 #pragma unroll
-    for (int i = c + 1; i < 6; ++i) {
+    for (int i = c + 1; i < NS; ++i) {
       double s = 0;
 #pragma unroll
-      for (int k = c; k < i; ++k) s -= A[i + 6 * k] * Li[k + 6 * c];
-      Li[i + 6 * c] = s / A[i + 6 * i];
+      for (int k = c; k < i; ++k) s -= A[i + NS * k] * Li[k + NS * c];
+      Li[i + NS * c] = s / A[i + NS * i];
     }
   }
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < NS; ++i)
 #pragma unroll
-    for (int j = i; j < 6; ++j) {
+    for (int j = i; j < NS; ++j) {
       double s = 0;
 #pragma unroll
-      for (int k = j; k < 6; ++k) s += Li[k + 6 * i] * Li[k + 6 * j];
-      Sinv[sym6(i, j) * nT + e] = ok ? s : NAN;
+      for (int k = j; k < NS; ++k) s += Li[k + NS * i] * Li[k + NS * j];
+      Sinv[E::sym(i, j) * nT + e] = ok ? s : NAN;
     }
 }
 
@@ -187,6 +205,7 @@ __global__ __launch_bounds__(kTetBS) void k_kkt_setup(TetMeshDev m, const Real* 
 // b = [-PM (x_full - x_tilde - h^2 g); -vol g; vol Sym (S - s)]
 // (linear_system.h:67-86): the inertia part per tet into fe (gathered per
 // free vertex by k_kkt_rhs_vtx), the element parts straight into b.
+template <int DIM>
 __global__ __launch_bounds__(kTetBS) void k_kkt_rhs_tet(TetMeshDev m, const double* __restrict__ x,
                                                         Inertia in, const double* __restrict__ g,
                                                         const double* __restrict__ S,
@@ -197,27 +216,29 @@ __global__ __launch_bounds__(kTetBS) void k_kkt_rhs_tet(TetMeshDev m, const doub
   const int e = xcd_index(m.nT);
   double acc = 0.0;
   if (e < m.nT) {
+    using E = Elem<DIM>;
+    constexpr int NC = E::NC;
     const int nT = m.nT;
     const double vol = m.vol[e];
     const int4 t = m.T[e];
     const int v[4] = {t.x, t.y, t.z, t.w};
-    double d[12], dsum[3] = {0, 0, 0};
+    double d[3 * NC], dsum[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NC; ++k)
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         const double dd = x[3 * v[k] + r] - in.x_tilde[3 * v[k] + r] - in.h2g[r];
         d[3 * k + r] = dd;
         dsum[r] += dd;
       }
-    const double mc = m.density * vol / 20.0;
+    const double mc = m.density * vol / E::kMassDiv;
     double* fo = fe + (size_t)e * 12;
 #pragma unroll
-    for (int j = 0; j < 12; ++j) fo[j] = -mc * (d[j] + dsum[j % 3]);
+    for (int j = 0; j < 3 * NC; ++j) fo[j] = -mc * (d[j] + dsum[j % 3]);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < E::NS; ++i) {
       const double bs = -vol * g[i * nT + e];
-      const double bl = vol * symw(i) * (S[i * nT + e] - s[i * nT + e]);
+      const double bl = vol * E::symw(i) * (S[i * nT + e] - s[i * nT + e]);
       b.s[i * nT + e] = bs;
       b.l[i * nT + e] = bl;
       acc += bs * bs + bl * bl;
@@ -284,19 +305,21 @@ struct MrVecs {
   KktVec x;   // solution
 };
 
-template <typename Real>
+template <typename Real, int DIM>
 __global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtrl* __restrict__ ctrl,
                                                    const Real* __restrict__ dsdF,
                                                    const Real* __restrict__ Hpsi, MrVecs V,
                                                    double* __restrict__ fe,
                                                    double* __restrict__ parts) {
+  using E = Elem<DIM>;
+  constexpr int NC = E::NC, NS = E::NS, NF = E::NF;
   __shared__ double sm[kTetBS / kWave];
   const MinresCtrl c = *ctrl;
   const int e = xcd_index(m.nT);
   const int nT = m.nT;
   if (c.pending && e < nT) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < NS; ++i) {
       pending_update(c, (long)i * nT + e, V.wo.s, V.pa.s, V.pb.s, V.pn.s, V.x.s);
       pending_update(c, (long)i * nT + e, V.wo.l, V.pa.l, V.pb.l, V.pn.l, V.x.l);
     }
@@ -307,65 +330,65 @@ __global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtr
     const double vol = m.vol[e];
     const double ib = 1.0 / c.beta;
     const double ratio = c.beta / c.beta_prev;
-    double ws[6], wl[6];
+    double ws[NS], wl[NS];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < NS; ++i) {
       ws[i] = V.wc.s[i * nT + e] * ib;
       wl[i] = V.wc.l[i * nT + e] * ib;
     }
     // Jloc w_x over the free corners
     const int4 t = m.T[e];
     const int v[4] = {t.x, t.y, t.z, t.w};
-    double Fw[9];
+    double Fw[NF];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) Fw[i] = 0.0;
+    for (int i = 0; i < NF; ++i) Fw[i] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NC; ++k) {
       const int f = m.free_map[v[k]];
-      double wk[3];
+      double wk[DIM];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) wk[r] = f >= 0 ? V.wc.x[3 * f + r] * ib : 0.0;
+      for (int r = 0; r < DIM; ++r) wk[r] = f >= 0 ? V.wc.x[3 * f + r] * ib : 0.0;
 #pragma unroll
-      for (int cc = 0; cc < 3; ++cc) {
+      for (int cc = 0; cc < DIM; ++cc) {
         const double d = m.dX[(3 * k + cc) * nT + e];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) Fw[r + 3 * cc] += wk[r] * d;
+        for (int r = 0; r < DIM; ++r) Fw[r + DIM * cc] += wk[r] * d;
       }
     }
     // u = dsdF Fw and tv = dsdF^T (Sym w_l), one pass over dsdF
-    double u[6], tv[9];
+    double u[NS], tv[NF];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) tv[i] = 0.0;
+    for (int i = 0; i < NF; ++i) tv[i] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) u[i] = 0.0;
+    for (int i = 0; i < NS; ++i) u[i] = 0.0;
 #pragma unroll
-    for (int mm = 0; mm < 9; ++mm)
+    for (int mm = 0; mm < NF; ++mm)
 #pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        const double js = (double)dsdF[(i + 6 * mm) * nT + e];
+      for (int i = 0; i < NS; ++i) {
+        const double js = (double)dsdF[(i + NS * mm) * nT + e];
         u[i] += js * Fw[mm];
-        tv[mm] += js * symw(i) * wl[i];
+        tv[mm] += js * E::symw(i) * wl[i];
       }
-    // x rows of this tet: -vol Jloc^T tv
+    // x rows of this element: -vol Jloc^T tv (the z rows of a triangle: 0)
     double* fo = fe + (size_t)e * 12;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NC; ++k)
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         double a = 0;
+        if (r < DIM)
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) a += m.dX[(3 * k + cc) * nT + e] * tv[r + 3 * cc];
+          for (int cc = 0; cc < DIM; ++cc) a += m.dX[(3 * k + cc) * nT + e] * tv[r + DIM * cc];
         fo[3 * k + r] = -vol * a;
       }
     // s rows: Hloc w_s + vol Sym w_l ; lambda rows: vol Sym (w_s - u)
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < NS; ++i) {
       double a = 0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j)
-        a += (double)Hpsi[sym6(i < j ? i : j, i < j ? j : i) * nT + e] * ws[j];
-      const double ys = vol * a + vol * symw(i) * wl[i];
-      const double yl = vol * symw(i) * (ws[i] - u[i]);
+      for (int j = 0; j < NS; ++j) a += (double)Hpsi[E::sym(i, j) * nT + e] * ws[j];
+      const double ys = vol * a + vol * E::symw(i) * wl[i];
+      const double yl = vol * E::symw(i) * (ws[i] - u[i]);
       const long is = (long)i * nT + e;
       // v_new = K w - beta v_old, v_old = V_{k-1} / beta_{k-1}
       const double vs = ys - ratio * V.vo.s[is];
@@ -467,12 +490,14 @@ __global__ __launch_bounds__(kFinBS) void k_mr_fin_alpha(MinresCtrl* ctrl, const
 
 // v_new -= alpha v_k (init: alpha = 0, v_k absent); w_new = P v_new on the
 // element parts: Hloc^-1 = Hpsi^-1 / vol on s, S_e^-1 on lambda.
-template <typename Real>
+template <typename Real, int DIM>
 __global__ __launch_bounds__(kTetBS) void k_mr_prec_tet(TetMeshDev m, const MinresCtrl* __restrict__ ctrl,
                                                         int init, const Real* __restrict__ Hinv,
                                                         const double* __restrict__ Sinv, KktVec vn,
                                                         KktVec vc, KktVec wn,
                                                         double* __restrict__ parts) {
+  using E = Elem<DIM>;
+  constexpr int NS = E::NS;
   __shared__ double sm[kTetBS / kWave];
   const MinresCtrl c = *ctrl;
   if (!init && c.done) return;
@@ -482,9 +507,9 @@ __global__ __launch_bounds__(kTetBS) void k_mr_prec_tet(TetMeshDev m, const Minr
   if (e < nT) {
     const double ab = init ? 0.0 : c.alpha / c.beta;
     const double vol = m.vol[e];
-    double vs[6], vl[6];
+    double vs[NS], vl[NS];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < NS; ++i) {
       const long is = (long)i * nT + e;
       vs[i] = init ? vn.s[is] : vn.s[is] - ab * vc.s[is];
       vl[i] = init ? vn.l[is] : vn.l[is] - ab * vc.l[is];
@@ -495,11 +520,11 @@ __global__ __launch_bounds__(kTetBS) void k_mr_prec_tet(TetMeshDev m, const Minr
     }
     const double iv = 1.0 / vol;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < NS; ++i) {
       double a = 0, b = 0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const int k = sym6(i < j ? i : j, i < j ? j : i);
+      for (int j = 0; j < NS; ++j) {
+        const int k = E::sym(i, j);
         a += (double)Hinv[k * nT + e] * vs[j];
         b += Sinv[k * nT + e] * vl[j];
       }
@@ -683,8 +708,12 @@ int kkt_vertex_blocks(int nfree) { return blocks_for(nfree, kVtxBS); }
 template <typename Real>
 void launch_kkt_setup(const KktSystem<Real>& S, hipStream_t st) {
   if (S.m.nT == 0) return;
-  hipLaunchKernelGGL(k_kkt_setup<Real>, dim3(kkt_tet_blocks(S.m.nT)), dim3(kTetBS), 0, st, S.m,
-                     S.dsdF, S.Hinv, S.mlump_inv, S.Sinv);
+  if (S.m.dim == 2)
+    hipLaunchKernelGGL((k_kkt_setup<Real, 2>), dim3(kkt_tet_blocks(S.m.nT)), dim3(kTetBS), 0, st,
+                       S.m, S.dsdF, S.Hinv, S.mlump_inv, S.Sinv);
+  else
+    hipLaunchKernelGGL((k_kkt_setup<Real, 3>), dim3(kkt_tet_blocks(S.m.nT)), dim3(kTetBS), 0, st,
+                       S.m, S.dsdF, S.Hinv, S.mlump_inv, S.Sinv);
 }
 
 // Chebyshev constants for the mass block over [1/2, 5/2]: theta 1.5,
@@ -698,8 +727,12 @@ void launch_precondition(const KktSystem<Real>& S, int init, KktVec vn, KktVec v
                          hipStream_t st) {
   const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
   if (nbT > 0)
-    hipLaunchKernelGGL(k_mr_prec_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, init,
-                       S.Hinv, S.Sinv, vn, vc, wn, S.parts[1]);
+    if (S.m.dim == 2)
+      hipLaunchKernelGGL((k_mr_prec_tet<Real, 2>), dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl,
+                         init, S.Hinv, S.Sinv, vn, vc, wn, S.parts[1]);
+    else
+      hipLaunchKernelGGL((k_mr_prec_tet<Real, 3>), dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl,
+                         init, S.Hinv, S.Sinv, vn, vc, wn, S.parts[1]);
   if (nbV == 0) return;
   hipLaunchKernelGGL(k_mr_prec_vtx0, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.ctrl, init,
                      S.mdinv, kChebTheta, vn, vc, wn, S.cheb_r, S.cheb_d[0]);
@@ -723,8 +756,12 @@ void launch_minres_init(const KktSystem<Real>& S, const double* x_full, const In
                         double tol, int max_iters, hipStream_t st) {
   const int nbT = kkt_tet_blocks(S.m.nT), nbV = kkt_vertex_blocks(S.m.nfree);
   if (nbT > 0)
-    hipLaunchKernelGGL(k_kkt_rhs_tet, dim3(nbT), dim3(kTetBS), 0, st, S.m, x_full, in, g, Smat, s,
-                       S.fe, b, S.parts[0]);
+    if (S.m.dim == 2)
+      hipLaunchKernelGGL((k_kkt_rhs_tet<2>), dim3(nbT), dim3(kTetBS), 0, st, S.m, x_full, in, g,
+                         Smat, s, S.fe, b, S.parts[0]);
+    else
+      hipLaunchKernelGGL((k_kkt_rhs_tet<3>), dim3(nbT), dim3(kTetBS), 0, st, S.m, x_full, in, g,
+                         Smat, s, S.fe, b, S.parts[0]);
   if (nbV > 0)
     hipLaunchKernelGGL(k_kkt_rhs_vtx, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.inc_ptr,
                        S.inc, S.fe, b, S.parts[0] + nbT);
@@ -748,8 +785,12 @@ void launch_minres_iteration(const KktSystem<Real>& S, const MinresBuffers& B, l
   V.x = B.x;
   if (tet_ev) (void)hipEventRecord(tet_ev[0], st);
   if (nbT > 0)
-    hipLaunchKernelGGL(k_mr_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
-                       S.Hpsi, V, S.fe, S.parts[0]);
+    if (S.m.dim == 2)
+      hipLaunchKernelGGL((k_mr_tet<Real, 2>), dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
+                         S.Hpsi, V, S.fe, S.parts[0]);
+    else
+      hipLaunchKernelGGL((k_mr_tet<Real, 3>), dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
+                         S.Hpsi, V, S.fe, S.parts[0]);
   if (tet_ev) (void)hipEventRecord(tet_ev[1], st);
   if (nbV > 0)
     hipLaunchKernelGGL(k_mr_vertex, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.rowptr, S.col,
@@ -780,8 +821,12 @@ void launch_kkt_apply(const KktSystem<Real>& S, int which, KktVec in, KktVec out
   V.vo = out;
   V.pa = V.pb = V.pn = V.x = scratch;
   if (nbT > 0)
-    hipLaunchKernelGGL(k_mr_tet<Real>, dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
-                       S.Hpsi, V, S.fe, S.parts[0]);
+    if (S.m.dim == 2)
+      hipLaunchKernelGGL((k_mr_tet<Real, 2>), dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
+                         S.Hpsi, V, S.fe, S.parts[0]);
+    else
+      hipLaunchKernelGGL((k_mr_tet<Real, 3>), dim3(nbT), dim3(kTetBS), 0, st, S.m, S.ctrl, S.dsdF,
+                         S.Hpsi, V, S.fe, S.parts[0]);
   if (nbV > 0)
     hipLaunchKernelGGL(k_mr_vertex, dim3(nbV), dim3(kVtxBS), 0, st, S.m.nfree, S.rowptr, S.col,
                        S.mass, S.inc_ptr, S.inc, S.fe, S.ctrl, V, S.parts[0] + nbT);

@@ -235,7 +235,8 @@ struct MinresCtrl {
   double r1, r2, r3, coef;           // update of the last iteration: p and x
   int iter, done, max_iters, pending, fail, pad_;
 };
-// A KKT vector: x part (3 nfree) and the element parts (6 x nT, stride nT).
+// A KKT vector: x part (3 nfree) and the element parts (ns x nT, stride nT;
+// ns = 6 on tetrahedra, 3 on triangles, storage 6 x nT either way).
 struct KktVec {
   double* x = nullptr;
   double* s = nullptr;

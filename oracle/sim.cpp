@@ -206,7 +206,7 @@ struct OracleSim {
   std::vector<double> mlump, mdinv;   // per free dof: row sum and 1/diag of PMP
   std::vector<double> mchol;          // dense Cholesky of PMP (exact preconditioner)
   std::vector<double> lchol;          // dense Cholesky of L (exact preconditioner)
-  std::vector<Mat<6, 6>> linv_blk;    // element blocks of L, inverted
+  std::vector<double> linv_blk;       // element blocks of L, inverted (ns x ns row-major each)
 };
 
 namespace {
@@ -959,8 +959,10 @@ int dense_llt(OracleSim* S_, std::vector<double>& xo) {
 }
 
 // ---------------- indefinite (KKT) solve: "minres-indefinite-block" -------
-// Unknowns [dx (n); ds (6 nT); lambda (6 nT)], 6-vectors element-major, as
-// SystemMatrixIndefinite lays them out (reference linear_system.h:60-99).
+// Unknowns [dx (n); ds (ns nT); lambda (ns nT)], ns = 6 in 3D and 3 in 2D,
+// element-major, as SystemMatrixIndefinite lays them out (reference
+// linear_system.h:60-99; the factory registers it for DIM = 2 and 3,
+// linear_solver_factory.cpp:117-133, 150-151).
 
 // In-place dense Cholesky (lower, row-major n x n); false if not PD.
 bool dense_chol(std::vector<double>& A, int n) {
@@ -994,22 +996,55 @@ void dense_chol_solve(const std::vector<double>& L, int n, const double* b, doub
   }
 }
 
+// The mixed variable's element data per dimension: 3D 6-vectors with a 9 x 6
+// dSdF, 2D 3-vectors with a 4 x 3 dSdF (the DIM = 3 and DIM = 2 instances of
+// MixedStretch).
+template <int D> struct MixedView;
+template <> struct MixedView<3> {
+  static constexpr int NS = 6, NF = 9, NC = 4;
+  static double sym(int i) { return kSym[i]; }
+  static const Mat<9, 6>& dSdF(const OracleSim* S_, int e) { return S_->dSdF[e]; }
+  static const Mat<6, 6>& Hloc(const OracleSim* S_, int e) { return S_->Hloc[e]; }
+  static const Mat<6, 6>& Hinv(const OracleSim* S_, int e) { return S_->Hinv[e]; }
+  static double g(const OracleSim* S_, int e, int i) { return S_->g[e][i]; }
+  static double S(const OracleSim* S_, int e, int i) { return S_->S[e][i]; }
+};
+template <> struct MixedView<2> {
+  static constexpr int NS = 3, NF = 4, NC = 3;
+  static double sym(int i) { return kSym3[i]; }
+  static const Mat<4, 3>& dSdF(const OracleSim* S_, int e) { return S_->dSdF2[e]; }
+  static const Mat<3, 3>& Hloc(const OracleSim* S_, int e) { return S_->Hloc3[e]; }
+  static const Mat<3, 3>& Hinv(const OracleSim* S_, int e) { return S_->Hinv3[e]; }
+  static double g(const OracleSim* S_, int e, int i) { return S_->g3[e][i]; }
+  static double S(const OracleSim* S_, int e, int i) { return S_->S3[e][i]; }
+};
+
+int kkt_ns(const OracleSim* S_) { return S_->dim == 2 ? 3 : 6; }
+
 // rhs = [-grad_x; -vol g; vol Sym (S - s)]: linear_system.h:67-86 with
 // Displacement::gradient (displacement.cpp:64-73), grad_ = vol g
 // (mixed_stretch.cpp:195-203) and grad_la_ = -vol Sym (S - s)
 // (mixed_stretch.cpp:127-131, 313-332).
-void kkt_rhs(OracleSim* S_, std::vector<double>& b) {
-  const int n = S_->n, nT = S_->nT;
-  b.assign((size_t)n + 12 * nT, 0.0);
+template <int D>
+void kkt_rhs_t(OracleSim* S_, std::vector<double>& b) {
+  using V = MixedView<D>;
+  const int n = S_->n, nT = S_->nT, ns = V::NS;
+  b.assign((size_t)n + 2 * ns * (size_t)nT, 0.0);
   disp_gradient(S_);
   for (int i = 0; i < n; ++i) b[i] = -S_->grad_x_disp[i];
   for (int e = 0; e < nT; ++e) {
     const double vol = S_->vols[e];
-    for (int i = 0; i < 6; ++i) {
-      b[n + 6 * e + i] = -vol * S_->g[e][i];
-      b[n + 6 * nT + 6 * e + i] = vol * kSym[i] * (S_->S[e][i] - S_->s[6 * e + i]);
+    for (int i = 0; i < ns; ++i) {
+      b[n + (size_t)ns * e + i] = -vol * V::g(S_, e, i);
+      b[n + (size_t)ns * nT + (size_t)ns * e + i] =
+          vol * V::sym(i) * (V::S(S_, e, i) - S_->s[(size_t)ns * e + i]);
     }
   }
+}
+
+void kkt_rhs(OracleSim* S_, std::vector<double>& b) {
+  if (S_->dim == 2) kkt_rhs_t<2>(S_, b);
+  else kkt_rhs_t<3>(S_, b);
 }
 
 // y = K w with K = [[PMP, 0, Gx], [0, Hloc, C], [Gx^T, C, 0]],
@@ -1018,26 +1053,29 @@ void kkt_rhs(OracleSim* S_, std::vector<double>& b) {
 // Displacement::product_hessian (displacement.h:59-63) and
 // MixedStretch::product_hessian / product_jacobian_x / product_jacobian_mixed
 // (mixed_stretch.cpp:373-434), accumulated in the same order.
-void kkt_product(const OracleSim* S_, const double* w, double* y) {
+template <int D>
+void kkt_product_t(const OracleSim* S_, const double* w, double* y) {
+  using V = MixedView<D>;
+  constexpr int ns = V::NS, nf = V::NF;
   const int n = S_->n, nT = S_->nT;
   const double* wx = w;
   const double* ws = w + n;
-  const double* wl = w + n + 6 * nT;
+  const double* wl = w + n + (size_t)ns * nT;
   double* yx = y;
   double* ys = y + n;
-  double* yl = y + n + 6 * nT;
-  std::fill(y, y + (size_t)n + 12 * nT, 0.0);
+  double* yl = y + n + (size_t)ns * nT;
+  std::fill(y, y + (size_t)n + 2 * ns * (size_t)nT, 0.0);
   // x rows: PMP dx, then -= PJW (dSdF la)
   {
     std::vector<double> xv(wx, wx + n), t;
     csr_mult(S_, S_->PMP, xv, t);
     for (int i = 0; i < n; ++i) yx[i] += t[i];
-    std::vector<double> tmp(9 * (size_t)nT);
+    std::vector<double> tmp((size_t)nf * nT);
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < nT; ++e) {
-      Vec<6> l; for (int i = 0; i < 6; ++i) l[i] = wl[6 * e + i];
-      Vec<9> v = S_->dSdF[e] * l;
-      for (int i = 0; i < 9; ++i) tmp[9 * (size_t)e + i] = v[i];
+      Vec<ns> l; for (int i = 0; i < ns; ++i) l[i] = wl[(size_t)ns * e + i];
+      Vec<nf> v = V::dSdF(S_, e) * l;
+      for (int i = 0; i < nf; ++i) tmp[(size_t)nf * e + i] = v[i];
     }
     std::vector<double> acc(n, 0.0);
     apply_PJW(S_, tmp, acc, -1.0);
@@ -1047,39 +1085,45 @@ void kkt_product(const OracleSim* S_, const double* w, double* y) {
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < nT; ++e) {
     const double vol = S_->vols[e];
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < ns; ++i) {
       double a = 0;
-      for (int j = 0; j < 6; ++j) a += S_->Hloc[e](i, j) * ws[6 * e + j];
-      ys[6 * e + i] += a;
+      for (int j = 0; j < ns; ++j) a += V::Hloc(S_, e)(i, j) * ws[(size_t)ns * e + j];
+      ys[(size_t)ns * e + i] += a;
     }
-    for (int i = 0; i < 6; ++i) ys[6 * e + i] += vol * kSym[i] * wl[6 * e + i];
+    for (int i = 0; i < ns; ++i) ys[(size_t)ns * e + i] += vol * V::sym(i) * wl[(size_t)ns * e + i];
   }
   // la rows: -= dSdF^T (PJW^T dx), then += vol Sym ds
   {
-    std::vector<double> full(S_->nV * 3, 0.0);
+    std::vector<double> full((size_t)S_->nV * D, 0.0);
     for (int v = 0; v < S_->nV; ++v) {
       int f = S_->free_map[v];
       if (f < 0) continue;
-      for (int r = 0; r < 3; ++r) full[3 * v + r] = wx[3 * f + r];
+      for (int r = 0; r < D; ++r) full[(size_t)D * v + r] = wx[(size_t)D * f + r];
     }
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < nT; ++e) {
       double Fd[9]; def_grad(S_, full, e, Fd);
-      Vec<9> t; for (int i = 0; i < 9; ++i) t[i] = S_->vols[e] * Fd[i];
-      Vec<6> u = S_->dSdF[e].T() * t;
-      for (int i = 0; i < 6; ++i) yl[6 * e + i] -= u[i];
+      Vec<nf> t; for (int i = 0; i < nf; ++i) t[i] = S_->vols[e] * Fd[i];
+      Vec<ns> u = V::dSdF(S_, e).T() * t;
+      for (int i = 0; i < ns; ++i) yl[(size_t)ns * e + i] -= u[i];
     }
     for (int e = 0; e < nT; ++e)
-      for (int i = 0; i < 6; ++i) yl[6 * e + i] += S_->vols[e] * kSym[i] * ws[6 * e + i];
+      for (int i = 0; i < ns; ++i)
+        yl[(size_t)ns * e + i] += S_->vols[e] * V::sym(i) * ws[(size_t)ns * e + i];
   }
+}
+
+void kkt_product(const OracleSim* S_, const double* w, double* y) {
+  if (S_->dim == 2) kkt_product_t<2>(S_, w, y);
+  else kkt_product_t<3>(S_, w, y);
 }
 
 // Chebyshev steps of the mass block: kMassCheb steps of Chebyshev
 // acceleration of Jacobi on PMP (Saad, Iterative Methods for Sparse Linear
 // Systems, 2nd ed., Alg. 12.1) over the interval [1/2, 5/2], which holds the
-// spectrum of diag(M)^-1 M for linear tets (Wathen 1987; the element matrix
-// (1 + delta_ab)/2 has eigenvalues 1/2 and 5/2, and a principal submatrix
-// stays inside). The result is p(D^-1 M) D^-1 b for a fixed polynomial p,
+// spectrum of diag(M)^-1 M for linear tets and triangles (Wathen 1987: the
+// scaled element matrix (1 + delta_ab)/2 has eigenvalues 1/2 and 5/2 on a
+// tet, 1/2 and 2 on a triangle, and a principal submatrix stays inside). The result is p(D^-1 M) D^-1 b for a fixed polynomial p,
 // positive on the interval, so the operator is symmetric positive definite.
 constexpr int kMassCheb = 4;
 
@@ -1125,18 +1169,20 @@ void kkt_mass_setup(OracleSim* S_, bool exact) {
   }
 }
 
-// Gx column block of element e restricted to its corners: G_e(3k + r, j) =
-// -vol dX(k, c) dSdF(r + 3c, j) for free corner k (Gx = -PJW diag(dSdF),
+// Gx column block of element e restricted to its corners: G_e(D k + r, j) =
+// -vol dX(k, c) dSdF(r + D c, j) for free corner k (Gx = -PJW diag(dSdF),
 // mixed_stretch.cpp:342-348).
+template <int D>
 void gx_block(const OracleSim* S_, int e, double G[12][6]) {
+  using V = MixedView<D>;
   const double vol = S_->vols[e];
-  const double* dx = &S_->dX[(size_t)e * 12];
-  for (int k = 0; k < 4; ++k)
-    for (int r = 0; r < 3; ++r)
-      for (int j = 0; j < 6; ++j) {
+  const double* dx = &S_->dX[(size_t)e * V::NC * D];
+  for (int k = 0; k < V::NC; ++k)
+    for (int r = 0; r < D; ++r)
+      for (int j = 0; j < V::NS; ++j) {
         double a = 0;
-        for (int c = 0; c < 3; ++c) a += dx[3 * k + c] * S_->dSdF[e](r + 3 * c, j);
-        G[3 * k + r][j] = -vol * a;
+        for (int c = 0; c < D; ++c) a += dx[D * k + c] * V::dSdF(S_, e)(r + D * c, j);
+        G[D * k + r][j] = -vol * a;
       }
 }
 
@@ -1147,40 +1193,46 @@ void gx_block(const OracleSim* S_, int e, double G[12][6]) {
 // (SIAM J. Sci. Comput. 21, 2000) with the Schur complement
 // Gx^T M^-1 Gx + C Hloc^-1 C replaced by its element blocks, lumped mass in
 // place of M: S_e = Gx_e^T Mlump^-1 Gx_e + vol Sym Hpsi^-1 Sym, inverted.
-void kkt_refresh_L(OracleSim* S_, bool exact) {
+template <int D>
+void kkt_refresh_L_t(OracleSim* S_, bool exact) {
+  using V = MixedView<D>;
+  constexpr int ns = V::NS, nc = V::NC;
   const int nT = S_->nT;
-  S_->linv_blk.assign(nT, Mat<6, 6>());
+  S_->linv_blk.assign((size_t)nT * ns * ns, 0.0);
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < nT; ++e) {
     double G[12][6];
-    gx_block(S_, e, G);
-    Mat<6, 6> Le;
-    for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) {
+    gx_block<D>(S_, e, G);
+    Mat<ns, ns> Le;
+    for (int i = 0; i < ns; ++i)
+      for (int j = 0; j < ns; ++j) {
         double a = 0;
-        for (int k = 0; k < 4; ++k) {
-          int f = S_->free_map[S_->T[4 * e + k]];
+        for (int k = 0; k < nc; ++k) {
+          int f = S_->free_map[S_->T[(size_t)nc * e + k]];
           if (f < 0) continue;
-          for (int r = 0; r < 3; ++r) a += G[3 * k + r][i] * G[3 * k + r][j] / S_->mlump[3 * f + r];
+          for (int r = 0; r < D; ++r)
+            a += G[D * k + r][i] * G[D * k + r][j] / S_->mlump[(size_t)D * f + r];
         }
         Le(i, j) = a + (exact ? (i == j ? 1e-8 : 0.0)
-                              : S_->vols[e] * kSym[i] * S_->Hinv[e](i, j) * kSym[j]);
+                              : S_->vols[e] * V::sym(i) * V::Hinv(S_, e)(i, j) * V::sym(j));
       }
-    S_->linv_blk[e] = lu_inverse<6>(Le);
+    const Mat<ns, ns> Li = lu_inverse<ns>(Le);
+    for (int i = 0; i < ns; ++i)
+      for (int j = 0; j < ns; ++j) S_->linv_blk[((size_t)e * ns + i) * ns + j] = Li(i, j);
   }
   if (!exact) return;
-  const int m = 6 * nT;
-  // dense Gx (n x 6nT), then L
+  const int m = ns * nT;
+  // dense Gx (n x ns nT), then L
   const int n = S_->n;
   std::vector<double> Gx((size_t)n * m, 0.0);
   for (int e = 0; e < nT; ++e) {
     double G[12][6];
-    gx_block(S_, e, G);
-    for (int k = 0; k < 4; ++k) {
-      int f = S_->free_map[S_->T[4 * e + k]];
+    gx_block<D>(S_, e, G);
+    for (int k = 0; k < nc; ++k) {
+      int f = S_->free_map[S_->T[(size_t)nc * e + k]];
       if (f < 0) continue;
-      for (int r = 0; r < 3; ++r)
-        for (int j = 0; j < 6; ++j) Gx[(size_t)(3 * f + r) * m + 6 * e + j] += G[3 * k + r][j];
+      for (int r = 0; r < D; ++r)
+        for (int j = 0; j < ns; ++j) Gx[(size_t)(D * f + r) * m + ns * e + j] += G[D * k + r][j];
     }
   }
   S_->lchol.assign((size_t)m * m, 0.0);
@@ -1196,35 +1248,50 @@ void kkt_refresh_L(OracleSim* S_, bool exact) {
   dense_chol(S_->lchol, m);
 }
 
+void kkt_refresh_L(OracleSim* S_, bool exact) {
+  if (S_->dim == 2) kkt_refresh_L_t<2>(S_, exact);
+  else kkt_refresh_L_t<3>(S_, exact);
+}
+
 // z = P b. exact: the reference's P = diag(PMP^-1, vol Hpsi^-1, L^-1)
 // (BlockDiagonalPreconditioner::_solve_impl, preconditioners.h:160-192, with
 // product_hessian_inv, mixed_stretch.cpp:385-396; Hpsi here is h^2 times the
 // material Hessian). GPU form: diag(Chebyshev(PMP), Hloc^-1 = Hpsi^-1 / vol,
 // element blocks S_e^-1).
-void kkt_precond(const OracleSim* S_, bool exact, const double* b, double* z) {
+template <int D>
+void kkt_precond_t(const OracleSim* S_, bool exact, const double* b, double* z) {
+  using V = MixedView<D>;
+  constexpr int ns = V::NS;
   const int n = S_->n, nT = S_->nT;
   if (exact) dense_chol_solve(S_->mchol, n, b, z);
   else mass_cheb(S_, b, z);
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < nT; ++e) {
     const double vol = S_->vols[e];
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < ns; ++i) {
       double a = 0;
-      for (int j = 0; j < 6; ++j) a += S_->Hinv[e](i, j) * b[n + 6 * e + j];
-      z[n + 6 * e + i] = exact ? vol * a : a / vol;
+      for (int j = 0; j < ns; ++j) a += V::Hinv(S_, e)(i, j) * b[n + (size_t)ns * e + j];
+      z[n + (size_t)ns * e + i] = exact ? vol * a : a / vol;
     }
   }
+  const size_t lo = n + (size_t)ns * nT;
   if (exact) {
-    dense_chol_solve(S_->lchol, 6 * nT, b + n + 6 * nT, z + n + 6 * nT);
+    dense_chol_solve(S_->lchol, ns * nT, b + lo, z + lo);
   } else {
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < nT; ++e)
-      for (int i = 0; i < 6; ++i) {
+      for (int i = 0; i < ns; ++i) {
         double a = 0;
-        for (int j = 0; j < 6; ++j) a += S_->linv_blk[e](i, j) * b[n + 6 * nT + 6 * e + j];
-        z[n + 6 * nT + 6 * e + i] = a;
+        for (int j = 0; j < ns; ++j)
+          a += S_->linv_blk[((size_t)e * ns + i) * ns + j] * b[lo + (size_t)ns * e + j];
+        z[lo + (size_t)ns * e + i] = a;
       }
   }
+}
+
+void kkt_precond(const OracleSim* S_, bool exact, const double* b, double* z) {
+  if (S_->dim == 2) kkt_precond_t<2>(S_, exact, b, z);
+  else kkt_precond_t<3>(S_, exact, b, z);
 }
 
 // Preconditioned MINRES of Eigen 3.4 (unsupported/Eigen/src/IterativeSolvers/
@@ -1310,7 +1377,7 @@ int kkt_solve(OracleSim* S_) {
   S_->kkt_solves = (S_->kkt_solves + 1) % std::max(1, S_->p.max_newton);
   std::vector<double> b;
   kkt_rhs(S_, b);
-  const int n = S_->n, nT = S_->nT, N = n + 12 * nT;
+  const int n = S_->n, nT = S_->nT, ns = kkt_ns(S_), N = n + 2 * ns * nT;
   std::vector<double> sol(N);
   int iters = 0;
   double err = minres(
@@ -1321,8 +1388,8 @@ int kkt_solve(OracleSim* S_) {
   S_->pcg_iters_last = iters;
   S_->pcg_err_last = err;
   S_->dx.assign(sol.begin(), sol.begin() + n);
-  S_->ds.assign(sol.begin() + n, sol.begin() + n + 6 * nT);
-  S_->lam.assign(sol.begin() + n + 6 * nT, sol.end());
+  S_->ds.assign(sol.begin() + n, sol.begin() + n + (size_t)ns * nT);
+  S_->lam.assign(sol.begin() + n + (size_t)ns * nT, sol.end());
   return 0;
 }
 
@@ -1344,7 +1411,7 @@ void update_system(OracleSim* S_) {
 int substep(OracleSim* S_, double& decrement) {
   if (S_->p.solver == 2 || S_->p.solver == 3) {
     // indefinite system (mixed scenes only)
-    if (!S_->p.mixed || S_->dim != 3) return -3;
+    if (!S_->p.mixed) return -3;
     int rc = kkt_solve(S_);
     if (rc != 0) return rc;
     S_->st.pcg_iters_total += S_->pcg_iters_last;
@@ -1899,12 +1966,12 @@ int oracle_sim_get(void* sim, const char* name, double* out) {
 
 long oracle_kkt_size(void* sim) {
   OracleSim* S_ = (OracleSim*)sim;
-  return (long)S_->n + 12L * S_->nT;
+  return (long)S_->n + 2L * kkt_ns(S_) * S_->nT;
 }
 
 int oracle_kkt_rhs(void* sim, double* out) {
   OracleSim* S_ = (OracleSim*)sim;
-  if (!S_->p.mixed || S_->dim != 3) return -3;
+  if (!S_->p.mixed) return -3;
   std::vector<double> b;
   kkt_rhs(S_, b);
   std::copy(b.begin(), b.end(), out);
@@ -1913,14 +1980,14 @@ int oracle_kkt_rhs(void* sim, double* out) {
 
 int oracle_kkt_product(void* sim, const double* w, double* y) {
   OracleSim* S_ = (OracleSim*)sim;
-  if (!S_->p.mixed || S_->dim != 3) return -3;
+  if (!S_->p.mixed) return -3;
   kkt_product(S_, w, y);
   return 0;
 }
 
 int oracle_kkt_precond(void* sim, int exact, const double* b, double* z) {
   OracleSim* S_ = (OracleSim*)sim;
-  if (!S_->p.mixed || S_->dim != 3) return -3;
+  if (!S_->p.mixed) return -3;
   kkt_mass_setup(S_, exact != 0);
   kkt_refresh_L(S_, exact != 0);
   kkt_precond(S_, exact != 0, b, z);

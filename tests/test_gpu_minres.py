@@ -155,3 +155,76 @@ def test_minres_errors(native):
     cg = native.Sim(_scene(solver="eigen-pcg-diag"), precision="fp64")
     with pytest.raises(native.MfemError):
         cg.kkt_size()
+
+
+# ---- 2D (triangles, 3-vector s): the DIM = 2 instance of the same solver
+def _scene2d(material="Stable-Neohookean", **kw):
+    base = dict(solver=SOLVER, max_newton=5)
+    base.update(kw)
+    sc = scenes.rect2d_scene(6, 3, 0.1, **base)
+    sc.mixed = True
+    if material != "Stable-Neohookean":
+        sc.mat[:] = scenes.MATERIALS[material]
+    return sc
+
+
+def _dx2(dev):
+    """Device dx (3 per free vertex, z rows zero) in the reference's 2D layout."""
+    d = dev.get("dx").reshape(-1, 3)
+    assert not d[:, 2].any()
+    return d[:, :2].reshape(-1)
+
+
+@pytest.mark.parametrize("moved", [False, True])
+def test_kkt_operator_2d_matches_oracle(native, oracle, moved):
+    sc = _scene2d(itr_tol=1e-10, max_iters=20000)
+    print(f"kkt operator 2d: moved={moved}, nT={sc.nT}", flush=True)
+    dev = native.Sim(sc, precision="fp64")
+    ref = oracle.Sim(sc)
+    if moved:
+        for _ in range(2):
+            dev.step()
+            ref.step()
+    dev.update_system()
+    ref.update_system()
+    N = ref.kkt_size()
+    assert dev.kkt_size() == N == 2 * int((ref.get_int("free_map", sc.nV) >= 0).sum()) + 6 * sc.nT
+    tol = 1e-6 if moved else 1e-11  # after motion the states differ by the solves' rounding
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        w = rng.standard_normal(N)
+        assert relerr(dev.kkt_apply(0, w), ref.kkt_product(w)) < tol
+        assert relerr(dev.kkt_apply(1, w), ref.kkt_precond(w, False)) < tol
+    b_ref = ref.kkt_rhs()
+    assert np.abs(b_ref).max() > 0
+    assert relerr(dev.kkt_apply(2), b_ref) < (1e-5 if moved else 1e-11)
+
+
+@pytest.mark.parametrize("material", ["Stable-Neohookean", "ARAP"])
+def test_minres_2d_fixed_iterations_match_oracle(native, oracle, material):
+    sc = _scene2d(material, max_newton=1, itr_tol=1e-300, max_iters=30)
+    print(f"minres 2d fixed iterations: {material}", flush=True)
+    dev = native.Sim(sc, precision="fp64")
+    ref = oracle.Sim(sc)
+    dev.update_system()
+    ref.update_system()
+    dev.solve()
+    ref.solve()
+    assert dev.get_int("pcg_iters")[0] == ref.get_int("pcg_iters", 1)[0] == 30
+    assert relerr(_dx2(dev), ref.get("dx")) < 1e-9
+    for name in ("ds", "lambda"):
+        assert relerr(dev.get(name), ref.get(name)) < 1e-9, name
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-7), ("mixed", 1e-6)])
+def test_minres_2d_timesteps_match_oracle(native, oracle, precision, tol):
+    sc = _scene2d(itr_tol=1e-10, max_iters=20000)
+    dev = native.Sim(sc, precision=precision)
+    ref = oracle.Sim(sc, solver=2)
+    for k in range(3):
+        print(f"minres 2d timestep {k} ({precision})", flush=True)
+        dev.step()
+        ref.step()
+        disp = np.abs(ref.positions() - sc.V[:, :2]).max()
+        assert disp > 1e-4
+        assert np.abs(dev.positions()[:, :2] - ref.positions()).max() < tol * disp

@@ -374,7 +374,8 @@ def test_direct_solver_size_limit(native):
 
 def test_2d_rejects_models_without_2d_api(native):
     """2D plain needs the 4-vector API (SNH only); 2D mixed the 3-vector API
-    (no Neo-Hookean or Fung); MINRES and batches are 3D only."""
+    (no Neo-Hookean or Fung); MINRES needs the mixed variable, in 2D as in
+    3D."""
     sc = scenes.config_c1()
     sc.solver = "eigen-pcg-diag"
     sc.mat = np.full(sc.nT, scenes.MATERIALS["Neohookean"], dtype=np.int32)
@@ -385,6 +386,8 @@ def test_2d_rejects_models_without_2d_api(native):
         native.Sim(sc)
     sc.mat[:] = scenes.MATERIALS["Stable-Neohookean"]
     sc.solver = "minres-indefinite-block"
+    native.Sim(sc)
+    sc.mixed = False
     with pytest.raises(native.MfemError):
         native.Sim(sc)
 

@@ -67,8 +67,21 @@ def test_minres_stopping_rule_and_count(oracle):
     assert bad < 0
 
 
-def _small(oracle, solver="eigen-llt", **kw):
-    sc = scenes.beam_scene(3, 2, 2, seed=4, mixed=True, solver=solver, max_newton=1, **kw)
+def _scene(dim, **kw):
+    """A 72-tet beam (3D) or a 36-triangle rectangle (2D), mixed SNH."""
+    if dim == 3:
+        return scenes.beam_scene(3, 2, 2, seed=4, mixed=True, **kw)
+    sc = scenes.rect2d_scene(6, 3, 0.1, **kw)
+    sc.mixed = True
+    return sc
+
+
+def _sym(dim):
+    return np.array([1, 1, 1, 2, 2, 2.0]) if dim == 3 else np.array([1, 1, 2.0])
+
+
+def _small(oracle, solver="eigen-llt", dim=3, **kw):
+    sc = _scene(dim, solver=solver, max_newton=1, **kw)
     sim = oracle.Sim(sc)
     # move off the rest state so that lambda, ds and the constraint are nonzero
     sim.step()
@@ -97,28 +110,31 @@ def _pmp_dense(sim, n):
     return M
 
 
-def test_kkt_operator_structure(oracle):
-    sc, sim = _small(oracle)
+@pytest.mark.parametrize("dim", [3, 2])
+def test_kkt_operator_structure(oracle, dim):
+    sc, sim = _small(oracle, dim=dim)
     n = sim.get("x").size
     nT = sc.nT
+    ns = 6 if dim == 3 else 3
+    assert n == dim * int((sim.get_int("free_map", sc.nV) >= 0).sum())
     K = _dense_kkt(sim)
-    assert K.shape == (n + 12 * nT,) * 2
+    assert K.shape == (n + 2 * ns * nT,) * 2
     scale = np.abs(K).max()
     assert np.abs(K - K.T).max() <= 1e-13 * scale
-    xs, ss, ls = slice(0, n), slice(n, n + 6 * nT), slice(n + 6 * nT, n + 12 * nT)
+    xs, ss, ls = slice(0, n), slice(n, n + ns * nT), slice(n + ns * nT, n + 2 * ns * nT)
     assert np.abs(K[xs, xs] - _pmp_dense(sim, n)).max() <= 1e-15 * scale
     assert not K[xs, ss].any() and not K[ls, ls].any()
     vols = sim.get("vols")
-    sym = np.array([1, 1, 1, 2, 2, 2.0])
-    assert np.array_equal(K[ls, ss], np.diag(np.repeat(vols, 6) * np.tile(sym, nT)))
+    sym = _sym(dim)
+    assert np.array_equal(K[ls, ss], np.diag(np.repeat(vols, ns) * np.tile(sym, nT)))
     # Hloc = vol h^2 Hpsi = vol^2 Sym Hm Sym, Hm = Syminv h^2 Hpsi Syminv / vol
-    Hm = sim.get("H").reshape(nT, 6, 6).transpose(0, 2, 1)
+    Hm = sim.get("H").reshape(nT, ns, ns).transpose(0, 2, 1)
     for e in range(nT):
-        blk = K[n + 6 * e:n + 6 * e + 6, n + 6 * e:n + 6 * e + 6]
+        blk = K[n + ns * e:n + ns * e + ns, n + ns * e:n + ns * e + ns]
         ref = vols[e] ** 2 * (sym[:, None] * Hm[e] * sym[None, :])
         assert np.abs(blk - ref).max() <= 1e-12 * np.abs(ref).max()
-        off = K[n + 6 * e:n + 6 * e + 6, ss].copy()
-        off[:, 6 * e:6 * e + 6] = 0
+        off = K[n + ns * e:n + ns * e + ns, ss].copy()
+        off[:, ns * e:ns * e + ns] = 0
         assert not off.any()
 
 
@@ -154,17 +170,51 @@ def test_kkt_constraint_block_is_constraint_jacobian(oracle):
         assert np.abs(G[:, j] - fd).max() <= 1e-6 * np.abs(G[:, j]).max()
 
 
-def test_kkt_solution_equals_schur_elimination(oracle):
+def test_kkt_constraint_block_2d(oracle):
+    """2D: the lambda-x block is -d(vol Sym s(x))/dx, s = (S00, S11, S01) of
+    the symmetric polar factor (central differences, S by SVD)."""
+    sc, sim = _small(oracle, dim=2)
+    n = sim.get("x").size
+    nT = sc.nT
+    K = _dense_kkt(sim)
+    G = K[n + 3 * nT:, :n]
+    pos = sim.positions().copy()[:, :2]
+    fm = sim.get_int("free_map", sc.nV)
+    vols = sim.get("vols")
+    dX = sim.get("dX").reshape(nT, 3, 2)
+    sym = _sym(2)
+    T = sc.T[:, :3]
+
+    def cvec(P):
+        F = np.einsum("ekr,ekc->erc", P[T], dX)
+        _, sig, Vt = np.linalg.svd(F)
+        S = np.einsum("eji,ej,ejk->eik", Vt, sig, Vt)
+        s = np.stack([S[:, 0, 0], S[:, 1, 1], S[:, 1, 0]], 1)
+        return (vols[:, None] * sym[None, :] * s).reshape(-1)
+
+    h = 1e-6
+    for j in range(n):
+        v = np.nonzero(fm == j // 2)[0][0]
+        Pp, Pm = pos.copy(), pos.copy()
+        Pp[v, j % 2] += h
+        Pm[v, j % 2] -= h
+        fd = -(cvec(Pp) - cvec(Pm)) / (2 * h)
+        assert np.abs(G[:, j] - fd).max() <= 1e-6 * np.abs(G[:, j]).max()
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_kkt_solution_equals_schur_elimination(oracle, dim):
     """The KKT solution is the positive-definite path's Newton direction:
     the same dx, and ds, lambda from its local solve."""
-    sc, a = _small(oracle, solver="eigen-llt")
-    _, b = _small(oracle, solver="eigen-llt")
+    sc, a = _small(oracle, solver="eigen-llt", dim=dim)
+    _, b = _small(oracle, solver="eigen-llt", dim=dim)
     a.solve()
     n = a.get("x").size
     nT = sc.nT
+    ns = 6 if dim == 3 else 3
     K = _dense_kkt(b)
     sol = np.linalg.solve(K, b.kkt_rhs())
-    dx, ds, lam = sol[:n], sol[n:n + 6 * nT], sol[n + 6 * nT:]
+    dx, ds, lam = sol[:n], sol[n:n + ns * nT], sol[n + ns * nT:]
     for got, ref in ((dx, a.get("dx")), (ds, a.get("ds")), (lam, a.get("lambda"))):
         assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
 
@@ -181,43 +231,46 @@ def _cheb_closed_form(M, b, steps=4):
     return Dm * (Q @ (p * (Q.T @ (Dm * b))))
 
 
+@pytest.mark.parametrize("dim", [3, 2])
 @pytest.mark.parametrize("exact", [True, False])
-def test_kkt_preconditioner_blocks(oracle, exact):
-    sc, sim = _small(oracle)
+def test_kkt_preconditioner_blocks(oracle, exact, dim):
+    sc, sim = _small(oracle, dim=dim)
     n = sim.get("x").size
     nT = sc.nT
+    ns = 6 if dim == 3 else 3
     N = sim.kkt_size()
     K = _dense_kkt(sim)
     M = _pmp_dense(sim, n)
-    G = K[:n, n + 6 * nT:]                               # Gx (n x 6nT)
+    G = K[:n, n + ns * nT:]                              # Gx (n x ns nT)
     GmG = G.T @ (G / M.sum(axis=1)[:, None])             # Gx^T Mlump^-1 Gx
-    C = np.diag(K[n + 6 * nT:, n:n + 6 * nT])            # vol Sym
+    C = np.diag(K[n + ns * nT:, n:n + ns * nT])          # vol Sym
     rng = np.random.default_rng(1)
     b = rng.standard_normal(N)
     z = sim.kkt_precond(b, exact)
-    zx, zs, zl = z[:n], z[n:n + 6 * nT], z[n + 6 * nT:]
-    bx, bs, bl = b[:n], b[n:n + 6 * nT], b[n + 6 * nT:]
+    zx, zs, zl = z[:n], z[n:n + ns * nT], z[n + ns * nT:]
+    bx, bs, bl = b[:n], b[n:n + ns * nT], b[n + ns * nT:]
     vols = sim.get("vols")
-    Hl = [K[n + 6 * e:n + 6 * e + 6, n + 6 * e:n + 6 * e + 6] for e in range(nT)]
+    blk = lambda e: slice(ns * e, ns * e + ns)
+    Hl = [K[n:, n:][blk(e), blk(e)] for e in range(nT)]
     if exact:
         # the reference: PMP^-1, L^-1 with L = Gx^T Mlump^-1 Gx + 1e-8 I
         refx = np.linalg.solve(M, bx)
-        refl = np.linalg.solve(GmG + 1e-8 * np.eye(6 * nT), bl)
+        refl = np.linalg.solve(GmG + 1e-8 * np.eye(ns * nT), bl)
     else:
         # element blocks of the Schur complement Gx^T Mlump^-1 Gx + C Hloc^-1 C
         refx = _cheb_closed_form(M, bx)
         refl = []
         for e in range(nT):
-            c = C[6 * e:6 * e + 6]
-            Se = GmG[6 * e:6 * e + 6, 6 * e:6 * e + 6] + c[:, None] * np.linalg.inv(Hl[e]) * c
-            refl.append(np.linalg.solve(Se, bl[6 * e:6 * e + 6]))
+            c = C[blk(e)]
+            Se = GmG[blk(e), blk(e)] + c[:, None] * np.linalg.inv(Hl[e]) * c
+            refl.append(np.linalg.solve(Se, bl[blk(e)]))
         refl = np.concatenate(refl)
     assert np.abs(zx - refx).max() <= 1e-10 * np.abs(refx).max()
     assert np.abs(zl - refl).max() <= 1e-7 * np.abs(refl).max()
     # s block: the reference's vol (h^2 Hpsi)^-1 = vol^2 Hloc^-1; GPU form Hloc^-1
     for e in range(nT):
-        ref = (vols[e] ** 2 if exact else 1.0) * np.linalg.solve(Hl[e], bs[6 * e:6 * e + 6])
-        assert np.abs(zs[6 * e:6 * e + 6] - ref).max() <= 1e-9 * np.abs(ref).max()
+        ref = (vols[e] ** 2 if exact else 1.0) * np.linalg.solve(Hl[e], bs[blk(e)])
+        assert np.abs(zs[blk(e)] - ref).max() <= 1e-9 * np.abs(ref).max()
     # symmetric positive definite as an operator
     c = rng.standard_normal(N)
     zc = sim.kkt_precond(c, exact)
@@ -225,13 +278,16 @@ def test_kkt_preconditioner_blocks(oracle, exact):
     assert b @ z > 0
 
 
+@pytest.mark.parametrize("dim", [3, 2])
 @pytest.mark.parametrize("solver", [2, 3])
-def test_minres_newton_steps_match_direct(oracle, solver):
+def test_minres_newton_steps_match_direct(oracle, solver, dim):
     """Solved tightly, the KKT path gives the direct solver's timesteps, to
     the accuracy at which the Newton loop stops (energy change 1e-12)."""
-    kw = dict(seed=6, mixed=True, max_newton=5, itr_tol=1e-12, max_iters=20000)
-    a = oracle.Sim(scenes.beam_scene(3, 2, 2, solver="eigen-llt", **kw))
-    b = oracle.Sim(scenes.beam_scene(3, 2, 2, **kw), solver=solver)
+    kw = dict(max_newton=5, itr_tol=1e-12, max_iters=20000)
+    mk = (lambda **k: scenes.beam_scene(3, 2, 2, seed=6, mixed=True, **k)) if dim == 3 else \
+        (lambda **k: _scene(2, **k))
+    a = oracle.Sim(mk(solver="eigen-llt", **kw))
+    b = oracle.Sim(mk(**kw), solver=solver)
     for _ in range(3):
         a.step()
         b.step()
