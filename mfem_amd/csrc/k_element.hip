@@ -173,26 +173,51 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_polar_material(
   }
 }
 
+// Element outputs of k_mixed_blocks_rhs leave through LDS: tet e's block
+// record (90 values) starts 90 values after tet e-1's, so a store from every
+// lane would touch 64 cache lines per instruction. Each wave stages a round
+// of its 64 tets' values (an odd pitch keeps the LDS banks apart) and writes
+// the round back as contiguous runs.
+constexpr int kStagePitch = 37;  // widest round: 36 values per tet
+
+template <int N, int PITCH, typename V>
+__device__ __forceinline__ void stage_flush(const V* stage, int lane, int e0, int nT, V* dst,
+                                            int stride, int off) {
+  __syncthreads();
+  for (int idx = lane; idx < kWave * N; idx += kWave) {
+    const int t = idx / N, j = idx - t * N;
+    if (e0 + t < nT) dst[(size_t)(e0 + t) * stride + off + j] = stage[t * PITCH + j];
+  }
+  __syncthreads();
+}
+
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
     TetMeshDev m, const double* __restrict__ x, Inertia in, const Real* __restrict__ dsdF,
     const Real* __restrict__ Hpsi, const double* __restrict__ gl, Real* __restrict__ blocks,
     double* __restrict__ fe) {
+  __shared__ __align__(16) unsigned char stage_raw[kElemBlock * kStagePitch * sizeof(Real)];
+  const int lane = threadIdx.x & (kWave - 1);
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m.nT) return;
+  const int e0 = e - lane;  // first tet of this wave
   const int nT = m.nT;
-  const double vol = m.vol[e];
+  // lanes past the end compute on the last tet and store nothing, so that
+  // every lane reaches the barriers of the rounds
+  const int ec = e < nT ? e : nT - 1;
+  unsigned char* wave_raw = stage_raw + (size_t)(threadIdx.x - lane) * kStagePitch * sizeof(Real);
+  Real* stage = reinterpret_cast<Real*>(wave_raw);
+  const double vol = m.vol[ec];
   // B = dsdF * Jloc (6 x 12): B[i][3k+r] = sum_c dsdF[i][r+3c] dX(k,c)
   Real B[6 * 12];
   {
     Real d[54];
 #pragma unroll
-    for (int i = 0; i < 54; ++i) d[i] = dsdF[i * nT + e];
+    for (int i = 0; i < 54; ++i) d[i] = dsdF[i * nT + ec];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      Real dx0 = (Real)m.dX[(3 * k + 0) * nT + e];
-      Real dx1 = (Real)m.dX[(3 * k + 1) * nT + e];
-      Real dx2 = (Real)m.dX[(3 * k + 2) * nT + e];
+      Real dx0 = (Real)m.dX[(3 * k + 0) * nT + ec];
+      Real dx1 = (Real)m.dX[(3 * k + 1) * nT + ec];
+      Real dx2 = (Real)m.dX[(3 * k + 2) * nT + ec];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -201,48 +226,57 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
                                    d[i + 6 * (r + 6)] * dx2;
     }
   }
-  Real H[36];
+  // C = Hpsi * B (6 x 12)
+  Real C[6 * 12];
+  {
+    Real H[36];
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
+    for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = i; j < 6; ++j) {
-      Real h = Hpsi[sym6(i, j) * nT + e];
-      H[i + 6 * j] = h;
-      H[j + 6 * i] = h;
-    }
-  Real* out = blocks + (size_t)e * 90;
-  const Real rv = (Real)vol;
+      for (int j = i; j < 6; ++j) {
+        Real h = Hpsi[sym6(i, j) * nT + ec];
+        H[i + 6 * j] = h;
+        H[j + 6 * i] = h;
+      }
 #pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    Real C[18];  // H * B_l (6 x 3)
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < 12; ++q)
 #pragma unroll
       for (int i = 0; i < 6; ++i) {
         Real a = 0;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) a += H[i + 6 * j] * B[j + 6 * (3 * l + q)];
+        for (int j = 0; j < 6; ++j) a += H[i + 6 * j] * B[j + 6 * q];
         C[i + 6 * q] = a;
       }
-#pragma unroll
-    for (int k = 0; k <= l; ++k) {
-      Real* o = out + 9 * blk_index(k, l);
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          Real a = 0;
-#pragma unroll
-          for (int i = 0; i < 6; ++i) a += B[i + 6 * (3 * k + r)] * C[i + 6 * q];
-          o[3 * r + q] = rv * a;
-        }
-    }
   }
+  const Real rv = (Real)vol;
+  // block (k, l) = vol B_k^T C_l into round slot `slot`
+  auto block_out = [&](int k, int l, int slot) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        Real a = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a += B[i + 6 * (3 * k + r)] * C[i + 6 * (3 * l + q)];
+        stage[lane * kStagePitch + 9 * slot + 3 * r + q] = rv * a;
+      }
+  };
+  // record order blk_index(k, l): (0,0..3) | (1,1..3) | (2,2), (2,3), (3,3)
+#pragma unroll
+  for (int l = 0; l < 4; ++l) block_out(0, l, l);
+  stage_flush<36, kStagePitch>(stage, lane, e0, nT, blocks, 90, 0);
+#pragma unroll
+  for (int l = 1; l < 4; ++l) block_out(1, l, l - 1);
+  stage_flush<27, kStagePitch>(stage, lane, e0, nT, blocks, 90, 36);
+  block_out(2, 2, 0);
+  block_out(2, 3, 1);
+  block_out(3, 3, 2);
+  stage_flush<27, kStagePitch>(stage, lane, e0, nT, blocks, 90, 63);
   // element residual: -vol B^T (Sym gl) - M_e (x - x_tilde - h^2 g)
   double w[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) w[i] = (i < 3 ? 1.0 : 2.0) * gl[i * nT + e];
-  int4 t = m.T[e];
+  for (int i = 0; i < 6; ++i) w[i] = (i < 3 ? 1.0 : 2.0) * gl[i * nT + ec];
+  int4 t = m.T[ec];
   int v[4] = {t.x, t.y, t.z, t.w};
   double d[12], dsum[3] = {0, 0, 0};
 #pragma unroll
@@ -254,14 +288,18 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
       dsum[r] += dd;
     }
   const double mc = m.density * vol / 20.0;
-  double* fo = fe + (size_t)e * 12;
+  constexpr int kFePitch = 13;  // 64 x 13 doubles fit in the wave's stage
+  static_assert(kWave * kFePitch * sizeof(double) <= kWave * kStagePitch * sizeof(float),
+                "f_e stage exceeds the wave's LDS");
+  double* fst = reinterpret_cast<double*>(wave_raw);
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
     double a = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) a += (double)B[i + 6 * j] * w[i];
-    fo[j] = -vol * a - mc * (d[j] + dsum[j % 3]);
+    fst[lane * kFePitch + j] = -vol * a - mc * (d[j] + dsum[j % 3]);
   }
+  stage_flush<12, kFePitch>(fst, lane, e0, nT, fe, 12, 0);
 }
 
 template <typename Real>
