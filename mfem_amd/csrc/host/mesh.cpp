@@ -241,6 +241,8 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
   d_col.upload(col_, st);
   d_slot_ptr.upload(slot_ptr_, st);
   d_slots.upload(slots_, st);
+  d_upper.upload(upper_, st);
+  d_tpos.upload(tpos_, st);
   d_diag_block.upload(diag_block_, st);
   d_inc_ptr.upload(inc_ptr_, st);
   d_inc.upload(inc_, st);
@@ -319,6 +321,21 @@ void Mesh::build_maps() {
       throw std::runtime_error("Mesh: free vertex not referenced by any tetrahedron (singular system)");
   }
   nnzb_ = (int)col_.size();
+  // blocks j >= i and the positions of their transposes (j, i): the pattern
+  // is the symmetric vertex adjacency, and the slots of (j, i) are those of
+  // (i, j) transposed, in the same (tet, local pair) order
+  upper_.clear();
+  tpos_.clear();
+  for (int i = 0; i < nfree_; ++i)
+    for (int p = rowptr_[i]; p < rowptr_[i + 1]; ++p) {
+      const int j = col_[p];
+      if (j < i) continue;
+      const auto b = col_.begin() + rowptr_[j], e = col_.begin() + rowptr_[j + 1];
+      const auto it = std::lower_bound(b, e, i);
+      if (it == e || *it != i) throw std::runtime_error("Mesh: block pattern is not symmetric");
+      upper_.push_back(p);
+      tpos_.push_back((int)(it - col_.begin()));
+    }
 }
 
 bool Mesh::bc_moves() const {

@@ -61,6 +61,7 @@ class Mesh {
   int nT() const { return nT_; }
   int nfree() const { return nfree_; }
   int nnzb() const { return nnzb_; }
+  int nupper() const { return (int)upper_.size(); }  // blocks j >= i of the pattern
 
   // host state (reference Mesh public members)
   std::vector<double> V_, Vref_, Vinit_, initial_velocity_;
@@ -78,6 +79,7 @@ class Mesh {
   DeviceArray<double> d_dX, d_vol, d_mu, d_la;
   DeviceArray<int> d_mat, d_free_map;
   DeviceArray<int> d_rowptr, d_col, d_slot_ptr, d_slots, d_diag_block, d_inc_ptr, d_inc;
+  DeviceArray<int> d_upper, d_tpos;  // upper blocks and the positions of their transposes
   DeviceArray<double> d_mass;
   std::vector<int> rowptr_, col_;  // host copy of the BSR pattern
   // material model id of every element (after init)
@@ -91,6 +93,7 @@ class Mesh {
   int dim_ = 3, nodes_ = 4;
   int nV_ = 0, nT_ = 0, nfree_ = 0, nnzb_ = 0;
   std::vector<int> slot_ptr_, slots_, diag_block_, inc_ptr_, inc_;
+  std::vector<int> upper_, tpos_;  // pattern positions of the blocks j >= i; of their transposes
   std::vector<double> mass_;
   std::vector<int> elem_type_;
   std::vector<int> pinned_;

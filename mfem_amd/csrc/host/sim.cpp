@@ -439,12 +439,14 @@ void SystemMatrixPD<DIM>::pre_solve(SimState<DIM>* state) {
   invdiag.resize(3 * (size_t)m.nfree(), f32);
   rhs.resize(3 * (size_t)m.nfree());
   if (f32) {
-    launch_assemble_bsr<float>(m.nnzb(), m.d_slot_ptr.data(), m.d_slots.data(), m.d_mass.data(),
+    launch_assemble_bsr<float>(m.nupper(), m.d_upper.data(), m.d_tpos.data(), m.d_slot_ptr.data(),
+                               m.d_slots.data(), m.d_mass.data(),
                                c->blocks.f32(), K.f32(), c->stream);
     launch_jacobi_diag<float, float>(m.nfree(), m.d_diag_block.data(), K.f32(),
                                      invdiag.f32(), c->stream);
   } else {
-    launch_assemble_bsr<double>(m.nnzb(), m.d_slot_ptr.data(), m.d_slots.data(), m.d_mass.data(),
+    launch_assemble_bsr<double>(m.nupper(), m.d_upper.data(), m.d_tpos.data(), m.d_slot_ptr.data(),
+                                m.d_slots.data(), m.d_mass.data(),
                                 c->blocks.f64(), K.f64(), c->stream);
     launch_jacobi_diag<double, double>(m.nfree(), m.d_diag_block.data(), K.f64(),
                                        invdiag.f64(), c->stream);

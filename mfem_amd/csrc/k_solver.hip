@@ -51,17 +51,24 @@ inline int xcd_grid(int n, int bs) {
 // 3x3) over the block's slots, so the four lanes of a block read the same
 // 36 bytes of each element block with at most 3 loads each, and the diagonal
 // entries (0, 4, 8) with the mass term all fall to lane 0. Every entry is
-// summed in slot order, as with one lane per block.
+// summed in slot order, as with one lane per block. Only the blocks j >= i
+// are summed; each is also written transposed into block (j, i), whose
+// slots are the same element blocks transposed in the same order, so both
+// halves are what summing them separately gives, and each element block is
+// read once instead of twice.
 template <typename Real>
-__global__ __launch_bounds__(kBS) void k_assemble_bsr(int nnzb, const int* __restrict__ slot_ptr,
+__global__ __launch_bounds__(kBS) void k_assemble_bsr(int nub, const int* __restrict__ upper,
+                                                      const int* __restrict__ tpos,
+                                                      const int* __restrict__ slot_ptr,
                                                       const int* __restrict__ slots,
                                                       const double* __restrict__ mass,
                                                       const Real* __restrict__ blocks,
                                                       Real* __restrict__ K) {
-  const int t = xcd_item(4 * nnzb);
+  const int t = xcd_item(4 * nub);
   if (t < 0) return;
   const int q = t & 3;
-  const int b = t >> 2;
+  const int b = upper[t >> 2];
+  const int bt = tpos[t >> 2];
   const int ne = q == 0 ? 3 : 2;
   double acc[3] = {0.0, 0.0, 0.0};
   const int p1 = slot_ptr[b + 1];
@@ -93,10 +100,16 @@ __global__ __launch_bounds__(kBS) void k_assemble_bsr(int nnzb, const int* __res
           if (j < ne) acc[j] += (double)val[u][j];
   }
   Real* out = K + (size_t)9 * b;
+  Real* out_t = K + (size_t)9 * bt;
   const double m = q == 0 ? mass[b] : 0.0;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
-    if (j < ne) out[q + 4 * j] = (Real)(acc[j] + m);
+    if (j < ne) {
+      const int o = q + 4 * j;
+      const Real v = (Real)(acc[j] + m);
+      out[o] = v;
+      if (bt != b) out_t[3 * (o % 3) + o / 3] = v;
+    }
 }
 
 template <typename Real, typename VT>
@@ -717,11 +730,12 @@ void launch_finalize(const double* partials, int n, int op, double* result, hipS
 }
 
 template <typename Real>
-void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const double* mass,
-                         const Real* blocks, Real* K, hipStream_t st) {
-  if (nnzb == 0) return;
-  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(xcd_grid(4 * nnzb, kBS)), dim3(kBS), 0, st, nnzb,
-                     slot_ptr, slots, mass, blocks, K);
+void launch_assemble_bsr(int nub, const int* upper, const int* tpos, const int* slot_ptr,
+                         const int* slots, const double* mass, const Real* blocks, Real* K,
+                         hipStream_t st) {
+  if (nub == 0) return;
+  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(xcd_grid(4 * nub, kBS)), dim3(kBS), 0, st, nub,
+                     upper, tpos, slot_ptr, slots, mass, blocks, K);
 }
 
 template <typename Real, typename VT>
@@ -858,10 +872,10 @@ template void launch_pcg_batch_init<float>(const BatchCgBuffers<float>&, double*
 template void launch_pcg_batch_iteration<double>(const BatchCgBuffers<double>&, int, hipStream_t);
 template void launch_pcg_batch_iteration<float>(const BatchCgBuffers<float>&, int, hipStream_t);
 
-template void launch_assemble_bsr<double>(int, const int*, const int*, const double*, const double*,
-                                          double*, hipStream_t);
-template void launch_assemble_bsr<float>(int, const int*, const int*, const double*, const float*,
-                                         float*, hipStream_t);
+template void launch_assemble_bsr<double>(int, const int*, const int*, const int*, const int*,
+                                          const double*, const double*, double*, hipStream_t);
+template void launch_assemble_bsr<float>(int, const int*, const int*, const int*, const int*,
+                                         const double*, const float*, float*, hipStream_t);
 template void launch_jacobi_diag<double, double>(int, const int*, const double*, double*, hipStream_t);
 template void launch_jacobi_diag<float, float>(int, const int*, const float*, float*, hipStream_t);
 template void launch_spmv<double, double>(int, const int*, const int*, const double*, const double*,

@@ -100,10 +100,13 @@ void launch_plain2d_energy(const TetMeshDev& m, const double* x_full, const doub
 
 // --- assembly (reference src/utils/sparse_utils.cpp:141-203) ---------------
 // K block b = mass[b] * I + sum over its slot list of element blocks.
-// slot = (e << 4) | (a << 2) | bb  (local vertex pair a, bb of tet e)
+// slot = (e << 4) | (a << 2) | bb  (local vertex pair a, bb of tet e).
+// Runs over the nub blocks j >= i (pattern positions `upper`) and writes each
+// also transposed at `tpos` (the block (j, i)).
 template <typename Real>
-void launch_assemble_bsr(int nnzb, const int* slot_ptr, const int* slots, const double* mass,
-                         const Real* blocks, Real* K, hipStream_t st);
+void launch_assemble_bsr(int nub, const int* upper, const int* tpos, const int* slot_ptr,
+                         const int* slots, const double* mass, const Real* blocks, Real* K,
+                         hipStream_t st);
 // invdiag[3i+r] = 1 / K_(3i+r),(3i+r) from the diagonal blocks (Jacobi).
 template <typename Real, typename VT>
 void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* invdiag,
