@@ -313,7 +313,9 @@ __global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtr
                                                    double* __restrict__ parts) {
   using E = Elem<DIM>;
   constexpr int NC = E::NC, NS = E::NS, NF = E::NF;
+  constexpr int kFePitch = 13;  // x-row contributions staged per tet (odd pitch: LDS banks)
   __shared__ double sm[kTetBS / kWave];
+  __shared__ double fe_stage[kTetBS * kFePitch];
   const MinresCtrl c = *ctrl;
   const int e = xcd_index(m.nT);
   const int nT = m.nT;
@@ -369,8 +371,9 @@ __global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtr
         u[i] += js * Fw[mm];
         tv[mm] += js * E::symw(i) * wl[i];
       }
-    // x rows of this element: -vol Jloc^T tv (the z rows of a triangle: 0)
-    double* fo = fe + (size_t)e * 12;
+    // x rows of this element: -vol Jloc^T tv (the z rows of a triangle: 0),
+    // staged in LDS and written below as contiguous runs
+    double* fo = fe_stage + threadIdx.x * kFePitch;
 #pragma unroll
     for (int k = 0; k < NC; ++k)
 #pragma unroll
@@ -396,6 +399,19 @@ __global__ __launch_bounds__(kTetBS) void k_mr_tet(TetMeshDev m, const MinresCtr
       V.vo.s[is] = vs;
       V.vo.l[is] = vl;
       acc += vs * ws[i] + vl * wl[i];
+    }
+  }
+  // the staged x-row contributions of this wave's live tets, which are a
+  // contiguous prefix of the wave (xcd_index hands out consecutive items)
+  {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nlive = __popcll(__ballot(e < nT));
+    const int e0 = __shfl(e, 0);
+    const double* ws = fe_stage + (threadIdx.x - lane) * kFePitch;
+    __syncthreads();
+    for (int idx = lane; idx < nlive * 3 * NC; idx += kWave) {
+      const int t = idx / (3 * NC), j = idx - t * (3 * NC);
+      fe[(size_t)(e0 + t) * 12 + j] = ws[t * kFePitch + j];
     }
   }
   block_partial<kTetBS, 0>(acc, parts, sm);
