@@ -298,23 +298,51 @@ void MixedStretch<DIM>::reset() {
   S_.zero(st);
   if (two && c->fp32())
     launch_mixed2d_polar_material<float>(m, x0.data(), R_.data(), S_.data(), s_.data(),
-                                         dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
-                                         gl_.data(), 0.0, st);
+                                         dsdF_.f32(), hess_out<float>(), g_.data(), gl_.data(),
+                                         0.0, st);
   else if (two)
     launch_mixed2d_polar_material<double>(m, x0.data(), R_.data(), S_.data(), s_.data(),
-                                          dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
-                                          gl_.data(), 0.0, st);
+                                          dsdF_.f64(), hess_out<double>(), g_.data(), gl_.data(),
+                                          0.0, st);
   else if (c->fp32())
     launch_mixed_polar_material<float>(m, x0.data(), R_.data(), S_.data(), s_.data(),
-                                       dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
-                                       g_.data(), gl_.data(), 0.0, st);
+                                       dsdF_.f32(), hess_out<float>(), g_.data(), gl_.data(), 0.0,
+                                       st);
   else
     launch_mixed_polar_material<double>(m, x0.data(), R_.data(), S_.data(), s_.data(),
-                                        dsdF_.f64(), Hpsi_.f64(),
-                                        Hinv_.f64(), g_.data(), gl_.data(), 0.0, st);
+                                        dsdF_.f64(), hess_out<double>(), g_.data(), gl_.data(),
+                                        0.0, st);
   MFEM_HIP_CHECK(hipGetLastError());
   MFEM_HIP_CHECK(hipMemcpyAsync(s_.data(), S_.data(), S_.bytes(), hipMemcpyDeviceToDevice, st));
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// Where the element Hessian phase writes. FP64 mode: Hpsi_ and Hinv_. FP32
+// storage: Hpsi_ in FP32 for the stiffness blocks or the KKT product; with the
+// Schur complement, the FP64 pair Hloc_ / Hlinv_ for the local solve (sized
+// here, on first use); with the KKT solver, the FP32 inverse for its
+// preconditioner.
+template <int DIM>
+template <typename Real>
+HessOut<Real> MixedStretch<DIM>::hess_out() {
+  HessOut<Real> ho;
+  ho.Hpsi = Hpsi_.template as<Real>();
+  if (!Hpsi_.fp32()) {
+    ho.Hinv = Hinv_.template as<Real>();
+    return ho;
+  }
+  if (!schur_blocks_) {
+    ho.Hinv = Hinv_.template as<Real>();
+    return ho;
+  }
+  const size_t n = 21 * (size_t)this->mesh_->nT();
+  if (Hloc_.size() != n) {
+    Hloc_.resize(n);
+    Hlinv_.resize(n);
+  }
+  ho.Hloc = Hloc_.data();
+  ho.Hlinv = Hlinv_.data();
+  return ho;
 }
 
 // update_rotations + update_derivatives + rhs (reference mixed_stretch.cpp:
@@ -327,14 +355,14 @@ void MixedStretch<DIM>::update(const double* x_full, double dt) {
   if (m.dim == 2) {
     if (c->fp32()) {
       launch_mixed2d_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
-                                           dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
-                                           gl_.data(), h2_, c->stream);
+                                           dsdF_.f32(), hess_out<float>(), g_.data(), gl_.data(),
+                                           h2_, c->stream);
       if (schur_blocks_)
         launch_mixed2d_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
                                        gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
     } else {
       launch_mixed2d_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
-                                            dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
+                                            dsdF_.f64(), hess_out<double>(), g_.data(),
                                             gl_.data(), h2_, c->stream);
       if (schur_blocks_)
         launch_mixed2d_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(), Hpsi_.f64(),
@@ -344,15 +372,15 @@ void MixedStretch<DIM>::update(const double* x_full, double dt) {
   }
   if (c->fp32()) {
     launch_mixed_polar_material<float>(m, x_full, R_.data(), S_.data(), s_.data(),
-                                       dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
-                                       g_.data(), gl_.data(), h2_, c->stream);
+                                       dsdF_.f32(), hess_out<float>(), g_.data(), gl_.data(), h2_,
+                                       c->stream);
     if (schur_blocks_)
       launch_mixed_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
                                      gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
   } else {
     launch_mixed_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
-                                        dsdF_.f64(), Hpsi_.f64(),
-                                        Hinv_.f64(), g_.data(), gl_.data(), h2_, c->stream);
+                                        dsdF_.f64(), hess_out<double>(), g_.data(), gl_.data(),
+                                        h2_, c->stream);
     if (schur_blocks_)
       launch_mixed_blocks_rhs<double>(m, x_full, c->inertia(), dsdF_.f64(),
                                       Hpsi_.f64(), gl_.data(), c->blocks.f64(),
@@ -366,23 +394,23 @@ void MixedStretch<DIM>::solve(const double* dx) {
   auto m = this->mesh_->dev();
   if (m.dim == 2) {
     if (c->fp32())
-      launch_mixed2d_local_solve<float>(m, dx, dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(), g_.data(),
-                                        gl_.data(), la_.data(), ds_.data(),
+      launch_mixed2d_local_solve<float>(m, dx, dsdF_.f32(), local_hpsi(), local_hinv(),
+                                        g_.data(), gl_.data(), la_.data(), ds_.data(),
                                         c->reducer.slot(RS_MAXDS), c->stream);
     else
-      launch_mixed2d_local_solve<double>(m, dx, dsdF_.f64(), Hpsi_.f64(), Hinv_.f64(), g_.data(),
-                                         gl_.data(), la_.data(), ds_.data(),
+      launch_mixed2d_local_solve<double>(m, dx, dsdF_.f64(), local_hpsi(), local_hinv(),
+                                         g_.data(), gl_.data(), la_.data(), ds_.data(),
                                          c->reducer.slot(RS_MAXDS), c->stream);
     return;
   }
   if (c->fp32())
-    launch_mixed_local_solve<float>(m, dx, dsdF_.f32(), Hpsi_.f32(), Hinv_.f32(),
+    launch_mixed_local_solve<float>(m, dx, dsdF_.f32(), local_hpsi(), local_hinv(),
                                     g_.data(), gl_.data(), la_.data(), ds_.data(),
                                     c->reducer.slot(RS_MAXDS), c->stream);
   else
-    launch_mixed_local_solve<double>(m, dx, dsdF_.f64(), Hpsi_.f64(),
-                                     Hinv_.f64(), g_.data(), gl_.data(), la_.data(),
-                                     ds_.data(), c->reducer.slot(RS_MAXDS), c->stream);
+    launch_mixed_local_solve<double>(m, dx, dsdF_.f64(), local_hpsi(), local_hinv(),
+                                     g_.data(), gl_.data(), la_.data(), ds_.data(),
+                                     c->reducer.slot(RS_MAXDS), c->stream);
 }
 
 template <int DIM>

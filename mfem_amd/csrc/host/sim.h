@@ -201,11 +201,20 @@ class MixedStretch : public MixedVariable<DIM> {
   // element state (structure of arrays, stride nT)
   DeviceArray<double> R_, S_, s_, la_, ds_, g_, gl_;
   RealArray dsdF_, Hpsi_, Hinv_;
+  // FP64 copies of Hpsi and its inverse for the local solve when the element
+  // arrays above are stored in FP32 (empty in FP64 mode, where Hpsi_ and
+  // Hinv_ are the FP64 matrices)
+  DeviceArray<double> Hloc_, Hlinv_;
+  // the FP64 Hessian and inverse the local solve reads
+  const double* local_hpsi() const { return Hpsi_.fp32() ? Hloc_.data() : Hpsi_.f64(); }
+  const double* local_hinv() const { return Hpsi_.fp32() ? Hlinv_.data() : Hinv_.f64(); }
   // false when the linear solver does not use the Schur-complement system
   // (its element stiffness blocks and residual are then not formed)
   bool schur_blocks_ = true;
 
  private:
+  template <typename Real>
+  mfemhip::HessOut<Real> hess_out();
   double h2_ = 0;
 };
 

@@ -96,21 +96,24 @@ __device__ __forceinline__ void lu_inverse6(const Real* Ain, Real* X) {
   }
 }
 
-// Projected, h^2-scaled material Hessian (full 6x6, Real) from s.
-template <typename Real>
-__device__ __forceinline__ void projected_hessian(int type, double mu, double la, const double s[6],
-                                                  double h2, double g[6], Real Hs[36]) {
+// Projected, h^2-scaled material Hessian (full 6x6) from s, in FP64 in every
+// precision mode: the projection clamps eigenvalues to 1e-6 against entries
+// of order lambda + mu (up to ~1e7), a spread FP32 cannot hold, and the local
+// solve needs this matrix and its inverse to be inverses of each other to the
+// working precision (ds = Hpsi^-1 (Hpsi v) cancels exactly only then).
+__device__ __forceinline__ void projected_hessian(int type, double mu, double la,
+                                                  const double s[6], double h2, double g[6],
+                                                  double Hs[36]) {
   double gg[6], HH[21];
   material_grad_hess(type, mu, la, s, gg, HH);
-  Real A[36];
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) A[i + 6 * j] = (Real)HH[sym6(i, j)];
-  if (material_needs_psd(type) && !chol_pd<Real, 6>(A, Real(1e-6)))
-    jacobi_psd_project<Real, 6>(A, Real(1e-6));
+    for (int j = 0; j < 6; ++j) Hs[i + 6 * j] = HH[sym6(i, j)];
+  if (material_needs_psd(type) && !chol_pd<double, 6>(Hs, 1e-6))
+    jacobi_psd_project<double, 6>(Hs, 1e-6);
 #pragma unroll
-  for (int i = 0; i < 36; ++i) Hs[i] = (Real)h2 * A[i];
+  for (int i = 0; i < 36; ++i) Hs[i] *= h2;
 #pragma unroll
   for (int i = 0; i < 6; ++i) g[i] = h2 * gg[i];
 }
@@ -123,8 +126,8 @@ __device__ __forceinline__ constexpr int blk_index(int k, int l) {
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_mixed_polar_material(
     TetMeshDev m, const double* __restrict__ x, double* __restrict__ R, double* __restrict__ S,
-    const double* __restrict__ s, Real* __restrict__ dsdF, Real* __restrict__ Hpsi,
-    Real* __restrict__ Hinv, double* __restrict__ g, double* __restrict__ gl, double h2) {
+    const double* __restrict__ s, Real* __restrict__ dsdF, HessOut<Real> ho,
+    double* __restrict__ g, double* __restrict__ gl, double h2) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.nT) return;
   const int nT = m.nT;
@@ -150,24 +153,27 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_polar_material(
   double se[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) se[i] = s[i * nT + e];
-  double gg[6];
-  Real Hs[36];
-  projected_hessian<Real>(m.mat[e], m.mu[e], m.la[e], se, h2, gg, Hs);
-  Real Hi[36];
-  lu_inverse6<Real>(Hs, Hi);
+  double gg[6], Hs[36];
+  projected_hessian(m.mat[e], m.mu[e], m.la[e], se, h2, gg, Hs);
+  double Hi[36];
+  lu_inverse6<double>(Hs, Hi);
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = i; j < 6; ++j) {
-      Hpsi[sym6(i, j) * nT + e] = Hs[i + 6 * j];
-      Hinv[sym6(i, j) * nT + e] = (Real)(0.5 * ((double)Hi[i + 6 * j] + (double)Hi[j + 6 * i]));
+      const int k = sym6(i, j) * nT + e;
+      const double hi = 0.5 * (Hi[i + 6 * j] + Hi[j + 6 * i]);
+      ho.Hpsi[k] = (Real)Hs[i + 6 * j];
+      if (ho.Hinv) ho.Hinv[k] = (Real)hi;
+      if (ho.Hloc) ho.Hloc[k] = Hs[i + 6 * j];
+      if (ho.Hlinv) ho.Hlinv[k] = hi;
     }
   const double syminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     double a = gg[i];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) a += (double)Hs[i + 6 * j] * (sv[j] - se[j]);
+    for (int j = 0; j < 6; ++j) a += Hs[i + 6 * j] * (sv[j] - se[j]);
     g[i * nT + e] = gg[i];
     gl[i * nT + e] = syminv[i] * a;
   }
@@ -305,7 +311,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
     TetMeshDev m, const double* __restrict__ dx, const Real* __restrict__ dsdF,
-    const Real* __restrict__ Hpsi, const Real* __restrict__ Hinv, const double* __restrict__ g,
+    const double* __restrict__ Hpsi, const double* __restrict__ Hinv, const double* __restrict__ g,
     const double* __restrict__ gl, double* __restrict__ lam, double* __restrict__ ds,
     ReduceSlot max_ds) {
   __shared__ double sm[kElemBlock / kWave];
@@ -348,7 +354,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
     for (int i = 0; i < 6; ++i) {
       double a = 0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) a += (double)Hpsi[sym6(i, j) * nT + e] * y[j];
+      for (int j = 0; j < 6; ++j) a += Hpsi[sym6(i, j) * nT + e] * y[j];
       l[i] = -gl[i * nT + e] - syminv[i] * a;
       rr[i] = sym[i] * l[i] + g[i * nT + e];
     }
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
     for (int i = 0; i < 6; ++i) {
       double a = 0;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) a += (double)Hinv[sym6(i, j) * nT + e] * rr[j];
+      for (int j = 0; j < 6; ++j) a += Hinv[sym6(i, j) * nT + e] * rr[j];
       lam[i * nT + e] = l[i];
       ds[i * nT + e] = -a;
       local_max = fmax(local_max, fabs(a));
@@ -804,8 +810,8 @@ __device__ __forceinline__ double sym3w(int i) { return i < 2 ? 1.0 : 2.0; }
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_mixed2d_polar_material(
     TetMeshDev m, const double* __restrict__ x, double* __restrict__ R, double* __restrict__ S,
-    const double* __restrict__ s, Real* __restrict__ dsdF, Real* __restrict__ Hpsi,
-    Real* __restrict__ Hinv, double* __restrict__ g, double* __restrict__ gl, double h2) {
+    const double* __restrict__ s, Real* __restrict__ dsdF, HessOut<Real> ho,
+    double* __restrict__ g, double* __restrict__ gl, double h2) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.nT) return;
   const int nT = m.nT;
@@ -826,15 +832,17 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed2d_polar_material(
   const int type = m.mat[e];
   double gg[3], HH[6];
   material2d_grad_hess(type, m.mu[e], m.la[e], se, gg, HH);
-  Real A[9];
+  // projection and inverse in FP64 in every precision mode (see
+  // projected_hessian above)
+  double A[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) A[i + 3 * j] = (Real)HH[sym3(i, j)];
-  if (type == MAT_SNH && !chol_pd<Real, 3>(A, Real(1e-6))) jacobi_psd_project<Real, 3>(A, Real(1e-6));
+    for (int j = 0; j < 3; ++j) A[i + 3 * j] = HH[sym3(i, j)];
+  if (type == MAT_SNH && !chol_pd<double, 3>(A, 1e-6)) jacobi_psd_project<double, 3>(A, 1e-6);
   double Hs[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) Hs[i] = h2 * (double)A[i];
+  for (int i = 0; i < 9; ++i) Hs[i] = h2 * A[i];
   // inverse of the symmetric 3x3 by cofactors
   const double c00 = Hs[4] * Hs[8] - Hs[7] * Hs[5], c01 = Hs[7] * Hs[2] - Hs[1] * Hs[8],
                c02 = Hs[1] * Hs[5] - Hs[4] * Hs[2];
@@ -851,14 +859,18 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed2d_polar_material(
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = i; j < 3; ++j) {
-      Hpsi[sym3(i, j) * nT + e] = (Real)Hs[i + 3 * j];
-      Hinv[sym3(i, j) * nT + e] = (Real)(0.5 * (Hi[i + 3 * j] + Hi[j + 3 * i]));
+      const int k = sym3(i, j) * nT + e;
+      const double hi = 0.5 * (Hi[i + 3 * j] + Hi[j + 3 * i]);
+      ho.Hpsi[k] = (Real)Hs[i + 3 * j];
+      if (ho.Hinv) ho.Hinv[k] = (Real)hi;
+      if (ho.Hloc) ho.Hloc[k] = Hs[i + 3 * j];
+      if (ho.Hlinv) ho.Hlinv[k] = hi;
     }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     double a = h2 * gg[i];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) a += (double)(Real)Hs[i + 3 * j] * (sv[j] - se[j]);
+    for (int j = 0; j < 3; ++j) a += Hs[i + 3 * j] * (sv[j] - se[j]);
     g[i * nT + e] = h2 * gg[i];
     gl[i * nT + e] = (i < 2 ? 1.0 : 0.5) * a;
   }
@@ -956,7 +968,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed2d_blocks_rhs(
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_mixed2d_local_solve(
     TetMeshDev m, const double* __restrict__ dx, const Real* __restrict__ dsdF,
-    const Real* __restrict__ Hpsi, const Real* __restrict__ Hinv, const double* __restrict__ g,
+    const double* __restrict__ Hpsi, const double* __restrict__ Hinv, const double* __restrict__ g,
     const double* __restrict__ gl, double* __restrict__ lam, double* __restrict__ ds,
     ReduceSlot max_ds) {
   __shared__ double sm[kElemBlock / kWave];
@@ -990,7 +1002,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed2d_local_solve(
     for (int i = 0; i < 3; ++i) {
       double a = 0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) a += (double)Hpsi[sym3(i, j) * nT + e] * y[j];
+      for (int j = 0; j < 3; ++j) a += Hpsi[sym3(i, j) * nT + e] * y[j];
       l[i] = -gl[i * nT + e] - (i < 2 ? 1.0 : 0.5) * a;
       rr[i] = sym3w(i) * l[i] + g[i * nT + e];
     }
@@ -998,7 +1010,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed2d_local_solve(
     for (int i = 0; i < 3; ++i) {
       double a = 0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) a += (double)Hinv[sym3(i, j) * nT + e] * rr[j];
+      for (int j = 0; j < 3; ++j) a += Hinv[sym3(i, j) * nT + e] * rr[j];
       lam[i * nT + e] = l[i];
       ds[i * nT + e] = -a;
       local_max = fmax(local_max, fabs(a));
@@ -1151,11 +1163,11 @@ int reduce_grid(long n) { return n > 0 ? grid_for(n, kElemBlock) : 1; }
 
 template <typename Real>
 void launch_mixed_polar_material(const TetMeshDev& m, const double* x_full, double* R, double* S,
-                                 const double* s, Real* dsdF, Real* Hpsi, Real* Hinv, double* g,
+                                 const double* s, Real* dsdF, const HessOut<Real>& ho, double* g,
                                  double* gl, double h2, hipStream_t st) {
   if (m.nT == 0) return;
   hipLaunchKernelGGL(k_mixed_polar_material<Real>, dim3(grid_for(m.nT, kElemBlock)),
-                     dim3(kElemBlock), 0, st, m, x_full, R, S, s, dsdF, Hpsi, Hinv, g, gl, h2);
+                     dim3(kElemBlock), 0, st, m, x_full, R, S, s, dsdF, ho, g, gl, h2);
 }
 
 template <typename Real>
@@ -1169,7 +1181,7 @@ void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const In
 
 template <typename Real>
 void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
-                              const Real* Hpsi, const Real* Hinv, const double* g,
+                              const double* Hpsi, const double* Hinv, const double* g,
                               const double* gl, double* lam, double* ds, ReduceSlot max_ds,
                               hipStream_t st) {
   hipLaunchKernelGGL(k_mixed_local_solve<Real>, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st,
@@ -1229,11 +1241,11 @@ void launch_plain2d_energy(const TetMeshDev& m, const double* x_full, const doub
 
 template <typename Real>
 void launch_mixed2d_polar_material(const TetMeshDev& m, const double* x_full, double* R, double* S,
-                                   const double* s, Real* dsdF, Real* Hpsi, Real* Hinv, double* g,
-                                   double* gl, double h2, hipStream_t st) {
+                                   const double* s, Real* dsdF, const HessOut<Real>& ho,
+                                   double* g, double* gl, double h2, hipStream_t st) {
   if (m.nT == 0) return;
   hipLaunchKernelGGL(k_mixed2d_polar_material<Real>, dim3(grid_for(m.nT, kElemBlock)),
-                     dim3(kElemBlock), 0, st, m, x_full, R, S, s, dsdF, Hpsi, Hinv, g, gl, h2);
+                     dim3(kElemBlock), 0, st, m, x_full, R, S, s, dsdF, ho, g, gl, h2);
 }
 
 template <typename Real>
@@ -1247,7 +1259,7 @@ void launch_mixed2d_blocks_rhs(const TetMeshDev& m, const double* x_full, const 
 
 template <typename Real>
 void launch_mixed2d_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
-                                const Real* Hpsi, const Real* Hinv, const double* g,
+                                const double* Hpsi, const double* Hinv, const double* g,
                                 const double* gl, double* lam, double* ds, ReduceSlot max_ds,
                                 hipStream_t st) {
   const int gsz = reduce_grid(m.nT);
@@ -1287,13 +1299,14 @@ void launch_batch_snh_f9(int n, const double* mu, const double* la, const double
 
 #define MFEM_INST(Real)                                                                          \
   template void launch_mixed_polar_material<Real>(const TetMeshDev&, const double*, double*,    \
-                                                  double*, const double*, Real*, Real*, Real*,  \
-                                                  double*, double*, double, hipStream_t);       \
+                                                  double*, const double*, Real*,                \
+                                                  const HessOut<Real>&, double*, double*, double, \
+                                                  hipStream_t);                                 \
   template void launch_mixed_blocks_rhs<Real>(const TetMeshDev&, const double*, const Inertia&, \
                                               const Real*, const Real*, const double*, Real*,   \
                                               double*, hipStream_t);                            \
   template void launch_mixed_local_solve<Real>(const TetMeshDev&, const double*, const Real*,   \
-                                               const Real*, const Real*, const double*,         \
+                                               const double*, const double*, const double*,     \
                                                const double*, double*, double*, ReduceSlot,     \
                                                hipStream_t);                                    \
   template void launch_plain_update<Real>(const TetMeshDev&, const double*, const Inertia&,     \
@@ -1301,13 +1314,14 @@ void launch_batch_snh_f9(int n, const double* mu, const double* la, const double
   template void launch_plain2d_update<Real>(const TetMeshDev&, const double*, const Inertia&,   \
                                             double, Real*, double*, hipStream_t);               \
   template void launch_mixed2d_polar_material<Real>(const TetMeshDev&, const double*, double*,  \
-                                                    double*, const double*, Real*, Real*, Real*, \
-                                                    double*, double*, double, hipStream_t);     \
+                                                    double*, const double*, Real*,              \
+                                                    const HessOut<Real>&, double*, double*,     \
+                                                    double, hipStream_t);                       \
   template void launch_mixed2d_blocks_rhs<Real>(const TetMeshDev&, const double*, const Inertia&, \
                                                 const Real*, const Real*, const double*, Real*,   \
                                                 double*, hipStream_t);                            \
   template void launch_mixed2d_local_solve<Real>(const TetMeshDev&, const double*, const Real*,   \
-                                                 const Real*, const Real*, const double*,         \
+                                                 const double*, const double*, const double*,     \
                                                  const double*, double*, double*, ReduceSlot,     \
                                                  hipStream_t);
 MFEM_INST(double)

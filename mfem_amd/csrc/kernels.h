@@ -35,23 +35,37 @@ struct Inertia {
 };
 
 // --- mixed stretch (reference src/variables/mixed_stretch.cpp) ------------
+// Outputs of the element Hessian phase, [21][nT] packed symmetric (6 packed
+// per triangle in 2D). The projection and the inverse are computed in FP64;
+// Hpsi (= h^2 Hessian) and Hinv are stored at the storage precision Real
+// (stiffness blocks, KKT product and preconditioner); Hloc / Hlinv, when not
+// null, receive the same two matrices in FP64 for the local solve of the
+// FP32 storage mode. Hinv may be null when nothing reads it.
+template <typename Real>
+struct HessOut {
+  Real* Hpsi = nullptr;
+  Real* Hinv = nullptr;
+  double* Hloc = nullptr;
+  double* Hlinv = nullptr;
+};
 // Phase 1: polar fit (warm start R), S, ds/dF, material gradient g, PSD-
-// projected Hessian Hpsi (= h^2 Hessian), its inverse, and gl.
+// projected Hessian Hpsi, its inverse, and gl.
 template <typename Real>
 void launch_mixed_polar_material(const TetMeshDev& m, const double* x_full, double* R,
-                                 double* S, const double* s, Real* dsdF, Real* Hpsi,
-                                 Real* Hinv, double* g, double* gl, double h2,
-                                 hipStream_t st);
+                                 double* S, const double* s, Real* dsdF, const HessOut<Real>& ho,
+                                 double* g, double* gl, double h2, hipStream_t st);
 // Phase 2: element stiffness blocks vol * B^T Hpsi B (10 unique 3x3 blocks
 // per tet) and the element residual f_e (12 per tet).
 template <typename Real>
 void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
                              const Real* dsdF, const Real* Hpsi, const double* gl,
                              Real* blocks, double* fe, hipStream_t st);
-// Local solve (reference mixed_stretch.cpp:244-259): lambda and ds from dx.
+// Local solve (reference mixed_stretch.cpp:244-259): lambda and ds from dx,
+// with the FP64 Hessian and inverse (Hpsi, Hinv as in HessOut: the Real
+// arrays in FP64 mode, Hloc / Hlinv in FP32 storage mode).
 template <typename Real>
 void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
-                              const Real* Hpsi, const Real* Hinv, const double* g,
+                              const double* Hpsi, const double* Hinv, const double* g,
                               const double* gl, double* lam, double* ds, ReduceSlot max_ds,
                               hipStream_t st);
 // Line-search energy E(alpha) of the mixed formulation (reference
@@ -74,15 +88,15 @@ void launch_plain_energy(const TetMeshDev& m, const double* x_full, const double
 // rows of the 3D arrays).
 template <typename Real>
 void launch_mixed2d_polar_material(const TetMeshDev& m, const double* x_full, double* R, double* S,
-                                   const double* s, Real* dsdF, Real* Hpsi, Real* Hinv, double* g,
-                                   double* gl, double h2, hipStream_t st);
+                                   const double* s, Real* dsdF, const HessOut<Real>& ho,
+                                   double* g, double* gl, double h2, hipStream_t st);
 template <typename Real>
 void launch_mixed2d_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
                                const Real* dsdF, const Real* Hpsi, const double* gl, Real* blocks,
                                double* fe, hipStream_t st);
 template <typename Real>
 void launch_mixed2d_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
-                                const Real* Hpsi, const Real* Hinv, const double* g,
+                                const double* Hpsi, const double* Hinv, const double* g,
                                 const double* gl, double* lam, double* ds, ReduceSlot max_ds,
                                 hipStream_t st);
 void launch_mixed2d_energy(const TetMeshDev& m, const double* x_full, const double* dx,
