@@ -9,8 +9,15 @@ timestep (NewtonOptimizer::step); the metric counts Newton iterations x tets
 per second over the timed timesteps.
 
 The mixed FP32/FP64 mode is the reported `value`; the FP64 device mode runs
-the same timesteps for the comparison; the FP64 CPU oracle is timed on a
-bounded sample (one Newton iteration of the same scene) on the host cores.
+the same timesteps for the comparison. A second FP64 run with another CG
+kernel shape (dot products summed in another order) measures how far
+rounding alone moves the trajectory. The run is marked invalid (exit status 3
+after the JSON line) when the mixed positions leave the FP64 ones by more
+than 1e-6 relative (2-norm), or by more than 3x that FP64 rounding spread
+where the spread is the larger, or when mixed mode fails more line searches
+than FP64. The FP64 CPU oracle is timed on a bounded sample of the
+same scene on the host cores: one warm-up timestep, then the median of three
+timesteps.
 
 Prints one JSON line.
 """
@@ -78,13 +85,20 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
     spmv_n = 0
     t0 = time.perf_counter()
     per_step = []
+    ls_fail = []
+    halvings = []
     for k in range(steps):
         ts = time.perf_counter()
         sim.step()
         st = sim.stats()
         per_step.append(time.perf_counter() - ts)
+        hist = sim.newton_history()
+        halv = int(hist[:, 9].sum()) if hist.size else 0
+        ls_fail.append(int(st["ls_fail"]))
+        halvings.append(halv)
         print(f"[bench] {precision} step {k}: {per_step[-1] * 1e3:.1f} ms, newton {st['newton_iters']}, "
-              f"cg {st['pcg_iters']}, sweeps {st['refinements']}", file=sys.stderr, flush=True)
+              f"cg {st['pcg_iters']}, sweeps {st['refinements']}, ls halvings {halv}, "
+              f"ls failures {st['ls_fail']}", file=sys.stderr, flush=True)
         newton += st["newton_iters"]
         pcg += st["pcg_iters"]
         spmv_ms += st["spmv_ms"]
@@ -102,7 +116,21 @@ def run_gpu(scene, precision, steps, warmup, profile=True):
     sim.close()
     return dict(elapsed=elapsed, newton=newton, pcg=pcg, spmv_ms=spmv_ms, spmv_n=spmv_n,
                 nfree=nfree, nnzb=nnzb, vb=vb, positions=x, per_step=per_step, last=st,
-                tuning=tuning)
+                tuning=tuning, ls_fail=ls_fail, halvings=halvings)
+
+
+def fp64_spread_run(scene, steps, warmup, tuned_shape):
+    """The FP64 run again with a different CG kernel shape (so its dot
+    products are summed in another order), untimed: its final positions give
+    the trajectory's sensitivity to rounding."""
+    from mfem_amd import native
+    sim = native.Sim(scene, precision="fp64")
+    sim.set_cg_shape(3 if tuple(tuned_shape[:3]) != (4, 4, 256) else 0, 0)
+    for _ in range(warmup + steps):
+        sim.step()
+    x = sim.positions()
+    sim.close()
+    return x
 
 
 def run_cpu_minres(scene, threads, caps=(4, 12)):
@@ -142,7 +170,7 @@ def run_minres(args, scene):
     its_per_newton = r["pcg"] / max(r["newton"], 1)
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = cpu_threads(args.cpu_threads)
         fixed, per_iter, raw = run_cpu_minres(scene, threads)
         t_newton = fixed + its_per_newton * per_iter
         cpu = {"value": nT / t_newton, "unit": "Newton-steps*tets/s", "cores": threads,
@@ -188,13 +216,8 @@ def run_ic(args, scene):
     value = r["newton"] * nT / r["elapsed"]
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        dt_cpu, st_cpu = run_cpu_sample(scene, threads)
-        cpu = {"value": st_cpu["newton_iters"] * nT / dt_cpu, "unit": "Newton-steps*tets/s",
-               "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle, 1 Newton iteration of the same {nT}-tet scene with its "
-                         f"IC-CG (serial triangular solves): {dt_cpu:.2f} s, "
-                         f"{st_cpu['pcg_iters_total']} CG iterations"}
+        cpu = cpu_baseline_entry(scene, cpu_threads(args.cpu_threads), args,
+                                 label="IC-CG, serial triangular solves")
     out = {
         "metric": "Newton-steps*tets/s, Schur complement with IC-preconditioned CG (eigen-pcg-IC)",
         "value": value, "unit": "Newton-steps*tets/s", "n_gpus": 1, "steps": args.steps,
@@ -214,20 +237,67 @@ def run_ic(args, scene):
     print(json.dumps(out), flush=True)
 
 
-def run_cpu_sample(scene, threads):
-    """FP64 oracle on one Newton iteration of the same scene (bounded sample)."""
+def cpu_threads(requested=0):
+    """Host threads for the CPU baseline: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when the environment sets it (the share of the
+    host given to one GPU)."""
+    if requested:
+        return requested
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        avail = min(avail, int(env))
+    return max(1, avail)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_sample(scene, threads, steps=3, warmup=1, max_newton=None):
+    """FP64 oracle on the same scene: `warmup` timesteps not counted, then the
+    median over `steps` timesteps of Newton iterations x tets / seconds.
+    Returns (median rate, list of (seconds, newton, cg) per measured step)."""
     from oracle import oracle as O
     import copy
     sc = copy.copy(scene)
-    sc.max_newton = 1
+    if max_newton is not None:
+        sc.max_newton = max_newton
     sim = O.Sim(sc, threads=threads)
-    print(f"[bench] CPU oracle sample on {threads} threads ...", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    sim.step()
-    dt = time.perf_counter() - t0
-    st = sim.stats()
-    print(f"[bench] CPU oracle sample: {dt:.2f} s", file=sys.stderr, flush=True)
-    return dt, st
+    print(f"[bench] CPU oracle sample on {threads} threads: {warmup} warm-up + {steps} timesteps",
+          file=sys.stderr, flush=True)
+    rows = []
+    for k in range(warmup + steps):
+        t0 = time.perf_counter()
+        sim.step()
+        dt = time.perf_counter() - t0
+        st = sim.stats()
+        print(f"[bench] CPU oracle timestep {k}{' (warm-up)' if k < warmup else ''}: {dt:.2f} s, "
+              f"newton {st['newton_iters']}, cg {st['pcg_iters_total']}", file=sys.stderr,
+              flush=True)
+        if k >= warmup:
+            rows.append((dt, st["newton_iters"], st["pcg_iters_total"]))
+    rates = sorted(n * scene.nT / t for t, n, _ in rows)
+    return rates[len(rates) // 2], rows
+
+
+def cpu_baseline_entry(scene, threads, args, label="Jacobi CG"):
+    rate, rows = run_cpu_sample(scene, threads, steps=args.cpu_steps)
+    secs = ", ".join(f"{t:.2f} s/{n} Newton/{c} CG" for t, n, c in rows)
+    return {"value": rate, "unit": "Newton-steps*tets/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "cpus_visible": os.cpu_count(),
+            "sample": f"FP64 oracle ({label}) on the same {scene.nT}-tet scene: 1 warm-up "
+                      f"timestep, then the median over {len(rows)} timesteps ({secs})"}
 
 
 def run_batch(scs, precision, steps, warmup):
@@ -292,7 +362,7 @@ def run_c4(args):
         work = 0
         errs = []
         for k in order:
-            o = O.Sim(scs[k], threads=args.cpu_threads or min(os.cpu_count() or 1, 16))
+            o = O.Sim(scs[k], threads=cpu_threads(args.cpu_threads))
             for _ in range(args.warmup + args.steps):
                 o.step()
                 work += o.stats()["newton_iters"] * scs[k].nT
@@ -300,7 +370,7 @@ def run_c4(args):
             errs.append(float(np.linalg.norm(mixed["positions"][k] - xo) / np.linalg.norm(xo)))
         dt_cpu = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": work / dt_cpu, "unit": "Newton-steps*tets/s",
-                               "cores": args.cpu_threads or min(os.cpu_count() or 1, 16),
+                               "cores": cpu_threads(args.cpu_threads),
                                "kind": "port",
                                "sample": f"FP64 oracle on the 8 smallest beams, "
                                          f"{args.warmup + args.steps} steps each, {dt_cpu:.2f} s"}
@@ -325,8 +395,8 @@ def workload_name(config, nx, ny, nz, nT, scene):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", choices=("c3", "c2", "c5", "c4"), default="c3",
                     help="c3: the metric's 960k-tet mixed beam (default); c2: plain FP64 "
                          "24k-tet beam; c5: 1.95M-tet stiff ARAP/SNH beam; c4: 256 independent "
@@ -338,6 +408,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timesteps of the CPU-baseline median (after one warm-up timestep)")
     ap.add_argument("--solver", choices=("eigen-pcg-diag", "eigen-pcg-IC",
                                          "minres-indefinite-block"),
                     default="eigen-pcg-diag",
@@ -375,6 +447,10 @@ def main():
     achieved = bytes_spmv / (avg_spmv_ms * 1e-3) / 1e9 if avg_spmv_ms > 0 else 0.0
 
     extra = {}
+    valid = True
+    reasons = []
+    extra["mixed_ls_failures"] = sum(mixed["ls_fail"])
+    extra["mixed_ls_halvings"] = sum(mixed["halvings"])
     if fp64 is not None:
         v64 = fp64["newton"] * nT / fp64["elapsed"]
         x64 = fp64["positions"]
@@ -384,20 +460,36 @@ def main():
         extra["fp64_gpu"] = {
             "value": v64, "ms_per_step": 1000.0 * fp64["elapsed"] / args.steps,
             "newton_iters": fp64["newton"], "pcg_iters": fp64["pcg"],
+            "ls_failures": sum(fp64["ls_fail"]), "ls_halvings": sum(fp64["halvings"]),
             "cg_kernel_gbs": b64 / (avg64 * 1e-3) / 1e9 if avg64 > 0 else 0.0,
             "cg_kernel_ms": avg64}
         extra["mixed_over_fp64"] = value / v64 if v64 > 0 else None
         extra["mixed_vs_fp64_rel_pos_err"] = err
+        # rounding floor: FP64 against FP64 with other dot-product orders
+        tuned = (fp64["tuning"]["lanes_per_row"], fp64["tuning"]["blocks_in_flight"],
+                 fp64["tuning"]["workgroup"]) if fp64["tuning"] else (4, 4, 256)
+        xb = fp64_spread_run(scene, args.steps, args.warmup, tuned)
+        spread = float(np.linalg.norm(xb - x64) / np.linalg.norm(x64))
+        extra["fp64_vs_fp64_rounding_spread"] = spread
+        bar = max(1e-6, 3.0 * spread)
+        extra["mixed_error_bar"] = bar
+        # the bar is the precision config's (c3); c2 and c5 report
+        if args.config == "c3" and not err <= bar:
+            valid = False
+            reasons.append(f"mixed positions {err:.3e} from FP64 (bar {bar:.1e}: 1e-6 or 3x the "
+                           f"FP64 rounding spread {spread:.1e})")
+        if args.config == "c3" and sum(mixed["ls_fail"]) > sum(fp64["ls_fail"]):
+            valid = False
+            reasons.append(f"mixed failed {sum(mixed['ls_fail'])} line searches, FP64 "
+                           f"{sum(fp64['ls_fail'])}")
+    extra["valid"] = valid
+    if reasons:
+        extra["invalid_reason"] = "; ".join(reasons)
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        dt_cpu, st_cpu = run_cpu_sample(scene, threads)
-        cpu_value = st_cpu["newton_iters"] * nT / dt_cpu
-        cpu = {"value": cpu_value, "unit": "Newton-steps*tets/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 oracle, 1 Newton iteration (1 timestep, max_newton=1) of the same "
-                         f"{nT}-tet scene: {dt_cpu:.2f} s, {st_cpu['pcg_iters_total']} CG iterations"}
-        extra["gpu_over_cpu"] = value / cpu_value if cpu_value > 0 else None
+        cpu = cpu_baseline_entry(scene, cpu_threads(args.cpu_threads), args)
+        extra["gpu_over_cpu"] = value / cpu["value"] if cpu["value"] > 0 else None
 
     out = {
         "metric": "Newton-steps*tets/s at 1M tets, FP32-mixed vs FP64 GPU; PCG HBM GB/s",
@@ -436,6 +528,9 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
+    if not valid:
+        print(f"[bench] INVALID: {extra['invalid_reason']}", file=sys.stderr, flush=True)
+        sys.exit(3)
 
 
 if __name__ == "__main__":
