@@ -93,6 +93,15 @@ int mfem_sim_destroy(mfem_sim* sim);
 
 /* NewtonOptimizer<3>::step() (reference newton_optimizer.cpp:11-89). */
 int mfem_sim_step(mfem_sim* sim);
+/* Optimizer<3>::callback (reference optimizers/optimizer.h:16-42), which
+ * NewtonOptimizer::step invokes once per Newton iteration, after the line
+ * search (newton_optimizer.cpp:68). `cb` runs on the calling thread inside
+ * mfem_sim_step with the simulation, the 1-based Newton iteration and `user`;
+ * it may read the simulation (mfem_sim_get_positions gives the positions
+ * after the iteration's accepted step, mfem_sim_get its arrays) but must not
+ * step or destroy it. cb == NULL removes the callback. */
+typedef void (*mfem_newton_callback)(mfem_sim* sim, int iteration, void* user);
+int mfem_sim_set_callback(mfem_sim* sim, mfem_newton_callback cb, void* user);
 /* Phases of step(): x_tilde + update_system() (:92-110), substep() (:113-124)
  * and the line-search energy E(alpha) (:34-49). */
 int mfem_sim_update_system(mfem_sim* sim);
@@ -205,6 +214,28 @@ int mfem_batch_stats(mfem_batch* batch, int* newton_iters, int* pcg_iters, int* 
  * variant -1 restores autotuning on the next solve. */
 int mfem_batch_set_cg_shape(mfem_batch* batch, int variant, int rows_per_workgroup);
 int mfem_batch_cg_shape(mfem_batch* batch, int* variant, int* rows_per_workgroup);
+
+/* LinearSolver::solve() of "eigen-pcg-diag" on a caller's system (reference
+ * linear_solvers/linear_solver.h:8-21; EigenIterativeSolver<ConjugateGradient<
+ * SpMat, Lower|Upper>> with Eigen's default DiagonalPreconditioner,
+ * eigen_iterative_solver.h:14-29, as linear_solver_factory.cpp:75-85
+ * configures it): x = K^-1 b by Jacobi-preconditioned CG in FP64 with Eigen's
+ * semantics -- x0 = 0; preconditioner 1 / K_jj (1 where the diagonal entry is
+ * zero or absent); stop when |b - K x|^2 < max(tol^2 |b|^2, DBL_MIN) or after
+ * max_iters iterations. K is symmetric positive definite, stored as 3x3-block
+ * CSR (the reference's assembled lhs holds full 3x3 blocks,
+ * utils/sparse_utils.cpp:19-138): nb block rows, rowptr[nb + 1] (rowptr[0] =
+ * 0, non-decreasing, at most nnzb), col[nnzb] block columns in [0, nb),
+ * vals[9 nnzb] with each block row-major. All arrays are device pointers; b
+ * and x (out) hold 3 nb values. The pattern is checked on the device first
+ * (an error for a malformed rowptr or an out-of-range column, before any CG
+ * work). iterations receives Eigen's iterations(), rel_residual its error()
+ * (|r| / |b|); either may be NULL. The work arrays (72 bytes per scalar row
+ * plus 320 KB) are allocated for the call. Runs on `stream` (NULL: the
+ * default stream) and returns after the solve. */
+int mfem_pcg_bsr3(int nb, int nnzb, const int* rowptr, const int* col, const double* vals,
+                  const double* b, double* x, double tol, int max_iters, int* iterations,
+                  double* rel_residual, void* stream);
 
 /* Batched MaterialModel evaluation on the 6-vector s (reference
  * energies/material_model.h:29-38): device pointers, n points, s n x 6,

@@ -21,6 +21,7 @@ SOURCES = [
     "k_minres.hip",
     "k_solver.hip",
     "k_vector.hip",
+    "pcg_abi.hip",
     "host/batch.cpp",
     "host/json.cpp",
     "host/material_model.cpp",

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "host/abi_error.h"
 #include "host/batch.h"
 #include "host/output.h"
 #include "host/sim.h"
@@ -27,20 +28,16 @@ struct mfem_batch {
 };
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+void mfem::set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
 
 template <typename F>
 int guarded(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-  } catch (...) {
-    g_last_error = "unknown error";
-  }
-  return 1;
+  return mfem::abi_guarded(std::forward<F>(f));
 }
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -81,9 +78,16 @@ std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
   if (k == "g") return soa_to_elem(need_ms()->g_.to_host(st), nT, ws);
   if (k == "gl") return soa_to_elem(need_ms()->gl_.to_host(st), nT, ws);
   if (k == "dsdF") return soa_to_elem(need_ms()->dsdF_.to_host_double(st), nT, two ? 12 : 54);
+  // H and Hinv: the FP64 matrices when the variable keeps them (the local
+  // solve's copies in FP32 storage mode), else the stored ones
+  auto hess_host = [&](bool inv) {
+    auto* ms = need_ms();
+    const auto& d = inv ? ms->Hlinv_ : ms->Hloc_;
+    if (d.size()) return d.to_host(st);
+    return inv ? ms->Hinv_.to_host_double(st) : ms->Hpsi_.to_host_double(st);
+  };
   if (two && (k == "H" || k == "Hinv")) {
-    auto P = soa_to_elem(k == "H" ? need_ms()->Hpsi_.to_host_double(st)
-                                  : need_ms()->Hinv_.to_host_double(st), nT, 6);
+    auto P = soa_to_elem(hess_host(k == "Hinv"), nT, 6);
     std::vector<double> o((size_t)nT * 9);
     const double syminv[3] = {1, 1, 0.5};
     for (int e = 0; e < nT; ++e)
@@ -97,8 +101,7 @@ std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
     return o;
   }
   if (k == "H" || k == "Hinv") {
-    auto P = soa_to_elem(k == "H" ? need_ms()->Hpsi_.to_host_double(st)
-                                  : need_ms()->Hinv_.to_host_double(st), nT, 21);
+    auto P = soa_to_elem(hess_host(k == "Hinv"), nT, 21);
     std::vector<double> o((size_t)nT * 36);
     const double syminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
     for (int e = 0; e < nT; ++e)
@@ -295,6 +298,20 @@ int mfem_sim_destroy(mfem_sim* sim) {
 
 int mfem_sim_step(mfem_sim* sim) {
   return guarded([&] { sim->opt->step(); });
+}
+
+int mfem_sim_set_callback(mfem_sim* sim, mfem_newton_callback cb, void* user) {
+  return guarded([&] {
+    if (!sim) throw std::invalid_argument("null simulation");
+    if (!cb) {
+      sim->opt->callback = [](const SimState<3>&) {};
+      return;
+    }
+    NewtonOptimizer<3>* opt = sim->opt.get();
+    sim->opt->callback = [sim, opt, cb, user](const SimState<3>&) {
+      cb(sim, (int)opt->history().size(), user);
+    };
+  });
 }
 
 int mfem_sim_update_system(mfem_sim* sim) {

@@ -29,8 +29,11 @@ EXPORTED = [
     "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
     "mfem_batch_create", "mfem_batch_destroy", "mfem_batch_step", "mfem_batch_sizes",
     "mfem_batch_get_positions", "mfem_batch_stats", "mfem_batch_set_cg_shape",
-    "mfem_batch_cg_shape",
+    "mfem_batch_cg_shape", "mfem_sim_set_callback", "mfem_pcg_bsr3",
 ]
+
+# void (*)(mfem_sim*, int iteration, void* user)
+NEWTON_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_void_p)
 
 
 class SimParams(C.Structure):
@@ -114,6 +117,9 @@ def lib():
         L.mfem_batch_stats.argtypes = [vp, ip, ip, ip, ip, dp, dp]
         L.mfem_batch_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
         L.mfem_batch_cg_shape.argtypes = [vp, ip, ip]
+        L.mfem_sim_set_callback.argtypes = [vp, NEWTON_CALLBACK, vp]
+        L.mfem_pcg_bsr3.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int,
+                                    ip, dp, vp]
         _lib = L
     return _lib
 
@@ -206,6 +212,16 @@ class Sim:
 
     def step(self):
         check(lib().mfem_sim_step(self.h))
+
+    def set_callback(self, fn):
+        """fn(iteration) after every Newton iteration of step() (the
+        reference's Optimizer::callback); None removes it."""
+        if fn is None:
+            self._cb = None
+            check(lib().mfem_sim_set_callback(self.h, NEWTON_CALLBACK(), None))
+            return
+        self._cb = NEWTON_CALLBACK(lambda _sim, it, _user: fn(it))
+        check(lib().mfem_sim_set_callback(self.h, self._cb, None))
 
     HISTORY_COLUMNS = ("iteration", "energy", "energy_res", "newton_dec", "alpha", "update_ms",
                        "solve_ms", "ls_ms", "solver_iters", "ls_halvings")
@@ -470,3 +486,17 @@ def boundary_faces(V, T):
     check(lib().mfem_boundary_faces(V.ctypes.data_as(dp), V.shape[0], T.ctypes.data_as(ip),
                                     T.shape[0], F.ctypes.data_as(ip), C.byref(n)))
     return F
+
+
+def pcg_bsr3(rowptr, col, vals, b, tol, max_iters):
+    """mfem_pcg_bsr3 on torch device tensors (rowptr, col int32; vals 9 nnzb
+    and b 3 nb float64): returns (x, iterations, relative residual)."""
+    import torch
+    nb = int(rowptr.numel()) - 1
+    x = torch.empty(3 * nb, dtype=torch.float64, device=b.device)
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    check(lib().mfem_pcg_bsr3(nb, int(col.numel()), rowptr.data_ptr(), col.data_ptr(),
+                              vals.data_ptr(), b.data_ptr(), x.data_ptr(), float(tol),
+                              int(max_iters), C.byref(it), C.byref(rr), _stream()))
+    return x, it.value, rr.value

@@ -141,6 +141,8 @@ int oracle_ic_factor_dense(int n, int bs, const int* rowptr, const int* colind, 
                            double* L, double* scale, double* shift, int* color, int* attempts);
 int oracle_ic_pcg(int n, int bs, const int* rowptr, const int* colind, const double* vals,
                   const double* b, double* x, double tol, int max_iters, double* err);
+int oracle_jacobi_pcg(int n, const int* rowptr, const int* colind, const double* vals,
+                      const double* b, double* x, double tol, int max_iters, double* err);
 double oracle_minres_dense(int N, const double* A, const double* P, const double* b, double* x,
                            int max_iters, double tol, int* iters);
 

@@ -103,6 +103,8 @@ def lib():
         L.oracle_ic_factor_dense.restype = C.c_int
         L.oracle_ic_pcg.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
         L.oracle_ic_pcg.restype = C.c_int
+        L.oracle_jacobi_pcg.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
+        L.oracle_jacobi_pcg.restype = C.c_int
         _lib = L
     return _lib
 
@@ -410,4 +412,17 @@ def ic_pcg(rowptr, colind, vals, bs, b, tol, max_iters):
                              int(max_iters), _dp(err))
     if it < 0:
         raise RuntimeError("incomplete Cholesky broke down")
+    return x, it, float(err[0])
+
+
+def jacobi_pcg(rowptr, colind, vals, b, tol, max_iters):
+    """Eigen's CG with the diagonal preconditioner ("eigen-pcg-diag") on a
+    scalar CSR matrix, x0 = 0: (x, iterations, |r| / |b|)."""
+    rp, ci, va = _csr(rowptr, colind, vals)
+    n = rp.shape[0] - 1
+    b = f64(b)
+    x = np.zeros(n)
+    err = np.zeros(1)
+    it = lib().oracle_jacobi_pcg(n, _ip(rp), _ip(ci), _dp(va), _dp(b), _dp(x), float(tol),
+                                 int(max_iters), _dp(err))
     return x, it, float(err[0])

@@ -2051,6 +2051,33 @@ int oracle_ic_factor_dense(int n, int bs, const int* rowptr, const int* colind, 
   return 0;
 }
 
+// Eigen's CG with its default DiagonalPreconditioner (reference
+// eigen_iterative_solver.h:14-29, "eigen-pcg-diag") on a scalar CSR matrix:
+// x0 = 0, z = D^-1 r with D^-1_jj = 1 / a_jj, or 1 where the diagonal entry is
+// zero or absent. Returns the iteration count; err = |r| / |b|.
+int oracle_jacobi_pcg(int n, const int* rowptr, const int* colind, const double* vals,
+                      const double* b, double* x, double tol, int max_iters, double* err) {
+  std::vector<double> invd(n, 1.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (colind[k] == i) {
+        if (vals[k] != 0.0) invd[i] = 1.0 / vals[k];
+        break;
+      }
+  auto mult = [&](const std::vector<double>& v, std::vector<double>& y) {
+    y.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      double a = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) a += vals[k] * v[colind[k]];
+      y[i] = a;
+    }
+  };
+  auto pre = [&](const double* r, double* z) {
+    for (int i = 0; i < n; ++i) z[i] = invd[i] * r[i];
+  };
+  return cg_loop(n, mult, pre, b, x, tol, max_iters, err);
+}
+
 // Eigen's CG with that preconditioner on a scalar CSR matrix (x0 = 0).
 // Returns the iteration count, or a negative code when the factor fails.
 int oracle_ic_pcg(int n, int bs, const int* rowptr, const int* colind, const double* vals,

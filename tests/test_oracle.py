@@ -565,3 +565,52 @@ def test_newton_history_rows(oracle):
         assert np.all(np.log2(h[:, 4]) == np.round(np.log2(h[:, 4])))
         assert h[-1, 3] == st["last_decrement"]
         assert np.all(np.diff(E) <= 1e-12 * np.abs(E[:-1]))
+
+
+def _random_spd_bsr(nb, seed, fill=0.3):
+    """A symmetric positive definite 3x3-block matrix with a random block
+    pattern (diagonal blocks always present): (dense A, scalar CSR arrays)."""
+    rng = np.random.default_rng(seed)
+    P = rng.random((nb, nb)) < fill
+    P = P | P.T | np.eye(nb, dtype=bool)
+    A = np.zeros((3 * nb, 3 * nb))
+    for i in range(nb):
+        for j in range(i, nb):
+            if P[i, j]:
+                B = rng.standard_normal((3, 3))
+                if i == j:
+                    B = B + B.T
+                A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = B
+                A[3 * j:3 * j + 3, 3 * i:3 * i + 3] = B.T
+    A = A + np.diag(np.abs(A).sum(axis=1) + 1.0)  # symmetric, diagonally dominant: SPD
+    mask = np.kron(P, np.ones((3, 3), dtype=bool))
+    rowptr = np.concatenate([[0], np.cumsum(mask.sum(axis=1))]).astype(np.int32)
+    colind = np.concatenate([np.nonzero(mask[i])[0] for i in range(3 * nb)]).astype(np.int32)
+    vals = np.concatenate([A[i, mask[i]] for i in range(3 * nb)])
+    return A, rowptr, colind, vals, P
+
+
+def test_jacobi_pcg_krylov_minimizer(oracle):
+    """Eigen's CG with the diagonal preconditioner: after k iterations x_k
+    minimizes the A-norm error over the Krylov space K_k(D^-1 A, D^-1 b)
+    (brute-force least squares); max_iters = 0 returns x = 0; a tight
+    tolerance reaches the dense solution; the error is |b - A x| / |b|."""
+    A, rp, ci, va, _ = _random_spd_bsr(8, 3)
+    n = A.shape[0]
+    b = np.random.default_rng(4).standard_normal(n)
+    xs = np.linalg.solve(A, b)
+    x, it, err = oracle.jacobi_pcg(rp, ci, va, b, 1e-14, 10 * n)
+    assert np.abs(x - xs).max() < 1e-10 * np.abs(xs).max()
+    assert abs(err - np.linalg.norm(b - A @ x) / np.linalg.norm(b)) < 1e-12
+    x0, it0, _ = oracle.jacobi_pcg(rp, ci, va, b, 1e-14, 0)
+    assert it0 == 0 and not x0.any()
+    Dinv = 1.0 / np.diag(A)
+    K = [Dinv * b]
+    for k in range(1, 6):
+        xk, itk, _ = oracle.jacobi_pcg(rp, ci, va, b, 0.0, k)
+        assert itk == k
+        Q = np.stack(K, axis=1)
+        # min over y of |x* - Q y|_A  <=>  (Q^T A Q) y = Q^T A x* = Q^T b
+        y = np.linalg.solve(Q.T @ A @ Q, Q.T @ b)
+        assert np.abs(xk - Q @ y).max() < 1e-9 * np.abs(xs).max(), k
+        K.append(Dinv * (A @ K[-1]))
