@@ -237,6 +237,21 @@ int mfem_pcg_bsr3(int nb, int nnzb, const int* rowptr, const int* col, const dou
                   const double* b, double* x, double tol, int max_iters, int* iterations,
                   double* rel_residual, void* stream);
 
+/* mfem_pcg_bsr3 without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code run
+ * while a stream is being captured, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, semantics and results as mfem_pcg_bsr3 (the
+ * same kernels on the same launch shape). The work arrays live in one fixed
+ * device array of the library per GPU (about 38 MB, held until the process
+ * ends), which limits the system to 3 nb <= MFEM_PCG_NOALLOC_MAX_ROWS scalar
+ * rows; larger systems are rejected with an error. All calls on one GPU share
+ * that array and its contents from the last call: calls that overlap on one
+ * GPU would race, so the library serializes them within a process. */
+#define MFEM_PCG_NOALLOC_MAX_ROWS 524288
+int mfem_pcg_bsr3_noalloc(int nb, int nnzb, const int* rowptr, const int* col,
+                          const double* vals, const double* b, double* x, double tol,
+                          int max_iters, int* iterations, double* rel_residual, void* stream);
+
 /* Batched MaterialModel evaluation on the 6-vector s (reference
  * energies/material_model.h:29-38): device pointers, n points, s n x 6,
  * E n, g n x 6, H n x 36 column-major; psd_fix as hessian(s, psd_fix). */

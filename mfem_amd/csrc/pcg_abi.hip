@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -176,5 +177,65 @@ extern "C" int mfem_pcg_bsr3(int nb, int nnzb, const int* rowptr, const int* col
       throw;
     }
     if (work) MFEM_HIP_CHECK(hipFree(work));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// The same solve without any device allocation: the work arrays live in a
+// fixed array of this library's code object, one per GPU, sized for
+// MFEM_PCG_NOALLOC_MAX_ROWS scalar rows. In host code `scratch` is only a
+// stand-in symbol; its device address comes from hipGetSymbolAddress, once per
+// device, and only that address is handed to kernels.
+namespace mfemhip {
+namespace {
+
+constexpr long kNoAllocRows = MFEM_PCG_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocBytes =
+    sizeof(double) * 9 * (size_t)kNoAllocRows + sizeof(double) * 2 * kCgParts * kCgMaxBlocks +
+    4 * 256;
+static_assert(kNoAllocBytes % sizeof(double) == 0, "scratch size");
+
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call there). Called with g_noalloc_mu held.
+char* scratch_on_current_device() {
+  int dev = 0;
+  MFEM_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+}  // namespace mfemhip
+
+extern "C" int mfem_pcg_bsr3_noalloc(int nb, int nnzb, const int* rowptr, const int* col,
+                                     const double* vals, const double* b, double* x, double tol,
+                                     int max_iters, int* iterations, double* rel_residual,
+                                     void* stream) {
+  return mfem::abi_guarded([&] {
+    if (nb < 0) throw std::invalid_argument("negative matrix size");
+    if (3L * nb > mfemhip::kNoAllocRows)
+      throw std::invalid_argument(
+          "system of " + std::to_string(3L * nb) + " rows exceeds the no-allocation limit of " +
+          std::to_string(mfemhip::kNoAllocRows) + " rows (MFEM_PCG_NOALLOC_MAX_ROWS)");
+    if (mfemhip::pcg_bsr3_work_bytes(3L * nb) > mfemhip::kNoAllocBytes)
+      throw std::logic_error("no-allocation work layout exceeds the scratch array");
+    std::lock_guard<std::mutex> lock(mfemhip::g_noalloc_mu);
+    char* work = nb > 0 ? mfemhip::scratch_on_current_device() : nullptr;
+    mfemhip::pcg_bsr3_run(work, nb, nnzb, rowptr, col, vals, b, x, tol, max_iters, iterations,
+                          rel_residual, reinterpret_cast<hipStream_t>(stream));
   });
 }

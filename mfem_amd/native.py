@@ -29,8 +29,11 @@ EXPORTED = [
     "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
     "mfem_batch_create", "mfem_batch_destroy", "mfem_batch_step", "mfem_batch_sizes",
     "mfem_batch_get_positions", "mfem_batch_stats", "mfem_batch_set_cg_shape",
-    "mfem_batch_cg_shape", "mfem_sim_set_callback", "mfem_pcg_bsr3",
+    "mfem_batch_cg_shape", "mfem_sim_set_callback", "mfem_pcg_bsr3", "mfem_pcg_bsr3_noalloc",
 ]
+
+# scalar-row limit of mfem_pcg_bsr3_noalloc (MFEM_PCG_NOALLOC_MAX_ROWS)
+PCG_NOALLOC_MAX_ROWS = 524288
 
 # void (*)(mfem_sim*, int iteration, void* user)
 NEWTON_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_void_p)
@@ -120,6 +123,7 @@ def lib():
         L.mfem_sim_set_callback.argtypes = [vp, NEWTON_CALLBACK, vp]
         L.mfem_pcg_bsr3.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int,
                                     ip, dp, vp]
+        L.mfem_pcg_bsr3_noalloc.argtypes = L.mfem_pcg_bsr3.argtypes
         _lib = L
     return _lib
 
@@ -488,15 +492,19 @@ def boundary_faces(V, T):
     return F
 
 
-def pcg_bsr3(rowptr, col, vals, b, tol, max_iters):
-    """mfem_pcg_bsr3 on torch device tensors (rowptr, col int32; vals 9 nnzb
-    and b 3 nb float64): returns (x, iterations, relative residual)."""
+def pcg_bsr3(rowptr, col, vals, b, tol, max_iters, noalloc=False, x=None):
+    """mfem_pcg_bsr3 (or, with noalloc, mfem_pcg_bsr3_noalloc) on torch device
+    tensors (rowptr, col int32; vals 9 nnzb and b 3 nb float64; x, 3 nb
+    float64, is allocated when not given): returns (x, iterations, relative
+    residual)."""
     import torch
     nb = int(rowptr.numel()) - 1
-    x = torch.empty(3 * nb, dtype=torch.float64, device=b.device)
+    if x is None:
+        x = torch.empty(3 * nb, dtype=torch.float64, device=b.device)
     it = C.c_int(0)
     rr = C.c_double(0)
-    check(lib().mfem_pcg_bsr3(nb, int(col.numel()), rowptr.data_ptr(), col.data_ptr(),
-                              vals.data_ptr(), b.data_ptr(), x.data_ptr(), float(tol),
-                              int(max_iters), C.byref(it), C.byref(rr), _stream()))
+    fn = lib().mfem_pcg_bsr3_noalloc if noalloc else lib().mfem_pcg_bsr3
+    check(fn(nb, int(col.numel()), rowptr.data_ptr(), col.data_ptr(), vals.data_ptr(),
+             b.data_ptr(), x.data_ptr(), float(tol), int(max_iters), C.byref(it), C.byref(rr),
+             _stream()))
     return x, it.value, rr.value
