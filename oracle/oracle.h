@@ -143,6 +143,8 @@ int oracle_ic_pcg(int n, int bs, const int* rowptr, const int* colind, const dou
                   const double* b, double* x, double tol, int max_iters, double* err);
 int oracle_jacobi_pcg(int n, const int* rowptr, const int* colind, const double* vals,
                       const double* b, double* x, double tol, int max_iters, double* err);
+int oracle_envelope_solve(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                          const double* b, double* x, int* perm);
 double oracle_minres_dense(int N, const double* A, const double* P, const double* b, double* x,
                            int max_iters, double tol, int* iters);
 

@@ -105,6 +105,8 @@ def lib():
         L.oracle_ic_pcg.restype = C.c_int
         L.oracle_jacobi_pcg.argtypes = [C.c_int, ip, ip, dp, dp, dp, C.c_double, C.c_int, dp]
         L.oracle_jacobi_pcg.restype = C.c_int
+        L.oracle_envelope_solve.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, ip]
+        L.oracle_envelope_solve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -426,3 +428,17 @@ def jacobi_pcg(rowptr, colind, vals, b, tol, max_iters):
     it = lib().oracle_jacobi_pcg(n, _ip(rp), _ip(ci), _dp(va), _dp(b), _dp(x), float(tol),
                                  int(max_iters), _dp(err))
     return x, it, float(err[0])
+
+
+def envelope_solve(rowptr, colind, vals, bs, b):
+    """The oracle's sparse direct solve (reverse Cuthill-McKee order, envelope
+    Cholesky) of a scalar CSR matrix with bs x bs blocks: (x, vertex order)."""
+    rp, ci, va = _csr(rowptr, colind, vals)
+    n = rp.shape[0] - 1
+    b = f64(b)
+    x = np.zeros(n)
+    perm = np.zeros(n // bs, dtype=np.int32)
+    if lib().oracle_envelope_solve(n, int(bs), _ip(rp), _ip(ci), _dp(va), _dp(b), _dp(x),
+                                   _ip(perm)) != 0:
+        raise RuntimeError("matrix is not positive definite")
+    return x, perm

@@ -958,6 +958,124 @@ int dense_llt(OracleSim* S_, std::vector<double>& xo) {
   return 0;
 }
 
+// Sparse direct solve for larger systems: the same Cholesky factorization of
+// K (the reference's direct solvers, eigen_solver.h:17-31, are Cholesky
+// factorizations; CHOLMOD and SimplicialLLT order the unknowns to limit fill),
+// here with a reverse Cuthill-McKee vertex ordering (Cuthill and McKee 1969,
+// George and Liu 1981) and a row-oriented envelope (profile) factor. The
+// ordering changes only the rounding, not the solution.
+//
+// RCM on the vertex graph of a bs x bs-blocked pattern: breadth-first search
+// from a minimum-degree vertex of each component, neighbours in increasing
+// degree, then the order reversed. Returns perm[new] = old block row.
+std::vector<int> rcm_order(int nb, int bs, const int* rowptr, const int* colind) {
+  std::vector<std::vector<int>> adj(nb);
+  for (int i = 0; i < nb; ++i) {
+    for (int k = rowptr[bs * i]; k < rowptr[bs * i + 1]; ++k) {
+      const int j = colind[k] / bs;
+      if (j != i && (adj[i].empty() || adj[i].back() != j)) adj[i].push_back(j);
+    }
+    std::sort(adj[i].begin(), adj[i].end());
+    adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+  }
+  std::vector<int> order;
+  order.reserve(nb);
+  std::vector<char> seen(nb, 0);
+  std::vector<int> by_degree(nb);
+  for (int i = 0; i < nb; ++i) by_degree[i] = i;
+  std::stable_sort(by_degree.begin(), by_degree.end(),
+                   [&](int a, int b) { return adj[a].size() < adj[b].size(); });
+  for (int start : by_degree) {
+    if (seen[start]) continue;
+    size_t head = order.size();
+    order.push_back(start);
+    seen[start] = 1;
+    while (head < order.size()) {
+      const int v = order[head++];
+      std::vector<int> nb_v;
+      for (int w : adj[v])
+        if (!seen[w]) {
+          seen[w] = 1;
+          nb_v.push_back(w);
+        }
+      std::stable_sort(nb_v.begin(), nb_v.end(),
+                       [&](int a, int b) { return adj[a].size() < adj[b].size(); });
+      order.insert(order.end(), nb_v.begin(), nb_v.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+// Solve A x = b (A symmetric positive definite, scalar CSR with bs x bs
+// blocks) by envelope Cholesky in RCM order. Returns -1 if A is not PD.
+int envelope_llt_solve(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                       const double* b, double* x) {
+  const int nb = n / bs;
+  const std::vector<int> perm = rcm_order(nb, bs, rowptr, colind);
+  std::vector<int> inv(nb);
+  for (int i = 0; i < nb; ++i) inv[perm[i]] = i;
+  auto newidx = [&](int old) { return bs * inv[old / bs] + old % bs; };
+  // first column of every permuted row, then the row offsets of L
+  std::vector<int> first(n);
+  for (int i = 0; i < n; ++i) {
+    const int o = bs * perm[i / bs] + i % bs;
+    int f = i;
+    for (int k = rowptr[o]; k < rowptr[o + 1]; ++k) f = std::min(f, newidx(colind[k]));
+    first[i] = f;
+  }
+  std::vector<long> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) off[i + 1] = off[i] + (i - first[i] + 1);
+  std::vector<double> L(off[n], 0.0);
+  auto at = [&](int i, int j) -> double& { return L[off[i] + (j - first[i])]; };
+  for (int i = 0; i < n; ++i) {
+    const int o = bs * perm[i / bs] + i % bs;
+    for (int k = rowptr[o]; k < rowptr[o + 1]; ++k) {
+      const int j = newidx(colind[k]);
+      if (j <= i) at(i, j) = vals[k];
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const int fi = first[i];
+    double* Li = &L[off[i]];
+    for (int j = fi; j < i; ++j) {
+      const int k0 = std::max(fi, first[j]);
+      const double* Lj = &L[off[j]];
+      double s = Li[j - fi];
+      for (int k = k0; k < j; ++k) s -= Li[k - fi] * Lj[k - first[j]];
+      Li[j - fi] = s / Lj[j - first[j]];
+    }
+    double d = Li[i - fi];
+    for (int k = fi; k < i; ++k) d -= Li[k - fi] * Li[k - fi];
+    if (!(d > 0)) return -1;
+    Li[i - fi] = std::sqrt(d);
+  }
+  std::vector<double> y(n);
+  for (int i = 0; i < n; ++i) {
+    double s = b[bs * perm[i / bs] + i % bs];
+    for (int k = first[i]; k < i; ++k) s -= at(i, k) * y[k];
+    y[i] = s / at(i, i);
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    const double xi = y[i] / at(i, i);
+    y[i] = xi;
+    for (int k = first[i]; k < i; ++k) y[k] -= at(i, k) * xi;
+  }
+  for (int i = 0; i < n; ++i) x[bs * perm[i / bs] + i % bs] = y[i];
+  return 0;
+}
+
+// Direct solve of the Newton system: dense LLT for small systems (the
+// golden trajectories were recorded with it), the RCM envelope factor above
+// beyond 3000 unknowns.
+int direct_solve(OracleSim* S_, std::vector<double>& xo) {
+  if (S_->n <= 3000) return dense_llt(S_, xo);
+  xo.assign(S_->n, 0.0);
+  S_->pcg_iters_last = 0;
+  return envelope_llt_solve(S_->n, S_->dim, S_->asmb.rowptr.data(), S_->asmb.colind.data(),
+                            S_->lhs.data(), S_->rhs.data(), xo.data());
+}
+
 // ---------------- indefinite (KKT) solve: "minres-indefinite-block" -------
 // Unknowns [dx (n); ds (ns nT); lambda (ns nT)], ns = 6 in 3D and 3 in 2D,
 // element-major, as SystemMatrixIndefinite lays them out (reference
@@ -1439,7 +1557,7 @@ int substep(OracleSim* S_, double& decrement) {
     for (int i = 0; i < S_->n; ++i) S_->rhs[i] += -S_->grad_plain[i];
   }
   std::vector<double> sol;
-  int rc = S_->p.solver == 0 ? dense_llt(S_, sol) : S_->p.solver == 4 ? ic_pcg(S_, sol) : pcg(S_, sol);
+  int rc = S_->p.solver == 0 ? direct_solve(S_, sol) : S_->p.solver == 4 ? ic_pcg(S_, sol) : pcg(S_, sol);
   if (rc != 0) return rc;
   S_->st.pcg_iters_total += S_->pcg_iters_last;
   // post_solve
@@ -2076,6 +2194,18 @@ int oracle_jacobi_pcg(int n, const int* rowptr, const int* colind, const double*
     for (int i = 0; i < n; ++i) z[i] = invd[i] * r[i];
   };
   return cg_loop(n, mult, pre, b, x, tol, max_iters, err);
+}
+
+// The envelope Cholesky solve of a scalar CSR matrix with bs x bs blocks
+// (0, or -1 when the matrix is not positive definite); perm receives the RCM
+// vertex order (n / bs entries) when not NULL.
+int oracle_envelope_solve(int n, int bs, const int* rowptr, const int* colind, const double* vals,
+                          const double* b, double* x, int* perm) {
+  if (perm) {
+    const auto p = rcm_order(n / bs, bs, rowptr, colind);
+    std::copy(p.begin(), p.end(), perm);
+  }
+  return envelope_llt_solve(n, bs, rowptr, colind, vals, b, x);
 }
 
 // Eigen's CG with that preconditioner on a scalar CSR matrix (x0 = 0).

@@ -614,3 +614,44 @@ def test_jacobi_pcg_krylov_minimizer(oracle):
         y = np.linalg.solve(Q.T @ A @ Q, Q.T @ b)
         assert np.abs(xk - Q @ y).max() < 1e-9 * np.abs(xs).max(), k
         K.append(Dinv * (A @ K[-1]))
+
+
+def test_envelope_direct_solve(oracle):
+    """The oracle's sparse direct solve (RCM order + envelope Cholesky): the
+    dense solution on random SPD block matrices and on an assembled beam
+    system; the order is a permutation that narrows the beam's bandwidth; an
+    indefinite matrix is reported."""
+    for seed in (5, 6):
+        A, rp, ci, va, P = _random_spd_bsr(12, seed, fill=0.2)
+        b = np.random.default_rng(seed).standard_normal(A.shape[0])
+        x, perm = oracle.envelope_solve(rp, ci, va, 3, b)
+        assert sorted(perm) == list(range(12))
+        xs = np.linalg.solve(A, b)
+        assert np.abs(x - xs).max() < 1e-12 * np.abs(xs).max()
+    # a beam with its vertices relabelled at random (wide bandwidth)
+    sc = scenes.beam_scene(6, 3, 3, seed=2, mixed=False)
+    q = np.random.default_rng(8).permutation(sc.nV)
+    qi = np.argsort(q)
+    sc.V, sc.fixed, sc.T = sc.V[q], sc.fixed[q], qi[sc.T].astype(np.int32)
+    o = oracle.Sim(sc)
+    o.update_system()
+    o.solve()
+    n = len(o.get("rhs"))
+    rp = o.get_int("rowptr", n + 1)
+    ci = o.get_int("colind", rp[-1])
+    va, rhs = o.get("lhs"), o.get("rhs")
+    x, perm = oracle.envelope_solve(rp, ci, va, 3, rhs)
+    K = np.zeros((n, n))
+    for i in range(n):
+        K[i, ci[rp[i]:rp[i + 1]]] = va[rp[i]:rp[i + 1]]
+    xs = np.linalg.solve(K, rhs)
+    assert np.abs(x - xs).max() < 1e-10 * np.abs(xs).max()
+    inv = np.argsort(perm)
+    rows = np.repeat(np.arange(n), np.diff(rp)) // 3
+    cols = ci // 3
+    assert np.abs(inv[rows] - inv[cols]).max() < 0.5 * np.abs(rows - cols).max()
+    A, rp2, ci2, va2, _ = _random_spd_bsr(6, 7)
+    bad = va2.copy()
+    bad[rp2[0]] = -1e3  # a negative diagonal entry
+    with pytest.raises(RuntimeError, match="positive definite"):
+        oracle.envelope_solve(rp2, ci2, bad, 3, np.ones(A.shape[0]))
