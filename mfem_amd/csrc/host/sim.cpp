@@ -1720,9 +1720,14 @@ bool SimState<DIM>::load(const Json& args, const std::string& base_dir, int dim)
     materials.push_back(std::shared_ptr<MaterialModel>(mf.create(cfg->material_model, cfg)));
   }
   material_models_ = materials;
+  // relative paths: against the scene file's folder, else as given (the
+  // reference opens them relative to the working directory,
+  // simulation_state.cpp:50-67)
   auto resolve = [&](const std::string& p) {
     std::filesystem::path q(p);
-    return q.is_absolute() ? p : (std::filesystem::path(base_dir) / q).string();
+    if (q.is_absolute()) return p;
+    const std::string here = (std::filesystem::path(base_dir) / q).string();
+    return std::filesystem::exists(here) || !std::filesystem::exists(q) ? here : p;
   };
   // objects: the device backend takes one object per simulation
   std::vector<double> V;

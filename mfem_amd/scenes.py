@@ -234,7 +234,13 @@ def scene_from_json(path: str, dim: int = 3) -> Scene:
     from . import meshio
     d = json.load(open(path))
     base = os.path.dirname(os.path.abspath(path))
-    res = lambda q: q if os.path.isabs(q) else os.path.join(base, q)
+    def res(q):
+        # against the scene file's folder, else as given (the reference opens
+        # paths relative to the working directory)
+        if os.path.isabs(q):
+            return q
+        here = os.path.join(base, q)
+        return here if os.path.exists(here) or not os.path.exists(q) else q
     objs = d.get("objects", [])
     if len(objs) != 1:
         raise ValueError("scene_from_json loads scenes with exactly one object")
