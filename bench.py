@@ -300,9 +300,12 @@ def cpu_baseline_entry(scene, threads, args, label="Jacobi CG"):
                       f"timestep, then the median over {len(rows)} timesteps ({secs})"}
 
 
-def run_batch(scs, precision, steps, warmup):
+def run_batch(scs, precision, steps, warmup, graphs=True, shape=None):
     from mfem_amd import native
     b = native.Batch(scs, precision=precision)
+    b.set_graphs(graphs)
+    if shape is not None:
+        b.set_cg_shape(*shape)
     for _ in range(warmup):
         b.step()
     work = 0
@@ -321,8 +324,9 @@ def run_batch(scs, precision, steps, warmup):
     elapsed = time.perf_counter() - t0
     pos = [b.positions(k) for k in range(b.count)]
     shape = b.cg_shape()
+    counts = b.launch_counts()
     b.close()
-    return dict(elapsed=elapsed, work=work, pcg=pcg, positions=pos, shape=shape)
+    return dict(elapsed=elapsed, work=work, pcg=pcg, positions=pos, shape=shape, counts=counts)
 
 
 def run_c4(args):
@@ -334,6 +338,7 @@ def run_c4(args):
     scs = scenes.config_c4(args.beams)
     nT = np.array([sc.nT for sc in scs])
     mixed = run_batch(scs, "mixed", args.steps, args.warmup)
+    eager = run_batch(scs, "mixed", args.steps, args.warmup, graphs=False, shape=mixed["shape"])
     fp64 = None if args.no_fp64 else run_batch(scs, "fp64", args.steps, args.warmup)
     value = mixed["work"] / mixed["elapsed"]
     out = {
@@ -348,6 +353,16 @@ def run_c4(args):
                                f"tol 1e-4 / 500 its, 5 Newton its, one batch",
                    "beams": len(scs), "tets": int(nT.sum()), "precision": "mixed"},
         "pcg_iters": mixed["pcg"], "cg_shape_variant_rows_per_workgroup": list(mixed["shape"]),
+        "graphs": {
+            "eager_value": eager["work"] / eager["elapsed"],
+            "eager_ms_per_step": 1000.0 * eager["elapsed"] / args.steps,
+            "bit_identical_to_eager": all(np.array_equal(a, b) for a, b in
+                                          zip(mixed["positions"], eager["positions"])),
+            "submissions_per_newton_iteration": {
+                "update_phase": {"graphs": mixed["counts"][0] / mixed["counts"][2],
+                                 "eager": eager["counts"][0] / eager["counts"][2]},
+                "cg": {"graphs": mixed["counts"][1] / mixed["counts"][2],
+                       "eager": eager["counts"][1] / eager["counts"][2]}}},
     }
     if fp64 is not None:
         v64 = fp64["work"] / fp64["elapsed"]

@@ -214,6 +214,17 @@ int mfem_batch_stats(mfem_batch* batch, int* newton_iters, int* pcg_iters, int* 
  * variant -1 restores autotuning on the next solve. */
 int mfem_batch_set_cg_shape(mfem_batch* batch, int variant, int rows_per_workgroup);
 int mfem_batch_cg_shape(mfem_batch* batch, int* variant, int* rows_per_workgroup);
+/* hipGraph replay of the batch's Newton phases (no counterpart in the
+ * reference; off by default): with on = 1 the element update + assembly phase of every
+ * Newton iteration is one captured graph from the second step on, and CG
+ * iterations are replayed in captured groups of 16; on = 0 launches every
+ * kernel separately. Results are identical bit for bit. */
+int mfem_batch_set_graphs(mfem_batch* batch, int on);
+/* Host submissions since creation: of the update + assembly phase, of the CG
+ * iterations, and the number of Newton iterations (per-simulation Newton
+ * loops advance together, so one per pass). Any pointer may be NULL. */
+int mfem_batch_launch_counts(mfem_batch* batch, long* update_submissions, long* cg_submissions,
+                             long* newton_iterations);
 
 /* LinearSolver::solve() of "eigen-pcg-diag" on a caller's system (reference
  * linear_solvers/linear_solver.h:8-21; EigenIterativeSolver<ConjugateGradient<

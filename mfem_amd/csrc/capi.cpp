@@ -703,6 +703,19 @@ int mfem_batch_set_cg_shape(mfem_batch* b, int variant, int rows_per_workgroup) 
   return guarded([&] { b->sim->force_cg_shape(variant, rows_per_workgroup); });
 }
 
+int mfem_batch_set_graphs(mfem_batch* b, int on) {
+  return guarded([&] { b->sim->set_graphs(on != 0); });
+}
+
+int mfem_batch_launch_counts(mfem_batch* b, long* update_submissions, long* cg_submissions,
+                             long* newton_iterations) {
+  return guarded([&] {
+    if (update_submissions) *update_submissions = b->sim->update_submissions();
+    if (cg_submissions) *cg_submissions = b->sim->cg_submissions();
+    if (newton_iterations) *newton_iterations = b->sim->newton_iterations();
+  });
+}
+
 int mfem_batch_cg_shape(mfem_batch* b, int* variant, int* rows_per_workgroup) {
   return guarded([&] {
     if (variant) *variant = b->sim->cg_variant();

@@ -17,6 +17,8 @@ enum { RES_DX = 0, RES_DS = 1, RES_E0 = 2, RES_E1 = 3, RES_ROWS = 4 };
 // pinned staging regions of the small uploads
 enum { ST_TOL, ST_BUDGET, ST_ON_CG, ST_ALPHA0, ST_ON0, ST_ALPHA1, ST_ON1, ST_ACCEPT, kStageRegions };
 constexpr int kTuneMinRows = 20000;
+// CG iteration pairs per replayed graph (16 iterations: one host poll's worth)
+constexpr int kCgGraphPairs = 8;
 }  // namespace
 
 BatchSim::BatchSim(const std::vector<SceneArrays>& sims, std::shared_ptr<SimConfig> config) {
@@ -99,8 +101,89 @@ BatchSim::BatchSim(const std::vector<SceneArrays>& sims, std::shared_ptr<SimConf
 }
 
 BatchSim::~BatchSim() {
+  clear_graphs();
   for (auto e : poll_ev_)
     if (e) (void)hipEventDestroy(e);
+}
+
+void BatchSim::clear_graphs() {
+  for (auto& g : cg_graphs_)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  cg_graphs_.clear();
+  if (update_graph_) (void)hipGraphExecDestroy(update_graph_);
+  update_graph_ = nullptr;
+  update_sig_.clear();
+}
+
+void BatchSim::set_graphs(bool on) {
+  if (!on) clear_graphs();
+  graphs_ = on;
+}
+
+// kCgGraphPairs pairs of CG iterations (even, then odd buffer parity)
+// captured once per (right-hand side, solution, shape, precision) and
+// replayed; the kernels take their iteration index from it_ctr_ (reset to 0
+// before the first replay of a solve), so one graph serves a whole solve.
+template <typename T>
+hipGraphExec_t BatchSim::cg_pair_graph(const BatchCgBuffers<T>& B) {
+  const bool f32 = sizeof(T) == 4;
+  for (auto& g : cg_graphs_)
+    if (g.b == B.b && g.x == B.x && g.variant == B.variant && g.f32 == f32) return g.exec;
+  hipStream_t st = state_->ctx_->stream;
+  if (it_ctr_.size() != 2) it_ctr_.resize(2);
+  hipGraph_t graph = nullptr;
+  MFEM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int p = 0; p < kCgGraphPairs; ++p) {
+    launch_pcg_batch_iteration(B, 0, st, it_ctr_.data(), it_ctr_.data() + 1);
+    launch_pcg_batch_iteration(B, 1, st, it_ctr_.data() + 1, it_ctr_.data());
+  }
+  MFEM_HIP_CHECK(hipStreamEndCapture(st, &graph));
+  CgGraph g;
+  g.b = B.b;
+  g.x = B.x;
+  g.variant = B.variant;
+  g.f32 = f32;
+  MFEM_HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  MFEM_HIP_CHECK(hipGraphDestroy(graph));
+  cg_graphs_.push_back(g);
+  return g.exec;
+}
+
+// update_system (reference newton_optimizer.cpp:92-110) and the assembly of
+// SystemMatrixPD::pre_solve (linear_system.h:18-31). From the second step on,
+// with graphs enabled, the five launches are one captured graph, re-captured
+// when a buffer or the time step changed.
+void BatchSim::update_and_assemble(double dt) {
+  auto& S = *state_;
+  hipStream_t st = S.ctx_->stream;
+  const bool f32 = S.ctx_->fp32();
+  if (!graphs_ || steps_done_ == 0) {
+    mixed_->update(S.x_->value().data(), dt);
+    system_.pre_solve(&S);
+    subs_update_ += 5;
+    return;
+  }
+  const std::vector<const void*> sig = {
+      S.x_->value().data(), S.ctx_->blocks.as<float>(), S.ctx_->blocks.as<double>(),
+      S.ctx_->fe.data(), system_.K.as<float>(), system_.K.as<double>(),
+      system_.invdiag.as<float>(), system_.invdiag.as<double>(), system_.rhs.data(),
+      mixed_->Hloc_.data(), mixed_->dsdF_.as<float>(), mixed_->dsdF_.as<double>()};
+  if (!update_graph_ || sig != update_sig_ || dt != update_dt_) {
+    if (update_graph_) MFEM_HIP_CHECK(hipGraphExecDestroy(update_graph_));
+    update_graph_ = nullptr;
+    hipGraph_t graph = nullptr;
+    MFEM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    mixed_->update(S.x_->value().data(), dt);
+    system_.pre_solve(&S);
+    MFEM_HIP_CHECK(hipStreamEndCapture(st, &graph));
+    MFEM_HIP_CHECK(hipGraphInstantiate(&update_graph_, graph, nullptr, nullptr, 0));
+    MFEM_HIP_CHECK(hipGraphDestroy(graph));
+    update_sig_ = sig;
+    update_dt_ = dt;
+  }
+  (void)f32;
+  MFEM_HIP_CHECK(hipGraphLaunch(update_graph_, st));
+  subs_update_ += 1;
 }
 
 std::vector<double> BatchSim::positions(int sim) const {
@@ -142,6 +225,9 @@ void BatchSim::layout_workgroups(int rows_per_wg) {
     wg_sys.insert(wg_sys.end(), nw, b);
   }
   nwg_ = wg0_.back();
+  for (auto& g : cg_graphs_)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  cg_graphs_.clear();
   d_wg_sys_.upload(wg_sys, st);
   d_wg0_.upload(wg0_, st);
   parts_.resize(2 * (size_t)kCgParts * nwg_);
@@ -197,10 +283,26 @@ void BatchSim::run_cg(const double* b, double* x, const std::vector<double>& tol
   auto* snap = pinned_ctrl_->as<PcgCtrl>();
   long launched = 0;
   int issued = 0, waited = 0;
+  // graph replay: pairs of iterations; a pair past the stop exits at once,
+  // like any launch queued after it
+  hipGraphExec_t pair = nullptr;
+  if (graphs_) {
+    pair = cg_pair_graph(B);
+    launch_set_int(it_ctr_.data(), 0, st);
+  }
   auto issue = [&] {
     const long chunk = std::min<long>(poll, limit - launched);
-    for (long j = 0; j < chunk; ++j) launch_pcg_batch_iteration(B, (int)(launched + j), st);
-    launched += chunk;
+    if (pair) {
+      const long per = 2L * kCgGraphPairs;
+      const long graphs = (chunk + per - 1) / per;
+      for (long j = 0; j < graphs; ++j) MFEM_HIP_CHECK(hipGraphLaunch(pair, st));
+      launched += per * graphs;
+      subs_cg_ += graphs;
+    } else {
+      for (long j = 0; j < chunk; ++j) launch_pcg_batch_iteration(B, (int)(launched + j), st);
+      launched += chunk;
+      subs_cg_ += chunk;
+    }
     MFEM_HIP_CHECK(hipGetLastError());
     const int slot = issued & 1;
     MFEM_HIP_CHECK(hipMemcpyAsync(snap + (size_t)slot * nsim_, ctrls_.data(),
@@ -292,7 +394,6 @@ void BatchSim::solve(const std::vector<int>& on) {
   auto& m = *S.mesh_;
   auto& cfg = *S.config_;
   hipStream_t st = S.ctx_->stream;
-  system_.pre_solve(&S);
   const int n = 3 * m.nfree();
   const bool f32 = S.ctx_->fp32();
   x_.resize(n);
@@ -411,8 +512,9 @@ void BatchSim::step() {
   const int ls_iters = cfg.ls_iters;
   for (int i = 0; i < cfg.outer_steps; ++i) {
     if (std::find(on.begin(), on.end(), 1) == on.end()) break;
-    // update_system (reference newton_optimizer.cpp:92-110)
-    mixed_->update(S.x_->value().data(), dt);
+    // update_system (reference newton_optimizer.cpp:92-110) and the assembly
+    update_and_assemble(dt);
+    ++newton_total_;
     // substep (:113-124) for the simulations that are on
     solve(on);
     // decrement, E(0) and E(1), one read
@@ -476,6 +578,7 @@ void BatchSim::step() {
   mixed_->post_solve();
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
   MFEM_HIP_CHECK(hipGetLastError());
+  ++steps_done_;
   step_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
                  .count();
 }

@@ -45,8 +45,30 @@ class BatchSim {
   int rows_per_workgroup() const { return rows_per_wg_; }
   // Force the CG shape and rows per workgroup (variant -1: autotune again).
   void force_cg_shape(int variant, int rows_per_wg);
+  // Replay the element-update + assembly phase and groups of 16 CG
+  // iterations as captured hipGraphs; results are identical either way.
+  void set_graphs(bool on);
+  bool graphs() const { return graphs_; }
+  // Host submissions since creation: of the update + assembly phase (kernel
+  // launches, or one graph launch), of the CG iterations (one launch per
+  // iteration, or one graph launch per 16), and Newton iterations run.
+  long update_submissions() const { return subs_update_; }
+  long cg_submissions() const { return subs_cg_; }
+  long newton_iterations() const { return newton_total_; }
 
  private:
+  struct CgGraph {
+    const void* b = nullptr;
+    const void* x = nullptr;
+    int variant = -1;
+    bool f32 = false;
+    hipGraphExec_t exec = nullptr;
+  };
+  void clear_graphs();
+  template <typename T>
+  hipGraphExec_t cg_pair_graph(const mfemhip::BatchCgBuffers<T>& B);
+  // update_system + SystemMatrixPD::pre_solve, eager or replayed
+  void update_and_assemble(double dt);
   template <typename T>
   mfemhip::BatchCgBuffers<T> buffers(const double* b, double* x);
   template <typename T>
@@ -92,6 +114,17 @@ class BatchSim {
   std::vector<SimStats> stats_;
   std::vector<int> solve_iters_, solve_sweeps_;
   double step_ms_ = 0;
+  // graph replay (off by default: it gives the same results and cuts host
+  // submissions ~16x, but the device is the bottleneck and it measured ~1%
+  // slower on the 256-beam batch)
+  bool graphs_ = false;
+  std::vector<CgGraph> cg_graphs_;
+  hipGraphExec_t update_graph_ = nullptr;
+  std::vector<const void*> update_sig_;
+  double update_dt_ = 0;
+  DeviceArray<int> it_ctr_;  // the CG iteration index of graph replays (two words)
+  long subs_update_ = 0, subs_cg_ = 0, newton_total_ = 0;
+  int steps_done_ = 0;
 };
 
 }  // namespace mfem

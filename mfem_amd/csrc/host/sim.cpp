@@ -1624,6 +1624,15 @@ void SimState<DIM>::finish_setup() {
     ctx_ = std::make_unique<DeviceContext>();
     ctx_->precision = config_->precision;
   }
+  // FP32 storage pays off only in the iterative solvers' sweeps over K; the
+  // direct solvers factor K densely in FP64, and an FP32-rounded K can stop
+  // being positive definite when element stiffnesses span many decades
+  // (E = 6e4 and 1e14 in the reference's gummy-bear scene). With a direct
+  // solver the mixed mode therefore stores everything in FP64.
+  const auto solver = config_->solver_type;
+  if (solver == SOLVER_EIGEN_LLT || solver == SOLVER_EIGEN_LDLT || solver == SOLVER_EIGEN_LU ||
+      solver == SOLVER_CHOLMOD)
+    ctx_->precision = Precision::FP64;
   if (!ctx_->stream) {
     MFEM_HIP_CHECK(hipStreamCreateWithFlags(&ctx_->stream, hipStreamNonBlocking));
     ctx_->own_stream = true;

@@ -443,8 +443,14 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
     double* __restrict__ parts_new, const unsigned long long* __restrict__ stamps_old,
     unsigned long long* __restrict__ stamps_new) {
   __shared__ double sm[kCgParts * (BS / kWave)];
+  __shared__ int s_done;
   const unsigned long long t_start = (unsigned long long)wall_clock64();
-  if (ctrl->done) return;
+  // one read of the stop flag per workgroup: workgroup 0 may set it during
+  // this launch, and waves that read it at different times would otherwise
+  // split, leaving the block sums below with the slots of exited waves unset
+  if (threadIdx.x == 0) s_done = ctrl->done;
+  __syncthreads();
+  if (s_done) return;
   const int nb = gridDim.x;
   double alpha = 0.0, beta = 0.0;
   if (it > 0) {
@@ -525,17 +531,28 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
 // [row0[b], row0[b+1]) into equal chunks; each system has its own control
 // block, scalars and Eigen stopping test (the single-system kernel above,
 // with the partial-sum reduction restricted to the system's workgroups).
+//
+// Graph replay: with it_in set, the iteration index is read from *it_in
+// instead of the argument, and workgroup 0 writes it + 1 to *it_out (another
+// word than it_in, so no workgroup of this launch reads what it writes), so
+// that a captured pair of launches advances two iterations per replay.
 template <int G, int U, int BS, typename T>
 __global__ __launch_bounds__(BS) void k_cg_iter_batch(
-    int it, const int* __restrict__ rowptr, const int* __restrict__ col,
+    int it_arg, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const CgDof<T>* __restrict__ s_old, CgDof<T>* __restrict__ s_new,
     double* __restrict__ x, PcgCtrl* ctrls, const double* __restrict__ parts_old,
     double* __restrict__ parts_new, int nwg, const int* __restrict__ wg_sys,
-    const int* __restrict__ wg0, const int* __restrict__ row0) {
+    const int* __restrict__ wg0, const int* __restrict__ row0, const int* it_in, int* it_out) {
   __shared__ double sm[kCgParts * (BS / kWave)];
+  const int it = it_in ? *it_in : it_arg;
+  if (it_out && blockIdx.x == 0 && threadIdx.x == 0) *it_out = it + 1;
   const int b = wg_sys[blockIdx.x];
   PcgCtrl* ctrl = ctrls + b;
-  if (ctrl->done) return;
+  // one read of the stop flag per workgroup (see k_cg_iter)
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = ctrl->done;
+  __syncthreads();
+  if (s_done) return;
   const int w0 = wg0[b], nw = wg0[b + 1] - w0;
   double alpha = 0.0, beta = 0.0;
   if (it > 0) {
@@ -847,8 +864,10 @@ void launch_pcg_batch_setup(int nsys, PcgCtrl* ctrls, const double* b2, const do
 }
 
 template <typename T>
-void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st) {
+void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st,
+                                const int* it_in, int* it_out) {
   if (B.nwg <= 0) return;
+  // with it_in set, `it` only selects the buffer parity
   const auto* so = reinterpret_cast<const CgDof<T>*>(B.state[it & 1]);
   auto* sn = reinterpret_cast<CgDof<T>*>(B.state[(it + 1) & 1]);
   const double* po = B.parts[(it + 1) & 1];
@@ -858,7 +877,7 @@ void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t 
   case i:                                                                                      \
     hipLaunchKernelGGL((k_cg_iter_batch<kCgShapes[i].G, kCgShapes[i].U, kCgShapes[i].BS, T>),    \
                        dim3(B.nwg), dim3(kCgShapes[i].BS), 0, st, it, B.rowptr, B.col, B.K, so,  \
-                       sn, B.x, B.ctrls, po, pn, B.nwg, B.wg_sys, B.wg0, B.row0);              \
+                       sn, B.x, B.ctrls, po, pn, B.nwg, B.wg_sys, B.wg0, B.row0, it_in, it_out); \
     break;
     MFEM_CGB(0) MFEM_CGB(1) MFEM_CGB(2) MFEM_CGB(3) MFEM_CGB(4) MFEM_CGB(5) MFEM_CGB(6) MFEM_CGB(7)
 #undef MFEM_CGB
@@ -867,10 +886,20 @@ void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t 
   }
 }
 
+namespace {
+__global__ void k_set_int(int* p, int v) { *p = v; }
+}  // namespace
+
+void launch_set_int(int* p, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p, v);
+}
+
 template void launch_pcg_batch_init<double>(const BatchCgBuffers<double>&, double*, hipStream_t);
 template void launch_pcg_batch_init<float>(const BatchCgBuffers<float>&, double*, hipStream_t);
-template void launch_pcg_batch_iteration<double>(const BatchCgBuffers<double>&, int, hipStream_t);
-template void launch_pcg_batch_iteration<float>(const BatchCgBuffers<float>&, int, hipStream_t);
+template void launch_pcg_batch_iteration<double>(const BatchCgBuffers<double>&, int, hipStream_t,
+                                                 const int*, int*);
+template void launch_pcg_batch_iteration<float>(const BatchCgBuffers<float>&, int, hipStream_t,
+                                                const int*, int*);
 
 template void launch_assemble_bsr<double>(int, const int*, const int*, const int*, const int*,
                                           const double*, const double*, double*, hipStream_t);

@@ -220,8 +220,14 @@ void launch_pcg_batch_init(const BatchCgBuffers<T>& B, double* scratch, hipStrea
 // control blocks from per-system |b|^2, tolerance, iteration budget, on flag
 void launch_pcg_batch_setup(int nsys, PcgCtrl* ctrls, const double* b2, const double* tol,
                             const int* max_iters, const int* on, hipStream_t st);
+// Batched CG kernel `it`. For graph replay pass it_in / it_out (two distinct
+// device words): the kernel then takes its iteration index from *it_in and
+// stores it + 1 in *it_out; `it` only selects the buffer parity.
 template <typename T>
-void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st);
+void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st,
+                                const int* it_in = nullptr, int* it_out = nullptr);
+// *p = v on the device (a kernel, so that it may sit in a captured graph)
+void launch_set_int(int* p, int v, hipStream_t st);
 
 // Per-segment reductions, one workgroup per segment in a fixed order:
 // out[b] = sum v (mode 0), sum v^2 (mode 1) or max |v| (mode 2) over

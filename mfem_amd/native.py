@@ -30,6 +30,7 @@ EXPORTED = [
     "mfem_batch_create", "mfem_batch_destroy", "mfem_batch_step", "mfem_batch_sizes",
     "mfem_batch_get_positions", "mfem_batch_stats", "mfem_batch_set_cg_shape",
     "mfem_batch_cg_shape", "mfem_sim_set_callback", "mfem_pcg_bsr3", "mfem_pcg_bsr3_noalloc",
+    "mfem_batch_set_graphs", "mfem_batch_launch_counts",
 ]
 
 # scalar-row limit of mfem_pcg_bsr3_noalloc (MFEM_PCG_NOALLOC_MAX_ROWS)
@@ -120,6 +121,9 @@ def lib():
         L.mfem_batch_stats.argtypes = [vp, ip, ip, ip, ip, dp, dp]
         L.mfem_batch_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
         L.mfem_batch_cg_shape.argtypes = [vp, ip, ip]
+        L.mfem_batch_set_graphs.argtypes = [vp, C.c_int]
+        L.mfem_batch_launch_counts.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long),
+                                               C.POINTER(C.c_long)]
         L.mfem_sim_set_callback.argtypes = [vp, NEWTON_CALLBACK, vp]
         L.mfem_pcg_bsr3.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double, C.c_int,
                                     ip, dp, vp]
@@ -407,6 +411,15 @@ class Batch:
         v, r = C.c_int(), C.c_int()
         check(lib().mfem_batch_cg_shape(self.h, C.byref(v), C.byref(r)))
         return v.value, r.value
+
+    def set_graphs(self, on):
+        check(lib().mfem_batch_set_graphs(self.h, int(bool(on))))
+
+    def launch_counts(self):
+        """(update-phase submissions, CG submissions, Newton iterations)."""
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        check(lib().mfem_batch_launch_counts(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
 
 # ---- batched element entry points (torch tensors as device memory) ---------

@@ -92,3 +92,34 @@ def test_batch_rejects_mismatched_settings(native):
     c = scenes.beam_scene(3, 2, 2, seed=3, mixed=False)
     with pytest.raises(native.MfemError, match="only mixed-stretch"):
         native.Batch([a, c])
+
+
+def test_batch_graph_replay_is_bit_identical_at_256_beams(native):
+    """The batching config as configured (256 beams, mixed precision): the
+    hipGraph replay of the update + assembly phase and of CG iteration pairs
+    gives bit-identical positions and counts to launching every kernel, over
+    several steps (each replay follows eager work and synchronisations), and
+    submits one update graph per Newton iteration instead of five launches
+    and one CG graph per 16 iterations."""
+    scs = scenes.config_c4()
+    g = native.Batch(scs, precision="mixed")
+    e = native.Batch(scs, precision="mixed")
+    g.set_graphs(True)
+    e.set_graphs(False)
+    for b in (g, e):  # one CG shape for both (the autotuner could pick two)
+        b.set_cg_shape(2, 128)
+    for step in range(4):
+        g.step()
+        e.step()
+        sg, se = g.stats(), e.stats()
+        for key in ("newton_iters", "pcg_iters", "refinements", "ls_fail"):
+            assert np.array_equal(sg[key], se[key]), (step, key)
+        for k in range(g.count):
+            assert np.array_equal(g.positions(k), e.positions(k)), (step, k)
+    ug, cg, ng = g.launch_counts()
+    ue, ce, ne = e.launch_counts()
+    assert ng == ne > 0
+    assert ue == 5 * ne
+    # the first step runs eagerly (its buffers are allocated), then one graph
+    assert ug < ue and ug >= ng
+    assert cg < 0.2 * ce
