@@ -566,6 +566,102 @@ MFEM_DEV void jacobi_psd_project(Real* A, Real tol) {
     }
 }
 
+// Packed symmetric storage, upper triangle row by row: (i, j), i <= j, at
+// i*N - i(i-1)/2 + (j - i) (sym6 for N = 6).
+template <int N>
+MFEM_DEV constexpr int sym_pk(int i, int j) {
+  return i <= j ? i * N - (i * (i - 1)) / 2 + (j - i) : j * N - (j * (j - 1)) / 2 + (i - j);
+}
+
+// chol_pd on packed symmetric storage.
+template <typename Real, int N>
+MFEM_DEV bool chol_pd_packed(const Real* A, Real tol) {
+  Real L[N * (N + 1) / 2];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+#pragma unroll
+    for (int i = j; i < N; ++i) {
+      Real s = A[sym_pk<N>(j, i)] - (i == j ? tol : Real(0));
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= L[sym_pk<N>(k, i)] * L[sym_pk<N>(k, j)];
+      if (i == j) {
+        if (!(s > Real(0))) return false;
+        L[sym_pk<N>(j, j)] = tsqrt<Real>(s);
+      } else {
+        L[sym_pk<N>(j, i)] = s / L[sym_pk<N>(j, j)];
+      }
+    }
+  }
+  return true;
+}
+
+// jacobi_psd_project on packed symmetric storage: the same cyclic sweeps and
+// rotation angles, with each rotation applied to both sides at once (Golub &
+// Van Loan Alg. 8.5.2: a_pq <- 0, a_pp <- a_pp - t a_pq, a_qq <- a_qq + t a_pq,
+// the other entries of rows p and q rotated). Holding 21 entries instead of
+// 36 keeps the FP64 projection within half the register file.
+template <typename Real, int N>
+MFEM_DEV void jacobi_psd_project_packed(Real* A, Real tol) {
+  Real Q[N * N];
+#pragma unroll
+  for (int i = 0; i < N * N; ++i) Q[i] = (i % (N + 1) == 0) ? Real(1) : Real(0);
+  const Real eps = sizeof(Real) == 8 ? Real(1e-30) : Real(1e-14);
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    Real off = 0, dia = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      dia += A[sym_pk<N>(i, i)] * A[sym_pk<N>(i, i)];
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) off += A[sym_pk<N>(i, j)] * A[sym_pk<N>(i, j)];
+    }
+    off *= Real(2);
+    if (off <= eps * (off + dia)) break;
+#pragma unroll
+    for (int p = 0; p < N - 1; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
+        const Real apq = A[sym_pk<N>(p, q)];
+        if (apq == Real(0)) continue;
+        const Real app = A[sym_pk<N>(p, p)], aqq = A[sym_pk<N>(q, q)];
+        const Real theta = (aqq - app) / (Real(2) * apq);
+        const Real t = (theta >= Real(0) ? Real(1) : Real(-1)) /
+                       (tabs(theta) + tsqrt<Real>(theta * theta + Real(1)));
+        const Real c = Real(1) / tsqrt<Real>(t * t + Real(1)), s = t * c;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          if (k == p || k == q) continue;
+          const Real akp = A[sym_pk<N>(k, p)], akq = A[sym_pk<N>(k, q)];
+          A[sym_pk<N>(k, p)] = c * akp - s * akq;
+          A[sym_pk<N>(k, q)] = s * akp + c * akq;
+        }
+        A[sym_pk<N>(p, p)] = app - t * apq;
+        A[sym_pk<N>(q, q)] = aqq + t * apq;
+        A[sym_pk<N>(p, q)] = Real(0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const Real qkp = Q[k + N * p], qkq = Q[k + N * q];
+          Q[k + N * p] = c * qkp - s * qkq;
+          Q[k + N * q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  Real w[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const Real d = A[sym_pk<N>(i, i)];
+    w[i] = d < tol ? tol : d;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i; j < N; ++j) {
+      Real s = 0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) s += Q[i + N * k] * w[k] * Q[j + N * k];
+      A[sym_pk<N>(i, j)] = s;
+    }
+}
+
 // Inverse of an SPD matrix via Cholesky (full column-major in/out).
 template <typename Real, int N>
 MFEM_DEV void spd_inverse(const Real* A, Real* X) {

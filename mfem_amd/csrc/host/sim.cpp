@@ -561,7 +561,7 @@ PcgBuffers<T> PcgSolver<DIM>::buffers(const double* b, double* x) {
 
 template <int DIM>
 template <typename T>
-int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters) {
+int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters, int sweep) {
   hipStream_t st = this->state_->ctx_->stream;
   const PcgBuffers<T> B = buffers<T>(b, x);
   launch_pcg_init(B, tol, max_iters, st);
@@ -570,8 +570,16 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   auto* snap = pinned_->as<PcgCtrl>();     // two control-block snapshots
   long launched = 0;
   int issued = 0, waited = 0;  // snapshots; snapshot k uses slot k & 1
+  // Kernels queued past the stop exit at once but still cost a dispatch of
+  // the whole grid. Near the iteration count of the last sweep in this slot
+  // the chunks shrink to a quarter, so that fewer of them are queued when
+  // the stop comes.
+  const int hs = sweep > 0 ? 1 : 0;
+  const long hint = iter_hint_[hs];
+  const int tail = std::max(2, poll / 4);
   auto issue = [&] {
-    const long chunk = std::min<long>(poll, limit - launched);
+    const long size = (hint > 0 && launched + 3L * poll >= hint) ? tail : poll;
+    const long chunk = std::min<long>(size, limit - launched);
     for (long j = 0; j < chunk; ++j) launch_pcg_iteration(B, (int)(launched + j), st);
     launched += chunk;
     MFEM_HIP_CHECK(hipGetLastError());
@@ -595,6 +603,7 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
     }
     if (waited == issued) throw std::runtime_error("CG control block did not terminate");
   }
+  iter_hint_[hs] = last_.stop_it + 1;  // kernels 0 .. stop_it ran
   if (this->profile_spmv) {
     this->stats_.spmv_ms += last_.kern_ticks / wall_khz_;
     this->stats_.spmv_count += last_.kern_launches;
@@ -755,7 +764,7 @@ void PcgSolver<DIM>::solve() {
       r2 = R.read(RS_RES, st);
       if (!(r2 >= thr) || budget <= 0 || sweeps >= cfg.max_refinements) break;
       const double inner = std::max(cfg.inner_tol, std::sqrt(thr / r2) * 0.5);
-      const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget);
+      const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget, sweeps);
       budget -= std::max(it, 1);
       this->stats_.iterations += it;
       launch_axpy(n, 1.0, dx64_.data(), x_.data(), st);

@@ -306,8 +306,11 @@ class PcgSolver : public LinearSolver<DIM> {
  private:
   template <typename T>
   mfemhip::PcgBuffers<T> buffers(const double* b, double* x);
+  // sweep: which CG sweep of the solve this is (0 = the first); the
+  // iteration count of the last first (or later) sweep sets where the launch
+  // chunks shrink.
   template <typename T>
-  int run_cg(const double* b, double* x, double tol, int max_iters);
+  int run_cg(const double* b, double* x, double tol, int max_iters, int sweep = 0);
   template <typename T>
   void tune_spmv();
   template <typename T>
@@ -324,6 +327,7 @@ class PcgSolver : public LinearSolver<DIM> {
   DeviceArray<unsigned long long> stamps_;  // 2 x 2 x kCgMaxBlocks clock stamps
   DeviceArray<mfemhip::PcgCtrl> ctrl_;
   mfemhip::PcgCtrl last_{};      // control block at the end of the last CG solve
+  int iter_hint_[2] = {0, 0};    // iterations of the last first / later sweep
   std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots
   hipEvent_t poll_ev_[2] = {nullptr, nullptr};
   double wall_khz_ = 100000.0;   // device wall-clock rate

@@ -106,14 +106,12 @@ __device__ __forceinline__ void projected_hessian(int type, double mu, double la
                                                   double Hs[36]) {
   double gg[6], HH[21];
   material_grad_hess(type, mu, la, s, gg, HH);
+  if (material_needs_psd(type) && !chol_pd_packed<double, 6>(HH, 1e-6))
+    jacobi_psd_project_packed<double, 6>(HH, 1e-6);
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) Hs[i + 6 * j] = HH[sym6(i, j)];
-  if (material_needs_psd(type) && !chol_pd<double, 6>(Hs, 1e-6))
-    jacobi_psd_project<double, 6>(Hs, 1e-6);
-#pragma unroll
-  for (int i = 0; i < 36; ++i) Hs[i] *= h2;
+    for (int j = 0; j < 6; ++j) Hs[i + 6 * j] = h2 * HH[sym6(i, j)];
 #pragma unroll
   for (int i = 0; i < 6; ++i) g[i] = h2 * gg[i];
 }
@@ -1123,10 +1121,10 @@ __global__ void k_batch_material(int n, const int* __restrict__ type, const doub
   if (E) E[i] = material_energy(type[i], mu[i], la[i], sv);
   double gg[6], HH[21];
   material_grad_hess(type[i], mu[i], la[i], sv, gg, HH);
+  if (psd && material_needs_psd(type[i]) && !chol_pd_packed<double, 6>(HH, 1e-6))
+    jacobi_psd_project_packed<double, 6>(HH, 1e-6);
   double A[36];
   unpack_sym6(HH, A);
-  if (psd && material_needs_psd(type[i]) && !chol_pd<double, 6>(A, 1e-6))
-    jacobi_psd_project<double, 6>(A, 1e-6);
   if (g)
 #pragma unroll
     for (int k = 0; k < 6; ++k) g[6 * i + k] = gg[k];

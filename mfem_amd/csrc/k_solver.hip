@@ -435,6 +435,59 @@ __device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts
   block_sum_n<BS, kCgParts>(v, sm);
 }
 
+// One pass of a lane over a block row in the CG kernel: the U column ids,
+// value blocks and gathered states {r, q, p, D^-1} of the block columns. The
+// loads depend on neither alpha nor beta, so the first pass of a workgroup is
+// issued before the partial sums of the previous iteration are reduced, and
+// the two memory round trips overlap.
+template <int U, typename T>
+struct CgPass {
+  int c[U];
+  T v[U][9];
+  CgDof<T> g[U][3];
+};
+
+template <int G, int U, typename T>
+__device__ __forceinline__ void cg_load_pass(int p, int p1, const int* __restrict__ col,
+                                             const T* __restrict__ val,
+                                             const CgDof<T>* __restrict__ s_old,
+                                             CgPass<U, T>& P) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int q = p + u * G;
+    P.c[u] = q < p1 ? col[q] : -1;
+    if (q < p1) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) P.v[u][k] = val[(size_t)9 * q + k];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (P.c[u] >= 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P.g[u][k] = s_old[3 * P.c[u] + k];
+    }
+}
+
+// y += K_pass p with p = D^-1 (r - alpha q) + beta p of every gathered column
+template <int U, typename T>
+__device__ __forceinline__ void cg_apply_pass(const CgPass<U, T>& P, T a, T bt, T& y0, T& y1,
+                                              T& y2) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (P.c[u] >= 0) {
+      T x[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        T r, z;
+        cg_form(P.g[u][k], a, bt, r, z, x[k]);
+      }
+      y0 += P.v[u][0] * x[0] + P.v[u][1] * x[1] + P.v[u][2] * x[2];
+      y1 += P.v[u][3] * x[0] + P.v[u][4] * x[1] + P.v[u][5] * x[2];
+      y2 += P.v[u][6] * x[0] + P.v[u][7] * x[1] + P.v[u][8] * x[2];
+    }
+}
+
 template <int G, int U, int BS, typename T>
 __global__ __launch_bounds__(BS) void k_cg_iter(
     int it, int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
@@ -452,6 +505,24 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
   __syncthreads();
   if (s_done) return;
   const int nb = gridDim.x;
+  const int lane = threadIdx.x & (G - 1);
+  // XCD x (= blockIdx % 8) owns rows [nrows x / 8, nrows (x+1) / 8); its nbx
+  // workgroups split that range into equal contiguous chunks.
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = nb >> 3;
+  const long long r0 = (long long)nrows * xcd / 8, r1 = (long long)nrows * (xcd + 1) / 8;
+  const int c0 = (int)(r0 + (r1 - r0) * lb / nbx);
+  const int c1 = (int)(r0 + (r1 - r0) * (lb + 1) / nbx);
+  // first pass of this lane's first row, in flight during the reduction
+  int row = c0 + (int)threadIdx.x / G;
+  int p0 = 0, p1 = 0;
+  CgPass<U, T> P;
+  CgDof<T> own;
+  if (row < c1) {
+    p0 = rowptr[row];
+    p1 = rowptr[row + 1];
+    cg_load_pass<G, U>(p0 + lane, p1, col, val, s_old, P);
+    if (lane < 3) own = s_old[3 * row + lane];
+  }
   double alpha = 0.0, beta = 0.0;
   if (it > 0) {
     double v[kCgParts];
@@ -474,28 +545,31 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
     if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->rr = rr;
   }
   const T a = (T)alpha, bt = (T)beta;
-  const int lane = threadIdx.x & (G - 1);
-  // XCD x (= blockIdx % 8) owns rows [nrows x / 8, nrows (x+1) / 8); its nbx
-  // workgroups split that range into equal contiguous chunks.
-  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = nb >> 3;
-  const long long r0 = (long long)nrows * xcd / 8, r1 = (long long)nrows * (xcd + 1) / 8;
-  const int c0 = (int)(r0 + (r1 - r0) * lb / nbx);
-  const int c1 = (int)(r0 + (r1 - r0) * (lb + 1) / nbx);
   double acc[kCgParts];
 #pragma unroll
   for (int k = 0; k < kCgParts; ++k) acc[k] = 0.0;
-  for (int row = c0 + (int)threadIdx.x / G; row < c1; row += BS / G) {
-    T y0, y1, y2;
-    bsr_row_batched<G, U>(row, lane, rowptr, col, val,
-                          [&](int c, int k) {
-                            T r, z, p;
-                            cg_form(s_old[3 * c + k], a, bt, r, z, p);
-                            return p;
-                          },
-                          y0, y1, y2);
+  for (bool first = true; row < c1; row += BS / G, first = false) {
+    if (!first) {
+      p0 = rowptr[row];
+      p1 = rowptr[row + 1];
+      cg_load_pass<G, U>(p0 + lane, p1, col, val, s_old, P);
+      if (lane < 3) own = s_old[3 * row + lane];
+    }
+    T y0 = 0, y1 = 0, y2 = 0;
+    cg_apply_pass<U>(P, a, bt, y0, y1, y2);
+    for (int p = p0 + lane + U * G; p < p1; p += U * G) {
+      cg_load_pass<G, U>(p, p1, col, val, s_old, P);
+      cg_apply_pass<U>(P, a, bt, y0, y1, y2);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      y0 += __shfl_xor(y0, off, G);
+      y1 += __shfl_xor(y1, off, G);
+      y2 += __shfl_xor(y2, off, G);
+    }
     if (lane < 3) {
       const int i = 3 * row + lane;
-      const CgDof<T> s = s_old[i];
+      const CgDof<T> s = own;
       T r, z, p;
       cg_form(s, a, bt, r, z, p);
       x[i] += alpha * (double)s.p;
