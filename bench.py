@@ -73,9 +73,11 @@ def measured_traffic(lanes, unroll, workgroup, default_workload):
     return vals["read"] + vals["write"]
 
 
-def run_gpu(scene, precision, steps, warmup, profile=True):
+def run_gpu(scene, precision, steps, warmup, profile=True, element_kernel=0):
     from mfem_amd import native
     sim = native.Sim(scene, precision=precision)
+    if element_kernel:
+        sim.set_element_kernel(element_kernel)
     for _ in range(warmup):
         sim.step()
     if profile:
@@ -431,6 +433,9 @@ def main():
                     help="linear solver of the Newton step: Schur complement + Jacobi CG "
                          "(default, the metric's), Schur complement + IC(0) CG, or the KKT "
                          "system with MINRES (FP64)")
+    ap.add_argument("--element-kernel", type=int, choices=(0, 1), default=0,
+                    help="mixed-precision run: stiffness-block products on the vector units (0, "
+                         "default) or on the matrix cores (1, MFMA)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -452,7 +457,7 @@ def main():
     if args.solver == "eigen-pcg-IC":
         return run_ic(args, scene)
 
-    mixed = run_gpu(scene, "mixed", args.steps, args.warmup)
+    mixed = run_gpu(scene, "mixed", args.steps, args.warmup, element_kernel=args.element_kernel)
     fp64 = None if args.no_fp64 else run_gpu(scene, "fp64", args.steps, args.warmup)
 
     value = mixed["newton"] * nT / mixed["elapsed"]

@@ -148,6 +148,12 @@ int mfem_sim_write_state(mfem_sim* sim, const char* x_path, const char* v_path);
  * F may be NULL to query the count; *nF is its capacity in triangles. */
 int mfem_boundary_faces(const double* V, int nV, const int* T, int nT, int* F, int* nF);
 
+/* Kernel of the mixed stiffness blocks vol B^T Hpsi B (no counterpart in the
+ * reference; the blocks are the Aloc of mixed_stretch.cpp:168-175): 0 (default)
+ * forms the products on the vector units, 1 on the matrix cores
+ * (v_mfma_f32_4x4x1f32) when the blocks are stored in FP32 (mixed
+ * precision, 3D); FP64 storage and 2D keep the vector-unit kernel. */
+int mfem_sim_set_element_kernel(mfem_sim* sim, int variant);
 /* bit 0: time every CG kernel launch with per-workgroup device clock
  * stamps (span of the launch, summed in the stats as spmv_ms/spmv_count) */
 int mfem_sim_set_profiling(mfem_sim* sim, int flags);

@@ -477,6 +477,17 @@ int mfem_sim_newton_history(mfem_sim* sim, double* out, int* rows) {
   });
 }
 
+int mfem_sim_set_element_kernel(mfem_sim* sim, int variant) {
+  return guarded([&] {
+    if (variant != 0 && variant != 1) throw std::runtime_error("element kernel variant must be 0 or 1");
+    auto& S = sim->state();
+    MixedStretch<3>* ms =
+        S.mixed_vars_.empty() ? nullptr : dynamic_cast<MixedStretch<3>*>(S.mixed_vars_[0].get());
+    if (!ms) throw std::runtime_error("the element kernel variant needs a mixed-stretch variable");
+    ms->mfma_blocks_ = variant == 1;
+  });
+}
+
 int mfem_sim_set_profiling(mfem_sim* sim, int flags) {
   return guarded([&] { sim->opt->solver()->profile_spmv = (flags & 1) != 0; });
 }

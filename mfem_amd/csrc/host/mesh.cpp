@@ -246,6 +246,7 @@ void Mesh::init(hipStream_t st, const std::vector<int>& fixed_override) {
   d_diag_block.upload(diag_block_, st);
   d_inc_ptr.upload(inc_ptr_, st);
   d_inc.upload(inc_, st);
+  d_asm_rows.upload(asm_rows_, st);
   d_mass.upload(mass_, st);
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -326,6 +327,7 @@ void Mesh::build_maps() {
   // (i, j) transposed, in the same (tet, local pair) order
   upper_.clear();
   tpos_.clear();
+  std::vector<int> upper_row;
   for (int i = 0; i < nfree_; ++i)
     for (int p = rowptr_[i]; p < rowptr_[i + 1]; ++p) {
       const int j = col_[p];
@@ -335,7 +337,55 @@ void Mesh::build_maps() {
       if (it == e || *it != i) throw std::runtime_error("Mesh: block pattern is not symmetric");
       upper_.push_back(p);
       tpos_.push_back((int)(it - col_.begin()));
+      upper_row.push_back(i);
     }
+  // Traversal order of the assembly gathers: rows by the Z-order (Morton)
+  // code of their vertex's rest position. A tet's record is read by the rows
+  // of its four corners; in row order those rows lie up to a cross-section
+  // of the mesh apart, so the record has left the L2 before its last read,
+  // while in Z-order most corners of a tet are visited close together. Only
+  // which lane sums which block changes, not the order of any sum.
+  asm_rows_.resize(nfree_);
+  {
+    std::vector<int> vert(nfree_);
+    for (int v = 0; v < nV_; ++v)
+      if (free_map_[v] >= 0) vert[free_map_[v]] = v;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int f = 0; f < nfree_; ++f)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::min(lo[c], V_[3 * (size_t)vert[f] + c]);
+        hi[c] = std::max(hi[c], V_[3 * (size_t)vert[f] + c]);
+      }
+    constexpr int kBits = 21;
+    std::vector<unsigned long long> code(nfree_);
+    for (int f = 0; f < nfree_; ++f) {
+      unsigned long long k = 0;
+      for (int c = 0; c < 3; ++c) {
+        const double ext = hi[c] - lo[c];
+        const double u = ext > 0 ? (V_[3 * (size_t)vert[f] + c] - lo[c]) / ext : 0.0;
+        const auto q = (unsigned long long)std::min<double>(
+            std::max(0.0, u * (double)((1 << kBits) - 1)), (double)((1 << kBits) - 1));
+        for (int bit = 0; bit < kBits; ++bit) k |= ((q >> bit) & 1ULL) << (3 * bit + c);
+      }
+      code[f] = k;
+    }
+    for (int f = 0; f < nfree_; ++f) asm_rows_[f] = f;
+    std::stable_sort(asm_rows_.begin(), asm_rows_.end(),
+                     [&](int a, int b) { return code[a] < code[b]; });
+    std::vector<int> rank(nfree_);
+    for (int r = 0; r < nfree_; ++r) rank[asm_rows_[r]] = r;
+    std::vector<int> idx(upper_.size());
+    for (size_t t = 0; t < idx.size(); ++t) idx[t] = (int)t;
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return rank[upper_row[a]] < rank[upper_row[b]]; });
+    std::vector<int> u2(idx.size()), t2(idx.size());
+    for (size_t t = 0; t < idx.size(); ++t) {
+      u2[t] = upper_[idx[t]];
+      t2[t] = tpos_[idx[t]];
+    }
+    upper_.swap(u2);
+    tpos_.swap(t2);
+  }
 }
 
 bool Mesh::bc_moves() const {

@@ -80,6 +80,9 @@ class Mesh {
   DeviceArray<int> d_mat, d_free_map;
   DeviceArray<int> d_rowptr, d_col, d_slot_ptr, d_slots, d_diag_block, d_inc_ptr, d_inc;
   DeviceArray<int> d_upper, d_tpos;  // upper blocks and the positions of their transposes
+  // free rows in the order the assembly gathers visit them (Z-order of the
+  // rest positions; the upper blocks are listed in the same row order)
+  DeviceArray<int> d_asm_rows;
   DeviceArray<double> d_mass;
   std::vector<int> rowptr_, col_;  // host copy of the BSR pattern
   // material model id of every element (after init)
@@ -94,6 +97,7 @@ class Mesh {
   int nV_ = 0, nT_ = 0, nfree_ = 0, nnzb_ = 0;
   std::vector<int> slot_ptr_, slots_, diag_block_, inc_ptr_, inc_;
   std::vector<int> upper_, tpos_;  // pattern positions of the blocks j >= i; of their transposes
+  std::vector<int> asm_rows_;      // assembly visiting order of the free rows
   std::vector<double> mass_;
   std::vector<int> elem_type_;
   std::vector<int> pinned_;

@@ -376,7 +376,8 @@ void MixedStretch<DIM>::update(const double* x_full, double dt) {
                                        c->stream);
     if (schur_blocks_)
       launch_mixed_blocks_rhs<float>(m, x_full, c->inertia(), dsdF_.f32(), Hpsi_.f32(),
-                                     gl_.data(), c->blocks.f32(), c->fe.data(), c->stream);
+                                     gl_.data(), c->blocks.f32(), c->fe.data(), c->stream,
+                                     mfma_blocks_);
   } else {
     launch_mixed_polar_material<double>(m, x_full, R_.data(), S_.data(), s_.data(),
                                         dsdF_.f64(), hess_out<double>(), g_.data(), gl_.data(),
@@ -479,8 +480,8 @@ void SystemMatrixPD<DIM>::pre_solve(SimState<DIM>* state) {
     launch_jacobi_diag<double, double>(m.nfree(), m.d_diag_block.data(), K.f64(),
                                        invdiag.f64(), c->stream);
   }
-  launch_assemble_rhs(m.nfree(), m.d_inc_ptr.data(), m.d_inc.data(), c->fe.data(), rhs.data(),
-                      c->stream);
+  launch_assemble_rhs(m.nfree(), m.d_asm_rows.data(), m.d_inc_ptr.data(), m.d_inc.data(),
+                      c->fe.data(), rhs.data(), c->stream);
 }
 
 template <int DIM>

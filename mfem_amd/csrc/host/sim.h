@@ -211,6 +211,9 @@ class MixedStretch : public MixedVariable<DIM> {
   // false when the linear solver does not use the Schur-complement system
   // (its element stiffness blocks and residual are then not formed)
   bool schur_blocks_ = true;
+  // 3D FP32 storage: form the stiffness blocks' products on the matrix
+  // cores (MFMA) instead of the vector units
+  bool mfma_blocks_ = false;
 
  private:
   template <typename Real>

@@ -2,6 +2,7 @@
 // arrays loads (stride nT) so that each load instruction of a wavefront is
 // coalesced; vertex positions are gathered through L2 (xyz interleaved).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "device_math.h"
 #include "kernels.h"
 #include "reduce.h"
@@ -195,24 +196,14 @@ __device__ __forceinline__ void stage_flush(const V* stage, int lane, int e0, in
   __syncthreads();
 }
 
+// B = dsdF * Jloc (6 x 12): B[i][3k+r] = sum_c dsdF[i][r+3c] dX(k,c), and
+// C = Hpsi * B (6 x 12), both column-major (entry (i, q) at i + 6 q).
 template <typename Real>
-__global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
-    TetMeshDev m, const double* __restrict__ x, Inertia in, const Real* __restrict__ dsdF,
-    const Real* __restrict__ Hpsi, const double* __restrict__ gl, Real* __restrict__ blocks,
-    double* __restrict__ fe) {
-  __shared__ __align__(16) unsigned char stage_raw[kElemBlock * kStagePitch * sizeof(Real)];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e0 = e - lane;  // first tet of this wave
+__device__ __forceinline__ void mixed_element_BC(const TetMeshDev& m, int ec,
+                                                 const Real* __restrict__ dsdF,
+                                                 const Real* __restrict__ Hpsi, Real (&B)[72],
+                                                 Real (&C)[72]) {
   const int nT = m.nT;
-  // lanes past the end compute on the last tet and store nothing, so that
-  // every lane reaches the barriers of the rounds
-  const int ec = e < nT ? e : nT - 1;
-  unsigned char* wave_raw = stage_raw + (size_t)(threadIdx.x - lane) * kStagePitch * sizeof(Real);
-  Real* stage = reinterpret_cast<Real*>(wave_raw);
-  const double vol = m.vol[ec];
-  // B = dsdF * Jloc (6 x 12): B[i][3k+r] = sum_c dsdF[i][r+3c] dX(k,c)
-  Real B[6 * 12];
   {
     Real d[54];
 #pragma unroll
@@ -230,53 +221,36 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
                                    d[i + 6 * (r + 6)] * dx2;
     }
   }
-  // C = Hpsi * B (6 x 12)
-  Real C[6 * 12];
-  {
-    Real H[36];
+  Real H[36];
 #pragma unroll
-    for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < 6; ++i)
 #pragma unroll
-      for (int j = i; j < 6; ++j) {
-        Real h = Hpsi[sym6(i, j) * nT + ec];
-        H[i + 6 * j] = h;
-        H[j + 6 * i] = h;
-      }
+    for (int j = i; j < 6; ++j) {
+      Real h = Hpsi[sym6(i, j) * nT + ec];
+      H[i + 6 * j] = h;
+      H[j + 6 * i] = h;
+    }
 #pragma unroll
-    for (int q = 0; q < 12; ++q)
+  for (int q = 0; q < 12; ++q)
 #pragma unroll
-      for (int i = 0; i < 6; ++i) {
-        Real a = 0;
+    for (int i = 0; i < 6; ++i) {
+      Real a = 0;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) a += H[i + 6 * j] * B[j + 6 * q];
-        C[i + 6 * q] = a;
-      }
-  }
-  const Real rv = (Real)vol;
-  // block (k, l) = vol B_k^T C_l into round slot `slot`
-  auto block_out = [&](int k, int l, int slot) {
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        Real a = 0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) a += B[i + 6 * (3 * k + r)] * C[i + 6 * (3 * l + q)];
-        stage[lane * kStagePitch + 9 * slot + 3 * r + q] = rv * a;
-      }
-  };
-  // record order blk_index(k, l): (0,0..3) | (1,1..3) | (2,2), (2,3), (3,3)
-#pragma unroll
-  for (int l = 0; l < 4; ++l) block_out(0, l, l);
-  stage_flush<36, kStagePitch>(stage, lane, e0, nT, blocks, 90, 0);
-#pragma unroll
-  for (int l = 1; l < 4; ++l) block_out(1, l, l - 1);
-  stage_flush<27, kStagePitch>(stage, lane, e0, nT, blocks, 90, 36);
-  block_out(2, 2, 0);
-  block_out(2, 3, 1);
-  block_out(3, 3, 2);
-  stage_flush<27, kStagePitch>(stage, lane, e0, nT, blocks, 90, 63);
-  // element residual: -vol B^T (Sym gl) - M_e (x - x_tilde - h^2 g)
+      for (int j = 0; j < 6; ++j) a += H[i + 6 * j] * B[j + 6 * q];
+      C[i + 6 * q] = a;
+    }
+}
+
+// Element residual -vol B^T (Sym gl) - M_e (x - x_tilde - h^2 g) of the
+// wave's tets, staged through the wave's LDS (`wave_raw`, at least
+// 64 x 13 doubles) and written as contiguous runs.
+template <typename Real>
+__device__ __forceinline__ void mixed_element_fe(const TetMeshDev& m, int ec, int lane, int e0,
+                                                 const double* __restrict__ x, const Inertia& in,
+                                                 const double* __restrict__ gl, const Real (&B)[72],
+                                                 double vol, unsigned char* wave_raw,
+                                                 double* __restrict__ fe) {
+  const int nT = m.nT;
   double w[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) w[i] = (i < 3 ? 1.0 : 2.0) * gl[i * nT + ec];
@@ -292,9 +266,7 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
       dsum[r] += dd;
     }
   const double mc = m.density * vol / 20.0;
-  constexpr int kFePitch = 13;  // 64 x 13 doubles fit in the wave's stage
-  static_assert(kWave * kFePitch * sizeof(double) <= kWave * kStagePitch * sizeof(float),
-                "f_e stage exceeds the wave's LDS");
+  constexpr int kFePitch = 13;
   double* fst = reinterpret_cast<double*>(wave_raw);
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
@@ -304,6 +276,161 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
     fst[lane * kFePitch + j] = -vol * a - mc * (d[j] + dsum[j % 3]);
   }
   stage_flush<12, kFePitch>(fst, lane, e0, nT, fe, 12, 0);
+}
+
+// FP32 records are staged whole (64 tets x 90 values per wave, 23 KB) and
+// leave as one contiguous run of the wave's 64 records: staged in parts, the
+// runs of a part cover partial cache lines, and the L2 wrote such lines back
+// more than once (WRITE_SIZE 1.3x the record bytes at the 960k-tet beam).
+// FP64 records would need twice that LDS, so they leave in three rounds of
+// 36, 27 and 27 values per tet.
+constexpr int kFullPitch = 91;  // floats per staged FP32 record (odd: no bank conflicts)
+
+template <typename Real>
+__global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
+    TetMeshDev m, const double* __restrict__ x, Inertia in, const Real* __restrict__ dsdF,
+    const Real* __restrict__ Hpsi, const double* __restrict__ gl, Real* __restrict__ blocks,
+    double* __restrict__ fe) {
+  constexpr bool kWhole = sizeof(Real) == sizeof(float);
+  constexpr int kPitch = kWhole ? kFullPitch : kStagePitch;
+  __shared__ __align__(16) unsigned char stage_raw[kElemBlock * kPitch * sizeof(Real)];
+  static_assert(kWave * 13 * sizeof(double) <= kWave * kPitch * sizeof(Real),
+                "f_e stage exceeds the wave's LDS");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e0 = e - lane;  // first tet of this wave
+  const int nT = m.nT;
+  // lanes past the end compute on the last tet and store nothing, so that
+  // every lane reaches the barriers of the rounds
+  const int ec = e < nT ? e : nT - 1;
+  unsigned char* wave_raw = stage_raw + (size_t)(threadIdx.x - lane) * kPitch * sizeof(Real);
+  Real* stage = reinterpret_cast<Real*>(wave_raw);
+  const double vol = m.vol[ec];
+  Real B[72], C[72];
+  mixed_element_BC<Real>(m, ec, dsdF, Hpsi, B, C);
+  const Real rv = (Real)vol;
+  // block (k, l) = vol B_k^T C_l into staged slot `slot`
+  auto block_out = [&](int k, int l, int slot) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        Real a = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) a += B[i + 6 * (3 * k + r)] * C[i + 6 * (3 * l + q)];
+        stage[lane * kPitch + 9 * slot + 3 * r + q] = rv * a;
+      }
+  };
+  // record order blk_index(k, l): (0,0..3) | (1,1..3) | (2,2), (2,3), (3,3)
+  if constexpr (kWhole) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int l = k; l < 4; ++l) block_out(k, l, blk_index(k, l));
+    stage_flush<90, kPitch>(stage, lane, e0, nT, blocks, 90, 0);
+  } else {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) block_out(0, l, l);
+    stage_flush<36, kPitch>(stage, lane, e0, nT, blocks, 90, 0);
+#pragma unroll
+    for (int l = 1; l < 4; ++l) block_out(1, l, l - 1);
+    stage_flush<27, kPitch>(stage, lane, e0, nT, blocks, 90, 36);
+    block_out(2, 2, 0);
+    block_out(2, 3, 1);
+    block_out(3, 3, 2);
+    stage_flush<27, kPitch>(stage, lane, e0, nT, blocks, 90, 63);
+  }
+  mixed_element_fe<Real>(m, ec, lane, e0, x, in, gl, B, vol, wave_raw, fe);
+}
+
+// The same element blocks with the products B_k^T C_l on the matrix cores
+// (selectable; the vector-unit kernel above is the default: at the 960k-tet
+// beam it takes 236 us per launch against 255 us for this one, both writing
+// the records as whole contiguous runs):
+// v_mfma_f32_4x4x1f32 runs 16 independent 4x4 outer products per wave, one
+// per 4-lane group. The 12 x 12 product B^T C of a tet is cut into 4 x 4
+// tiles; its 6 upper tiles (I <= J) hold every entry of the 10 blocks
+// (k <= l) except the lower entries of the diagonal blocks that fall into a
+// lower tile, which are taken from their transposes. A wave stages B and C of
+// 16 tets at a time in LDS and runs, per upper tile, 6 accumulating MFMAs
+// (k = 0..5) for those 16 tets, lane 4t + m supplying B_t(k, 4I + m) and
+// C_t(k, 4J + m) and receiving column 4J + m of the tile (rows in its four
+// accumulators). Accumulation runs over k in order in FP32, as on the
+// vector units; the results may differ from them in the last bit.
+constexpr int kMfmaTets = 16;              // tets per MFMA round of a wave
+constexpr int kMfmaBCPitch = 73;           // floats per tet of staged B (and C)
+constexpr int kMfmaOutPitch = 91;          // floats per tet of staged blocks
+constexpr int kMfmaWaveFloats = kMfmaTets * (2 * kMfmaBCPitch + kMfmaOutPitch);
+
+__global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs_mfma(
+    TetMeshDev m, const double* __restrict__ x, Inertia in, const float* __restrict__ dsdF,
+    const float* __restrict__ Hpsi, const double* __restrict__ gl, float* __restrict__ blocks,
+    double* __restrict__ fe) {
+  __shared__ __align__(16) float lds[(kElemBlock / kWave) * kMfmaWaveFloats];
+  static_assert(kWave * 13 * sizeof(double) <= kMfmaWaveFloats * sizeof(float),
+                "f_e stage exceeds the wave's LDS");
+  const int lane = threadIdx.x & (kWave - 1);
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e0 = e - lane;
+  const int nT = m.nT;
+  const int ec = e < nT ? e : nT - 1;
+  float* wl = lds + (threadIdx.x / kWave) * kMfmaWaveFloats;
+  float* sB = wl;
+  float* sC = wl + kMfmaTets * kMfmaBCPitch;
+  float* sOut = wl + 2 * kMfmaTets * kMfmaBCPitch;
+  const double vol = m.vol[ec];
+  float B[72], C[72];
+  mixed_element_BC<float>(m, ec, dsdF, Hpsi, B, C);
+  const int t = lane >> 2, mm = lane & 3;  // MFMA block (tet of the round) and lane in it
+  for (int round = 0; round < kWave / kMfmaTets; ++round) {
+    // stage B and C (row k = 0..5, column 0..11) of this round's 16 tets
+    if ((lane >> 4) == round) {
+      const int tt = lane & (kMfmaTets - 1);
+#pragma unroll
+      for (int i = 0; i < 72; ++i) {
+        const int k = i % 6, col = i / 6;
+        sB[tt * kMfmaBCPitch + 12 * k + col] = B[i];
+        sC[tt * kMfmaBCPitch + 12 * k + col] = C[i];
+      }
+    }
+    __syncthreads();
+    const int et = e0 + round * kMfmaTets + t;  // tet of this lane's MFMA block
+    const float rv = (float)m.vol[et < nT ? et : nT - 1];
+#pragma unroll
+    for (int I = 0; I < 3; ++I)
+#pragma unroll
+      for (int J = I; J < 3; ++J) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const float a = sB[t * kMfmaBCPitch + 12 * k + 4 * I + mm];
+          const float b = sC[t * kMfmaBCPitch + 12 * k + 4 * J + mm];
+          acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, acc, 0, 0, 0);
+        }
+        // acc[r] = (B^T C)(4I + r, 4J + mm)
+        const int col = 4 * J + mm, l = col / 3, q = col % 3;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 4 * I + r, k = row / 3, rr = row % 3;
+          const float v = rv * acc[r];
+          if (k < l || (k == l && I <= J))
+            sOut[t * kMfmaOutPitch + 9 * blk_index(k, l) + 3 * rr + q] = v;
+          // the transposed entry of a diagonal block when its tile is lower
+          if (k == l && (row / 4) < (col / 4) && rr != q)
+            sOut[t * kMfmaOutPitch + 9 * blk_index(k, k) + 3 * q + rr] = v;
+        }
+      }
+    __syncthreads();
+    const int eb = e0 + round * kMfmaTets;
+    for (int idx = lane; idx < kMfmaTets * 90; idx += kWave) {
+      const int tt = idx / 90, j = idx - tt * 90;
+      if (eb + tt < nT) blocks[(size_t)eb * 90 + idx] = sOut[tt * kMfmaOutPitch + j];
+    }
+    __syncthreads();
+  }
+  mixed_element_fe<float>(m, ec, lane, e0, x, in, gl, B, vol,
+                          reinterpret_cast<unsigned char*>(wl), fe);
 }
 
 template <typename Real>
@@ -1171,8 +1298,15 @@ void launch_mixed_polar_material(const TetMeshDev& m, const double* x_full, doub
 template <typename Real>
 void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
                              const Real* dsdF, const Real* Hpsi, const double* gl, Real* blocks,
-                             double* fe, hipStream_t st) {
+                             double* fe, hipStream_t st, bool mfma) {
   if (m.nT == 0) return;
+  if constexpr (std::is_same_v<Real, float>) {
+    if (mfma) {
+      hipLaunchKernelGGL(k_mixed_blocks_rhs_mfma, dim3(grid_for(m.nT, kElemBlock)),
+                         dim3(kElemBlock), 0, st, m, x_full, in, dsdF, Hpsi, gl, blocks, fe);
+      return;
+    }
+  }
   hipLaunchKernelGGL(k_mixed_blocks_rhs<Real>, dim3(grid_for(m.nT, kElemBlock)), dim3(kElemBlock),
                      0, st, m, x_full, in, dsdF, Hpsi, gl, blocks, fe);
 }
@@ -1302,7 +1436,7 @@ void launch_batch_snh_f9(int n, const double* mu, const double* la, const double
                                                   hipStream_t);                                 \
   template void launch_mixed_blocks_rhs<Real>(const TetMeshDev&, const double*, const Inertia&, \
                                               const Real*, const Real*, const double*, Real*,   \
-                                              double*, hipStream_t);                            \
+                                              double*, hipStream_t, bool);                      \
   template void launch_mixed_local_solve<Real>(const TetMeshDev&, const double*, const Real*,   \
                                                const double*, const double*, const double*,     \
                                                const double*, double*, double*, ReduceSlot,     \

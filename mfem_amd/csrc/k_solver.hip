@@ -125,10 +125,14 @@ __global__ void k_jacobi_diag(int nfree, const int* __restrict__ diag_block, con
   }
 }
 
-__global__ void k_assemble_rhs(int nfree, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+// Rows are visited in the order `rows` lists them (each XCD a contiguous
+// part of the list).
+__global__ void k_assemble_rhs(int nfree, const int* __restrict__ rows,
+                               const int* __restrict__ inc_ptr, const int* __restrict__ inc,
                                const double* __restrict__ fe, double* __restrict__ rhs) {
-  const int i = xcd_item(nfree);
-  if (i < 0) return;
+  const int item = xcd_item(nfree);
+  if (item < 0) return;
+  const int i = rows[item];
   double a0 = 0, a1 = 0, a2 = 0;
   const int p1 = inc_ptr[i + 1];
   // incidences in batches: ids, then element values, in flight together
@@ -867,11 +871,11 @@ template void launch_bsr_to_dense<float>(int, const int*, const int*, const floa
 template void launch_bsr_to_dense<double>(int, const int*, const int*, const double*, double*,
                                           long, hipStream_t);
 
-void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const double* fe,
-                         double* rhs, hipStream_t st) {
+void launch_assemble_rhs(int nfree, const int* rows, const int* inc_ptr, const int* inc,
+                         const double* fe, double* rhs, hipStream_t st) {
   if (nfree == 0) return;
-  hipLaunchKernelGGL(k_assemble_rhs, dim3(xcd_grid(nfree, kBS)), dim3(kBS), 0, st, nfree, inc_ptr, inc,
-                     fe, rhs);
+  hipLaunchKernelGGL(k_assemble_rhs, dim3(xcd_grid(nfree, kBS)), dim3(kBS), 0, st, nfree, rows,
+                     inc_ptr, inc, fe, rhs);
 }
 
 template <typename MT, typename VT>

@@ -55,11 +55,12 @@ void launch_mixed_polar_material(const TetMeshDev& m, const double* x_full, doub
                                  double* S, const double* s, Real* dsdF, const HessOut<Real>& ho,
                                  double* g, double* gl, double h2, hipStream_t st);
 // Phase 2: element stiffness blocks vol * B^T Hpsi B (10 unique 3x3 blocks
-// per tet) and the element residual f_e (12 per tet).
+// per tet) and the element residual f_e (12 per tet). mfma (FP32 only):
+// the B^T C products on the matrix cores instead of the vector units.
 template <typename Real>
 void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const Inertia& in,
                              const Real* dsdF, const Real* Hpsi, const double* gl,
-                             Real* blocks, double* fe, hipStream_t st);
+                             Real* blocks, double* fe, hipStream_t st, bool mfma = false);
 // Local solve (reference mixed_stretch.cpp:244-259): lambda and ds from dx,
 // with the FP64 Hessian and inverse (Hpsi, Hinv as in HessOut: the Real
 // arrays in FP64 mode, Hloc / Hlinv in FP32 storage mode).
@@ -129,9 +130,10 @@ void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* inv
 template <typename Real>
 void launch_bsr_to_dense(int nrows, const int* rowptr, const int* col, const Real* K, double* A,
                          long lda, hipStream_t st);
-// rhs at free vertex i = sum over its (tet, corner) incidences of f_e[corner]
-void launch_assemble_rhs(int nfree, const int* inc_ptr, const int* inc, const double* fe,
-                         double* rhs, hipStream_t st);
+// rhs at free vertex i = sum over its (tet, corner) incidences of f_e[corner];
+// rows: the free rows in visiting order (a permutation of 0 .. nfree-1)
+void launch_assemble_rhs(int nfree, const int* rows, const int* inc_ptr, const int* inc,
+                         const double* fe, double* rhs, hipStream_t st);
 
 // --- BSR 3x3 SpMV / Jacobi PCG (reference Eigen ConjugateGradient semantics)
 // Control block of one CG solve (device memory, read by the host at polls).

@@ -25,6 +25,7 @@ EXPORTED = [
     "mfem_sim_energy", "mfem_sim_stats", "mfem_sim_get_positions", "mfem_sim_size",
     "mfem_sim_get", "mfem_sim_get_int", "mfem_sim_set_profiling", "mfem_sim_matrix_info",
     "mfem_sim_time_spmv", "mfem_sim_spmv_tuning", "mfem_sim_set_cg_shape", "mfem_sim_cg_tuning",
+    "mfem_sim_set_element_kernel",
     "mfem_sim_kkt_size", "mfem_sim_kkt_apply",
     "mfem_material_s6_batch", "mfem_snh_f9_batch", "mfem_polar_batch",
     "mfem_batch_create", "mfem_batch_destroy", "mfem_batch_step", "mfem_batch_sizes",
@@ -107,6 +108,7 @@ def lib():
         L.mfem_sim_time_spmv.argtypes = [vp, C.c_int, dp]
         L.mfem_sim_spmv_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip]
         L.mfem_sim_set_cg_shape.argtypes = [vp, C.c_int, C.c_int]
+        L.mfem_sim_set_element_kernel.argtypes = [vp, C.c_int]
         L.mfem_sim_cg_tuning.argtypes = [vp, dp, ip, ip, ip, ip, ip, ip, ip, ip]
         L.mfem_sim_kkt_size.argtypes = [vp, C.POINTER(C.c_long)]
         L.mfem_sim_kkt_apply.argtypes = [vp, C.c_int, dp, dp]
@@ -316,6 +318,11 @@ class Sim:
         check(lib().mfem_sim_spmv_tuning(self.h, t.ctypes.data_as(dp), C.byref(n), C.byref(ch),
                                          C.byref(G), C.byref(U), C.byref(NT)))
         return t, ch.value, (G.value, U.value, NT.value)
+
+    def set_element_kernel(self, variant):
+        """0: stiffness-block products on the vector units (default); 1: on
+        the matrix cores (FP32 storage, 3D)."""
+        check(lib().mfem_sim_set_element_kernel(self.h, int(variant)))
 
     def set_cg_shape(self, variant, blocks=0):
         """Force the CG kernel shape and grid (variant -1: autotune again)."""
