@@ -110,14 +110,21 @@ template <> inline const float* RealArray::as<float>() const { return f_.data();
 // Small pinned host buffer for scalar read-backs.
 class PinnedBuffer {
  public:
-  explicit PinnedBuffer(size_t bytes = 256) { MFEM_HIP_CHECK(hipHostMalloc(&p_, bytes)); }
+  // flags: hipHostMalloc flags; with hipHostMallocMapped, device() is the
+  // address kernels write the buffer through
+  explicit PinnedBuffer(size_t bytes = 256, unsigned flags = hipHostMallocDefault) {
+    MFEM_HIP_CHECK(hipHostMalloc(&p_, bytes, flags));
+    if (flags & hipHostMallocMapped) MFEM_HIP_CHECK(hipHostGetDevicePointer(&d_, p_, 0));
+  }
   ~PinnedBuffer() { if (p_) (void)hipHostFree(p_); }
   PinnedBuffer(const PinnedBuffer&) = delete;
   PinnedBuffer& operator=(const PinnedBuffer&) = delete;
   template <typename T> T* as() { return reinterpret_cast<T*>(p_); }
+  template <typename T> T* device() { return reinterpret_cast<T*>(d_); }
 
  private:
   void* p_ = nullptr;
+  void* d_ = nullptr;
 };
 
 }  // namespace mfem

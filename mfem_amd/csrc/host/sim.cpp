@@ -3,6 +3,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -496,7 +497,8 @@ void SystemMatrixPD<DIM>::post_solve(SimState<DIM>* state, const double* dx) {
 // ============================================================ CG
 template <int DIM>
 PcgSolver<DIM>::PcgSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
-  pinned_ = std::make_unique<PinnedBuffer>(2 * sizeof(PcgCtrl) + 64);
+  pinned_ = std::make_unique<PinnedBuffer>(2 * sizeof(PcgCtrl) + 64,
+                                           hipHostMallocMapped | hipHostMallocCoherent);
   ctrl_.resize(1);
   parts_.resize(2 * (size_t)kCgParts * kCgMaxBlocks);
   stamps_.resize(2 * 2 * (size_t)kCgMaxBlocks);
@@ -564,13 +566,18 @@ template <int DIM>
 template <typename T>
 int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters, int sweep) {
   hipStream_t st = this->state_->ctx_->stream;
-  const PcgBuffers<T> B = buffers<T>(b, x);
+  PcgBuffers<T> B = buffers<T>(b, x);
+  // The kernel that stops the solve writes the control block into mapped
+  // pinned memory; the host reads it there after each chunk's event, so a
+  // poll costs no copy on the stream.
+  auto* mirror = pinned_->as<PcgCtrl>();
+  mirror->done = 0;
+  B.mirror = pinned_->device<PcgCtrl>();
   launch_pcg_init(B, tol, max_iters, st);
   const int poll = std::max(1, this->state_->config_->pcg_poll);
   const long limit = (long)max_iters + 2;  // kernels 0 .. max_iters + 1
-  auto* snap = pinned_->as<PcgCtrl>();     // two control-block snapshots
   long launched = 0;
-  int issued = 0, waited = 0;  // snapshots; snapshot k uses slot k & 1
+  int issued = 0, waited = 0;  // chunks; chunk k signals event k & 1
   // Kernels queued past the stop exit at once but still cost a dispatch of
   // the whole grid. Near the iteration count of the last sweep in this slot
   // the chunks shrink to a quarter, so that fewer of them are queued when
@@ -584,10 +591,7 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
     for (long j = 0; j < chunk; ++j) launch_pcg_iteration(B, (int)(launched + j), st);
     launched += chunk;
     MFEM_HIP_CHECK(hipGetLastError());
-    const int slot = issued & 1;
-    MFEM_HIP_CHECK(hipMemcpyAsync(snap + slot, ctrl_.data(), sizeof(PcgCtrl),
-                                  hipMemcpyDeviceToHost, st));
-    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[slot], st));
+    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[issued & 1], st));
     ++issued;
   };
   // One chunk stays queued behind the one being polled, so the device never
@@ -595,11 +599,11 @@ int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters
   issue();
   while (true) {
     if (launched < limit && issued - waited < 2) issue();
-    const int slot = waited & 1;
-    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[slot]));
+    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[waited & 1]));
     ++waited;
-    if (snap[slot].done) {
-      last_ = snap[slot];
+    if (reinterpret_cast<volatile const int&>(mirror->done)) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      last_ = *mirror;
       break;
     }
     if (waited == issued) throw std::runtime_error("CG control block did not terminate");

@@ -496,7 +496,7 @@ template <int G, int U, int BS, typename T>
 __global__ __launch_bounds__(BS) void k_cg_iter(
     int it, int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
     const T* __restrict__ val, const CgDof<T>* __restrict__ s_old, CgDof<T>* __restrict__ s_new,
-    double* __restrict__ x, PcgCtrl* ctrl, const double* __restrict__ parts_old,
+    double* __restrict__ x, PcgCtrl* ctrl, PcgCtrl* mirror, const double* __restrict__ parts_old,
     double* __restrict__ parts_new, const unsigned long long* __restrict__ stamps_old,
     unsigned long long* __restrict__ stamps_new) {
   __shared__ double sm[kCgParts * (BS / kWave)];
@@ -541,6 +541,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
         ctrl->iter = converged ? (j > 0 ? j - 1 : 0) : j;
         ctrl->stop_it = it;
         ctrl->done = 1;
+        if (mirror) *mirror = *ctrl;
       }
       return;
     }
@@ -743,7 +744,8 @@ __global__ __launch_bounds__(kBS) void k_cg_init(int n, const double* __restrict
 
 // |b|^2 and the stopping threshold max(tol^2 |b|^2, DBL_MIN).
 __global__ __launch_bounds__(kBS) void k_cg_setup(PcgCtrl* ctrl, const double* __restrict__ parts,
-                                                  int np, double tol, int max_iters) {
+                                                  int np, double tol, int max_iters,
+                                                  PcgCtrl* mirror) {
   __shared__ double sm[kBS / kWave];
   const double b2 = reduce_array<kBS, 0>(parts, np, 1, 0, sm);
   if (threadIdx.x == 0) {
@@ -756,6 +758,7 @@ __global__ __launch_bounds__(kBS) void k_cg_setup(PcgCtrl* ctrl, const double* _
     ctrl->stop_it = 0;
     ctrl->kern_launches = 0;
     ctrl->done = (b2 == 0.0) ? 1 : 0;  // x = 0 (Eigen returns at once)
+    if (mirror && ctrl->done) *mirror = *ctrl;
   }
 }
 
@@ -899,7 +902,8 @@ void launch_pcg_init(const PcgBuffers<T>& B, double tol, int max_iters, hipStrea
   const int gv = std::min(grid(B.n, kBS), 256);
   hipLaunchKernelGGL(k_cg_init<T>, dim3(gv), dim3(kBS), 0, st, B.n, B.b, B.invdiag, B.x,
                      reinterpret_cast<CgDof<T>*>(B.state[0]), B.parts[1]);
-  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kBS), 0, st, B.ctrl, B.parts[1], gv, tol, max_iters);
+  hipLaunchKernelGGL(k_cg_setup, dim3(1), dim3(kBS), 0, st, B.ctrl, B.parts[1], gv, tol, max_iters,
+                     B.mirror);
 }
 
 template <typename T>
@@ -918,7 +922,7 @@ void launch_pcg_iteration(const PcgBuffers<T>& B, int it, hipStream_t st) {
   case i:                                                                                      \
     hipLaunchKernelGGL((k_cg_iter<kCgShapes[i].G, kCgShapes[i].U, kCgShapes[i].BS, T>),          \
                        dim3(B.blocks), dim3(kCgShapes[i].BS), 0, st, it, nrows, B.rowptr, B.col, \
-                       B.K, so, sn, B.x, B.ctrl, po, pn, to, tn);                              \
+                       B.K, so, sn, B.x, B.ctrl, B.mirror, po, pn, to, tn);                    \
     break;
     MFEM_CG(0) MFEM_CG(1) MFEM_CG(2) MFEM_CG(3) MFEM_CG(4) MFEM_CG(5) MFEM_CG(6) MFEM_CG(7)
 #undef MFEM_CG

@@ -180,6 +180,9 @@ struct PcgBuffers {
   double* x = nullptr;         // solution (FP64 accumulation)
   T* state[2] = {nullptr, nullptr};            // 4 n values each
   PcgCtrl* ctrl = nullptr;
+  // optional: mapped pinned host memory that receives the control block when
+  // the solve stops (the host polls it instead of copying ctrl)
+  PcgCtrl* mirror = nullptr;
   double* parts[2] = {nullptr, nullptr};       // kCgParts * blocks each
   unsigned long long* stamps[2] = {nullptr, nullptr};  // 2 * blocks each (profiling) or null
   int variant = 0;  // CG kernel shape
