@@ -305,12 +305,14 @@ int mfem_sim_set_callback(mfem_sim* sim, mfem_newton_callback cb, void* user) {
     if (!sim) throw std::invalid_argument("null simulation");
     if (!cb) {
       sim->opt->callback = [](const SimState<3>&) {};
+      sim->opt->sync_each_iteration = false;
       return;
     }
     NewtonOptimizer<3>* opt = sim->opt.get();
     sim->opt->callback = [sim, opt, cb, user](const SimState<3>&) {
       cb(sim, (int)opt->history().size(), user);
     };
+    sim->opt->sync_each_iteration = true;
   });
 }
 

@@ -113,7 +113,8 @@ struct SimConfig {
   std::set<VariableType> mixed_variables = {VAR_MIXED_STRETCH, VAR_MIXED_COLLISION};
   // device backend options
   Precision precision = Precision::FP64;
-  int pcg_poll = 16;         // CG iterations launched between host polls
+  int pcg_poll = 16;         // CG iterations launched between host polls (the
+                             // Jacobi CG keeps a quarter as many queued ahead)
   double inner_tol = 1e-4;   // relative tolerance of each FP32 inner CG solve (MIXED)
   int max_refinements = 50;  // defect-correction sweeps (MIXED)
   int spmv_variant = -1;     // BSR SpMV shape; -1: autotune on the first solve

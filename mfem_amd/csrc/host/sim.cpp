@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <immintrin.h>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,16 +27,18 @@ void Reducer::init(long max_items) {
   long g = std::max<long>(reduce_grid(max_items), max_items / 32 + 1);
   stride_ = std::max<long>(2 * g + 64, 4096);
   partials_.resize((size_t)kSlots * stride_);
-  results_.resize(kSlots);
-  results_.zero();
-  pinned_ = std::make_unique<PinnedBuffer>(256);
+  // the finalize kernels write the results straight into mapped pinned
+  // memory: a read is a stream synchronisation, with no copy on the stream
+  results_ = std::make_unique<PinnedBuffer>(kSlots * sizeof(double),
+                                            hipHostMallocMapped | hipHostMallocCoherent);
+  for (int k = 0; k < kSlots; ++k) results_->as<double>()[k] = 0.0;
   MFEM_HIP_CHECK(hipDeviceSynchronize());
 }
 
 ReduceSlot Reducer::slot(int k) {
   ReduceSlot r;
   r.partials = partials_.data() + (size_t)k * stride_;
-  r.result = results_.data() + k;
+  r.result = results_->device<double>() + k;
   return r;
 }
 
@@ -47,10 +50,9 @@ double Reducer::read(int k, hipStream_t st) {
 
 void Reducer::read_range(int k0, int n, double* out, hipStream_t st) {
   if (k0 < 0 || n < 0 || k0 + n > kSlots) throw std::logic_error("reducer slot out of range");
-  MFEM_HIP_CHECK(hipMemcpyAsync(pinned_->as<double>(), results_.data() + k0, n * sizeof(double),
-                                hipMemcpyDeviceToHost, st));
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
-  for (int i = 0; i < n; ++i) out[i] = pinned_->as<double>()[i];
+  const volatile double* r = results_->as<double>() + k0;
+  for (int i = 0; i < n; ++i) out[i] = r[i];
 }
 
 Inertia DeviceContext::inertia() const {
@@ -502,7 +504,6 @@ PcgSolver<DIM>::PcgSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
   ctrl_.resize(1);
   parts_.resize(2 * (size_t)kCgParts * kCgMaxBlocks);
   stamps_.resize(2 * 2 * (size_t)kCgMaxBlocks);
-  for (auto& e : poll_ev_) MFEM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   int dev = 0, khz = 0;
   MFEM_HIP_CHECK(hipGetDevice(&dev));
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
@@ -510,10 +511,7 @@ PcgSolver<DIM>::PcgSolver(SimState<DIM>* state) : LinearSolver<DIM>(state) {
 }
 
 template <int DIM>
-PcgSolver<DIM>::~PcgSolver() {
-  for (auto e : poll_ev_)
-    if (e) (void)hipEventDestroy(e);
-}
+PcgSolver<DIM>::~PcgSolver() = default;
 
 template <int DIM>
 void PcgSolver<DIM>::force_cg_shape(int variant, int blocks) {
@@ -564,51 +562,49 @@ PcgBuffers<T> PcgSolver<DIM>::buffers(const double* b, double* x) {
 
 template <int DIM>
 template <typename T>
-int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters, int sweep) {
+int PcgSolver<DIM>::run_cg(const double* b, double* x, double tol, int max_iters) {
   hipStream_t st = this->state_->ctx_->stream;
   PcgBuffers<T> B = buffers<T>(b, x);
-  // The kernel that stops the solve writes the control block into mapped
-  // pinned memory; the host reads it there after each chunk's event, so a
-  // poll costs no copy on the stream.
-  auto* mirror = pinned_->as<PcgCtrl>();
+  // Every CG kernel that does work writes its index into mapped pinned
+  // memory, and the one that stops the solve writes the control block there.
+  // The host keeps `ahead` kernels queued beyond the last one that began, so
+  // the device never waits for it and at most that many kernels are queued
+  // past the stop (they exit at once); no event or copy enters the stream.
+  volatile PcgCtrl* mirror = pinned_->as<PcgCtrl>();
   mirror->done = 0;
+  mirror->started = -1;
   B.mirror = pinned_->device<PcgCtrl>();
   launch_pcg_init(B, tol, max_iters, st);
-  const int poll = std::max(1, this->state_->config_->pcg_poll);
+  const long ahead = std::max(2, this->state_->config_->pcg_poll / 4);
   const long limit = (long)max_iters + 2;  // kernels 0 .. max_iters + 1
   long launched = 0;
-  int issued = 0, waited = 0;  // chunks; chunk k signals event k & 1
-  // Kernels queued past the stop exit at once but still cost a dispatch of
-  // the whole grid. Near the iteration count of the last sweep in this slot
-  // the chunks shrink to a quarter, so that fewer of them are queued when
-  // the stop comes.
-  const int hs = sweep > 0 ? 1 : 0;
-  const long hint = iter_hint_[hs];
-  const int tail = std::max(2, poll / 4);
-  auto issue = [&] {
-    const long size = (hint > 0 && launched + 3L * poll >= hint) ? tail : poll;
-    const long chunk = std::min<long>(size, limit - launched);
-    for (long j = 0; j < chunk; ++j) launch_pcg_iteration(B, (int)(launched + j), st);
-    launched += chunk;
+  auto launch_to = [&](long n) {
+    n = std::min(n, limit);
+    if (launched >= n) return;
+    for (; launched < n; ++launched) launch_pcg_iteration(B, (int)launched, st);
     MFEM_HIP_CHECK(hipGetLastError());
-    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[issued & 1], st));
-    ++issued;
   };
-  // One chunk stays queued behind the one being polled, so the device never
-  // waits for the host; kernels queued after the stop exit at once.
-  issue();
-  while (true) {
-    if (launched < limit && issued - waited < 2) issue();
-    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[waited & 1]));
-    ++waited;
-    if (reinterpret_cast<volatile const int&>(mirror->done)) {
-      std::atomic_thread_fence(std::memory_order_acquire);
-      last_ = *mirror;
-      break;
+  launch_to(ahead);
+  while (!mirror->done) {
+    if (launched < limit) {
+      launch_to((long)mirror->started + 1 + ahead);
+      _mm_pause();
+      continue;
     }
-    if (waited == issued) throw std::runtime_error("CG control block did not terminate");
+    // all kernels queued: the last one stops the solve at the latest
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));
+    if (!mirror->done) throw std::runtime_error("CG control block did not terminate");
   }
-  iter_hint_[hs] = last_.stop_it + 1;  // kernels 0 .. stop_it ran
+  std::atomic_thread_fence(std::memory_order_acquire);
+  last_.rr = mirror->rr;
+  last_.b2 = mirror->b2;
+  last_.threshold = mirror->threshold;
+  last_.kern_ticks = mirror->kern_ticks;
+  last_.iter = mirror->iter;
+  last_.done = 1;
+  last_.max_iters = mirror->max_iters;
+  last_.stop_it = mirror->stop_it;
+  last_.kern_launches = mirror->kern_launches;
   if (this->profile_spmv) {
     this->stats_.spmv_ms += last_.kern_ticks / wall_khz_;
     this->stats_.spmv_count += last_.kern_launches;
@@ -769,7 +765,7 @@ void PcgSolver<DIM>::solve() {
       r2 = R.read(RS_RES, st);
       if (!(r2 >= thr) || budget <= 0 || sweeps >= cfg.max_refinements) break;
       const double inner = std::max(cfg.inner_tol, std::sqrt(thr / r2) * 0.5);
-      const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget, sweeps);
+      const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget);
       budget -= std::max(it, 1);
       this->stats_.iterations += it;
       launch_axpy(n, 1.0, dx64_.data(), x_.data(), st);
@@ -1377,8 +1373,6 @@ void NewtonOptimizer<DIM>::reset() {
   LinearSolverFactory<DIM> f;
   solver_ = f.create(S.config_->solver_type, &S);
   if (!solver_) throw std::runtime_error("unknown linear solver");
-  for (auto& e : ev_)
-    if (!e) MFEM_HIP_CHECK(hipEventCreate(&e));
 }
 
 // reference newton_optimizer.cpp:92-110
@@ -1484,13 +1478,40 @@ void NewtonOptimizer<DIM>::step() {
   double grad_norm = 0, E = 0, E_prev = 0, res = 0;
   const bool has_mixed = !S.mixed_vars_.empty();
   const int ls_iters = S.config_->ls_iters;
+  // Phase timings come from events read once the step has ended: nothing in
+  // the next iteration waits for the accepted step, so its first kernels are
+  // queued behind it without a synchronisation in between.
+  auto events = [&](int it) {
+    while ((int)ev_.size() < 4 * (it + 1)) {
+      hipEvent_t e = nullptr;
+      MFEM_HIP_CHECK(hipEventCreate(&e));
+      ev_.push_back(e);
+    }
+    return &ev_[4 * it];
+  };
+  auto phase_times = [&](int it) {
+    const hipEvent_t* e = &ev_[4 * it];
+    float t01 = 0, t12 = 0, t23 = 0;
+    MFEM_HIP_CHECK(hipEventElapsedTime(&t01, e[0], e[1]));
+    MFEM_HIP_CHECK(hipEventElapsedTime(&t12, e[1], e[2]));
+    MFEM_HIP_CHECK(hipEventElapsedTime(&t23, e[2], e[3]));
+    auto& h = history_[it];
+    h.update_ms = t01;
+    h.solve_ms = t12;
+    h.ls_ms = t23;
+    stats_.update_ms += t01;
+    stats_.solve_ms += t12;
+    stats_.ls_ms += t23;
+  };
+  int timed = 0;  // iterations whose phase times are in stats_
   do {
-    MFEM_HIP_CHECK(hipEventRecord(ev_[0], st));
+    hipEvent_t* ev = events(i);
+    MFEM_HIP_CHECK(hipEventRecord(ev[0], st));
     update_system();
-    MFEM_HIP_CHECK(hipEventRecord(ev_[1], st));
+    MFEM_HIP_CHECK(hipEventRecord(ev[1], st));
     solver_->solve();
     const int nd = launch_decrement();
-    MFEM_HIP_CHECK(hipEventRecord(ev_[2], st));
+    MFEM_HIP_CHECK(hipEventRecord(ev[2], st));
     const int ne = launch_energy(0.0, RS_E0);
     if (ls_iters > 0) launch_energy(1.0, RS_E1);
     double v[6] = {0, 0, 0, 0, 0, 0};
@@ -1526,20 +1547,18 @@ void NewtonOptimizer<DIM>::step() {
       stats_.ls_fail++;
     }
     stats_.ls_iters += iter;
-    MFEM_HIP_CHECK(hipEventRecord(ev_[3], st));
-    MFEM_HIP_CHECK(hipEventSynchronize(ev_[3]));
-    float t01 = 0, t12 = 0, t23 = 0;
-    MFEM_HIP_CHECK(hipEventElapsedTime(&t01, ev_[0], ev_[1]));
-    MFEM_HIP_CHECK(hipEventElapsedTime(&t12, ev_[1], ev_[2]));
-    MFEM_HIP_CHECK(hipEventElapsedTime(&t23, ev_[2], ev_[3]));
-    stats_.update_ms += t01;
-    stats_.solve_ms += t12;
-    stats_.ls_ms += t23;
+    MFEM_HIP_CHECK(hipEventRecord(ev[3], st));
     stats_.alpha = alpha;
-    history_.push_back({i + 1, E, res, grad_norm, alpha, t01, t12, t23, ss.iterations, iter});
+    history_.push_back({i + 1, E, res, grad_norm, alpha, 0.0, 0.0, 0.0, ss.iterations, iter});
+    if (this->sync_each_iteration) {
+      MFEM_HIP_CHECK(hipEventSynchronize(ev[3]));
+      for (; timed <= i; ++timed) phase_times(timed);
+    }
     ++i;
     this->callback(S);
   } while (i < S.config_->outer_steps && grad_norm > S.config_->newton_tol && res > 1e-12);
+  MFEM_HIP_CHECK(hipEventSynchronize(ev_[4 * (i - 1) + 3]));
+  for (; timed < i; ++timed) phase_times(timed);
   stats_.newton_iters = i;
   stats_.decrement = grad_norm;
   stats_.energy = E;

@@ -38,8 +38,8 @@ class Reducer {
 
  private:
   long stride_ = 0;
-  DeviceArray<double> partials_, results_;
-  std::unique_ptr<PinnedBuffer> pinned_;
+  DeviceArray<double> partials_;
+  std::unique_ptr<PinnedBuffer> results_;
 };
 
 // Device resources shared by the variables and solvers of one simulation.
@@ -309,11 +309,8 @@ class PcgSolver : public LinearSolver<DIM> {
  private:
   template <typename T>
   mfemhip::PcgBuffers<T> buffers(const double* b, double* x);
-  // sweep: which CG sweep of the solve this is (0 = the first); the
-  // iteration count of the last first (or later) sweep sets where the launch
-  // chunks shrink.
   template <typename T>
-  int run_cg(const double* b, double* x, double tol, int max_iters, int sweep = 0);
+  int run_cg(const double* b, double* x, double tol, int max_iters);
   template <typename T>
   void tune_spmv();
   template <typename T>
@@ -330,9 +327,7 @@ class PcgSolver : public LinearSolver<DIM> {
   DeviceArray<unsigned long long> stamps_;  // 2 x 2 x kCgMaxBlocks clock stamps
   DeviceArray<mfemhip::PcgCtrl> ctrl_;
   mfemhip::PcgCtrl last_{};      // control block at the end of the last CG solve
-  int iter_hint_[2] = {0, 0};    // iterations of the last first / later sweep
-  std::unique_ptr<PinnedBuffer> pinned_;  // two control-block snapshots
-  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  std::unique_ptr<PinnedBuffer> pinned_;  // mapped host copy of the control block
   double wall_khz_ = 100000.0;   // device wall-clock rate
 };
 
@@ -477,6 +472,9 @@ class Optimizer {
   virtual void step() = 0;
   SimState<DIM>& state() { return *state_; }
   std::function<void(const SimState<DIM>&)> callback = [](const SimState<DIM>&) {};
+  // set when `callback` is the caller's: each Newton iteration then ends with
+  // a synchronisation, so the callback sees that iteration's timings
+  bool sync_each_iteration = false;
 
  protected:
   std::unique_ptr<SimState<DIM>> state_;
@@ -520,7 +518,8 @@ class NewtonOptimizer : public Optimizer<DIM> {
   int launch_decrement();
   std::unique_ptr<LinearSolver<DIM>> solver_;
   StepStats stats_;
-  hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // four timing events per Newton iteration of the current step
+  std::vector<hipEvent_t> ev_;
 };
 
 template <int DIM>

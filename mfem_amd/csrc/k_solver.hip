@@ -439,6 +439,22 @@ __device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts
   block_sum_n<BS, kCgParts>(v, sm);
 }
 
+// The control block of a stopped solve into the host's mapped copy: every
+// field, then (after a system-scope fence) done, so that a host that sees
+// done set reads the final fields.
+__device__ __forceinline__ void publish_stop(const PcgCtrl& c, PcgCtrl* mirror) {
+  mirror->rr = c.rr;
+  mirror->b2 = c.b2;
+  mirror->threshold = c.threshold;
+  mirror->kern_ticks = c.kern_ticks;
+  mirror->iter = c.iter;
+  mirror->max_iters = c.max_iters;
+  mirror->stop_it = c.stop_it;
+  mirror->kern_launches = c.kern_launches;
+  __threadfence_system();
+  __hip_atomic_store(&mirror->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // One pass of a lane over a block row in the CG kernel: the U column ids,
 // value blocks and gathered states {r, q, p, D^-1} of the block columns. The
 // loads depend on neither alpha nor beta, so the first pass of a workgroup is
@@ -509,6 +525,9 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
   __syncthreads();
   if (s_done) return;
   const int nb = gridDim.x;
+  // progress for the host, which keeps a few kernels queued ahead of it
+  if (mirror && blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(&mirror->started, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   const int lane = threadIdx.x & (G - 1);
   // XCD x (= blockIdx % 8) owns rows [nrows x / 8, nrows (x+1) / 8); its nbx
   // workgroups split that range into equal contiguous chunks.
@@ -541,7 +560,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter(
         ctrl->iter = converged ? (j > 0 ? j - 1 : 0) : j;
         ctrl->stop_it = it;
         ctrl->done = 1;
-        if (mirror) *mirror = *ctrl;
+        if (mirror) publish_stop(*ctrl, mirror);
       }
       return;
     }
@@ -715,7 +734,7 @@ __global__ void k_cg_setup_batch(int nsys, PcgCtrl* ctrls, const double* __restr
   c.max_iters = max_iters[b];
   c.stop_it = 0;
   c.kern_launches = 0;
-  c.pad_ = 0;
+  c.started = 0;
   c.done = (!on[b] || c.b2 == 0.0) ? 1 : 0;
   ctrls[b] = c;
 }
@@ -757,8 +776,9 @@ __global__ __launch_bounds__(kBS) void k_cg_setup(PcgCtrl* ctrl, const double* _
     ctrl->max_iters = max_iters;
     ctrl->stop_it = 0;
     ctrl->kern_launches = 0;
+    ctrl->started = -1;
     ctrl->done = (b2 == 0.0) ? 1 : 0;  // x = 0 (Eigen returns at once)
-    if (mirror && ctrl->done) *mirror = *ctrl;
+    if (mirror && ctrl->done) publish_stop(*ctrl, mirror);
   }
 }
 

@@ -142,7 +142,7 @@ struct PcgCtrl {
   double kern_ticks;  // profiling: summed wall-clock ticks of the CG kernels
   int iter, done, max_iters, stop_it;
   int kern_launches;  // profiling: CG kernels accounted in kern_ticks
-  int pad_;
+  int started;        // index of the last CG kernel that did not exit at once
 };
 // SpMV shapes (lanes per block row G, blocks per lane per trip U,
 // non-temporal value loads NT), chosen per matrix by the host autotuner.
