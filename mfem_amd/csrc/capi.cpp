@@ -4,6 +4,7 @@
 #include "../../include/mfem_hip.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -55,6 +56,39 @@ int sym6(int i, int j) {
   return i * 6 - (i * (i - 1)) / 2 + (j - i);
 }
 
+// Inverses of packed symmetric 6x6 matrices (21 values, stride nT) by
+// Gauss-Jordan elimination with partial pivoting; the symmetric part of each
+// inverse, same layout.
+std::vector<double> packed_sym_inverse6(const std::vector<double>& P, int nT) {
+  std::vector<double> out(P.size());
+  for (int e = 0; e < nT; ++e) {
+    double A[6][12];
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) {
+        A[i][j] = P[(size_t)sym6(i, j) * nT + e];
+        A[i][6 + j] = i == j ? 1.0 : 0.0;
+      }
+    for (int c = 0; c < 6; ++c) {
+      int p = c;
+      for (int r = c + 1; r < 6; ++r)
+        if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
+      if (p != c)
+        for (int j = 0; j < 12; ++j) std::swap(A[c][j], A[p][j]);
+      const double d = 1.0 / A[c][c];
+      for (int j = 0; j < 12; ++j) A[c][j] *= d;
+      for (int r = 0; r < 6; ++r)
+        if (r != c && A[r][c] != 0.0) {
+          const double f = A[r][c];
+          for (int j = 0; j < 12; ++j) A[r][j] -= f * A[c][j];
+        }
+    }
+    for (int i = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j)
+        out[(size_t)sym6(i, j) * nT + e] = 0.5 * (A[i][6 + j] + A[j][6 + i]);
+  }
+  return out;
+}
+
 std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
   auto& S = sim->state();
   auto& m = *S.mesh_;
@@ -84,6 +118,8 @@ std::vector<double> fetch(mfem_sim* sim, const std::string& k) {
     auto* ms = need_ms();
     const auto& d = inv ? ms->Hlinv_ : ms->Hloc_;
     if (d.size()) return d.to_host(st);
+    // FP32 storage on tets keeps no inverse: that of the FP64 Hessian
+    if (inv && ms->Hloc_.size()) return packed_sym_inverse6(ms->Hloc_.to_host(st), nT);
     return inv ? ms->Hinv_.to_host_double(st) : ms->Hpsi_.to_host_double(st);
   };
   if (two && (k == "H" || k == "Hinv")) {

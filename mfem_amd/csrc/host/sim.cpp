@@ -322,9 +322,9 @@ void MixedStretch<DIM>::reset() {
 
 // Where the element Hessian phase writes. FP64 mode: Hpsi_ and Hinv_. FP32
 // storage: Hpsi_ in FP32 for the stiffness blocks or the KKT product; with the
-// Schur complement, the FP64 pair Hloc_ / Hlinv_ for the local solve (sized
-// here, on first use); with the KKT solver, the FP32 inverse for its
-// preconditioner.
+// Schur complement, FP64 Hloc_ for the local solve (sized here, on first
+// use), which on tets it solves with by LU, and on triangles the FP64 inverse
+// Hlinv_; with the KKT solver, the FP32 inverse for its preconditioner.
 template <int DIM>
 template <typename Real>
 HessOut<Real> MixedStretch<DIM>::hess_out() {
@@ -339,12 +339,11 @@ HessOut<Real> MixedStretch<DIM>::hess_out() {
     return ho;
   }
   const size_t n = 21 * (size_t)this->mesh_->nT();
-  if (Hloc_.size() != n) {
-    Hloc_.resize(n);
-    Hlinv_.resize(n);
-  }
+  const bool tet = this->mesh_->dev().dim == 3;
+  if (Hloc_.size() != n) Hloc_.resize(n);
+  if (!tet && Hlinv_.size() != n) Hlinv_.resize(n);
   ho.Hloc = Hloc_.data();
-  ho.Hlinv = Hlinv_.data();
+  if (!tet) ho.Hlinv = Hlinv_.data();
   return ho;
 }
 

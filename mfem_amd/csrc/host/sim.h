@@ -201,13 +201,18 @@ class MixedStretch : public MixedVariable<DIM> {
   // element state (structure of arrays, stride nT)
   DeviceArray<double> R_, S_, s_, la_, ds_, g_, gl_;
   RealArray dsdF_, Hpsi_, Hinv_;
-  // FP64 copies of Hpsi and its inverse for the local solve when the element
-  // arrays above are stored in FP32 (empty in FP64 mode, where Hpsi_ and
-  // Hinv_ are the FP64 matrices)
+  // FP64 copies of Hpsi (and on triangles its inverse) for the local solve
+  // when the element arrays above are stored in FP32 (empty in FP64 mode,
+  // where Hpsi_ and Hinv_ are the FP64 matrices); the tet local solve
+  // solves with Hloc_ by LU
   DeviceArray<double> Hloc_, Hlinv_;
-  // the FP64 Hessian and inverse the local solve reads
+  // the FP64 Hessian and inverse the local solve reads (inverse null: the
+  // kernel forms it)
   const double* local_hpsi() const { return Hpsi_.fp32() ? Hloc_.data() : Hpsi_.f64(); }
-  const double* local_hinv() const { return Hpsi_.fp32() ? Hlinv_.data() : Hinv_.f64(); }
+  const double* local_hinv() const {
+    if (!Hpsi_.fp32()) return Hinv_.f64();
+    return Hlinv_.size() ? Hlinv_.data() : nullptr;
+  }
   // false when the linear solver does not use the Schur-complement system
   // (its element stiffness blocks and residual are then not formed)
   bool schur_blocks_ = true;

@@ -97,6 +97,47 @@ __device__ __forceinline__ void lu_inverse6(const Real* Ain, Real* X) {
   }
 }
 
+// x = A^-1 b for a general 6x6 A by LU with partial pivoting (the factor of
+// lu_inverse6, applied to one right-hand side instead of forming A^-1).
+__device__ __forceinline__ void lu_solve6(const double* Ain, const double* b, double* x) {
+  double A[36], y[6];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) A[i] = Ain[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) y[i] = b[i];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int p = k;
+    double best = fabs(A[k + 6 * k]);
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i)
+      if (fabs(A[i + 6 * k]) > best) { best = fabs(A[i + 6 * k]); p = i; }
+    // predicated swaps with compile-time indices (as in lu_inverse6)
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i)
+      if (p == i) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) { double t = A[k + 6 * j]; A[k + 6 * j] = A[i + 6 * j]; A[i + 6 * j] = t; }
+        double t = y[k]; y[k] = y[i]; y[i] = t;
+      }
+    const double id = 1.0 / A[k + 6 * k];
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      const double l = A[i + 6 * k] * id;
+#pragma unroll
+      for (int j = k + 1; j < 6; ++j) A[i + 6 * j] -= l * A[k + 6 * j];
+      y[i] -= l * y[k];
+    }
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) s -= A[i + 6 * k] * x[k];
+    x[i] = s / A[i + 6 * i];
+  }
+}
+
 // Projected, h^2-scaled material Hessian (full 6x6) from s, in FP64 in every
 // precision mode: the projection clamps eigenvalues to 1e-6 against entries
 // of order lambda + mu (up to ~1e7), a spread FP32 cannot hold, and the local
@@ -154,19 +195,27 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_polar_material(
   for (int i = 0; i < 6; ++i) se[i] = s[i * nT + e];
   double gg[6], Hs[36];
   projected_hessian(m.mat[e], m.mu[e], m.la[e], se, h2, gg, Hs);
-  double Hi[36];
-  lu_inverse6<double>(Hs, Hi);
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = i; j < 6; ++j) {
       const int k = sym6(i, j) * nT + e;
-      const double hi = 0.5 * (Hi[i + 6 * j] + Hi[j + 6 * i]);
       ho.Hpsi[k] = (Real)Hs[i + 6 * j];
-      if (ho.Hinv) ho.Hinv[k] = (Real)hi;
       if (ho.Hloc) ho.Hloc[k] = Hs[i + 6 * j];
-      if (ho.Hlinv) ho.Hlinv[k] = hi;
     }
+  if (ho.Hinv || ho.Hlinv) {  // uniform over the launch
+    double Hi[36];
+    lu_inverse6<double>(Hs, Hi);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = i; j < 6; ++j) {
+        const int k = sym6(i, j) * nT + e;
+        const double hi = 0.5 * (Hi[i + 6 * j] + Hi[j + 6 * i]);
+        if (ho.Hinv) ho.Hinv[k] = (Real)hi;
+        if (ho.Hlinv) ho.Hlinv[k] = hi;
+      }
+  }
   const double syminv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
@@ -483,14 +532,30 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_local_solve(
       l[i] = -gl[i * nT + e] - syminv[i] * a;
       rr[i] = sym[i] * l[i] + g[i * nT + e];
     }
+    // ds = -Hpsi^-1 rr: the stored inverse, or (Hinv null) an LU solve
+    // with the same FP64 matrix
+    double sol[6];
+    if (Hinv) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double a = 0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a += Hinv[sym6(i, j) * nT + e] * rr[j];
+        sol[i] = a;
+      }
+    } else {
+      double Hs[36];
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) Hs[i + 6 * j] = Hpsi[sym6(i, j) * nT + e];
+      lu_solve6(Hs, rr, sol);
+    }
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-      double a = 0;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) a += Hinv[sym6(i, j) * nT + e] * rr[j];
       lam[i * nT + e] = l[i];
-      ds[i * nT + e] = -a;
-      local_max = fmax(local_max, fabs(a));
+      ds[i * nT + e] = -sol[i];
+      local_max = fmax(local_max, fabs(sol[i]));
     }
   }
   block_partial<kElemBlock, 1>(local_max, max_ds.partials, sm);

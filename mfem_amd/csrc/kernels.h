@@ -63,7 +63,8 @@ void launch_mixed_blocks_rhs(const TetMeshDev& m, const double* x_full, const In
                              Real* blocks, double* fe, hipStream_t st, bool mfma = false);
 // Local solve (reference mixed_stretch.cpp:244-259): lambda and ds from dx,
 // with the FP64 Hessian and inverse (Hpsi, Hinv as in HessOut: the Real
-// arrays in FP64 mode, Hloc / Hlinv in FP32 storage mode).
+// arrays in FP64 mode, Hloc in FP32 storage mode, where Hinv is null and the
+// kernel solves with Hloc by LU with partial pivoting).
 template <typename Real>
 void launch_mixed_local_solve(const TetMeshDev& m, const double* dx, const Real* dsdF,
                               const double* Hpsi, const double* Hinv, const double* g,
