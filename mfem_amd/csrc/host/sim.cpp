@@ -441,6 +441,17 @@ bool MixedStretch<DIM>::energy_launch(const double* x_full, const double* dx, do
 }
 
 template <int DIM>
+bool MixedStretch<DIM>::energy_launch2(const double* x_full, const double* dx, double alpha0,
+                                       ReduceSlot out0, double alpha1, ReduceSlot out1) {
+  auto* c = this->ctx_;
+  const auto m = this->mesh_->dev();
+  if (m.dim == 2) return Variable<DIM>::energy_launch2(x_full, dx, alpha0, out0, alpha1, out1);
+  launch_mixed_energy2(m, x_full, dx, alpha0, alpha1, c->inertia(), R_.data(), s_.data(),
+                       ds_.data(), la_.data(), h2_, out0, out1, c->stream);
+  return true;
+}
+
+template <int DIM>
 VariableFactory<DIM>::VariableFactory() {
   this->register_type(VAR_STRETCH, Stretch<DIM>::name(),
                       [](std::shared_ptr<Mesh> m, std::shared_ptr<SimConfig> c,
@@ -1450,6 +1461,28 @@ int NewtonOptimizer<DIM>::launch_energy(double alpha, int base) {
 }
 
 template <int DIM>
+int NewtonOptimizer<DIM>::launch_energy2(double alpha0, int base0, double alpha1, int base1) {
+  auto& S = *this->state_;
+  auto* c = S.ctx_.get();
+  const double* x = S.x_->value().data();
+  const double* dx = S.x_->delta().data();
+  auto& R = c->reducer;
+  int k = 0;
+  for (auto& v : S.mixed_vars_)
+    if (v->energy_launch2(x, dx, alpha0, R.slot(base0 + k), alpha1, R.slot(base1 + k))) ++k;
+  for (auto& v : S.vars_)
+    if (k < 2 && v->energy_launch2(x, dx, alpha0, R.slot(base0 + k), alpha1, R.slot(base1 + k)))
+      ++k;
+  if (S.mixed_vars_.size() + S.vars_.size() > 2)
+    throw std::runtime_error("at most two elastic variables are supported");
+  if (k == 0) {
+    launch_energy(alpha0, base0);
+    return launch_energy(alpha1, base1);
+  }
+  return k;
+}
+
+template <int DIM>
 double NewtonOptimizer<DIM>::energy(double alpha) {
   auto& S = *this->state_;
   double v[2] = {0, 0};
@@ -1511,8 +1544,7 @@ void NewtonOptimizer<DIM>::step() {
     solver_->solve();
     const int nd = launch_decrement();
     MFEM_HIP_CHECK(hipEventRecord(ev[2], st));
-    const int ne = launch_energy(0.0, RS_E0);
-    if (ls_iters > 0) launch_energy(1.0, RS_E1);
+    const int ne = ls_iters > 0 ? launch_energy2(0.0, RS_E0, 1.0, RS_E1) : launch_energy(0.0, RS_E0);
     double v[6] = {0, 0, 0, 0, 0, 0};
     c->reducer.read_range(RS_MAXDX, ls_iters > 0 ? RS_E1 + 2 : RS_E0 + 2, v, st);
     grad_norm = nd == 2 ? std::max(v[0], v[1]) : v[0];

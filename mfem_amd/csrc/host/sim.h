@@ -117,6 +117,14 @@ class Variable {
                              mfemhip::ReduceSlot out) {
     return false;
   }
+  // energies at alpha0 and alpha1 into out0 and out1 (the same values as two
+  // energy_launch calls)
+  virtual bool energy_launch2(const double* x_full, const double* dx, double alpha0,
+                              mfemhip::ReduceSlot out0, double alpha1, mfemhip::ReduceSlot out1) {
+    if (!energy_launch(x_full, dx, alpha0, out0)) return false;
+    energy_launch(x_full, dx, alpha1, out1);
+    return true;
+  }
   std::shared_ptr<Mesh> mesh() const { return mesh_; }
 
  protected:
@@ -193,6 +201,8 @@ class MixedStretch : public MixedVariable<DIM> {
   void solve(const double* dx) override;
   bool energy_launch(const double* x_full, const double* dx, double alpha,
                      mfemhip::ReduceSlot out) override;
+  bool energy_launch2(const double* x_full, const double* dx, double alpha0,
+                      mfemhip::ReduceSlot out0, double alpha1, mfemhip::ReduceSlot out1) override;
   DeviceArray<double>& value() override { return s_; }
   DeviceArray<double>& delta() override { return ds_; }
   DeviceArray<double>& lambda() override { return la_; }
@@ -518,6 +528,9 @@ class NewtonOptimizer : public Optimizer<DIM> {
   // E(alpha) as one partial per elastic variable in reducer slots base, ...;
   // returns the number of slots written (at most 2)
   int launch_energy(double alpha, int base);
+  // E(alpha0) and E(alpha1) into slots base0.. and base1.. in one pass where
+  // a variable supports it (same values as two launch_energy calls)
+  int launch_energy2(double alpha0, int base0, double alpha1, int base1);
   // the decrement's slots after a solve: max |dx| and, with a mixed
   // variable, max |ds| (returns the number of leading slots to read)
   int launch_decrement();

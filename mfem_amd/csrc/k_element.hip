@@ -630,6 +630,30 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_energy(
   block_partial<kElemBlock, 0>(en, out.partials, sm);
 }
 
+// E(alpha0) and E(alpha1) of the line search's first two trials in one
+// launch: a workgroup holds the 128 tets of one k_mixed_energy workgroup
+// twice, waves 0-1 at alpha0 and waves 2-3 at alpha1, so the second pair
+// reads the tets' data from the CU's caches, and each half is reduced as
+// k_mixed_energy reduces its workgroup: both sums are bit-identical to two
+// separate launches.
+__global__ __launch_bounds__(2 * kElemBlock) void k_mixed_energy2(
+    TetMeshDev m, const double* __restrict__ x, const double* __restrict__ dx, double alpha0,
+    double alpha1, Inertia in, const double* __restrict__ R, const double* __restrict__ s,
+    const double* __restrict__ ds, const double* __restrict__ lam, double h2, ReduceSlot out0,
+    ReduceSlot out1) {
+  static_assert(kElemBlock == 2 * kWave, "each half of the workgroup is two waves");
+  __shared__ double sm[2 * kElemBlock / kWave];
+  const int half = threadIdx.x / kElemBlock;
+  const int e = blockIdx.x * kElemBlock + (threadIdx.x - half * kElemBlock);
+  double en = 0.0;
+  if (e < m.nT) en = mixed_energy_tet(m, e, x, dx, half ? alpha1 : alpha0, in, R, s, ds, lam, h2);
+  en = wave_sum(en);
+  if ((threadIdx.x & (kWave - 1)) == 0) sm[threadIdx.x / kWave] = en;
+  __syncthreads();
+  if (threadIdx.x == 0) out0.partials[blockIdx.x] = sm[0] + sm[1];
+  if (threadIdx.x == kElemBlock) out1.partials[blockIdx.x] = sm[2] + sm[3];
+}
+
 // Batched simulations: energy of every tet with the step length of its
 // simulation (tets of simulations that are off get 0).
 __global__ __launch_bounds__(kElemBlock) void k_mixed_energy_batch(
@@ -1402,6 +1426,16 @@ void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double
   hipLaunchKernelGGL(k_mixed_energy, dim3(reduce_grid(m.nT)), dim3(kElemBlock), 0, st, m, x_full,
                      dx, alpha, in, R, s, ds, lam, h2, out);
   launch_finalize(out.partials, reduce_grid(m.nT), 0, out.result, st);
+}
+
+void launch_mixed_energy2(const TetMeshDev& m, const double* x_full, const double* dx,
+                          double alpha0, double alpha1, const Inertia& in, const double* R,
+                          const double* s, const double* ds, const double* lam, double h2,
+                          ReduceSlot out0, ReduceSlot out1, hipStream_t st) {
+  hipLaunchKernelGGL(k_mixed_energy2, dim3(reduce_grid(m.nT)), dim3(2 * kElemBlock), 0, st, m,
+                     x_full, dx, alpha0, alpha1, in, R, s, ds, lam, h2, out0, out1);
+  launch_finalize(out0.partials, reduce_grid(m.nT), 0, out0.result, st);
+  launch_finalize(out1.partials, reduce_grid(m.nT), 0, out1.result, st);
 }
 
 template <typename Real>

@@ -77,6 +77,12 @@ void launch_mixed_energy(const TetMeshDev& m, const double* x_full, const double
                          double alpha, const Inertia& in, const double* R, const double* s,
                          const double* ds, const double* lam, double h2, ReduceSlot out,
                          hipStream_t st);
+// E(alpha0) into out0 and E(alpha1) into out1 in one pass, each bit-identical
+// to launch_mixed_energy's
+void launch_mixed_energy2(const TetMeshDev& m, const double* x_full, const double* dx,
+                          double alpha0, double alpha1, const Inertia& in, const double* R,
+                          const double* s, const double* ds, const double* lam, double h2,
+                          ReduceSlot out0, ReduceSlot out1, hipStream_t st);
 
 // --- plain stretch (reference src/variables/stretch.cpp) -------------------
 template <typename Real>

@@ -146,6 +146,25 @@ def test_update_solve_energy_phases(native, mixed):
 
 # ------------------------------------------------------------------ whole steps
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_line_search_energies_match_single_evaluations(native, precision):
+    """The Newton loop evaluates E(0) and E(1) in one launch; the phases
+    driven one at a time through the C ABI evaluate E(alpha) alone. The
+    first Newton iteration of a step starts from the same state, so the
+    recorded E(0) and the ABI's energy(0) must be bit-identical, and E(1)
+    decides the line search the same way (no halving when E(1) < E(0))."""
+    sc = scenes.beam_scene(5, 3, 2, seed=4, mixed=True)
+    a = native.Sim(sc, precision=precision)
+    b = native.Sim(sc, precision=precision)
+    a.update_system()
+    a.solve()
+    e0, e1 = a.energy(0.0), a.energy(1.0)
+    b.step()
+    h = b.newton_history()
+    assert h[0, 1] == e0
+    assert (h[0, 9] == 0) == (e1 < e0)
+
+
 @pytest.mark.parametrize("mixed", [True, False])
 def test_trajectory_fp64(native, mixed):
     sc = scenes.beam_scene(8, 3, 3, seed=7, mixed=mixed, E_range=(1e5, 1e6), itr_tol=1e-12,
