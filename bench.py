@@ -339,9 +339,11 @@ def run_c4(args):
     from oracle import oracle as O
     scs = scenes.config_c4(args.beams)
     nT = np.array([sc.nT for sc in scs])
-    mixed = run_batch(scs, "mixed", args.steps, args.warmup)
-    eager = run_batch(scs, "mixed", args.steps, args.warmup, graphs=False, shape=mixed["shape"])
-    fp64 = None if args.no_fp64 else run_batch(scs, "fp64", args.steps, args.warmup)
+    # the library's default (eager launches) is the value; the hipGraph replay
+    # of the same batch with the same CG shape is reported beside it
+    mixed = run_batch(scs, "mixed", args.steps, args.warmup, graphs=False)
+    graph = run_batch(scs, "mixed", args.steps, args.warmup, graphs=True, shape=mixed["shape"])
+    fp64 = None if args.no_fp64 else run_batch(scs, "fp64", args.steps, args.warmup, graphs=False)
     value = mixed["work"] / mixed["elapsed"]
     out = {
         "metric": "Newton-steps*tets/s at 1M tets, FP32-mixed vs FP64 GPU; PCG HBM GB/s",
@@ -356,15 +358,15 @@ def run_c4(args):
                    "beams": len(scs), "tets": int(nT.sum()), "precision": "mixed"},
         "pcg_iters": mixed["pcg"], "cg_shape_variant_rows_per_workgroup": list(mixed["shape"]),
         "graphs": {
-            "eager_value": eager["work"] / eager["elapsed"],
-            "eager_ms_per_step": 1000.0 * eager["elapsed"] / args.steps,
+            "graph_value": graph["work"] / graph["elapsed"],
+            "graph_ms_per_step": 1000.0 * graph["elapsed"] / args.steps,
             "bit_identical_to_eager": all(np.array_equal(a, b) for a, b in
-                                          zip(mixed["positions"], eager["positions"])),
+                                          zip(graph["positions"], mixed["positions"])),
             "submissions_per_newton_iteration": {
-                "update_phase": {"graphs": mixed["counts"][0] / mixed["counts"][2],
-                                 "eager": eager["counts"][0] / eager["counts"][2]},
-                "cg": {"graphs": mixed["counts"][1] / mixed["counts"][2],
-                       "eager": eager["counts"][1] / eager["counts"][2]}}},
+                "update_phase": {"graphs": graph["counts"][0] / graph["counts"][2],
+                                 "eager": mixed["counts"][0] / mixed["counts"][2]},
+                "cg": {"graphs": graph["counts"][1] / graph["counts"][2],
+                       "eager": mixed["counts"][1] / mixed["counts"][2]}}},
     }
     if fp64 is not None:
         v64 = fp64["work"] / fp64["elapsed"]

@@ -283,6 +283,44 @@ void BatchSim::run_cg(const double* b, double* x, const std::vector<double>& tol
   auto* snap = pinned_ctrl_->as<PcgCtrl>();
   long launched = 0;
   int issued = 0, waited = 0;
+  // Eager launches run only the workgroups of systems that had not stopped
+  // at the last poll: a launch late in a solve, when a few slow systems are
+  // left, dispatches their workgroups instead of the whole batch's. Every
+  // workgroup of a listed system runs, so each system's iteration is the
+  // same kernel work as with the full grid.
+  BatchCgBuffers<T> Bc = B;
+  if (!graphs_) {
+    for (auto& d : d_active_) d.resize(nwg_);
+    if (pinned_active_n_ != nwg_) {
+      pinned_active_ = std::make_unique<PinnedBuffer>(3 * sizeof(int) * (size_t)nwg_);
+      pinned_active_n_ = nwg_;
+    }
+  }
+  int list_slot = -1;
+  long retired_at[3] = {0, 0, 0};  // chunks issued before a list was replaced
+  auto compact = [&](const PcgCtrl* c) {
+    if (graphs_) return;
+    const int cur = Bc.active ? Bc.nactive : nwg_;
+    int count = 0;
+    for (int k = 0; k < nsim_; ++k)
+      if (!c[k].done) count += wg0_[k + 1] - wg0_[k];
+    if (count == 0 || count > cur * 3 / 4) return;
+    const int slot = (list_slot + 1) % 3;
+    // every chunk that read this slot's previous list has completed, and so
+    // has the chunk after its copy (so the copy has read its pinned source)
+    if (waited <= retired_at[slot]) return;
+    int* h = pinned_active_->as<int>() + (size_t)slot * nwg_;
+    int n = 0;
+    for (int k = 0; k < nsim_; ++k)
+      if (!c[k].done)
+        for (int w = wg0_[k]; w < wg0_[k + 1]; ++w) h[n++] = w;
+    MFEM_HIP_CHECK(hipMemcpyAsync(d_active_[slot].data(), h, sizeof(int) * (size_t)n,
+                                  hipMemcpyHostToDevice, st));
+    if (list_slot >= 0) retired_at[list_slot] = issued;
+    list_slot = slot;
+    Bc.active = d_active_[slot].data();
+    Bc.nactive = n;
+  };
   // graph replay: pairs of iterations; a pair past the stop exits at once,
   // like any launch queued after it
   hipGraphExec_t pair = nullptr;
@@ -299,7 +337,7 @@ void BatchSim::run_cg(const double* b, double* x, const std::vector<double>& tol
       launched += per * graphs;
       subs_cg_ += graphs;
     } else {
-      for (long j = 0; j < chunk; ++j) launch_pcg_batch_iteration(B, (int)(launched + j), st);
+      for (long j = 0; j < chunk; ++j) launch_pcg_batch_iteration(Bc, (int)(launched + j), st);
       launched += chunk;
       subs_cg_ += chunk;
     }
@@ -324,6 +362,7 @@ void BatchSim::run_cg(const double* b, double* x, const std::vector<double>& tol
       break;
     }
     if (waited == issued) throw std::runtime_error("batched CG control blocks did not terminate");
+    compact(c);
   }
   iters.assign(nsim_, 0);
   for (int k = 0; k < nsim_; ++k) iters[k] = on[k] ? ctrl_host_[k].iter : 0;

@@ -105,6 +105,12 @@ class BatchSim {
   std::vector<mfemhip::PcgCtrl> ctrl_host_;
   std::unique_ptr<PinnedBuffer> pinned_ctrl_;  // two snapshots of all control blocks
   hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  // Compacted lists of the workgroups of the systems still iterating (eager
+  // CG only), a ring of three (device lists and their pinned sources) so
+  // that a list is rewritten only after the launches that read it have run.
+  DeviceArray<int> d_active_[3];
+  std::unique_ptr<PinnedBuffer> pinned_active_;
+  int pinned_active_n_ = 0;
   // line search / decrement
   DeviceArray<double> e_tet_, d_alpha_, d_alpha1_, d_res_;
   DeviceArray<int> d_on1_;

@@ -640,11 +640,14 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
     const T* __restrict__ val, const CgDof<T>* __restrict__ s_old, CgDof<T>* __restrict__ s_new,
     double* __restrict__ x, PcgCtrl* ctrls, const double* __restrict__ parts_old,
     double* __restrict__ parts_new, int nwg, const int* __restrict__ wg_sys,
-    const int* __restrict__ wg0, const int* __restrict__ row0, const int* it_in, int* it_out) {
+    const int* __restrict__ wg0, const int* __restrict__ row0, const int* __restrict__ active,
+    const int* it_in, int* it_out) {
   __shared__ double sm[kCgParts * (BS / kWave)];
   const int it = it_in ? *it_in : it_arg;
   if (it_out && blockIdx.x == 0 && threadIdx.x == 0) *it_out = it + 1;
-  const int b = wg_sys[blockIdx.x];
+  // the workgroup this block runs: itself, or the active list's entry
+  const int wg = active ? active[blockIdx.x] : (int)blockIdx.x;
+  const int b = wg_sys[wg];
   PcgCtrl* ctrl = ctrls + b;
   // one read of the stop flag per workgroup (see k_cg_iter)
   __shared__ int s_done;
@@ -660,7 +663,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
     const int j = it - 1;
     const bool converged = rr < ctrl->threshold;
     if (converged || j >= ctrl->max_iters) {
-      if (blockIdx.x == w0 && threadIdx.x == 0) {
+      if (wg == w0 && threadIdx.x == 0) {
         ctrl->rr = rr;
         ctrl->iter = converged ? (j > 0 ? j - 1 : 0) : j;
         ctrl->stop_it = it;
@@ -670,11 +673,11 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
     }
     alpha = rz / pq;
     beta = (rz - 2.0 * alpha * zq + alpha * alpha * qdq) / rz;
-    if (blockIdx.x == w0 && threadIdx.x == 0) ctrl->rr = rr;
+    if (wg == w0 && threadIdx.x == 0) ctrl->rr = rr;
   }
   const T a = (T)alpha, bt = (T)beta;
   const int lane = threadIdx.x & (G - 1);
-  const int lb = blockIdx.x - w0;
+  const int lb = wg - w0;
   const long long r0 = row0[b], r1 = row0[b + 1];
   const int c0 = (int)(r0 + (r1 - r0) * lb / nw);
   const int c1 = (int)(r0 + (r1 - r0) * (lb + 1) / nw);
@@ -714,7 +717,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
   block_sum_n<BS, kCgParts>(acc, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < kCgParts; ++k) parts_new[(size_t)k * nwg + blockIdx.x] = acc[k];
+    for (int k = 0; k < kCgParts; ++k) parts_new[(size_t)k * nwg + wg] = acc[k];
   }
 }
 
@@ -969,6 +972,9 @@ template <typename T>
 void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t st,
                                 const int* it_in, int* it_out) {
   if (B.nwg <= 0) return;
+  if (B.active && (B.nactive <= 0 || B.nactive > B.nwg))
+    throw std::invalid_argument("batched CG: active workgroup count out of range");
+  const int grid = B.active ? B.nactive : B.nwg;
   // with it_in set, `it` only selects the buffer parity
   const auto* so = reinterpret_cast<const CgDof<T>*>(B.state[it & 1]);
   auto* sn = reinterpret_cast<CgDof<T>*>(B.state[(it + 1) & 1]);
@@ -978,8 +984,9 @@ void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t 
 #define MFEM_CGB(i)                                                                             \
   case i:                                                                                      \
     hipLaunchKernelGGL((k_cg_iter_batch<kCgShapes[i].G, kCgShapes[i].U, kCgShapes[i].BS, T>),    \
-                       dim3(B.nwg), dim3(kCgShapes[i].BS), 0, st, it, B.rowptr, B.col, B.K, so,  \
-                       sn, B.x, B.ctrls, po, pn, B.nwg, B.wg_sys, B.wg0, B.row0, it_in, it_out); \
+                       dim3(grid), dim3(kCgShapes[i].BS), 0, st, it, B.rowptr, B.col, B.K, so,   \
+                       sn, B.x, B.ctrls, po, pn, B.nwg, B.wg_sys, B.wg0, B.row0, B.active, it_in, \
+                       it_out);                                                                \
     break;
     MFEM_CGB(0) MFEM_CGB(1) MFEM_CGB(2) MFEM_CGB(3) MFEM_CGB(4) MFEM_CGB(5) MFEM_CGB(6) MFEM_CGB(7)
 #undef MFEM_CGB

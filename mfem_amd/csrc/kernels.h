@@ -224,6 +224,10 @@ struct BatchCgBuffers {
   const int* wg_sys = nullptr;  // [nwg] system of each workgroup
   const int* wg0 = nullptr;     // [nsys + 1] first workgroup of each system
   const int* row0 = nullptr;    // [nsys + 1] first block row of each system
+  // optional: the workgroups to run ([nactive], every workgroup of each
+  // system still iterating); null runs all nwg
+  const int* active = nullptr;
+  int nactive = 0;
   int variant = 0;
 };
 // x = 0 and the initial state for all rows (scratch: >= 256 doubles)
