@@ -121,13 +121,29 @@ def run_gpu(scene, precision, steps, warmup, profile=True, element_kernel=0):
                 tuning=tuning, ls_fail=ls_fail, halvings=halvings)
 
 
-def fp64_spread_run(scene, steps, warmup, tuned_shape):
-    """The FP64 run again with a different CG kernel shape (so its dot
-    products are summed in another order), untimed: its final positions give
-    the trajectory's sensitivity to rounding."""
+# CG kernel shapes (lanes per block row, blocks in flight, workgroup size)
+# by variant index, as k_solver.hip lists them
+CG_SHAPES = [(4, 4, 256), (4, 4, 512), (8, 2, 256), (8, 2, 1024), (4, 2, 256), (4, 2, 1024),
+             (16, 1, 1024), (8, 1, 256)]
+
+
+def alternative_shapes(tuned_shape, n=2):
+    """n CG kernel shapes other than the tuned one (other lanes per row and
+    workgroup sizes, so the dot products are summed in other orders)."""
+    out = []
+    for v in (3, 6, 0, 2):
+        if CG_SHAPES[v] != tuple(tuned_shape[:3]) and len(out) < n:
+            out.append(v)
+    return out
+
+
+def fp64_spread_run(scene, steps, warmup, variant):
+    """The FP64 run again with CG kernel shape `variant` (its dot products
+    summed in another order), untimed: its final positions give the
+    trajectory's sensitivity to rounding."""
     from mfem_amd import native
     sim = native.Sim(scene, precision="fp64")
-    sim.set_cg_shape(3 if tuple(tuned_shape[:3]) != (4, 4, 256) else 0, 0)
+    sim.set_cg_shape(variant, 0)
     for _ in range(warmup + steps):
         sim.step()
     x = sim.positions()
@@ -490,9 +506,16 @@ def main():
         # rounding floor: FP64 against FP64 with other dot-product orders
         tuned = (fp64["tuning"]["lanes_per_row"], fp64["tuning"]["blocks_in_flight"],
                  fp64["tuning"]["workgroup"]) if fp64["tuning"] else (4, 4, 256)
-        xb = fp64_spread_run(scene, args.steps, args.warmup, tuned)
-        spread = float(np.linalg.norm(xb - x64) / np.linalg.norm(x64))
+        # two more FP64 runs with other shapes: one pair is one sample of the
+        # trajectory's rounding sensitivity; the largest of the three pairs
+        # is the spread
+        runs = [x64] + [fp64_spread_run(scene, args.steps, args.warmup, v)
+                        for v in alternative_shapes(tuned)]
+        pairs = [float(np.linalg.norm(a - b) / np.linalg.norm(x64))
+                 for i, a in enumerate(runs) for b in runs[i + 1:]]
+        spread = max(pairs)
         extra["fp64_vs_fp64_rounding_spread"] = spread
+        extra["fp64_vs_fp64_pairs"] = pairs
         bar = max(1e-6, 3.0 * spread)
         extra["mixed_error_bar"] = bar
         # the bar is the precision config's (c3); c2 and c5 report
