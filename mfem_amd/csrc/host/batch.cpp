@@ -151,7 +151,7 @@ hipGraphExec_t BatchSim::cg_pair_graph(const BatchCgBuffers<T>& B) {
 
 // update_system (reference newton_optimizer.cpp:92-110) and the assembly of
 // SystemMatrixPD::pre_solve (linear_system.h:18-31). From the second step on,
-// with graphs enabled, the five launches are one captured graph, re-captured
+// with graphs enabled, the four launches are one captured graph, re-captured
 // when a buffer or the time step changed.
 void BatchSim::update_and_assemble(double dt) {
   auto& S = *state_;
@@ -160,7 +160,7 @@ void BatchSim::update_and_assemble(double dt) {
   if (!graphs_ || steps_done_ == 0) {
     mixed_->update(S.x_->value().data(), dt);
     system_.pre_solve(&S);
-    subs_update_ += 5;
+    subs_update_ += 4;
     return;
   }
   const std::vector<const void*> sig = {

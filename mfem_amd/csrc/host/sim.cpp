@@ -480,21 +480,24 @@ void SystemMatrixPD<DIM>::pre_solve(SimState<DIM>* state) {
   K.resize((size_t)m.nnzb() * 9, f32);
   invdiag.resize(3 * (size_t)m.nfree(), f32);
   rhs.resize(3 * (size_t)m.nfree());
+  // K and rhs gathers in one launch, then the Jacobi diagonal of K
   if (f32) {
-    launch_assemble_bsr<float>(m.nupper(), m.d_upper.data(), m.d_tpos.data(), m.d_slot_ptr.data(),
-                               m.d_slots.data(), m.d_mass.data(),
-                               c->blocks.f32(), K.f32(), c->stream);
+    launch_assemble_bsr_rhs<float>(m.nupper(), m.d_upper.data(), m.d_tpos.data(),
+                                   m.d_slot_ptr.data(), m.d_slots.data(), m.d_mass.data(),
+                                   c->blocks.f32(), K.f32(), m.nfree(), m.d_asm_rows.data(),
+                                   m.d_inc_ptr.data(), m.d_inc.data(), c->fe.data(), rhs.data(),
+                                   c->stream);
     launch_jacobi_diag<float, float>(m.nfree(), m.d_diag_block.data(), K.f32(),
                                      invdiag.f32(), c->stream);
   } else {
-    launch_assemble_bsr<double>(m.nupper(), m.d_upper.data(), m.d_tpos.data(), m.d_slot_ptr.data(),
-                                m.d_slots.data(), m.d_mass.data(),
-                                c->blocks.f64(), K.f64(), c->stream);
+    launch_assemble_bsr_rhs<double>(m.nupper(), m.d_upper.data(), m.d_tpos.data(),
+                                    m.d_slot_ptr.data(), m.d_slots.data(), m.d_mass.data(),
+                                    c->blocks.f64(), K.f64(), m.nfree(), m.d_asm_rows.data(),
+                                    m.d_inc_ptr.data(), m.d_inc.data(), c->fe.data(), rhs.data(),
+                                    c->stream);
     launch_jacobi_diag<double, double>(m.nfree(), m.d_diag_block.data(), K.f64(),
                                        invdiag.f64(), c->stream);
   }
-  launch_assemble_rhs(m.nfree(), m.d_asm_rows.data(), m.d_inc_ptr.data(), m.d_inc.data(),
-                      c->fe.data(), rhs.data(), c->stream);
 }
 
 template <int DIM>

@@ -36,12 +36,13 @@ __device__ __forceinline__ constexpr int blk_index(int k, int l) {
 // runs on XCD b % 8), each walked in order by that XCD's workgroups; -1 past
 // the range. Rows that share tets then share one L2. The grid is
 // xcd_grid(n, bs).
-__device__ __forceinline__ int xcd_item(int n) {
-  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3;
+__device__ __forceinline__ int xcd_item(int n, int block) {
+  const int xcd = block & 7, lb = block >> 3;
   const long long start = (long long)n * xcd / 8, end = (long long)n * (xcd + 1) / 8;
   const long long i = start + (long long)lb * blockDim.x + threadIdx.x;
   return i < end ? (int)i : -1;
 }
+__device__ __forceinline__ int xcd_item(int n) { return xcd_item(n, (int)blockIdx.x); }
 inline int xcd_grid(int n, int bs) {
   const int per_xcd = (n + 7) / 8;
   return 8 * std::max(1, (per_xcd + bs - 1) / bs);
@@ -57,15 +58,13 @@ inline int xcd_grid(int n, int bs) {
 // halves are what summing them separately gives, and each element block is
 // read once instead of twice.
 template <typename Real>
-__global__ __launch_bounds__(kBS) void k_assemble_bsr(int nub, const int* __restrict__ upper,
-                                                      const int* __restrict__ tpos,
-                                                      const int* __restrict__ slot_ptr,
-                                                      const int* __restrict__ slots,
-                                                      const double* __restrict__ mass,
-                                                      const Real* __restrict__ blocks,
-                                                      Real* __restrict__ K) {
-  const int t = xcd_item(4 * nub);
-  if (t < 0) return;
+__device__ __forceinline__ void assemble_bsr_item(int t, const int* __restrict__ upper,
+                                                  const int* __restrict__ tpos,
+                                                  const int* __restrict__ slot_ptr,
+                                                  const int* __restrict__ slots,
+                                                  const double* __restrict__ mass,
+                                                  const Real* __restrict__ blocks,
+                                                  Real* __restrict__ K) {
   const int q = t & 3;
   const int b = upper[t >> 2];
   const int bt = tpos[t >> 2];
@@ -112,6 +111,18 @@ __global__ __launch_bounds__(kBS) void k_assemble_bsr(int nub, const int* __rest
     }
 }
 
+template <typename Real>
+__global__ __launch_bounds__(kBS) void k_assemble_bsr(int nub, const int* __restrict__ upper,
+                                                      const int* __restrict__ tpos,
+                                                      const int* __restrict__ slot_ptr,
+                                                      const int* __restrict__ slots,
+                                                      const double* __restrict__ mass,
+                                                      const Real* __restrict__ blocks,
+                                                      Real* __restrict__ K) {
+  const int t = xcd_item(4 * nub);
+  if (t >= 0) assemble_bsr_item<Real>(t, upper, tpos, slot_ptr, slots, mass, blocks, K);
+}
+
 template <typename Real, typename VT>
 __global__ void k_jacobi_diag(int nfree, const int* __restrict__ diag_block, const Real* __restrict__ K,
                               VT* __restrict__ invdiag) {
@@ -127,12 +138,10 @@ __global__ void k_jacobi_diag(int nfree, const int* __restrict__ diag_block, con
 
 // Rows are visited in the order `rows` lists them (each XCD a contiguous
 // part of the list).
-__global__ void k_assemble_rhs(int nfree, const int* __restrict__ rows,
-                               const int* __restrict__ inc_ptr, const int* __restrict__ inc,
-                               const double* __restrict__ fe, double* __restrict__ rhs) {
-  const int item = xcd_item(nfree);
-  if (item < 0) return;
-  const int i = rows[item];
+__device__ __forceinline__ void assemble_rhs_row(int i, const int* __restrict__ inc_ptr,
+                                                 const int* __restrict__ inc,
+                                                 const double* __restrict__ fe,
+                                                 double* __restrict__ rhs) {
   double a0 = 0, a1 = 0, a2 = 0;
   const int p1 = inc_ptr[i + 1];
   // incidences in batches: ids, then element values, in flight together
@@ -162,6 +171,33 @@ __global__ void k_assemble_rhs(int nfree, const int* __restrict__ rows,
   rhs[3 * i] = a0;
   rhs[3 * i + 1] = a1;
   rhs[3 * i + 2] = a2;
+}
+
+__global__ void k_assemble_rhs(int nfree, const int* __restrict__ rows,
+                               const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+                               const double* __restrict__ fe, double* __restrict__ rhs) {
+  const int item = xcd_item(nfree);
+  if (item >= 0) assemble_rhs_row(rows[item], inc_ptr, inc, fe, rhs);
+}
+
+// Both gathers in one launch: workgroups [0, nb_bsr) assemble K, the rest
+// the right-hand side. Neither is bound by bandwidth (each waits on its
+// id -> value load chains), so the rhs gather runs in the K gather's
+// shadow. nb_bsr is a multiple of 8, so both parts keep their XCD mapping.
+template <typename Real>
+__global__ __launch_bounds__(kBS) void k_assemble_bsr_rhs(
+    int nub, const int* __restrict__ upper, const int* __restrict__ tpos,
+    const int* __restrict__ slot_ptr, const int* __restrict__ slots,
+    const double* __restrict__ mass, const Real* __restrict__ blocks, Real* __restrict__ K,
+    int nb_bsr, int nfree, const int* __restrict__ rows, const int* __restrict__ inc_ptr,
+    const int* __restrict__ inc, const double* __restrict__ fe, double* __restrict__ rhs) {
+  if ((int)blockIdx.x < nb_bsr) {
+    const int t = xcd_item(4 * nub, (int)blockIdx.x);
+    if (t >= 0) assemble_bsr_item<Real>(t, upper, tpos, slot_ptr, slots, mass, blocks, K);
+  } else {
+    const int item = xcd_item(nfree, (int)blockIdx.x - nb_bsr);
+    if (item >= 0) assemble_rhs_row(rows[item], inc_ptr, inc, fe, rhs);
+  }
 }
 
 
@@ -897,6 +933,19 @@ template void launch_bsr_to_dense<float>(int, const int*, const int*, const floa
 template void launch_bsr_to_dense<double>(int, const int*, const int*, const double*, double*,
                                           long, hipStream_t);
 
+template <typename Real>
+void launch_assemble_bsr_rhs(int nub, const int* upper, const int* tpos, const int* slot_ptr,
+                             const int* slots, const double* mass, const Real* blocks, Real* K,
+                             int nfree, const int* rows, const int* inc_ptr, const int* inc,
+                             const double* fe, double* rhs, hipStream_t st) {
+  const int nb_bsr = nub > 0 ? xcd_grid(4 * nub, kBS) : 0;
+  const int nb_rhs = nfree > 0 ? xcd_grid(nfree, kBS) : 0;
+  if (nb_bsr + nb_rhs == 0) return;
+  hipLaunchKernelGGL(k_assemble_bsr_rhs<Real>, dim3(nb_bsr + nb_rhs), dim3(kBS), 0, st, nub,
+                     upper, tpos, slot_ptr, slots, mass, blocks, K, nb_bsr, nfree, rows, inc_ptr,
+                     inc, fe, rhs);
+}
+
 void launch_assemble_rhs(int nfree, const int* rows, const int* inc_ptr, const int* inc,
                          const double* fe, double* rhs, hipStream_t st) {
   if (nfree == 0) return;
@@ -1014,6 +1063,14 @@ template void launch_assemble_bsr<double>(int, const int*, const int*, const int
                                           const double*, const double*, double*, hipStream_t);
 template void launch_assemble_bsr<float>(int, const int*, const int*, const int*, const int*,
                                          const double*, const float*, float*, hipStream_t);
+template void launch_assemble_bsr_rhs<double>(int, const int*, const int*, const int*,
+                                              const int*, const double*, const double*, double*,
+                                              int, const int*, const int*, const int*,
+                                              const double*, double*, hipStream_t);
+template void launch_assemble_bsr_rhs<float>(int, const int*, const int*, const int*,
+                                             const int*, const double*, const float*, float*,
+                                             int, const int*, const int*, const int*,
+                                             const double*, double*, hipStream_t);
 template void launch_jacobi_diag<double, double>(int, const int*, const double*, double*, hipStream_t);
 template void launch_jacobi_diag<float, float>(int, const int*, const float*, float*, hipStream_t);
 template void launch_spmv<double, double>(int, const int*, const int*, const double*, const double*,

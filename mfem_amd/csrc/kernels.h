@@ -141,6 +141,12 @@ void launch_bsr_to_dense(int nrows, const int* rowptr, const int* col, const Rea
 // rows: the free rows in visiting order (a permutation of 0 .. nfree-1)
 void launch_assemble_rhs(int nfree, const int* rows, const int* inc_ptr, const int* inc,
                          const double* fe, double* rhs, hipStream_t st);
+// launch_assemble_bsr and launch_assemble_rhs as one launch (same results)
+template <typename Real>
+void launch_assemble_bsr_rhs(int nub, const int* upper, const int* tpos, const int* slot_ptr,
+                             const int* slots, const double* mass, const Real* blocks, Real* K,
+                             int nfree, const int* rows, const int* inc_ptr, const int* inc,
+                             const double* fe, double* rhs, hipStream_t st);
 
 // --- BSR 3x3 SpMV / Jacobi PCG (reference Eigen ConjugateGradient semantics)
 // Control block of one CG solve (device memory, read by the host at polls).
