@@ -729,11 +729,17 @@ void PcgSolver<DIM>::solve() {
   system_.pre_solve(this->state_);
   const int n = 3 * m.nfree();
   const bool f32 = c->fp32();
+  // the solution is written straight into the displacement step, so
+  // post_solve copies nothing (x_ is the autotuner's scratch solution)
+  auto& dx = this->state_->x_->delta();
+  if ((long)dx.size() != (long)n) throw std::logic_error("displacement step has the wrong size");
+  double* xo = dx.data();
   x_.resize(n);
   q_.resize(n, f32);
   state0_.resize(4 * (size_t)n, f32);
   state1_.resize(4 * (size_t)n, f32);
   this->stats_ = SolveStats();
+  rf_ = Refinement();
   if (variant_ < 0) {
     if (f32) tune_spmv<float>();
     else tune_spmv<double>();
@@ -742,9 +748,8 @@ void PcgSolver<DIM>::solve() {
     if (f32) tune_cg<float>();
     else tune_cg<double>();
   }
-  auto& R = c->reducer;
   if (!f32) {
-    this->stats_.iterations = run_cg<double>(system_.rhs.data(), x_.data(), cfg.itr_tol,
+    this->stats_.iterations = run_cg<double>(system_.rhs.data(), xo, cfg.itr_tol,
                                              cfg.max_iterative_solver_iters);
     this->stats_.refinements = 1;
     this->stats_.rel_residual = last_.b2 > 0 ? std::sqrt(last_.rr / last_.b2) : 0.0;
@@ -756,38 +761,71 @@ void PcgSolver<DIM>::solve() {
     // budget, or after max_refinements sweeps.
     r64_.resize(n);
     dx64_.resize(n);
-    int budget = cfg.max_iterative_solver_iters;
-    int sweeps = 0;
-    double b2 = 0, r2 = 0;
+    rf_.budget = cfg.max_iterative_solver_iters;
     if (cfg.max_refinements > 0) {
       // sqrt(threshold / |r|^2) = tol while r = b
       const double inner = std::min(std::max(cfg.inner_tol, 0.5 * cfg.itr_tol), 0.5);
-      const int it = run_cg<float>(system_.rhs.data(), x_.data(), inner, budget);
-      budget -= std::max(it, 1);
+      const int it = run_cg<float>(system_.rhs.data(), xo, inner, rf_.budget);
+      rf_.budget -= std::max(it, 1);
       this->stats_.iterations += it;
-      b2 = last_.b2;
-      r2 = last_.rr;
-      ++sweeps;
+      rf_.b2 = last_.b2;
+      rf_.r2 = last_.rr;
+      ++rf_.sweeps;
     } else {
-      x_.zero(st);
+      dx.zero(st);
     }
-    const double thr = std::max(cfg.itr_tol * cfg.itr_tol * b2, 2.2250738585072014e-308);
-    while (b2 > 0) {
-      launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
-                             x_.data(), system_.rhs.data(), r64_.data(), R.slot(RS_RES), st);
-      r2 = R.read(RS_RES, st);
-      if (!(r2 >= thr) || budget <= 0 || sweeps >= cfg.max_refinements) break;
-      const double inner = std::max(cfg.inner_tol, std::sqrt(thr / r2) * 0.5);
-      const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), budget);
-      budget -= std::max(it, 1);
-      this->stats_.iterations += it;
-      launch_axpy(n, 1.0, dx64_.data(), x_.data(), st);
-      ++sweeps;
+    rf_.thr = std::max(cfg.itr_tol * cfg.itr_tol * rf_.b2, 2.2250738585072014e-308);
+    if (rf_.b2 > 0) {
+      launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(), xo,
+                             system_.rhs.data(), r64_.data(), c->reducer.slot(RS_RES), st);
+      if (this->defer_) rf_.pending = true;
+      else refine(xo);
     }
-    this->stats_.refinements = sweeps;
-    this->stats_.rel_residual = b2 > 0 ? std::sqrt(r2 / b2) : 0.0;
+    this->stats_.refinements = rf_.sweeps;
+    this->stats_.rel_residual = rf_.b2 > 0 ? std::sqrt(rf_.r2 / rf_.b2) : 0.0;
   }
-  system_.post_solve(this->state_, x_.data());
+  system_.post_solve(this->state_, xo);
+}
+
+// The defect-correction loop from the residual check on: reads |r|^2 of the
+// last residual launch and runs further sweeps while Eigen's rule fails.
+template <int DIM>
+void PcgSolver<DIM>::refine(double* xo) {
+  auto* c = this->state_->ctx_.get();
+  auto& m = *this->state_->mesh_;
+  auto& cfg = *this->state_->config_;
+  hipStream_t st = c->stream;
+  const int n = 3 * m.nfree();
+  for (;;) {
+    rf_.r2 = c->reducer.read(RS_RES, st);
+    if (!(rf_.r2 >= rf_.thr) || rf_.budget <= 0 || rf_.sweeps >= cfg.max_refinements) break;
+    const double inner = std::max(cfg.inner_tol, std::sqrt(rf_.thr / rf_.r2) * 0.5);
+    const int it = run_cg<float>(r64_.data(), dx64_.data(), std::min(inner, 0.5), rf_.budget);
+    rf_.budget -= std::max(it, 1);
+    this->stats_.iterations += it;
+    launch_axpy(n, 1.0, dx64_.data(), xo, st);
+    ++rf_.sweeps;
+    launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(), xo,
+                           system_.rhs.data(), r64_.data(), c->reducer.slot(RS_RES), st);
+  }
+}
+
+// A deferred solve ends here: the residual check that solve() queued is
+// read now, behind whatever the caller queued meanwhile. More sweeps change
+// the solution; post_solve then runs again and the caller's work has to be
+// queued again.
+template <int DIM>
+bool PcgSolver<DIM>::settle() {
+  if (!rf_.pending) return true;
+  rf_.pending = false;
+  const int sweeps = rf_.sweeps;
+  double* xo = this->state_->x_->delta().data();
+  refine(xo);
+  this->stats_.refinements = rf_.sweeps;
+  this->stats_.rel_residual = rf_.b2 > 0 ? std::sqrt(rf_.r2 / rf_.b2) : 0.0;
+  if (rf_.sweeps == sweeps) return true;
+  system_.post_solve(this->state_, xo);
+  return false;
 }
 
 // ============================================================ direct
@@ -1544,10 +1582,16 @@ void NewtonOptimizer<DIM>::step() {
     MFEM_HIP_CHECK(hipEventRecord(ev[0], st));
     update_system();
     MFEM_HIP_CHECK(hipEventRecord(ev[1], st));
-    solver_->solve();
-    const int nd = launch_decrement();
-    MFEM_HIP_CHECK(hipEventRecord(ev[2], st));
-    const int ne = ls_iters > 0 ? launch_energy2(0.0, RS_E0, 1.0, RS_E1) : launch_energy(0.0, RS_E0);
+    // The decrement and the first line-search energies depend only on the
+    // step: they are queued behind the solver's last residual check and
+    // queued again in the rare case that the check asks for another sweep.
+    solver_->solve_deferred();
+    int nd = 0, ne = 0;
+    do {
+      nd = launch_decrement();
+      MFEM_HIP_CHECK(hipEventRecord(ev[2], st));
+      ne = ls_iters > 0 ? launch_energy2(0.0, RS_E0, 1.0, RS_E1) : launch_energy(0.0, RS_E0);
+    } while (!solver_->settle());
     double v[6] = {0, 0, 0, 0, 0, 0};
     c->reducer.read_range(RS_MAXDX, ls_iters > 0 ? RS_E1 + 2 : RS_E0 + 2, v, st);
     grad_norm = nd == 2 ? std::max(v[0], v[1]) : v[0];

@@ -282,12 +282,24 @@ class LinearSolver {
   explicit LinearSolver(SimState<DIM>* state) : state_(state) {}
   virtual ~LinearSolver() = default;
   virtual void solve() = 0;
+  // solve() with the solver's last acceptance test (if it reads a result
+  // back) left for settle(): work that depends only on the solution can be
+  // queued behind that test meanwhile, with no host round trip before it.
+  void solve_deferred() {
+    defer_ = true;
+    solve();
+    defer_ = false;
+  }
+  // Completes a deferred solve. False: the solution changed after solve()
+  // returned, and work queued behind it has to be queued again.
+  virtual bool settle() { return true; }
   const SolveStats& stats() const { return stats_; }
   bool profile_spmv = false;
 
  protected:
   SimState<DIM>* state_;
   SolveStats stats_;
+  bool defer_ = false;
 };
 
 // "eigen-pcg-diag": Jacobi-preconditioned CG with Eigen's stopping rule.
@@ -299,6 +311,7 @@ class PcgSolver : public LinearSolver<DIM> {
   explicit PcgSolver(SimState<DIM>* state);
   ~PcgSolver() override;
   void solve() override;
+  bool settle() override;
   SystemMatrixPD<DIM>& system() { return system_; }
   // Plain SpMV shape chosen by its autotuner (-1 before the first solve) and
   // the mean device time of every shape on this matrix.
@@ -330,6 +343,13 @@ class PcgSolver : public LinearSolver<DIM> {
   void tune_spmv();
   template <typename T>
   void tune_cg();
+  void refine(double* xo);
+  // the FP32 defect-correction loop's state, kept across a deferred check
+  struct Refinement {
+    int budget = 0, sweeps = 0;
+    double b2 = 0, r2 = 0, thr = 0;
+    bool pending = false;
+  } rf_;
   int variant_ = -1;
   std::vector<double> tune_ms_;
   int cg_variant_ = -1, cg_blocks_ = 0;
