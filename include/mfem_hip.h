@@ -70,7 +70,9 @@ typedef struct {
   double alpha;           /* last accepted step */
   double res;             /* last energy residual */
   double rel_residual;    /* |b - K dx| / |b| of the last solve */
-  double update_ms, solve_ms, ls_ms, step_ms; /* device time per phase (hipEvents) */
+  double update_ms, solve_ms, ls_ms, step_ms; /* device time per phase (hipEvents; 0
+                                                 unless phase timing is on, see
+                                                 mfem_sim_set_profiling) */
   double spmv_ms;         /* summed device time of the CG kernels (when profiled) */
   int spmv_count;
 } mfem_step_stats;
@@ -155,7 +157,9 @@ int mfem_boundary_faces(const double* V, int nV, const int* T, int nT, int* F, i
  * precision, 3D); FP64 storage and 2D keep the vector-unit kernel. */
 int mfem_sim_set_element_kernel(mfem_sim* sim, int variant);
 /* bit 0: time every CG kernel launch with per-workgroup device clock
- * stamps (span of the launch, summed in the stats as spmv_ms/spmv_count) */
+ * stamps (span of the launch, summed in the stats as spmv_ms/spmv_count);
+ * bit 1: time each Newton iteration's phases with device events (update_ms,
+ * solve_ms, ls_ms; also on with "print_timing" or a callback installed) */
 int mfem_sim_set_profiling(mfem_sim* sim, int flags);
 /* Sizes used by the roofline model of the CG SpMV: free vertices, nonzero
  * 3x3 blocks, bytes per stored value. */

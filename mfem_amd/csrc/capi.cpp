@@ -527,7 +527,10 @@ int mfem_sim_set_element_kernel(mfem_sim* sim, int variant) {
 }
 
 int mfem_sim_set_profiling(mfem_sim* sim, int flags) {
-  return guarded([&] { sim->opt->solver()->profile_spmv = (flags & 1) != 0; });
+  return guarded([&] {
+    sim->opt->solver()->profile_spmv = (flags & 1) != 0;
+    sim->opt->phase_timing = (flags & 2) != 0;
+  });
 }
 
 int mfem_sim_matrix_info(mfem_sim* sim, int* nfree, int* nnzb, int* value_bytes) {

@@ -1576,12 +1576,16 @@ void NewtonOptimizer<DIM>::step() {
     stats_.solve_ms += t12;
     stats_.ls_ms += t23;
   };
+  const bool timing = this->phase_timing || this->sync_each_iteration || S.config_->show_timing;
+  auto mark = [&](hipEvent_t* ev, int k) {
+    if (ev) MFEM_HIP_CHECK(hipEventRecord(ev[k], st));
+  };
   int timed = 0;  // iterations whose phase times are in stats_
   do {
-    hipEvent_t* ev = events(i);
-    MFEM_HIP_CHECK(hipEventRecord(ev[0], st));
+    hipEvent_t* ev = timing ? events(i) : nullptr;
+    mark(ev, 0);
     update_system();
-    MFEM_HIP_CHECK(hipEventRecord(ev[1], st));
+    mark(ev, 1);
     // The decrement and the first line-search energies depend only on the
     // step: they are queued behind the solver's last residual check and
     // queued again in the rare case that the check asks for another sweep.
@@ -1589,7 +1593,7 @@ void NewtonOptimizer<DIM>::step() {
     int nd = 0, ne = 0;
     do {
       nd = launch_decrement();
-      MFEM_HIP_CHECK(hipEventRecord(ev[2], st));
+      mark(ev, 2);
       ne = ls_iters > 0 ? launch_energy2(0.0, RS_E0, 1.0, RS_E1) : launch_energy(0.0, RS_E0);
     } while (!solver_->settle());
     double v[6] = {0, 0, 0, 0, 0, 0};
@@ -1625,7 +1629,7 @@ void NewtonOptimizer<DIM>::step() {
       stats_.ls_fail++;
     }
     stats_.ls_iters += iter;
-    MFEM_HIP_CHECK(hipEventRecord(ev[3], st));
+    mark(ev, 3);
     stats_.alpha = alpha;
     history_.push_back({i + 1, E, res, grad_norm, alpha, 0.0, 0.0, 0.0, ss.iterations, iter});
     if (this->sync_each_iteration) {
@@ -1635,8 +1639,10 @@ void NewtonOptimizer<DIM>::step() {
     ++i;
     this->callback(S);
   } while (i < S.config_->outer_steps && grad_norm > S.config_->newton_tol && res > 1e-12);
-  MFEM_HIP_CHECK(hipEventSynchronize(ev_[4 * (i - 1) + 3]));
-  for (; timed < i; ++timed) phase_times(timed);
+  if (timing) {
+    MFEM_HIP_CHECK(hipEventSynchronize(ev_[4 * (i - 1) + 3]));
+    for (; timed < i; ++timed) phase_times(timed);
+  }
   stats_.newton_iters = i;
   stats_.decrement = grad_norm;
   stats_.energy = E;

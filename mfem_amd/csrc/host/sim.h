@@ -510,6 +510,11 @@ class Optimizer {
   // set when `callback` is the caller's: each Newton iteration then ends with
   // a synchronisation, so the callback sees that iteration's timings
   bool sync_each_iteration = false;
+  // device-event timings of each Newton iteration's phases (update, solve,
+  // line search); also on with show_timing or a caller's callback. Each
+  // event is a marker in the stream that costs the device a few
+  // microseconds, so they are off by default.
+  bool phase_timing = false;
 
  protected:
   std::unique_ptr<SimState<DIM>> state_;
