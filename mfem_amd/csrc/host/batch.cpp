@@ -1,7 +1,10 @@
 // Batched simulations: see batch.h.
 #include "batch.h"
 
+#include <immintrin.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -83,12 +86,14 @@ BatchSim::BatchSim(const std::vector<SceneArrays>& sims, std::shared_ptr<SimConf
   d_row0_.upload(f0_, st);
   ctrls_.resize(nsim_);
   ctrl_host_.resize(nsim_);
-  pinned_ctrl_ = std::make_unique<PinnedBuffer>(2 * sizeof(PcgCtrl) * nsim_ + 64);
+  pinned_ctrl_ = std::make_unique<PinnedBuffer>(sizeof(PcgCtrl) * (nsim_ + 1) + 64,
+                                                hipHostMallocMapped | hipHostMallocCoherent);
   pinned_res_ = std::make_unique<PinnedBuffer>(sizeof(double) * RES_ROWS * nsim_ + 64);
   staging_ = std::make_unique<PinnedBuffer>(sizeof(double) * kStageRegions * nsim_ + 64);
   d_res_.resize((size_t)RES_ROWS * nsim_);
   d_alpha_.resize(nsim_);
   d_b2_.resize(nsim_);
+  seg_part_.resize((size_t)kSegSlices * nsim_);
   d_tol_.resize(nsim_);
   d_budget_.resize(nsim_);
   d_on_.resize(nsim_);
@@ -96,15 +101,10 @@ BatchSim::BatchSim(const std::vector<SceneArrays>& sims, std::shared_ptr<SimConf
   scratch_.resize(256);
   res_.assign((size_t)RES_ROWS * nsim_, 0.0);
   stats_.resize(nsim_);
-  for (auto& e : poll_ev_) MFEM_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   MFEM_HIP_CHECK(hipStreamSynchronize(st));
 }
 
-BatchSim::~BatchSim() {
-  clear_graphs();
-  for (auto e : poll_ev_)
-    if (e) (void)hipEventDestroy(e);
-}
+BatchSim::~BatchSim() { clear_graphs(); }
 
 void BatchSim::clear_graphs() {
   for (auto& g : cg_graphs_)
@@ -255,6 +255,8 @@ BatchCgBuffers<T> BatchSim::buffers(const double* b, double* x) {
   B.wg0 = d_wg0_.data();
   B.row0 = d_row0_.data();
   B.variant = std::max(variant_, 0);
+  B.mirror = pinned_ctrl_->device<PcgCtrl>();
+  B.progress = B.mirror + nsim_;
   return B;
 }
 
@@ -268,26 +270,34 @@ void BatchSim::run_cg(const double* b, double* x, const std::vector<double>& tol
   auto& S = *state_;
   hipStream_t st = S.ctx_->stream;
   const BatchCgBuffers<T> B = buffers<T>(b, x);
+  // Every system's control block is mirrored in mapped host memory (the
+  // setup writes each one, the kernel that stops a system writes it again),
+  // and the first workgroup of every CG kernel writes the kernel's index
+  // there. The host keeps a few kernels queued beyond the last one that
+  // began and puts no event or copy into the stream.
+  volatile PcgCtrl* mirror = pinned_ctrl_->as<PcgCtrl>();
+  volatile PcgCtrl* progress = mirror + nsim_;
+  for (int k = 0; k < nsim_; ++k) mirror[k].done = 0;
+  progress->started = -1;
   launch_pcg_batch_init(B, scratch_.data(), st);
-  launch_seg_reduce(nsim_, b, 1, 0, d_seg_rows_.data(), 1, d_b2_.data(), st);
+  launch_seg_reduce(nsim_, b, 1, 0, d_seg_rows_.data(), 1, d_b2_.data(), seg_part_.data(),
+                    st);
   stage(d_tol_, tol, ST_TOL);
   stage(d_budget_, budget, ST_BUDGET);
   stage(d_on_, on, ST_ON_CG);
   launch_pcg_batch_setup(nsim_, ctrls_.data(), d_b2_.data(), d_tol_.data(), d_budget_.data(),
-                         d_on_.data(), st);
+                         d_on_.data(), st, B.mirror);
   int max_budget = 0;
   for (int k = 0; k < nsim_; ++k)
     if (on[k]) max_budget = std::max(max_budget, budget[k]);
   const long limit = (long)max_budget + 2;
   const int poll = std::max(1, S.config_->pcg_poll);
-  auto* snap = pinned_ctrl_->as<PcgCtrl>();
   long launched = 0;
-  int issued = 0, waited = 0;
-  // Eager launches run only the workgroups of systems that had not stopped
-  // at the last poll: a launch late in a solve, when a few slow systems are
-  // left, dispatches their workgroups instead of the whole batch's. Every
-  // workgroup of a listed system runs, so each system's iteration is the
-  // same kernel work as with the full grid.
+  // Eager launches run only the workgroups of systems that had stopped at
+  // no earlier look at the mirror: a launch late in a solve, when a few slow
+  // systems are left, dispatches their workgroups instead of the whole
+  // batch's. Every workgroup of a listed system runs, so each system's
+  // iteration is the same kernel work as with the full grid.
   BatchCgBuffers<T> Bc = B;
   if (!graphs_) {
     for (auto& d : d_active_) d.resize(nwg_);
@@ -297,73 +307,90 @@ void BatchSim::run_cg(const double* b, double* x, const std::vector<double>& tol
     }
   }
   int list_slot = -1;
-  long retired_at[3] = {0, 0, 0};  // chunks issued before a list was replaced
-  auto compact = [&](const PcgCtrl* c) {
+  // a list slot is rewritten only once the kernel with this index has begun:
+  // every kernel that read the slot's previous list has then completed, and
+  // so has the copy of that list from its pinned source
+  long free_from[3] = {0, 0, 0};
+  auto compact = [&](long started) {
     if (graphs_) return;
     const int cur = Bc.active ? Bc.nactive : nwg_;
     int count = 0;
     for (int k = 0; k < nsim_; ++k)
-      if (!c[k].done) count += wg0_[k + 1] - wg0_[k];
+      if (!mirror[k].done) count += wg0_[k + 1] - wg0_[k];
     if (count == 0 || count > cur * 3 / 4) return;
     const int slot = (list_slot + 1) % 3;
-    // every chunk that read this slot's previous list has completed, and so
-    // has the chunk after its copy (so the copy has read its pinned source)
-    if (waited <= retired_at[slot]) return;
+    if (started < free_from[slot]) return;
     int* h = pinned_active_->as<int>() + (size_t)slot * nwg_;
     int n = 0;
     for (int k = 0; k < nsim_; ++k)
-      if (!c[k].done)
+      if (!mirror[k].done)
         for (int w = wg0_[k]; w < wg0_[k + 1]; ++w) h[n++] = w;
     MFEM_HIP_CHECK(hipMemcpyAsync(d_active_[slot].data(), h, sizeof(int) * (size_t)n,
                                   hipMemcpyHostToDevice, st));
-    if (list_slot >= 0) retired_at[list_slot] = issued;
+    if (list_slot >= 0) free_from[list_slot] = launched;
     list_slot = slot;
     Bc.active = d_active_[slot].data();
     Bc.nactive = n;
   };
-  // graph replay: pairs of iterations; a pair past the stop exits at once,
-  // like any launch queued after it
+  // graph replay: groups of kCgGraphPairs pairs of iterations; a group past
+  // the stop exits at once, like any launch queued after it
   hipGraphExec_t pair = nullptr;
   if (graphs_) {
     pair = cg_pair_graph(B);
     launch_set_int(it_ctr_.data(), 0, st);
   }
-  auto issue = [&] {
-    const long chunk = std::min<long>(poll, limit - launched);
-    if (pair) {
-      const long per = 2L * kCgGraphPairs;
-      const long graphs = (chunk + per - 1) / per;
-      for (long j = 0; j < graphs; ++j) MFEM_HIP_CHECK(hipGraphLaunch(pair, st));
-      launched += per * graphs;
-      subs_cg_ += graphs;
-    } else {
-      for (long j = 0; j < chunk; ++j) launch_pcg_batch_iteration(Bc, (int)(launched + j), st);
-      launched += chunk;
-      subs_cg_ += chunk;
+  const long per = pair ? 2L * kCgGraphPairs : 1;
+  const long ahead = std::max<long>(std::max(2, poll / 4), per);
+  auto launch_to = [&](long n) {
+    n = std::min(n, limit);
+    if (launched >= n) return;
+    for (; launched < n; launched += per) {
+      if (pair) MFEM_HIP_CHECK(hipGraphLaunch(pair, st));
+      else launch_pcg_batch_iteration(Bc, (int)launched, st);
+      ++subs_cg_;
     }
     MFEM_HIP_CHECK(hipGetLastError());
-    const int slot = issued & 1;
-    MFEM_HIP_CHECK(hipMemcpyAsync(snap + (size_t)slot * nsim_, ctrls_.data(),
-                                  sizeof(PcgCtrl) * nsim_, hipMemcpyDeviceToHost, st));
-    MFEM_HIP_CHECK(hipEventRecord(poll_ev_[slot], st));
-    ++issued;
   };
-  issue();
+  auto all_done = [&] {
+    for (int k = 0; k < nsim_; ++k)
+      if (!mirror[k].done) return false;
+    return true;
+  };
+  launch_to(ahead);
+  long seen = -2;
   while (true) {
-    if (launched < limit && issued - waited < 2) issue();
-    const int slot = waited & 1;
-    MFEM_HIP_CHECK(hipEventSynchronize(poll_ev_[slot]));
-    ++waited;
-    const PcgCtrl* c = snap + (size_t)slot * nsim_;
-    bool all = true;
-    for (int k = 0; k < nsim_ && all; ++k) all = c[k].done != 0;
-    if (all) {
-      std::copy(c, c + nsim_, ctrl_host_.begin());
-      break;
+    const long started = progress->started;
+    if (started != seen) {  // a kernel began: look at the systems' flags
+      seen = started;
+      if (all_done()) break;
+      compact(started);
     }
-    if (waited == issued) throw std::runtime_error("batched CG control blocks did not terminate");
-    compact(c);
+    if (launched < limit) {
+      launch_to(started + 1 + ahead);
+      _mm_pause();
+      continue;
+    }
+    // all kernels queued: the last one stops every system at the latest
+    MFEM_HIP_CHECK(hipStreamSynchronize(st));
+    if (!all_done()) throw std::runtime_error("batched CG control blocks did not terminate");
+    break;
   }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  for (int k = 0; k < nsim_; ++k) {
+    PcgCtrl& c = ctrl_host_[k];
+    c.rr = mirror[k].rr;
+    c.b2 = mirror[k].b2;
+    c.threshold = mirror[k].threshold;
+    c.kern_ticks = mirror[k].kern_ticks;
+    c.iter = mirror[k].iter;
+    c.done = 1;
+    c.max_iters = mirror[k].max_iters;
+    c.stop_it = mirror[k].stop_it;
+    c.kern_launches = mirror[k].kern_launches;
+  }
+  // the kernels queued past the last stop exit at once; they finish before
+  // the mirror is reset for the next solve
+  MFEM_HIP_CHECK(hipStreamSynchronize(st));
   iters.assign(nsim_, 0);
   for (int k = 0; k < nsim_; ++k) iters[k] = on[k] ? ctrl_host_[k].iter : 0;
 }
@@ -396,7 +423,8 @@ void BatchSim::tune() {
       layout_workgroups(rpw);
       BatchCgBuffers<T> B = buffers<T>(system_.rhs.data(), x_.data());
       launch_pcg_batch_init(B, scratch_.data(), st);
-      launch_seg_reduce(nsim_, B.b, 1, 0, d_seg_rows_.data(), 1, d_b2_.data(), st);
+      launch_seg_reduce(nsim_, B.b, 1, 0, d_seg_rows_.data(), 1, d_b2_.data(),
+                        seg_part_.data(), st);
       stage(d_tol_, tol, ST_TOL);
       stage(d_budget_, budget, ST_BUDGET);
       stage(d_on_, on, ST_ON_CG);
@@ -478,7 +506,8 @@ void BatchSim::solve(const std::vector<int>& on) {
     while (true) {
       launch_residual<float>(m.nfree(), m.d_rowptr.data(), m.d_col.data(), system_.K.f32(),
                              x_.data(), system_.rhs.data(), r64_.data(), R.slot(8), st);
-      launch_seg_reduce(nsim_, r64_.data(), 1, 0, d_seg_rows_.data(), 1, d_res_.data(), st);
+      launch_seg_reduce(nsim_, r64_.data(), 1, 0, d_seg_rows_.data(), 1, d_res_.data(),
+                        seg_part_.data(), st);
       MFEM_HIP_CHECK(hipMemcpyAsync(pinned_res_->as<double>(), d_res_.data(),
                                     sizeof(double) * nsim_, hipMemcpyDeviceToHost, st));
       MFEM_HIP_CHECK(hipStreamSynchronize(st));
@@ -546,7 +575,7 @@ void BatchSim::step() {
                               mixed_->s_.data(), mixed_->ds_.data(), mixed_->la_.data(), h2,
                               e_tet_.data(), st);
     launch_seg_reduce(nsim_, e_tet_.data(), 1, 0, d_seg_tets_.data(), 0,
-                      d_res_.data() + (size_t)row * nsim_, st);
+                      d_res_.data() + (size_t)row * nsim_, seg_part_.data(), st);
   };
   const int ls_iters = cfg.ls_iters;
   for (int i = 0; i < cfg.outer_steps; ++i) {
@@ -558,9 +587,9 @@ void BatchSim::step() {
     solve(on);
     // decrement, E(0) and E(1), one read
     launch_seg_reduce(nsim_, S.x_->delta().data(), 1, 0, d_seg_rows_.data(), 2,
-                      d_res_.data() + (size_t)RES_DX * nsim_, st);
+                      d_res_.data() + (size_t)RES_DX * nsim_, seg_part_.data(), st);
     launch_seg_reduce(nsim_, mixed_->ds_.data(), 6, nT, d_seg_tets_.data(), 2,
-                      d_res_.data() + (size_t)RES_DS * nsim_, st);
+                      d_res_.data() + (size_t)RES_DS * nsim_, seg_part_.data(), st);
     energies(std::vector<double>(nsim_, 0.0), on, RES_E0, false);
     if (ls_iters > 0) energies(std::vector<double>(nsim_, 1.0), on, RES_E1, true);
     read_results(RES_ROWS * nsim_);

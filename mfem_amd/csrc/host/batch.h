@@ -103,8 +103,8 @@ class BatchSim {
   DeviceArray<mfemhip::PcgCtrl> ctrls_;
   DeviceArray<int> d_budget_, d_on_;
   std::vector<mfemhip::PcgCtrl> ctrl_host_;
-  std::unique_ptr<PinnedBuffer> pinned_ctrl_;  // two snapshots of all control blocks
-  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  // mapped copy of every system's control block, then the progress word
+  std::unique_ptr<PinnedBuffer> pinned_ctrl_;
   // Compacted lists of the workgroups of the systems still iterating (eager
   // CG only), a ring of three (device lists and their pinned sources) so
   // that a list is rewritten only after the launches that read it have run.
@@ -113,6 +113,7 @@ class BatchSim {
   int pinned_active_n_ = 0;
   // line search / decrement
   DeviceArray<double> e_tet_, d_alpha_, d_alpha1_, d_res_;
+  DeviceArray<double> seg_part_;  // slice partials of the per-simulation reductions
   DeviceArray<int> d_on1_;
   std::unique_ptr<PinnedBuffer> pinned_res_;
   std::unique_ptr<PinnedBuffer> staging_;  // kStageRegions x nsim doubles

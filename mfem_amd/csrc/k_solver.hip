@@ -475,9 +475,10 @@ __device__ __forceinline__ void cg_reduce_parts(const double* __restrict__ parts
   block_sum_n<BS, kCgParts>(v, sm);
 }
 
-// The control block of a stopped solve into the host's mapped copy: every
-// field, then (after a system-scope fence) done, so that a host that sees
-// done set reads the final fields.
+// A control block into the host's mapped copy: every field, then (after a
+// system-scope fence) done, so that a host that sees done set reads the
+// final fields. Called when a solve stops, and by the batched setup for
+// every system (done then tells whether the system starts stopped).
 __device__ __forceinline__ void publish_stop(const PcgCtrl& c, PcgCtrl* mirror) {
   mirror->rr = c.rr;
   mirror->b2 = c.b2;
@@ -488,7 +489,7 @@ __device__ __forceinline__ void publish_stop(const PcgCtrl& c, PcgCtrl* mirror) 
   mirror->stop_it = c.stop_it;
   mirror->kern_launches = c.kern_launches;
   __threadfence_system();
-  __hip_atomic_store(&mirror->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mirror->done, c.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // One pass of a lane over a block row in the CG kernel: the U column ids,
@@ -677,10 +678,15 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
     double* __restrict__ x, PcgCtrl* ctrls, const double* __restrict__ parts_old,
     double* __restrict__ parts_new, int nwg, const int* __restrict__ wg_sys,
     const int* __restrict__ wg0, const int* __restrict__ row0, const int* __restrict__ active,
-    const int* it_in, int* it_out) {
+    const int* it_in, int* it_out, PcgCtrl* mirror, PcgCtrl* progress) {
   __shared__ double sm[kCgParts * (BS / kWave)];
   const int it = it_in ? *it_in : it_arg;
-  if (it_out && blockIdx.x == 0 && threadIdx.x == 0) *it_out = it + 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it_out) *it_out = it + 1;
+    // progress for the host, which keeps a few kernels queued ahead of it
+    if (progress)
+      __hip_atomic_store(&progress->started, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
   // the workgroup this block runs: itself, or the active list's entry
   const int wg = active ? active[blockIdx.x] : (int)blockIdx.x;
   const int b = wg_sys[wg];
@@ -704,6 +710,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
         ctrl->iter = converged ? (j > 0 ? j - 1 : 0) : j;
         ctrl->stop_it = it;
         ctrl->done = 1;
+        if (mirror) publish_stop(*ctrl, mirror + b);
       }
       return;
     }
@@ -761,7 +768,7 @@ __global__ __launch_bounds__(BS) void k_cg_iter_batch(
 // per-system |b|^2; systems that are off (or have b = 0) start done.
 __global__ void k_cg_setup_batch(int nsys, PcgCtrl* ctrls, const double* __restrict__ b2,
                                  const double* __restrict__ tol, const int* __restrict__ max_iters,
-                                 const int* __restrict__ on) {
+                                 const int* __restrict__ on, PcgCtrl* mirror) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nsys) return;
   PcgCtrl c;
@@ -776,6 +783,7 @@ __global__ void k_cg_setup_batch(int nsys, PcgCtrl* ctrls, const double* __restr
   c.started = 0;
   c.done = (!on[b] || c.b2 == 0.0) ? 1 : 0;
   ctrls[b] = c;
+  if (mirror) publish_stop(c, mirror + b);
 }
 
 // x = 0, state = {b, 0, 0, D^-1}; partials of b.b
@@ -1011,10 +1019,11 @@ void launch_pcg_batch_init(const BatchCgBuffers<T>& B, double* scratch, hipStrea
 }
 
 void launch_pcg_batch_setup(int nsys, PcgCtrl* ctrls, const double* b2, const double* tol,
-                            const int* max_iters, const int* on, hipStream_t st) {
+                            const int* max_iters, const int* on, hipStream_t st,
+                            PcgCtrl* mirror) {
   if (nsys == 0) return;
   hipLaunchKernelGGL(k_cg_setup_batch, dim3(grid(nsys, kBS)), dim3(kBS), 0, st, nsys, ctrls, b2, tol,
-                     max_iters, on);
+                     max_iters, on, mirror);
 }
 
 template <typename T>
@@ -1035,7 +1044,7 @@ void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t 
     hipLaunchKernelGGL((k_cg_iter_batch<kCgShapes[i].G, kCgShapes[i].U, kCgShapes[i].BS, T>),    \
                        dim3(grid), dim3(kCgShapes[i].BS), 0, st, it, B.rowptr, B.col, B.K, so,   \
                        sn, B.x, B.ctrls, po, pn, B.nwg, B.wg_sys, B.wg0, B.row0, B.active, it_in, \
-                       it_out);                                                                \
+                       it_out, B.mirror, B.progress);                                          \
     break;
     MFEM_CGB(0) MFEM_CGB(1) MFEM_CGB(2) MFEM_CGB(3) MFEM_CGB(4) MFEM_CGB(5) MFEM_CGB(6) MFEM_CGB(7)
 #undef MFEM_CGB

@@ -144,13 +144,18 @@ void launch_axpy(long n, double alpha, const double* x, double* y, hipStream_t s
 
 namespace {
 
-__global__ __launch_bounds__(kBS) void k_seg_reduce(const double* __restrict__ v, int ncomp,
-                                                    long comp_stride, const int* __restrict__ seg,
-                                                    int mode, double* __restrict__ out) {
+// Stage 1 of the per-segment reductions: workgroup (b, p) reduces the p-th
+// of kSegSlices equal slices of segment b (every component) into
+// part[b * kSegSlices + p].
+__global__ __launch_bounds__(kBS) void k_seg_reduce_slices(const double* __restrict__ v, int ncomp,
+                                                           long comp_stride,
+                                                           const int* __restrict__ seg, int mode,
+                                                           double* __restrict__ part) {
   __shared__ double sm[kBS / kWave];
-  const int b = blockIdx.x;
-  const int lo = seg[b], hi = seg[b + 1];
-  double acc = mode == 2 ? 0.0 : 0.0;
+  const int b = blockIdx.x / kSegSlices, p = blockIdx.x % kSegSlices;
+  const long lo0 = seg[b], len = seg[b + 1] - lo0;
+  const int lo = (int)(lo0 + len * p / kSegSlices), hi = (int)(lo0 + len * (p + 1) / kSegSlices);
+  double acc = 0.0;
   for (int c = 0; c < ncomp; ++c) {
     const double* vc = v + (size_t)c * comp_stride;
     for (int i = lo + (int)threadIdx.x; i < hi; i += kBS) {
@@ -161,7 +166,20 @@ __global__ __launch_bounds__(kBS) void k_seg_reduce(const double* __restrict__ v
     }
   }
   const double r = mode == 2 ? block_reduce_all<kBS, 1>(acc, sm) : block_reduce_all<kBS, 0>(acc, sm);
-  if (threadIdx.x == 0) out[b] = r;
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Stage 2: one thread per segment combines its slices in slice order.
+__global__ __launch_bounds__(kBS) void k_seg_reduce_combine(int nseg, const double* __restrict__ part,
+                                                            int mode, double* __restrict__ out) {
+  const int b = blockIdx.x * kBS + threadIdx.x;
+  if (b >= nseg) return;
+  double acc = part[(size_t)b * kSegSlices];
+  for (int p = 1; p < kSegSlices; ++p) {
+    const double a = part[(size_t)b * kSegSlices + p];
+    acc = mode == 2 ? fmax(acc, a) : acc + a;
+  }
+  out[b] = acc;
 }
 
 __global__ __launch_bounds__(kBS) void k_accept_batch(int nV, const int* __restrict__ free_map,
@@ -193,10 +211,12 @@ __global__ __launch_bounds__(kBS) void k_accept_batch(int nV, const int* __restr
 }  // namespace
 
 void launch_seg_reduce(int nseg, const double* v, int ncomp, long comp_stride, const int* seg,
-                       int mode, double* out, hipStream_t st) {
+                       int mode, double* out, double* part, hipStream_t st) {
   if (nseg <= 0) return;
-  hipLaunchKernelGGL(k_seg_reduce, dim3(nseg), dim3(kBS), 0, st, v, ncomp, comp_stride, seg, mode,
-                     out);
+  hipLaunchKernelGGL(k_seg_reduce_slices, dim3(nseg * kSegSlices), dim3(kBS), 0, st, v, ncomp,
+                     comp_stride, seg, mode, part);
+  hipLaunchKernelGGL(k_seg_reduce_combine, dim3(grid(nseg, kBS)), dim3(kBS), 0, st, nseg, part,
+                     mode, out);
 }
 
 void launch_accept_batch(int nV, const int* free_map, const int* vert_sim, int nT,

@@ -235,13 +235,20 @@ struct BatchCgBuffers {
   const int* active = nullptr;
   int nactive = 0;
   int variant = 0;
+  // optional, in mapped host memory: a copy of every control block, written
+  // when the system's solve starts (its threshold, or done) and when it
+  // stops (every field, then done), and progress->started, the index of the
+  // last kernel that began
+  PcgCtrl* mirror = nullptr;
+  PcgCtrl* progress = nullptr;
 };
 // x = 0 and the initial state for all rows (scratch: >= 256 doubles)
 template <typename T>
 void launch_pcg_batch_init(const BatchCgBuffers<T>& B, double* scratch, hipStream_t st);
 // control blocks from per-system |b|^2, tolerance, iteration budget, on flag
 void launch_pcg_batch_setup(int nsys, PcgCtrl* ctrls, const double* b2, const double* tol,
-                            const int* max_iters, const int* on, hipStream_t st);
+                            const int* max_iters, const int* on, hipStream_t st,
+                            PcgCtrl* mirror = nullptr);
 // Batched CG kernel `it`. For graph replay pass it_in / it_out (two distinct
 // device words): the kernel then takes its iteration index from *it_in and
 // stores it + 1 in *it_out; `it` only selects the buffer parity.
@@ -251,12 +258,14 @@ void launch_pcg_batch_iteration(const BatchCgBuffers<T>& B, int it, hipStream_t 
 // *p = v on the device (a kernel, so that it may sit in a captured graph)
 void launch_set_int(int* p, int v, hipStream_t st);
 
-// Per-segment reductions, one workgroup per segment in a fixed order:
-// out[b] = sum v (mode 0), sum v^2 (mode 1) or max |v| (mode 2) over
-// indices [seg[b], seg[b+1]) of each of ncomp components (component c at
-// v + c * comp_stride).
+// Per-segment reductions in a fixed order: out[b] = sum v (mode 0), sum v^2
+// (mode 1) or max |v| (mode 2) over indices [seg[b], seg[b+1]) of each of
+// ncomp components (component c at v + c * comp_stride). Each segment is cut
+// into kSegSlices equal slices, one workgroup each; `part` holds
+// nseg * kSegSlices partial results, combined per segment in slice order.
+constexpr int kSegSlices = 16;
 void launch_seg_reduce(int nseg, const double* v, int ncomp, long comp_stride, const int* seg,
-                       int mode, double* out, hipStream_t st);
+                       int mode, double* out, double* part, hipStream_t st);
 // Batched line-search acceptance: x += alpha[sim of vertex] dx on free
 // vertices, s += alpha[sim of tet] ds.
 void launch_accept_batch(int nV, const int* free_map, const int* vert_sim, int nT,

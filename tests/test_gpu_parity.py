@@ -632,6 +632,7 @@ def test_newton_history_matches_oracle(native, oracle, mixed):
     sc = scenes.beam_scene(3, 2, 2, seed=2, mixed=mixed, itr_tol=1e-12, max_iters=5000,
                            max_newton=10, newton_tol=1e-9)
     g, o = native.Sim(sc), oracle.Sim(sc)
+    g.set_profiling(2)  # per-iteration phase times (device events)
     for k in range(3):
         g.step()
         o.step()
