@@ -111,18 +111,6 @@ __device__ __forceinline__ void assemble_bsr_item(int t, const int* __restrict__
     }
 }
 
-template <typename Real>
-__global__ __launch_bounds__(kBS) void k_assemble_bsr(int nub, const int* __restrict__ upper,
-                                                      const int* __restrict__ tpos,
-                                                      const int* __restrict__ slot_ptr,
-                                                      const int* __restrict__ slots,
-                                                      const double* __restrict__ mass,
-                                                      const Real* __restrict__ blocks,
-                                                      Real* __restrict__ K) {
-  const int t = xcd_item(4 * nub);
-  if (t >= 0) assemble_bsr_item<Real>(t, upper, tpos, slot_ptr, slots, mass, blocks, K);
-}
-
 template <typename Real, typename VT>
 __global__ void k_jacobi_diag(int nfree, const int* __restrict__ diag_block, const Real* __restrict__ K,
                               VT* __restrict__ invdiag) {
@@ -171,13 +159,6 @@ __device__ __forceinline__ void assemble_rhs_row(int i, const int* __restrict__ 
   rhs[3 * i] = a0;
   rhs[3 * i + 1] = a1;
   rhs[3 * i + 2] = a2;
-}
-
-__global__ void k_assemble_rhs(int nfree, const int* __restrict__ rows,
-                               const int* __restrict__ inc_ptr, const int* __restrict__ inc,
-                               const double* __restrict__ fe, double* __restrict__ rhs) {
-  const int item = xcd_item(nfree);
-  if (item >= 0) assemble_rhs_row(rows[item], inc_ptr, inc, fe, rhs);
 }
 
 // Both gathers in one launch: workgroups [0, nb_bsr) assemble K, the rest
@@ -894,15 +875,6 @@ void launch_finalize(const double* partials, int n, int op, double* result, hipS
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBS), 0, st, partials, n, op, result);
 }
 
-template <typename Real>
-void launch_assemble_bsr(int nub, const int* upper, const int* tpos, const int* slot_ptr,
-                         const int* slots, const double* mass, const Real* blocks, Real* K,
-                         hipStream_t st) {
-  if (nub == 0) return;
-  hipLaunchKernelGGL(k_assemble_bsr<Real>, dim3(xcd_grid(4 * nub, kBS)), dim3(kBS), 0, st, nub,
-                     upper, tpos, slot_ptr, slots, mass, blocks, K);
-}
-
 template <typename Real, typename VT>
 void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* invdiag,
                         hipStream_t st) {
@@ -952,13 +924,6 @@ void launch_assemble_bsr_rhs(int nub, const int* upper, const int* tpos, const i
   hipLaunchKernelGGL(k_assemble_bsr_rhs<Real>, dim3(nb_bsr + nb_rhs), dim3(kBS), 0, st, nub,
                      upper, tpos, slot_ptr, slots, mass, blocks, K, nb_bsr, nfree, rows, inc_ptr,
                      inc, fe, rhs);
-}
-
-void launch_assemble_rhs(int nfree, const int* rows, const int* inc_ptr, const int* inc,
-                         const double* fe, double* rhs, hipStream_t st) {
-  if (nfree == 0) return;
-  hipLaunchKernelGGL(k_assemble_rhs, dim3(xcd_grid(nfree, kBS)), dim3(kBS), 0, st, nfree, rows,
-                     inc_ptr, inc, fe, rhs);
 }
 
 template <typename MT, typename VT>
@@ -1068,10 +1033,6 @@ template void launch_pcg_batch_iteration<double>(const BatchCgBuffers<double>&, 
 template void launch_pcg_batch_iteration<float>(const BatchCgBuffers<float>&, int, hipStream_t,
                                                 const int*, int*);
 
-template void launch_assemble_bsr<double>(int, const int*, const int*, const int*, const int*,
-                                          const double*, const double*, double*, hipStream_t);
-template void launch_assemble_bsr<float>(int, const int*, const int*, const int*, const int*,
-                                         const double*, const float*, float*, hipStream_t);
 template void launch_assemble_bsr_rhs<double>(int, const int*, const int*, const int*,
                                               const int*, const double*, const double*, double*,
                                               int, const int*, const int*, const int*,

@@ -121,14 +121,6 @@ void launch_plain2d_energy(const TetMeshDev& m, const double* x_full, const doub
                            hipStream_t st);
 
 // --- assembly (reference src/utils/sparse_utils.cpp:141-203) ---------------
-// K block b = mass[b] * I + sum over its slot list of element blocks.
-// slot = (e << 4) | (a << 2) | bb  (local vertex pair a, bb of tet e).
-// Runs over the nub blocks j >= i (pattern positions `upper`) and writes each
-// also transposed at `tpos` (the block (j, i)).
-template <typename Real>
-void launch_assemble_bsr(int nub, const int* upper, const int* tpos, const int* slot_ptr,
-                         const int* slots, const double* mass, const Real* blocks, Real* K,
-                         hipStream_t st);
 // invdiag[3i+r] = 1 / K_(3i+r),(3i+r) from the diagonal blocks (Jacobi).
 template <typename Real, typename VT>
 void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* invdiag,
@@ -137,11 +129,14 @@ void launch_jacobi_diag(int nfree, const int* diag_block, const Real* K, VT* inv
 template <typename Real>
 void launch_bsr_to_dense(int nrows, const int* rowptr, const int* col, const Real* K, double* A,
                          long lda, hipStream_t st);
-// rhs at free vertex i = sum over its (tet, corner) incidences of f_e[corner];
-// rows: the free rows in visiting order (a permutation of 0 .. nfree-1)
-void launch_assemble_rhs(int nfree, const int* rows, const int* inc_ptr, const int* inc,
-                         const double* fe, double* rhs, hipStream_t st);
-// launch_assemble_bsr and launch_assemble_rhs as one launch (same results)
+// The K and rhs gathers in one launch.
+// K: block b = mass[b] * I + sum over its slot list of element blocks, with
+// slot = (e << 4) | (a << 2) | bb (local vertex pair a, bb of tet e); runs
+// over the nub blocks j >= i (pattern positions `upper`) and writes each also
+// transposed at `tpos` (the block (j, i)).
+// rhs: at free vertex i, the sum over its (tet, corner) incidences of
+// f_e[corner]; rows lists the free rows in visiting order (a permutation of
+// 0 .. nfree-1).
 template <typename Real>
 void launch_assemble_bsr_rhs(int nub, const int* upper, const int* tpos, const int* slot_ptr,
                              const int* slots, const double* mass, const Real* blocks, Real* K,
