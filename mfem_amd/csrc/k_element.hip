@@ -234,11 +234,11 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_polar_material(
 // the round back as contiguous runs.
 constexpr int kStagePitch = 37;  // widest round: 36 values per tet
 
-template <int N, int PITCH, typename V>
+template <int N, int PITCH, typename V, int TETS = kWave>
 __device__ __forceinline__ void stage_flush(const V* stage, int lane, int e0, int nT, V* dst,
                                             int stride, int off) {
   __syncthreads();
-  for (int idx = lane; idx < kWave * N; idx += kWave) {
+  for (int idx = lane; idx < TETS * N; idx += kWave) {
     const int t = idx / N, j = idx - t * N;
     if (e0 + t < nT) dst[(size_t)(e0 + t) * stride + off + j] = stage[t * PITCH + j];
   }
@@ -327,13 +327,15 @@ __device__ __forceinline__ void mixed_element_fe(const TetMeshDev& m, int ec, in
   stage_flush<12, kFePitch>(fst, lane, e0, nT, fe, 12, 0);
 }
 
-// FP32 records are staged whole (64 tets x 90 values per wave, 23 KB) and
-// leave as one contiguous run of the wave's 64 records: staged in parts, the
-// runs of a part cover partial cache lines, and the L2 wrote such lines back
-// more than once (WRITE_SIZE 1.3x the record bytes at the 960k-tet beam).
-// FP64 records would need twice that LDS, so they leave in three rounds of
-// 36, 27 and 27 values per tet.
+// FP32 records are staged whole and leave as contiguous runs, 32 tets at a
+// time (lanes 0-31, then 32-63): 32 records are 11520 B, exactly 90 lines of
+// 128 B, so every run covers whole lines. Staged in parts of a record, the
+// runs of a part covered partial cache lines, and the L2 wrote such lines
+// back more than once (WRITE_SIZE 1.3x the record bytes at the 960k-tet
+// beam); 64 records at a time need twice the LDS (fewer workgroups per CU).
+// FP64 records leave in three rounds of 36, 27 and 27 values per tet.
 constexpr int kFullPitch = 91;  // floats per staged FP32 record (odd: no bank conflicts)
+constexpr int kHalfWave = kWave / 2;
 
 template <typename Real>
 __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
@@ -342,9 +344,10 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
     double* __restrict__ fe) {
   constexpr bool kWhole = sizeof(Real) == sizeof(float);
   constexpr int kPitch = kWhole ? kFullPitch : kStagePitch;
-  __shared__ __align__(16) unsigned char stage_raw[kElemBlock * kPitch * sizeof(Real)];
-  static_assert(kWave * 13 * sizeof(double) <= kWave * kPitch * sizeof(Real),
-                "f_e stage exceeds the wave's LDS");
+  constexpr int kStageTets = kWhole ? kHalfWave : kWave;  // tets staged per wave at a time
+  constexpr int kWaveBytes = kStageTets * kPitch * sizeof(Real);
+  __shared__ __align__(16) unsigned char stage_raw[(kElemBlock / kWave) * kWaveBytes];
+  static_assert(kWave * 13 * sizeof(double) <= kWaveBytes, "f_e stage exceeds the wave's LDS");
   const int lane = threadIdx.x & (kWave - 1);
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int e0 = e - lane;  // first tet of this wave
@@ -352,12 +355,14 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
   // lanes past the end compute on the last tet and store nothing, so that
   // every lane reaches the barriers of the rounds
   const int ec = e < nT ? e : nT - 1;
-  unsigned char* wave_raw = stage_raw + (size_t)(threadIdx.x - lane) * kPitch * sizeof(Real);
+  unsigned char* wave_raw = stage_raw + (size_t)(threadIdx.x / kWave) * kWaveBytes;
   Real* stage = reinterpret_cast<Real*>(wave_raw);
   const double vol = m.vol[ec];
   Real B[72], C[72];
   mixed_element_BC<Real>(m, ec, dsdF, Hpsi, B, C);
   const Real rv = (Real)vol;
+  // the lane's staged record
+  const int srow = (kWhole ? (lane & (kHalfWave - 1)) : lane) * kPitch;
   // block (k, l) = vol B_k^T C_l into staged slot `slot`
   auto block_out = [&](int k, int l, int slot) {
 #pragma unroll
@@ -367,16 +372,22 @@ __global__ __launch_bounds__(kElemBlock) void k_mixed_blocks_rhs(
         Real a = 0;
 #pragma unroll
         for (int i = 0; i < 6; ++i) a += B[i + 6 * (3 * k + r)] * C[i + 6 * (3 * l + q)];
-        stage[lane * kPitch + 9 * slot + 3 * r + q] = rv * a;
+        stage[srow + 9 * slot + 3 * r + q] = rv * a;
       }
   };
   // record order blk_index(k, l): (0,0..3) | (1,1..3) | (2,2), (2,3), (3,3)
   if constexpr (kWhole) {
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      if (lane / kHalfWave == h) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k)
 #pragma unroll
-      for (int l = k; l < 4; ++l) block_out(k, l, blk_index(k, l));
-    stage_flush<90, kPitch>(stage, lane, e0, nT, blocks, 90, 0);
+          for (int l = k; l < 4; ++l) block_out(k, l, blk_index(k, l));
+      }
+      stage_flush<90, kPitch, Real, kHalfWave>(stage, lane, e0 + kHalfWave * h, nT, blocks, 90,
+                                               0);
+    }
   } else {
 #pragma unroll
     for (int l = 0; l < 4; ++l) block_out(0, l, l);
