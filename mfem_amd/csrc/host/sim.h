@@ -286,9 +286,12 @@ class LinearSolver {
   // back) left for settle(): work that depends only on the solution can be
   // queued behind that test meanwhile, with no host round trip before it.
   void solve_deferred() {
+    struct Reset {
+      bool& f;
+      ~Reset() { f = false; }
+    } reset{defer_};
     defer_ = true;
     solve();
-    defer_ = false;
   }
   // Completes a deferred solve. False: the solution changed after solve()
   // returned, and work queued behind it has to be queued again.
