@@ -111,7 +111,7 @@ def test_c3_mixed_and_fp64_25_steps(native):
             print(f"c3 step {k}: fp64 vs oracle next to the clamp {e_hot:.1e}, fp64 spread "
                   f"there {spread_on(hot):.1e}")
         worst.append((k, err, spread, e_sub))
-    # where the FP64 spread is small, the north-star bar itself holds
+    # where the FP64 spread is small, the plain 1e-6 bar holds
     for k, err, spread, _ in worst:
         if spread < 1e-7:
             assert err < 1e-6, (k, err)
