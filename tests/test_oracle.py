@@ -314,6 +314,70 @@ def test_assembly_matches_bruteforce(oracle, mixed):
     assert np.linalg.norm(K @ dx - rhs) <= 1e-11 * np.linalg.norm(rhs)
 
 
+def test_element_block_is_energy_hessian_at_rest(oracle):
+    """The mixed element block (the JᵀHJ block of the displacement system,
+    reference mixed_stretch.cpp:152-175) pinned by the energy it comes from:
+    at the rest state (s = I, so grad Psi = 0 and the curvature of s(F)
+    drops out) it equals the Hessian of vol h² Psi(s(F(q))) over the tet's
+    12 nodal coordinates, by central differences, with F = sum_k q_k dphi_kᵀ
+    (the reference's dF/dx, tests/test_jacobian.cpp:8-30) and s the stretch
+    of an exact rotation fit. Also: symmetric, and rigid translations in its
+    null space. The differences agree to ~3e-11 of the largest entry; a
+    wrong volume or h² factor, or a doubled weight on the shear entries of
+    s (tried on the CPU: 0.49), moves them by O(1)."""
+    sc = scenes.beam_scene(2, 1, 1, seed=4, mixed=True, E_range=(1e5, 1e6))
+    sim = oracle.Sim(sc)
+    sim.update_system()
+    A = sim.get("Aloc").reshape(sc.nT, 12, 12)
+    dX = sim.get("dX").reshape(sc.nT, 4, 3)
+    vols = sim.get("vols")
+    h2 = sc.dt * sc.dt
+
+    def energy(e, q):
+        F = sum(np.outer(q[3 * k:3 * k + 3], dX[e, k]) for k in range(4))
+        R, _, _, _ = oracle.newton_procrustes(np.eye(3), np.eye(3), F, tol=1e-13, max_iter=200)
+        psi, _, _ = oracle.material_s6(sc.mat[e], sc.mu[e], sc.la[e], s_of(R, F), psd_fix=False)
+        return vols[e] * h2 * psi
+
+    d = 1e-4
+    for e in (0, 5, sc.nT - 1):
+        Ae = A[e].T  # column-major store
+        scale = np.abs(Ae).max()
+        assert np.abs(Ae - Ae.T).max() <= 1e-12 * scale
+        for t in np.eye(3):
+            assert np.abs(Ae @ np.tile(t, 4)).max() <= 1e-10 * scale
+        q0 = sc.V[sc.T[e]].reshape(12)
+        H = np.zeros((12, 12))
+        for i in range(12):
+            for j in range(i, 12):
+                ei, ej = d * np.eye(12)[i], d * np.eye(12)[j]
+                H[i, j] = H[j, i] = (energy(e, q0 + ei + ej) - energy(e, q0 + ei - ej)
+                                     - energy(e, q0 - ei + ej) + energy(e, q0 - ei - ej)) / (4 * d * d)
+        assert np.abs(H - Ae).max() <= 1e-7 * scale, (e, np.abs(H - Ae).max() / scale)
+
+
+def test_rotation_and_stretch_reassemble_F(oracle):
+    """On a deformed state the oracle's rotation and stretch multiply back to
+    the deformation gradient built from its positions and dphi/dX (the
+    reference's F = Jloc q, tests/test_jacobian.cpp:8-30): F = R S with S the
+    symmetric stretch, to the rotation fit's tolerance."""
+    sc = scenes.beam_scene(2, 1, 1, seed=4, mixed=True, E_range=(1e5, 1e6))
+    sim = oracle.Sim(sc)
+    sim.step()
+    sim.update_system()
+    x = sim.get("positions").reshape(-1, 3)
+    dX = sim.get("dX").reshape(sc.nT, 4, 3)
+    R = sim.get("R").reshape(sc.nT, 3, 3)
+    S = sim.get("S").reshape(sc.nT, 6)
+    for e in range(sc.nT):
+        F = sum(np.outer(x[sc.T[e, k]], dX[e, k]) for k in range(4))
+        Re = R[e].T  # column-major store
+        s = S[e]
+        Sm = np.array([[s[0], s[3], s[4]], [s[3], s[1], s[5]], [s[4], s[5], s[2]]])
+        assert np.abs(Re @ Re.T - np.eye(3)).max() < 1e-12
+        assert np.abs(Re @ Sm - F).max() <= 1e-5 * np.abs(F).max(), e
+
+
 # ---------------------------------------------------------------- whole step
 
 def test_free_fall_closed_form(oracle):
