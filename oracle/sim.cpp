@@ -2026,7 +2026,8 @@ long oracle_sim_size(void* sim, const char* name) {
   if (k == "Aloc") return (long)S_->Aloc.size();
   if (k == "Hplain") return (long)S_->Hp.size();
   if (k == "gplain") return (long)S_->gp.size();
-  if (k == "lhs" || k == "PMP") return (long)S_->lhs.size();
+  if (k == "lhs") return (long)S_->lhs.size();
+  if (k == "PMP") return (long)S_->PMP.size();
   if (k == "rhs" || k == "dx" || k == "x") return S_->n;
   if (k == "vols") return nT;
   if (k == "dX") return (long)S_->dX.size();

@@ -356,6 +356,39 @@ def test_element_block_is_energy_hessian_at_rest(oracle):
         assert np.abs(H - Ae).max() <= 1e-7 * scale, (e, np.abs(H - Ae).max() / scale)
 
 
+def test_consistent_mass_by_quadrature(oracle):
+    """The assembled mass matrix (reference tet_mesh.cpp:36-41, Bartels'
+    linear_tetmesh_mass_matrix) against rho * integral of phi_a phi_b by the
+    4-point degree-2 tet rule (exact for products of linear basis functions),
+    with no vertex pinned so that P M Pᵀ is all of M: equal entries, total
+    mass rho * sum(vol) per component, symmetric positive definite."""
+    sc = scenes.beam_scene(2, 2, 1, seed=6, mixed=False)
+    sc.fixed = np.zeros_like(sc.fixed)
+    sim = oracle.Sim(sc)
+    n = 3 * sc.nV
+    rp = sim.get_int("rowptr", n + 1)
+    v = sim.get("PMP")
+    ci = sim.get_int("colind", v.size)
+    M = np.zeros((n, n))
+    for i in range(n):
+        M[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    vols, _ = oracle.tet_geometry(sc.V, sc.T)
+    a, b = 0.5854101966249685, 0.1381966011250105  # barycentric points of the rule
+    bary = np.full((4, 4), b) + np.eye(4) * (a - b)  # row q: phi_k at point q
+    Mq = np.zeros((n, n))
+    for e in range(sc.nT):
+        me = sc.density * vols[e] / 4.0 * bary.T @ bary  # sum_q w_q phi_a phi_b, w_q = V/4
+        for i in range(4):
+            for j in range(4):
+                for r in range(3):
+                    Mq[3 * sc.T[e, i] + r, 3 * sc.T[e, j] + r] += me[i, j]
+    assert np.abs(M - Mq).max() <= 1e-13 * np.abs(Mq).max()
+    for r in range(3):
+        assert abs(M[r::3, r::3].sum() - sc.density * vols.sum()) <= 1e-12 * sc.density * vols.sum()
+    assert np.abs(M - M.T).max() == 0.0
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
 def test_rotation_and_stretch_reassemble_F(oracle):
     """On a deformed state the oracle's rotation and stretch multiply back to
     the deformation gradient built from its positions and dphi/dX (the
