@@ -107,6 +107,8 @@ def lib():
         L.oracle_jacobi_pcg.restype = C.c_int
         L.oracle_envelope_solve.argtypes = [C.c_int, C.c_int, ip, ip, dp, dp, dp, ip]
         L.oracle_envelope_solve.restype = C.c_int
+        L.oracle_bdf_tables.argtypes = [C.c_int, dp, dp]
+        L.oracle_bdf_tables.restype = C.c_int
         _lib = L
     return _lib
 
@@ -228,6 +230,14 @@ def bc_fixed(kind_id, V, axis, ratio):
     lib().oracle_bc_fixed(int(kind_id), _dp(V), V.shape[0], V.shape[1], int(axis), float(ratio),
                           _ip(out))
     return out
+
+
+def bdf_tables(order):
+    """BDF weights a (x_tilde = sum_j a_j x_{n-j} + ...) and beta of an order 1-6."""
+    a = np.zeros(6)
+    beta = np.zeros(1)
+    n = lib().oracle_bdf_tables(int(order), _dp(a), _dp(beta))
+    return a[:n].copy(), beta[0]
 
 
 def tet_geometry(V, T):

@@ -1700,6 +1700,14 @@ void bdf_tables(int I, std::vector<double>& a, double& beta) {
 
 extern "C" {
 
+// the BDF table of `order` (1-6): a[0 .. order-1] and beta; returns order
+int oracle_bdf_tables(int order, double* a, double* beta) {
+  std::vector<double> av;
+  bdf_tables(order, av, *beta);
+  std::copy(av.begin(), av.end(), a);
+  return (int)av.size();
+}
+
 int oracle_newton_procrustes(double* R, const double* A, const double* B, int cg,
                              double* dRdF, double tol, int max_iter, int* iters) {
   M3 Rm, Am, Bm;

@@ -413,6 +413,29 @@ def test_rotation_and_stretch_reassemble_F(oracle):
 
 # ---------------------------------------------------------------- whole step
 
+@pytest.mark.parametrize("order", range(1, 7))
+def test_bdf_tables_are_the_bdf_formulas(oracle, order):
+    """The BDF tables (reference time_integrators/BDF.cpp:45-93), pinned by
+    what defines BDF-k: x_{n+1} = sum_j a_j x_{n-j} + beta h x'_{n+1} holds
+    exactly for every polynomial of degree <= k. With h = 1 and t_{n+1} = 0
+    these are k + 1 linear conditions on (a_0 .. a_{k-1}, beta):
+    1 = sum a_j; 0 = -sum a_j (j + 1) + beta; 0 = sum a_j (-(j + 1))^m,
+    m = 2..k; solved here and compared with the table."""
+    a, beta = oracle.bdf_tables(order)
+    k = order
+    A = np.zeros((k + 1, k + 1))
+    rhs = np.zeros(k + 1)
+    for m in range(k + 1):
+        for j in range(k):
+            A[m, j] = (-(j + 1.0)) ** m
+        A[m, k] = 1.0 if m == 1 else 0.0
+    rhs[0] = 1.0
+    # row m reads sum_j a_j t_j^m + beta m 0^(m-1) = 0^m
+    sol = np.linalg.solve(A, rhs)
+    assert np.allclose(a, sol[:k], rtol=0, atol=1e-13), (a, sol[:k])
+    assert abs(beta - sol[k]) < 1e-13
+
+
 def test_free_fall_closed_form(oracle):
     """No pins, no deformation: BDF1 gives x_n = x_0 + h^2 g n(n+1)/2."""
     sc = scenes.beam_scene(2, 1, 1, jitter=0.0, mixed=True, solver="eigen-llt",
