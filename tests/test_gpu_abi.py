@@ -96,6 +96,25 @@ def test_pcg_bsr3_iteration_cap_and_empty(native, oracle):
     assert it == 0 and rr == 0.0 and x.numel() == 0
 
 
+def test_pcg_bsr3_zero_rhs_and_one_block_row(native, oracle):
+    """Eigen's rule for b = 0 (x = 0, no iteration) on a real system, and the
+    smallest system, one 3x3 block row, against the oracle and a dense solve."""
+    sc = scenes.beam_scene(3, 2, 2, seed=24, mixed=False)
+    rp, ci, vals, rhs = system_of(native, sc)
+    brp, bcol, bval = csr_to_bsr3(rp, ci, vals)
+    x, it, rr = native.pcg_bsr3(*to_dev(brp, bcol, bval, np.zeros_like(rhs)), 1e-4, 100)
+    assert it == 0 and rr == 0.0 and not x.cpu().numpy().any()
+    A = np.array([[4.0, 1.0, 0.5], [1.0, 3.0, 0.2], [0.5, 0.2, 2.0]])
+    b = np.array([1.0, -2.0, 0.5])
+    x, it, rr = native.pcg_bsr3(*to_dev(np.array([0, 1], np.int32), np.array([0], np.int32),
+                                        A.reshape(-1), b), 1e-12, 100)
+    xo, ito, _ = oracle.jacobi_pcg(np.array([0, 3, 6, 9], np.int32),
+                                   np.array([0, 1, 2] * 3, np.int32), A.reshape(-1), b, 1e-12, 100)
+    assert it == ito
+    assert np.abs(x.cpu().numpy() - np.linalg.solve(A, b)).max() <= 1e-12
+    assert np.abs(x.cpu().numpy() - xo).max() <= 1e-12
+
+
 def test_pcg_bsr3_rejects_bad_patterns(native):
     sc = scenes.beam_scene(3, 2, 2, seed=23, mixed=False)
     rp, ci, vals, rhs = system_of(native, sc)
