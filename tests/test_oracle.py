@@ -256,6 +256,23 @@ def test_tet_geometry_fixtures(oracle):
             assert np.allclose(dX[e].sum(axis=0), 0, atol=1e-12)
 
 
+def test_tri_geometry_conventions(oracle):
+    """2D geometry (reference tri2d_mesh.cpp:25-34): the per-triangle "volume"
+    is igl::doublearea, twice the area (|det Dm|), dphi/dX makes F(X_ref) = I,
+    and its rows sum to 0."""
+    sc = scenes.rect2d_scene(6, 3, 0.1)
+    sim = oracle.Sim(sc)
+    dX = sim.get("dX").reshape(sc.nT, 3, 2)
+    vols = sim.get("vols")
+    for e in range(sc.nT):
+        X = sc.V[sc.T[e]][:, :2]
+        Dm = (X[1:] - X[0]).T
+        assert abs(vols[e] - abs(np.linalg.det(Dm))) < 1e-15
+        F = sum(np.outer(X[k], dX[e, k]) for k in range(3))
+        assert np.allclose(F, np.eye(2), atol=1e-12)
+        assert np.allclose(dX[e].sum(axis=0), 0, atol=1e-12)
+
+
 def test_kuhn_beam_counts_and_clamp(oracle):
     V, T = scenes.kuhn_beam(40, 10, 10, 0.1, 0.1, seed=0)
     assert V.shape == (4961, 3) and T.shape == (24000, 4)
